@@ -1,0 +1,208 @@
+// dst_core.hpp -- batched 1-D DST-I passes for gfx950.
+//
+// A 2-D DST-I of an (m-1) x (n-1) vertex field (reference src/myfft.f90:83-94,
+// FFTW RODFT00 on both axes, unnormalised) is two 1-D passes: one along the
+// contiguous axis i ("row pass") and one along j ("column pass").  Both passes
+// read and write the natural i-fastest layout, so the field never needs a
+// transposed copy.
+//
+// One 1-D DST-I of length N-1 is
+//     Y_k = 2 sum_{j=1}^{N-1} x_j sin(pi j k / N),   k = 1..N-1.
+// Two real sequences a, b are packed as z = a + i b and extended oddly to
+// length 2N (z_0 = z_N = 0, z_{2N-j} = -z_j).  Then the forward DFT gives
+// Z_k = -i Y(a)_k + Y(b)_k, so Y(b)_k = Re Z_k and Y(a)_k = -Im Z_k with no
+// post-processing.  The complex DFT is the Stockham autosort formulation of
+// Govindaraju et al., SC08, done in LDS with radix 4/2/3/5 stages.
+//
+// A workgroup (256 threads = 4 wave64) holds P = EMAX / (2N) such FFTs in LDS
+// (EMAX complex values).  Every stage first pulls all butterfly inputs of a
+// thread into registers, then (after a barrier) writes the outputs back in
+// place, so one LDS buffer suffices.  The radix schedule and the twiddle
+// table (exp(-2 pi i k / 2N), k < 2N) are runtime data, so one kernel
+// instantiation serves every supported size; the per-radix stage code is a
+// template so that register arrays are statically indexed.
+//
+// Fusion: the pass takes an operator object with
+//     T    load(int b, int g, int s, int P, int idx)
+//              value of local sequence s (0..2P-1) of group g, batch b, at
+//              position idx (0..N-2); 0 for sequences that do not exist
+//     void store(int b, int g, int p, int P, int k, T ya, T yb)
+//              outputs of FFT p (0..P-1) at k (1..N-1): ya is the transform
+//              of local sequence p, yb that of local sequence p+P
+// so that pointwise prologue/epilogue work (the AB2 combine, boundary
+// terms, eigenvalue scaling, the CN combine) runs inside the pass.  The row
+// pass (AXIS 0) walks loads/stores along idx/k, which is contiguous in
+// memory; the column pass (AXIS 1) walks them along s/p, i.e. across
+// adjacent columns of one row.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ibfs {
+
+constexpr int kThreads = 256;
+constexpr int kMaxStages = 16;
+
+template <typename T>
+struct cplx {
+  T re, im;
+};
+
+template <typename T>
+struct FftPlan {
+  int n2;                  // complex FFT length 2N
+  int nst;                 // number of stages
+  int rad[kMaxStages];     // radix of each stage
+  const cplx<T>* tw;       // n2 twiddles exp(-2 pi i k / n2)
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// In-register DFT of radix R (forward, exp(-2 pi i q r / R)).
+template <typename T, int R>
+__device__ __forceinline__ void dft_small(cplx<T>* v) {
+  if constexpr (R == 2) {
+    cplx<T> a = v[0], b = v[1];
+    v[0] = {a.re + b.re, a.im + b.im};
+    v[1] = {a.re - b.re, a.im - b.im};
+  } else if constexpr (R == 4) {
+    cplx<T> t0 = {v[0].re + v[2].re, v[0].im + v[2].im};
+    cplx<T> t1 = {v[0].re - v[2].re, v[0].im - v[2].im};
+    cplx<T> t2 = {v[1].re + v[3].re, v[1].im + v[3].im};
+    cplx<T> t3 = {v[1].re - v[3].re, v[1].im - v[3].im};
+    v[0] = {t0.re + t2.re, t0.im + t2.im};
+    v[2] = {t0.re - t2.re, t0.im - t2.im};
+    v[1] = {t1.re + t3.im, t1.im - t3.re};  // t1 - i t3
+    v[3] = {t1.re - t3.im, t1.im + t3.re};  // t1 + i t3
+  } else if constexpr (R == 3) {
+    // c = cos(2 pi/3) = -1/2, s = sin(2 pi/3) = sqrt(3)/2
+    const T s3 = T(0.86602540378443864676372317075293618);
+    cplx<T> a = v[0], b = v[1], c = v[2];
+    cplx<T> sum = {b.re + c.re, b.im + c.im};
+    cplx<T> dif = {b.re - c.re, b.im - c.im};
+    cplx<T> mid = {a.re - T(0.5) * sum.re, a.im - T(0.5) * sum.im};
+    v[0] = {a.re + sum.re, a.im + sum.im};
+    // v1 = mid - i s dif, v2 = mid + i s dif
+    v[1] = {mid.re + s3 * dif.im, mid.im - s3 * dif.re};
+    v[2] = {mid.re - s3 * dif.im, mid.im + s3 * dif.re};
+  } else if constexpr (R == 5) {
+    const T c1 = T(0.30901699437494742410229341718281906);   // cos(2 pi/5)
+    const T c2 = T(-0.80901699437494742410229341718281906);  // cos(4 pi/5)
+    const T s1 = T(0.95105651629515357211643933337938214);   // sin(2 pi/5)
+    const T s2 = T(0.58778525229247312916870595463907277);   // sin(4 pi/5)
+    cplx<T> a = v[0];
+    cplx<T> p1 = {v[1].re + v[4].re, v[1].im + v[4].im};
+    cplx<T> m1 = {v[1].re - v[4].re, v[1].im - v[4].im};
+    cplx<T> p2 = {v[2].re + v[3].re, v[2].im + v[3].im};
+    cplx<T> m2 = {v[2].re - v[3].re, v[2].im - v[3].im};
+    v[0] = {a.re + p1.re + p2.re, a.im + p1.im + p2.im};
+    cplx<T> r1 = {a.re + c1 * p1.re + c2 * p2.re, a.im + c1 * p1.im + c2 * p2.im};
+    cplx<T> r2 = {a.re + c2 * p1.re + c1 * p2.re, a.im + c2 * p1.im + c1 * p2.im};
+    // i-parts: q1 = s1 m1 + s2 m2, q2 = s2 m1 - s1 m2 ; v1 = r1 - i q1, v4 = r1 + i q1
+    cplx<T> q1 = {s1 * m1.re + s2 * m2.re, s1 * m1.im + s2 * m2.im};
+    cplx<T> q2 = {s2 * m1.re - s1 * m2.re, s2 * m1.im - s1 * m2.im};
+    v[1] = {r1.re + q1.im, r1.im - q1.re};
+    v[4] = {r1.re - q1.im, r1.im + q1.re};
+    v[2] = {r2.re + q2.im, r2.im - q2.re};
+    v[3] = {r2.re - q2.im, r2.im + q2.re};
+  }
+}
+
+// One Stockham stage of radix R over nfft FFTs of length n2 held in z.
+template <typename T, int R, int EMAX>
+__device__ __forceinline__ void fft_stage(cplx<T>* z, int nfft, int n2, int ns, const cplx<T>* __restrict__ tw) {
+  constexpr int NB = (EMAX / R + kThreads - 1) / kThreads;
+  const int nr = n2 / R;
+  const int nbf = nfft * nr;
+  const int twstep = n2 / (ns * R);
+  cplx<T> v[NB][R];
+  int dst[NB];
+#pragma unroll
+  for (int t = 0; t < NB; t++) {
+    const int b = threadIdx.x + t * kThreads;
+    dst[t] = -1;
+    if (b < nbf) {
+      const int p = b / nr;
+      const int j = b - p * nr;
+      const int k = j % ns;
+      const cplx<T>* src = z + p * n2 + j;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[t][r] = src[r * nr];
+#pragma unroll
+      for (int r = 1; r < R; r++) v[t][r] = cmul(v[t][r], tw[k * r * twstep]);
+      dst[t] = p * n2 + (j - k) * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NB; t++) {
+    if (dst[t] >= 0) {
+      dft_small<T, R>(v[t]);
+#pragma unroll
+      for (int r = 0; r < R; r++) z[dst[t] + r * ns] = v[t][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int EMAX>
+__device__ __forceinline__ void fft_run(cplx<T>* z, int nfft, const FftPlan<T>& plan) {
+  int ns = 1;
+  for (int s = 0; s < plan.nst; s++) {
+    const int R = plan.rad[s];
+    switch (R) {
+      case 4: fft_stage<T, 4, EMAX>(z, nfft, plan.n2, ns, plan.tw); break;
+      case 2: fft_stage<T, 2, EMAX>(z, nfft, plan.n2, ns, plan.tw); break;
+      case 3: fft_stage<T, 3, EMAX>(z, nfft, plan.n2, ns, plan.tw); break;
+      default: fft_stage<T, 5, EMAX>(z, nfft, plan.n2, ns, plan.tw); break;
+    }
+    ns *= R;
+  }
+}
+
+// AXIS 0: sequences are contiguous rows; AXIS 1: sequences are columns.
+// grid.x = groups of sequences, grid.y = batch.
+template <typename T, int EMAX, int AXIS, class Op>
+__global__ __launch_bounds__(kThreads) void dst_pass(Op op, FftPlan<T> plan) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* z = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int n2 = plan.n2;
+  const int N = n2 / 2;
+  const int len = N - 1;
+  const int P = EMAX / n2;          // FFTs per workgroup
+  const int b = blockIdx.y;
+  const int g = blockIdx.x;
+
+  // zero the two fixed points of every odd extension
+  for (int p = threadIdx.x; p < P; p += kThreads) {
+    z[p * n2] = {T(0), T(0)};
+    z[p * n2 + N] = {T(0), T(0)};
+  }
+  // load: local sequence s -> FFT (s mod P), real part if s < P else imaginary
+  const int tot = 2 * P * len;
+  for (int e = threadIdx.x; e < tot; e += kThreads) {
+    int s, idx;
+    if (AXIS == 0) { s = e / len; idx = e - s * len; }
+    else { idx = e / (2 * P); s = e - idx * (2 * P); }
+    const T x = op.load(b, g, s, P, idx);
+    const int p = s < P ? s : s - P;
+    const int h = s < P ? 0 : 1;
+    reinterpret_cast<T*>(z + p * n2 + idx + 1)[h] = x;
+    reinterpret_cast<T*>(z + p * n2 + n2 - idx - 1)[h] = -x;
+  }
+  __syncthreads();
+  fft_run<T, EMAX>(z, P, plan);
+  // store: Y(a)_k = -Im Z_k, Y(b)_k = Re Z_k
+  const int tot2 = P * len;
+  for (int e = threadIdx.x; e < tot2; e += kThreads) {
+    int p, k;
+    if (AXIS == 0) { p = e / len; k = e - p * len + 1; }
+    else { k = e / P + 1; p = e - (k - 1) * P; }
+    const cplx<T> Z = z[p * n2 + k];
+    op.store(b, g, p, P, k, -Z.im, Z.re);
+  }
+}
+
+}  // namespace ibfs

@@ -1,0 +1,226 @@
+"""ctypes binding of the gfx950 IBFS library (include/ibfs.h).
+
+``Solver`` mirrors the reference's module procedures: ``advance`` is
+src/operators.f90:62-188, ``vort2flux``/``curl``/``rot``/``coarsify``/
+``get_bc``/``reg``/``regT``/``a_times``/``cholsl`` are the operators of
+src/operators.f90 and ``dst``/``ctci``/``ainv`` those of src/myfft.f90.
+Arrays are numpy float64 in the reference layouts (Fortran column-major,
+flattened).  Errors raise ``IBFSError`` with the library's message, the way
+the reference STOPs.
+
+There is no CPU fallback: without ``libibfs_hip.so`` the import fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libibfs_hip.so")
+
+PLAIN_FP64 = 0
+MIXED = 1
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+class IBFSError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m ibfs_amd.build` "
+                          "(the IBFS step has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    L.ibfs_create.restype = C.c_int
+    L.ibfs_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                              C.c_double, _D, _D, _I, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.ibfs_dims.argtypes = [C.c_void_p, _I]
+    L.ibfs_set_state.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
+    L.ibfs_get_state.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
+    L.ibfs_set_cholesky.argtypes = [C.c_void_p, _D, _D]
+    L.ibfs_build_cholesky.argtypes = [C.c_void_p, _D, _D]
+    L.ibfs_step.argtypes = [C.c_void_p, _I, _D, _D, _D, _D, _D]
+    L.ibfs_advance.argtypes = [C.c_void_p, _I, C.c_int, _D, _D]
+    for nm in ("ibfs_op_dst", "ibfs_op_ctci", "ibfs_op_ainv", "ibfs_op_rot", "ibfs_op_reg", "ibfs_op_regT",
+               "ibfs_op_a_times", "ibfs_op_cholsl"):
+        getattr(L, nm).argtypes = [C.c_void_p, _D, _D]
+    L.ibfs_op_curl.argtypes = [C.c_void_p, _D, _D, _D]
+    L.ibfs_op_coarsify.argtypes = [C.c_void_p, _D, _D, _D]
+    L.ibfs_op_get_bc.argtypes = [C.c_void_p, _D, C.c_double, _D]
+    L.ibfs_op_rot_nonlinear.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
+    L.ibfs_op_vort2flux.argtypes = [C.c_void_p, _D, _D, C.c_int]
+    L.ibfs_last_advance_ms.argtypes = [C.c_void_p, _D]
+    L.ibfs_time_dst_pass.argtypes = [C.c_void_p, C.c_int, C.c_int, _D]
+    L.ibfs_last_error.restype = C.c_char_p
+    L.ibfs_last_error.argtypes = [C.c_void_p]
+    L.ibfs_destroy.argtypes = [C.c_void_p]
+    return L
+
+
+_lib = _load()
+EXPORTED = [n for n in dir(_lib) if n.startswith("ibfs_")]
+
+
+def lib():
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):
+        raise TypeError("expected a C-contiguous float64 numpy array")
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Solver:
+    """One case on the GPU (one handle, one HIP stream)."""
+
+    def __init__(self, m, n, mgridlev, dt, re, len_, offx, offy, xb, yb, bdyno=None, unif_flow=True,
+                 precision=PLAIN_FP64):
+        xb = _f64(xb)
+        yb = _f64(yb)
+        nb = xb.size
+        bd = None if bdyno is None else np.ascontiguousarray(bdyno, dtype=np.int32)
+        h = C.c_void_p()
+        rc = _lib.ibfs_create(m, n, mgridlev, nb, dt, re, len_, offx, offy, _p(xb), _p(yb),
+                              None if bd is None else bd.ctypes.data_as(_I), 1 if unif_flow else 0, precision,
+                              C.byref(h))
+        self._h = h
+        if rc:
+            msg = _lib.ibfs_last_error(h).decode() if h.value else "invalid arguments"
+            self.close()
+            raise IBFSError(f"ibfs_create failed ({rc}): {msg}")
+        d = (C.c_int * 12)()
+        _lib.ibfs_dims(h, d)
+        (self.m, self.n, self.L, self.nb, self.nf, self.nq, self.nw, self.nbc, self.nbody, self.bmin, self.bmax,
+         self.nsmear) = list(d)
+        self.dt = dt
+        self.itime = 0
+        self.ncols = 2 * self.nbody + self.L + 1
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.ibfs_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise IBFSError(f"{what} failed ({rc}): {_lib.ibfs_last_error(self._h).decode()}")
+
+    # ---- state ----
+    def set_state(self, omega=None, rhs_old=None, q=None, q0=None, fb=None):
+        a = [_f64(v) for v in (omega, rhs_old, q, q0, fb)]
+        self._chk(_lib.ibfs_set_state(self._h, *[_p(v) for v in a]), "ibfs_set_state")
+
+    def get_state(self):
+        om = np.zeros(self.nw * self.L)
+        ro = np.zeros(self.nw * self.L)
+        q = np.zeros(self.nq * self.L)
+        q0 = np.zeros(self.nq * self.L)
+        fb = np.zeros(self.nf)
+        self._chk(_lib.ibfs_get_state(self._h, _p(om), _p(ro), _p(q), _p(q0), _p(fb)), "ibfs_get_state")
+        return {"omega": om, "rhs_old": ro, "q": q, "q0": q0, "fb": fb}
+
+    def set_cholesky(self, cholmat, cholvec):
+        self._chk(_lib.ibfs_set_cholesky(self._h, _p(_f64(cholmat)), _p(_f64(cholvec))), "ibfs_set_cholesky")
+
+    def build_cholesky(self):
+        cm = np.zeros(self.nf * self.nf)
+        cv = np.zeros(self.nf)
+        self._chk(_lib.ibfs_build_cholesky(self._h, _p(cm), _p(cv)), "ibfs_build_cholesky")
+        return cm, cv
+
+    # ---- time stepping ----
+    def advance(self, nsteps=1, motion=None):
+        """nsteps calls of advance; returns an (nsteps, 2*nbody+L+1) history of
+        [fx.., fy.., circ(1..L), slip]."""
+        it = C.c_int(self.itime)
+        hist = np.zeros(nsteps * self.ncols)
+        self._chk(_lib.ibfs_advance(self._h, C.byref(it), nsteps, _p(_f64(motion)), _p(hist)), "ibfs_advance")
+        self.itime = it.value
+        return hist.reshape(nsteps, self.ncols)
+
+    def advance_timed(self, nsteps, motion=None, hist=False):
+        """nsteps steps; returns device milliseconds measured with HIP events."""
+        it = C.c_int(self.itime)
+        h = np.zeros(nsteps * self.ncols) if hist else None
+        self._chk(_lib.ibfs_advance(self._h, C.byref(it), nsteps, _p(_f64(motion)), _p(h)), "ibfs_advance")
+        self.itime = it.value
+        ms = C.c_double()
+        _lib.ibfs_last_advance_ms(self._h, C.byref(ms))
+        return ms.value
+
+    def time_dst_pass(self, axis, reps=200):
+        """Mean device time (ms) of one 1-D DST-I pass over a W field."""
+        ms = C.c_double()
+        self._chk(_lib.ibfs_time_dst_pass(self._h, axis, reps, C.byref(ms)), "ibfs_time_dst_pass")
+        return ms.value
+
+    def split(self, row):
+        nb = self.nbody
+        return row[:nb], row[nb:2 * nb], row[2 * nb:2 * nb + self.L], row[-1]
+
+    # ---- operators ----
+    def _op(self, name, *ins, nout):
+        out = np.zeros(nout)
+        self._chk(getattr(_lib, name)(self._h, *[_p(_f64(v)) for v in ins], _p(out)), name)
+        return out
+
+    def dst(self, x):
+        return self._op("ibfs_op_dst", x, nout=self.nw)
+
+    def ctci(self, x):
+        return self._op("ibfs_op_ctci", x, nout=self.nw)
+
+    def ainv(self, x):
+        return self._op("ibfs_op_ainv", x, nout=self.nw)
+
+    def rot(self, x):
+        return self._op("ibfs_op_rot", x, nout=self.nw)
+
+    def reg(self, h0):
+        return self._op("ibfs_op_reg", h0, nout=self.nq)
+
+    def regT(self, h):
+        return self._op("ibfs_op_regT", h, nout=self.nf)
+
+    def a_times(self, x):
+        return self._op("ibfs_op_a_times", x, nout=self.nf)
+
+    def cholsl(self, b):
+        return self._op("ibfs_op_cholsl", b, nout=self.nf)
+
+    def coarsify(self, crhs, rhs):
+        return self._op("ibfs_op_coarsify", crhs, rhs, nout=self.nw)
+
+    def rot_nonlinear(self, omega, q, q0, bc):
+        return self._op("ibfs_op_rot_nonlinear", omega, q, q0, bc, nout=self.nw)
+
+    def curl(self, x, coarse=None):
+        out = np.zeros(self.nq)
+        self._chk(_lib.ibfs_op_curl(self._h, _p(_f64(x)), _p(_f64(coarse)), _p(out)), "ibfs_op_curl")
+        return out
+
+    def get_bc(self, r, fac):
+        out = np.zeros(self.nbc)
+        self._chk(_lib.ibfs_op_get_bc(self._h, _p(_f64(r)), fac, _p(out)), "ibfs_op_get_bc")
+        return out
+
+    def vort2flux(self, omega, nlev):
+        om = np.array(omega, dtype=np.float64, copy=True)
+        vel = np.zeros(self.nq * max(nlev, 1))
+        self._chk(_lib.ibfs_op_vort2flux(self._h, _p(om), _p(vel), nlev), "ibfs_op_vort2flux")
+        return om, vel[: self.nq * nlev]

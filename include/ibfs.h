@@ -1,0 +1,139 @@
+/*
+ * ibfs.h -- C-ABI of the MI355X (gfx950) IBFS time-step library.
+ *
+ * The reference has no FFI: its seam is the Fortran call
+ *     CALL advance(it)                       (src/main.f90:47-50)
+ * with every input and output held in module globals (src/variables.f90:13-23,
+ * src/grid.f90:12-28, src/myfft.f90:12-17).  This library owns those arrays on
+ * the device; a Fortran host binds these entry points with ISO_C_BINDING and
+ * keeps the namelist input, the advance loop and the output files.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a non-zero IBFS_E* code on
+ *     failure; ibfs_last_error() gives the message (the Fortran shim turns it
+ *     into STOP, as the reference's own failures are STOPs).
+ *   - All arrays are host memory in the reference's layouts (Fortran
+ *     column-major, 1-based in Fortran):
+ *       omega, rhs_old : (m-1)*(n-1)*L, i fastest      (variables.f90:46)
+ *       q, q0          : Nq*L, u faces then v faces     (grid.f90:119-134)
+ *       fb, motion     : Nf = 2*Nb, x comps then y comps (grid.f90:105-116)
+ *       cholmat        : Nf*Nf column-major, cholvec: Nf (variables.f90:22-23)
+ *     They are copied, never retained past the call.
+ *   - One handle owns one HIP stream and its device memory.  Calls on one
+ *     handle are not thread-safe (the reference is single-threaded too).
+ */
+#ifndef IBFS_H
+#define IBFS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ibfs_handle ibfs_handle;
+
+enum {
+  IBFS_OK = 0,
+  IBFS_EINVAL = 1,  /* bad argument or unsupported size */
+  IBFS_EHIP = 2,    /* a HIP runtime call failed */
+  IBFS_ECHOL = 3,   /* Cholesky factorisation failed (matrix not SPD) */
+  IBFS_ESTATE = 4   /* call out of order (e.g. a step before the factor exists) */
+};
+
+/* Precision modes. */
+enum {
+  IBFS_PLAIN_FP64 = 0, /* FP64 throughout: the drop-in */
+  IBFS_MIXED = 1       /* FP32 transforms/nonlinear/smearing; FP64 force solve,
+                          reductions and one refinement sweep */
+};
+
+/* Creates the solver for one case.  Replaces the set-up chain
+ * input -> setup -> setup_variables (src/main.f90:42-44):
+ *   grid spacing delta = len/m and the %4 check     (src/grid.f90:64-69)
+ *   level clamp mgridlev <= nlevmax                 (src/grid.f90:79-82)
+ *   index maps and smearing tables (compact form)   (src/grid.f90:99-137,142-230)
+ *   eigenvalue / AB2 / CN tables                    (src/myfft.f90:21-70)
+ *   zero-flow or uniform-flow initial condition     (src/variables.f90:54-67)
+ * xb, yb, bdyno: the Nb immersed-boundary points as read from geom.inp
+ * (src/grid.f90:270-276).  bdyno may be NULL (all 0). */
+int ibfs_create(int m, int n, int mgridlev, int nb, double dt, double re, double len, double offsetx,
+                double offsety, const double* xb, const double* yb, const int* bdyno, int unif_flow,
+                int precision, ibfs_handle** out);
+
+/* dims[0..11] = m, n, L (after the clamp), Nb, Nf, Nq, W=(m-1)(n-1),
+ * nbc=2(m+1)+2(n+1), number of bodies, min bdyno, max bdyno, Nsmear. */
+int ibfs_dims(const ibfs_handle* h, int* dims);
+
+/* Replaces read_variables / write_variables' state exchange
+ * (src/variables.f90:141-175).  NULL pointers are skipped. */
+int ibfs_set_state(ibfs_handle* h, const double* omega, const double* rhs_old, const double* q, const double* q0,
+                   const double* fb);
+int ibfs_get_state(ibfs_handle* h, double* omega, double* rhs_old, double* q, double* q0, double* fb);
+
+/* Replaces read_cholesky (src/variables.f90:124-137): loads a factor in the
+ * ibfs.chd layout (strict lower triangle = L, diagonal in cholvec). */
+int ibfs_set_cholesky(ibfs_handle* h, const double* cholmat, const double* cholvec);
+
+/* Replaces preprocess (src/operators.f90:29-59): builds
+ * A = E C (C^T C)^-1_multidomain ainv C^T E^T column by column on the device
+ * (a_times, src/operators.f90:444-462, batched over columns) and factors it
+ * as choldc does (src/operators.f90:766-791).  If cholmat/cholvec are not
+ * NULL the factor is returned in the ibfs.chd layout for the host to cache. */
+int ibfs_build_cholesky(ibfs_handle* h, double* cholmat, double* cholvec);
+
+/* One time step: replaces advance (src/operators.f90:62-188) for stationary
+ * bodies.  *itime is incremented on entry as in the reference (line 77); at
+ * *itime == 1 the explicit Euler start is taken (lines 115-117).  motion
+ * (Nf, may be NULL = 0) is what move_body/actuation would return, already
+ * scaled by delta (src/grid.f90:573-574); the library subtracts regT(q0)
+ * (line 155).  Outputs (each may be NULL):
+ *   fb    : Nf forces, the reference's fb
+ *   force : [fx(bmin..bmax), fy(bmin..bmax)] as written by write_force (line 184)
+ *   circ  : L circulations as written by write_circ (line 185)
+ *   slip  : max |E q - motion| as written by write_slip (line 186) */
+int ibfs_step(ibfs_handle* h, int* itime, const double* motion, double* fb, double* force, double* circ,
+              double* slip);
+
+/* nsteps consecutive calls of advance with a constant motion vector.  hist
+ * (may be NULL) receives nsteps rows of [fx.., fy.., circ(1..L), slip]
+ * (2*nbody + L + 1 values per row), i.e. the lines of force.dat, circ.dat
+ * and slip.dat.  Steps after the first are replayed from one captured HIP
+ * graph. */
+int ibfs_advance(ibfs_handle* h, int* itime, int nsteps, const double* motion, double* hist);
+
+/* Operator entry points, one per reference module procedure; used by the
+ * parity tests.  Each copies its inputs in, runs the device kernels on the
+ * handle's stream, and copies the result out; none touches the state. */
+int ibfs_op_dst(ibfs_handle* h, const double* in, double* out);                 /* src/myfft.f90:83-94   */
+int ibfs_op_ctci(ibfs_handle* h, const double* in, double* out);                /* src/myfft.f90:97-109  */
+int ibfs_op_ainv(ibfs_handle* h, const double* in, double* out);                /* src/myfft.f90:112-124 */
+int ibfs_op_curl(ibfs_handle* h, const double* x, const double* coarse, double* vel);
+/* curl (src/operators.f90:243-288) with xbc = get_bc(coarse, 1.0); coarse == NULL means xbc = 0 */
+int ibfs_op_rot(ibfs_handle* h, const double* x, double* out);                  /* src/operators.f90:291-304 */
+int ibfs_op_coarsify(ibfs_handle* h, const double* crhs, const double* rhs, double* out); /* :307-333 */
+int ibfs_op_get_bc(ibfs_handle* h, const double* r, double fac, double* rbc);   /* src/operators.f90:336-367 */
+int ibfs_op_rot_nonlinear(ibfs_handle* h, const double* omega, const double* q, const double* q0, const double* bc,
+                          double* out); /* rot(nonlinear(...)), src/operators.f90:465-510 + 291-304 */
+int ibfs_op_vort2flux(ibfs_handle* h, double* omega, double* vel, int nlev);     /* src/operators.f90:191-240 */
+int ibfs_op_reg(ibfs_handle* h, const double* h0, double* hq);                  /* src/operators.f90:513-528 */
+int ibfs_op_regT(ibfs_handle* h, const double* hq, double* h0);                 /* src/operators.f90:531-546 */
+int ibfs_op_a_times(ibfs_handle* h, const double* x, double* out);              /* src/operators.f90:444-462 */
+int ibfs_op_cholsl(ibfs_handle* h, const double* b, double* x);                 /* src/operators.f90:794-819 */
+
+/* Device time of the last ibfs_advance call in milliseconds (HIP events on
+ * the handle's stream), for measurement. */
+int ibfs_last_advance_ms(const ibfs_handle* h, double* ms);
+
+/* Measurement only (no counterpart in the reference): launches one 1-D
+ * DST-I pass over a W field reps times on the handle's stream and returns
+ * the mean launch duration from HIP events.  axis 0 = along i (rows),
+ * 1 = along j (columns). */
+int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch);
+
+const char* ibfs_last_error(const ibfs_handle* h);
+int ibfs_destroy(ibfs_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IBFS_H */
