@@ -1,0 +1,135 @@
+MODULE operators
+  !---------------------------------------------------------------------------!
+  ! The time step seen from the host: advance(itime) keeps the reference's
+  ! interface (src/operators.f90:62-188) and output files, and hands the
+  ! step itself to the GPU library.
+  !---------------------------------------------------------------------------!
+  USE ISO_C_BINDING
+  USE parameters
+  USE grid
+  USE ibfs_c
+  USE variables
+  IMPLICIT NONE
+
+  LOGICAL :: files_open = .FALSE.
+  LOGICAL :: have_factor = .FALSE.
+  REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE :: motion, hist
+
+CONTAINS
+
+  SUBROUTINE preprocess
+    !-------------------------------------------------------------------------!
+    ! src/operators.f90:29-59: reuse input/ibfs.chd if present, else build
+    ! the force-system matrix and its Cholesky factor (on the GPU) and cache
+    ! it.  As in the reference, a cached factor is trusted as is.
+    !-------------------------------------------------------------------------!
+    LOGICAL :: readchol
+    REAL(C_DOUBLE), DIMENSION(:,:), ALLOCATABLE :: cm
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE :: cv
+
+    IF (.NOT. stationary) THEN
+       ! The CG path of moving bodies is not offloaded; a body with bdyno 0
+       ! is stationary, so the same matrix is factored once, in memory only.
+       ALLOCATE(cm(MAX(nf,1),MAX(nf,1)), cv(MAX(nf,1)))
+       CALL ibfs_check(ibfs_build_cholesky(handle, cm, cv), 'ibfs_build_cholesky')
+       DEALLOCATE(cm, cv)
+       have_factor = .TRUE.
+       RETURN
+    END IF
+    INQUIRE(file='input/ibfs.chd',exist=readchol)
+    IF (readchol) THEN
+       CALL read_cholesky
+       CALL ibfs_check(ibfs_set_cholesky(handle, cholmat, cholvec), 'ibfs_set_cholesky')
+    ELSE
+       WRITE(*,*) 'Precomputing body matrix on the GPU...'
+       WRITE(111,*) 'Precomputing body matrix...'
+       CALL ibfs_check(ibfs_build_cholesky(handle, cholmat, cholvec), 'ibfs_build_cholesky')
+       CALL write_cholesky
+    END IF
+    have_factor = .TRUE.
+  END SUBROUTINE preprocess
+
+  SUBROUTINE open_outputs(first_it)
+    INTEGER :: first_it
+    CALL open_one(100, 'output/force.dat', first_it)
+    CALL open_one(101, 'output/circ.dat', first_it)
+    CALL open_one(106, 'output/slip.dat', first_it)
+    files_open = .TRUE.
+  END SUBROUTINE open_outputs
+
+  SUBROUTINE open_one(u, fname, first_it)
+    ! same status choice as the per-step open of the reference
+    ! (src/operators.f90:566-576): replace at it==1, else append or create
+    INTEGER :: u, first_it
+    CHARACTER(*) :: fname
+    LOGICAL :: ex
+    INQUIRE(file=fname, exist=ex)
+    IF (first_it == 1) THEN
+       OPEN(unit=u,file=fname,form="formatted",status="replace")
+    ELSE IF (ex) THEN
+       OPEN(unit=u,file=fname,form="formatted",status="old",position="append")
+    ELSE
+       OPEN(unit=u,file=fname,form="formatted",status="new")
+    END IF
+  END SUBROUTINE open_one
+
+  SUBROUTINE close_outputs
+    IF (files_open) THEN
+       CLOSE(100); CLOSE(101); CLOSE(106)
+       files_open = .FALSE.
+    END IF
+  END SUBROUTINE close_outputs
+
+  SUBROUTINE advance(itime)
+    !-------------------------------------------------------------------------!
+    ! Advances by one step, or by up to nblock steps in one library call when
+    ! requested (never past istop or a restart save).  Writes the lines of
+    ! force.dat (it, fx(bmin..bmax), fy(bmin..bmax)), circ.dat (it,
+    ! circ(1..L)) and slip.dat (it, slip) for every step, and the restart file
+    ! every isave steps and at istop (src/operators.f90:180-186).
+    !-------------------------------------------------------------------------!
+    INTEGER :: itime
+    INTEGER(C_INT) :: it_c, nsteps
+    INTEGER :: k, s, ncol, t, b
+
+    nsteps = MIN(nblock, istop - itime)
+    DO t = itime + 1, itime + nsteps
+       IF (MOD(t, isave) == 0) THEN
+          nsteps = t - itime
+          EXIT
+       END IF
+    END DO
+    IF (nsteps < 1) nsteps = 1
+
+    IF (.NOT. have_factor) CALL preprocess
+    IF (.NOT. files_open) CALL open_outputs(itime + 1)
+
+    ncol = 2*nbody + mgridlev + 1
+    IF (.NOT. ALLOCATED(motion)) THEN
+       ALLOCATE(motion(MAX(nf,1)))
+       motion = 0.d0
+    END IF
+    IF (ALLOCATED(hist)) THEN
+       IF (SIZE(hist) < ncol*nsteps) DEALLOCATE(hist)
+    END IF
+    IF (.NOT. ALLOCATED(hist)) ALLOCATE(hist(ncol*nsteps))
+
+    it_c = itime
+    CALL ibfs_check(ibfs_advance(handle, it_c, nsteps, motion, hist), 'ibfs_advance')
+
+    DO s = 1, nsteps
+       k = (s-1)*ncol
+       WRITE(100,*) itime+s, (hist(k+b), b=1,nbody), (hist(k+nbody+b), b=1,nbody)
+       WRITE(101,*) itime+s, (hist(k+2*nbody+b), b=1,mgridlev)
+       WRITE(106,*) itime+s, hist(k+ncol)
+    END DO
+    itime = it_c
+    WRITE(*,*) "* itime =", itime, " SLIP =", hist((nsteps-1)*ncol+ncol)
+    WRITE(111,*) "...Advanced to itime =", itime
+
+    IF ((MOD(itime,isave).eq.0).or.(itime==istop)) THEN
+       CALL write_variables(itime)
+    END IF
+  END SUBROUTINE advance
+
+END MODULE operators

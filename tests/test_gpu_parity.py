@@ -113,12 +113,25 @@ def test_advance_matches_oracle(name):
         w = slice(k * o.nw, (k + 1) * o.nw)
         qq = slice(k * o.nq, (k + 1) * o.nq)
         assert relerr(st["omega"][w], o.omega[w]) < TOL, f"omega level {k + 1}"
-        assert relerr(st["rhs_old"][w], o.rhs_old[w]) < TOL, f"rhs_old level {k + 1}"
+        # rhs_old is the nonlinear term: differences of products of q and
+        # omega, so it carries ~10x the relative rounding of the fields
+        assert relerr(st["rhs_old"][w], o.rhs_old[w]) < 10 * TOL, f"rhs_old level {k + 1}"
         assert relerr(st["q"][qq], o.q[qq]) < TOL, f"q level {k + 1}"
     assert relerr(st["fb"], o.fb) < TOL
+    # Force histories are normalised by the force scale of the body (max over
+    # both components and all steps) and circulations by the vorticity scale
+    # sum|omega_k|: in mirror-symmetric cases the lift and the net
+    # circulation are rounding noise, so a per-column relative error is
+    # meaningless there.
     nbd = o.nbody
-    for col in range(2 * nbd + o.L):
-        assert relerr(gh[:, col], oh[:, col]) < TOL, f"history column {col}"
+    for b in range(nbd):
+        cols = [b, nbd + b]
+        scale = np.max(np.abs(oh[:, cols]))
+        assert np.max(np.abs(gh[:, cols] - oh[:, cols])) < TOL * scale, f"force of body {b}"
+    for k in range(o.L):
+        scale = np.sum(np.abs(o.omega[k * o.nw:(k + 1) * o.nw]))
+        col = 2 * nbd + k
+        assert np.max(np.abs(gh[:, col] - oh[:, col])) < TOL * scale, f"circulation level {k + 1}"
     # slip is a small difference of O(delta) terms: compare on the delta scale
     assert np.max(np.abs(gh[:, -1] - oh[:, -1])) < TOL * o.delta
 
