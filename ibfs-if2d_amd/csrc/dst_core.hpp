@@ -205,4 +205,131 @@ __global__ __launch_bounds__(kThreads) void dst_pass(Op op, FftPlan<T> plan) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Size-specialised pass (the sizes of the production cases).  N2 = 2N is a
+// compile-time constant, so the radix schedule, every index division and the
+// twiddle-table offsets fold into shifts and constants.  One FFT is served by
+// TPF = N2/4 threads (one radix-4 butterfly per thread per stage); a
+// workgroup holds P FFTs.  Twiddles come from a per-stage table laid out
+// [stage][k][r-1] so that a butterfly reads its R-1 factors contiguously.
+// For the column pass the workgroup order is made XCD-aware: consecutive
+// workgroups are dispatched round-robin over the 8 XCDs, so group g is
+// remapped to give every XCD one contiguous range of columns and the column
+// segments shared by neighbouring groups stay in that XCD's L2.
+// ---------------------------------------------------------------------------
+namespace spec {
+
+struct Sched {
+  int nst;
+  int rad[kMaxStages];
+  int ns[kMaxStages];   // product of the radices before stage s
+  int toff[kMaxStages]; // offset of stage s in the twiddle table
+  int ntw;              // table length
+};
+
+constexpr Sched make_sched(int n2) {
+  Sched s{};
+  int r = n2, k = 0;
+  while (r % 4 == 0) { s.rad[k++] = 4; r /= 4; }
+  while (r % 2 == 0) { s.rad[k++] = 2; r /= 2; }
+  while (r % 3 == 0) { s.rad[k++] = 3; r /= 3; }
+  while (r % 5 == 0) { s.rad[k++] = 5; r /= 5; }
+  s.nst = k;
+  int ns = 1, off = 0;
+  for (int i = 0; i < k; i++) {
+    s.ns[i] = ns;
+    s.toff[i] = off;
+    off += ns * (s.rad[i] - 1);
+    ns *= s.rad[i];
+  }
+  s.ntw = off;
+  return s;
+}
+
+template <typename T, int N2, int P, int S>
+__device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* __restrict__ tw, int t) {
+  constexpr Sched sc = make_sched(N2);
+  constexpr int R = sc.rad[S];
+  constexpr int NS = sc.ns[S];
+  constexpr int NR = N2 / R;       // butterflies per FFT
+  constexpr int TPF = N2 / 4;
+  constexpr int NBT = (NR + TPF - 1) / TPF;
+  const int p = t / TPF, lt = t % TPF;
+  cplx<T>* zp = z + p * N2;
+  cplx<T> v[NBT][R];
+#pragma unroll
+  for (int i = 0; i < NBT; i++) {
+    const int j = lt + i * TPF;
+    if (NBT * TPF == NR || j < NR) {
+#pragma unroll
+      for (int r = 0; r < R; r++) v[i][r] = zp[j + r * NR];
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+        const cplx<T>* w = tw + sc.toff[S] + k * (R - 1);
+#pragma unroll
+        for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], w[r - 1]);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NBT; i++) {
+    const int j = lt + i * TPF;
+    if (NBT * TPF == NR || j < NR) {
+      dft_small<T, R>(v[i]);
+      const int k = j % NS;
+      const int d = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) zp[d + r * NS] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N2, int P, int S>
+__device__ __forceinline__ void run(cplx<T>* z, const cplx<T>* tw, int t) {
+  constexpr Sched sc = make_sched(N2);
+  if constexpr (S < sc.nst) {
+    stage<T, N2, P, S>(z, tw, t);
+    run<T, N2, P, S + 1>(z, tw, t);
+  }
+}
+
+template <typename T, int N2, int P, int AXIS, class Op>
+__global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* __restrict__ tw) {
+  constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
+  __shared__ cplx<T> z[P * N2];
+  const int t = threadIdx.x;
+  const int b = blockIdx.y;
+  int g = blockIdx.x;
+  if (AXIS == 1 && (gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  if (t < P) {
+    z[t * N2] = {T(0), T(0)};
+    z[t * N2 + N] = {T(0), T(0)};
+  }
+#pragma unroll 4
+  for (int e = t; e < 2 * P * LEN; e += NT) {
+    int s, idx;
+    if (AXIS == 0) { s = e / LEN; idx = e - s * LEN; }
+    else { idx = e / (2 * P); s = e - idx * (2 * P); }
+    const T x = op.load(b, g, s, P, idx);
+    const int p = s < P ? s : s - P;
+    const int h = s < P ? 0 : 1;
+    reinterpret_cast<T*>(z + p * N2 + idx + 1)[h] = x;
+    reinterpret_cast<T*>(z + p * N2 + N2 - idx - 1)[h] = -x;
+  }
+  __syncthreads();
+  run<T, N2, P, 0>(z, tw, t);
+#pragma unroll 4
+  for (int e = t; e < P * LEN; e += NT) {
+    int p, k;
+    if (AXIS == 0) { p = e / LEN; k = e - p * LEN + 1; }
+    else { k = e / P + 1; p = e - (k - 1) * P; }
+    const cplx<T> Z = z[p * N2 + k];
+    op.store(b, g, p, P, k, -Z.im, Z.re);
+  }
+}
+
+}  // namespace spec
+
 }  // namespace ibfs

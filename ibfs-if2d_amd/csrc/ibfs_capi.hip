@@ -103,7 +103,7 @@ struct ibfs_handle {
 
   // tables
   std::vector<double> con1, con2, vfac;
-  DevBuf<double> laminv, lam1, lam1i;
+  DevBuf<double> laminv, lam1, lam1i, lam1n;
 
   // state (reference layout)
   DevBuf<double> omega, rhs_old, q, q0, fb;
@@ -144,19 +144,55 @@ struct ibfs_handle {
   static size_t lds() { return kEmax * sizeof(cplx<double>); }
 
   // ------------------------------------------------------------------ passes
+  // FFTs per workgroup of the size-specialised passes (row pass / column pass)
+  static constexpr int spec_p(int n2, int axis) {
+    return axis == 0 ? (n2 >= 1024 ? 1 : (n2 >= 512 ? 2 : 4)) : (n2 >= 2048 ? 1 : (n2 >= 1024 ? 2 : 4));
+  }
+  DevBuf<cplx<double>> stw_m, stw_n;  // per-stage twiddles of the specialised sizes
+  bool spec_m = false, spec_n = false;
+
+  // Launch one pass.  nseq sequences along the other axis; mult = 2 when a
+  // workgroup's P FFTs take 2P sequences of one field, 1 when they pair two
+  // fields (X + iY).
+  template <int AXIS, class Op>
+  void launch(const Op& op, int nseq, int mult, int batch) {
+    const int n2 = AXIS == 0 ? pm.n2 : pn.n2;
+    const bool sp = AXIS == 0 ? spec_m : spec_n;
+    const cplx<double>* stw = AXIS == 0 ? stw_m.p : stw_n.p;
+    if (sp) {
+      switch (n2) {
+#define IBFS_SPEC_CASE(NN)                                                                                  \
+  case NN: {                                                                                                \
+    constexpr int P = spec_p(NN, AXIS);                                                                     \
+    const int groups = (nseq + mult * P - 1) / (mult * P);                                                  \
+    hipLaunchKernelGGL((spec::dst_pass_n<double, NN, P, AXIS, Op>), dim3(groups, batch), dim3(P * NN / 4), \
+                       0, stream, op, stw);                                                                 \
+    return;                                                                                                 \
+  }
+        IBFS_SPEC_CASE(400)
+        IBFS_SPEC_CASE(512)
+        IBFS_SPEC_CASE(640)
+        IBFS_SPEC_CASE(768)
+        IBFS_SPEC_CASE(1024)
+        IBFS_SPEC_CASE(2048)
+#undef IBFS_SPEC_CASE
+        default: break;
+      }
+    }
+    const FftPlan<double>& plan = AXIS == 0 ? pm : pn;
+    const int P = kEmax / n2;
+    const int groups = (nseq + mult * P - 1) / (mult * P);
+    hipLaunchKernelGGL((dst_pass<double, kEmax, AXIS, Op>), dim3(groups, batch), dim3(kThreads), lds(), stream, op,
+                       plan);
+  }
+
   template <class Src, class Dst>
   void row_pass(const Src& src, const Dst& dst, int batch) {
-    RowOp<double, Src, Dst> op{geo, src, dst};
-    const int groups = (geo.nm1 + 2 * P_m() - 1) / (2 * P_m());
-    hipLaunchKernelGGL((dst_pass<double, kEmax, 0, RowOp<double, Src, Dst>>), dim3(groups, batch), dim3(kThreads),
-                       lds(), stream, op, pm);
+    launch<0>(RowOp<double, Src, Dst>{geo, src, dst}, geo.nm1, 2, batch);
   }
   template <class Src, class Dst>
   void col_pass(const Src& src, const Dst& dst, int batch) {
-    ColOp<double, Src, Dst> op{geo, src, dst};
-    const int groups = (geo.mm1 + 2 * P_n() - 1) / (2 * P_n());
-    hipLaunchKernelGGL((dst_pass<double, kEmax, 1, ColOp<double, Src, Dst>>), dim3(groups, batch), dim3(kThreads),
-                       lds(), stream, op, pn);
+    launch<1>(ColOp<double, Src, Dst>{geo, src, dst}, geo.mm1, 2, batch);
   }
 
   // out = S * diag(tab) * S (src) with the given first-pass source and final
@@ -209,12 +245,11 @@ struct ibfs_handle {
                             con1[lev],
                             con2[lev],
                             vfac[lev],
+                            normalize,
                             first};
-      hipLaunchKernelGGL((dst_pass<double, kEmax, 0, Step1RowOp<double>>),
-                         dim3((geo.nm1 + P_m() - 1) / P_m(), 1), dim3(kThreads), lds(), stream, r1, pm);
-      Step1ColOp<double> c1{geo, T1.p, T1.p + nw, lam1.p + (long)lev * nw, lam1i.p + (long)lev * nw, T2.p};
-      hipLaunchKernelGGL((dst_pass<double, kEmax, 1, Step1ColOp<double>>),
-                         dim3((geo.mm1 + P_n() - 1) / P_n(), 1), dim3(kThreads), lds(), stream, c1, pn);
+      launch<0>(r1, geo.nm1, 1, 1);
+      Step1ColOp<double> c1{geo, T1.p, T1.p + nw, lam1n.p + (long)lev * nw, lam1i.p + (long)lev * nw, T2.p};
+      launch<1>(c1, geo.mm1, 1, 1);
       row_pass(SrcField<double>{T2.p, 0}, DstField<double>{T3.p, 0}, 1);
       col_pass(SrcField<double>{T3.p, 0}, DstField<double>{omega.p + (long)lev * nw, 0}, 1);
     }
@@ -436,6 +471,26 @@ void make_plan(int n2, FftPlan<double>& plan, DevBuf<cplx<double>>& tw, hipStrea
   plan.tw = tw.p;
 }
 
+// Per-stage twiddles of a size-specialised pass: stage s, butterfly index
+// k < ns, factor r = 1..R-1 holds exp(-2 pi i k r / (ns R)).
+bool make_spec_twiddles(int n2, DevBuf<cplx<double>>& tw, hipStream_t st) {
+  static const int kSizes[] = {400, 512, 640, 768, 1024, 2048};
+  if (std::find(std::begin(kSizes), std::end(kSizes), n2) == std::end(kSizes)) return false;
+  const spec::Sched sc = spec::make_sched(n2);
+  std::vector<cplx<double>> t(std::max(sc.ntw, 1));
+  for (int s = 0; s < sc.nst; s++) {
+    const int R = sc.rad[s], ns = sc.ns[s];
+    for (int k = 0; k < ns; k++)
+      for (int r = 1; r < R; r++) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) / (long double)(ns * R);
+        t[sc.toff[s] + k * (R - 1) + (r - 1)] = {(double)cosl(a), (double)sinl(a)};
+      }
+  }
+  tw.alloc(t.size());
+  tw.upload(t.data(), t.size(), st);
+  return true;
+}
+
 int fail(ibfs_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
   return code;
@@ -500,6 +555,8 @@ int ibfs_create(int m, int n, int mgridlev, int nb, double dt, double re, double
     hipStream_t s = h->stream;
     make_plan(2 * m, h->pm, h->tw_m, s);
     make_plan(2 * n, h->pn, h->tw_n, s);
+    h->spec_m = make_spec_twiddles(2 * m, h->stw_m, s);
+    h->spec_n = make_spec_twiddles(2 * n, h->stw_n, s);
 
     // setup_fft tables (src/myfft.f90:42-69), same expressions and order
     const int L = h->L;
@@ -513,7 +570,7 @@ int ibfs_create(int m, int n, int mgridlev, int nb, double dt, double re, double
       h->con1[k - 1] = 1.5 * dt / del22 / h->normalize;
       h->con2[k - 1] = -0.5 * dt / del22 / h->normalize;
     }
-    std::vector<double> laminv(nw), lam1(nw * L), lam1i(nw * L);
+    std::vector<double> laminv(nw), lam1(nw * L), lam1i(nw * L), lam1n(nw * L);
     for (int j = 1; j <= n - 1; j++)
       for (int i = 1; i <= m - 1; i++) {
         const size_t e = (size_t)(i - 1) + (size_t)(j - 1) * (m - 1);
@@ -522,13 +579,15 @@ int ibfs_create(int m, int n, int mgridlev, int nb, double dt, double re, double
         laminv[e] = 1.0 / lam / h->normalize;
         for (int k = 1; k <= L; k++) {
           const double del22 = del2 * std::pow(4.0, k - 1);
-          lam1[e + nw * (k - 1)] = (1.0 - 0.5 * dt * lam / del22 / re) / h->normalize;
+          lam1n[e + nw * (k - 1)] = 1.0 - 0.5 * dt * lam / del22 / re;
+          lam1[e + nw * (k - 1)] = lam1n[e + nw * (k - 1)] / h->normalize;
           lam1i[e + nw * (k - 1)] = 1.0 / (1.0 + 0.5 * dt * lam / del22 / re);
         }
       }
     up(h->laminv, laminv, s);
     up(h->lam1, lam1, s);
     up(h->lam1i, lam1i, s);
+    up(h->lam1n, lam1n, s);
 
     build_smear(h, xb, yb);
     up(h->face_id, h->h_face_id, s);

@@ -196,9 +196,13 @@ struct ColOp {
 };
 
 // Step 1, first pass at level k: X = con1*rhs + con2*rhs_old + rhsbc with
-// rhs = rot(nonlinear(omega_k, q_k, q0_k, lastbc_k)) and Y = omega_k, rows of
-// both transformed together (FFT p = X row + i Y row).  rhs_old_k <- rhs.
-// src/operators.f90:110-133.
+// rhs = rot(nonlinear(omega_k, q_k, q0_k, lastbc_k)) and Y = omega_k / (4mn),
+// rows of both transformed together (FFT p = X row + i Y row).
+// rhs_old_k <- rhs.  src/operators.f90:110-133.
+// Y is pre-divided by the normalisation 4mn so that both halves of a packed
+// FFT have the scale of the final result: an FFT's rounding is relative to
+// the larger half, and packing omega at full scale against X (which carries
+// the 1/(4mn) of con1/con2) would cost a factor 4mn in the accuracy of S(X).
 template <typename T>
 struct Step1RowOp {
   Geo g;
@@ -208,13 +212,13 @@ struct Step1RowOp {
   const T* omc;     // new omega_{k+1} (nullptr on the coarsest level)
   T* rhs_old;
   T* tx; T* ty;     // row-transformed X and Y
-  T con1, con2, vfac;
+  T con1, con2, vfac, norm;
   int first;        // itime == 1: explicit Euler start
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int row = gr * P + (s < P ? s : s - P);
     if (row >= g.nm1) return T(0);
     const int e = idx + row * g.mm1;
-    if (s >= P) return om[e];
+    if (s >= P) return om[e] / norm;
     const int i = idx + 2, j = row + 2;
     const T rhs = rot_nonlinear<T>(g, i, j, om, q, q0, lastbc);
     const T ro = first ? rhs : rhs_old[e];
@@ -236,12 +240,13 @@ struct Step1RowOp {
 };
 
 // Step 1, second pass: columns of X~ and Y~ together, then
-// Z = lam1i_k * (X^ + lam1_k * Y^) (src/operators.f90:126-130).
+// Z = lam1i_k * (X^ + lam1_k * S(omega)) (src/operators.f90:126-130), where
+// lam1_k * S(omega) = lam1n_k * S(omega / 4mn) with lam1n = lam1 * 4mn.
 template <typename T>
 struct Step1ColOp {
   Geo g;
   const T* tx; const T* ty;
-  const T* lam1; const T* lam1i;  // level-k tables
+  const T* lam1n; const T* lam1i;  // level-k tables (lam1n = lam1 * 4mn)
   T* z;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int col = gr * P + (s < P ? s : s - P);
@@ -253,7 +258,7 @@ struct Step1ColOp {
     const int col = gr * P + p;
     if (col >= g.mm1) return;
     const int e = col + (k - 1) * g.mm1;
-    z[e] = lam1i[e] * (ya + lam1[e] * yb);
+    z[e] = lam1i[e] * (ya + lam1n[e] * yb);
   }
 };
 

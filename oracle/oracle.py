@@ -72,6 +72,7 @@ def lib():
         L.orc_advance.restype = C.c_int
         L.orc_advance.argtypes = [C.c_void_p, _I, _D, C.c_int, _D, _D, _D]
         L.orc_step1.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_dst_order.argtypes = [C.c_void_p, C.c_int]
         L.orc_dst1.argtypes = [_D, _D, C.c_int]
         L.orc_fft.argtypes = [_D, _D, C.c_int]
         _lib = L
@@ -300,6 +301,10 @@ class Oracle:
         y = np.zeros(self.L)
         lib().orc_circ(self._h, _p(y))
         return y
+
+    def set_dst_order(self, order):
+        """0: transform along i first (default); 1: along j first."""
+        lib().orc_set_dst_order(self._h, int(order))
 
     def step1(self, first):
         """STEP 1 of advance only (intermediate vorticity on every level)."""

@@ -1,0 +1,71 @@
+"""Parity at the production grid sizes (the size-specialised DST passes):
+transforms at every configured FFT length, and a few full steps of the
+200^2 x 3-level cylinder (the CPU reference case) and of the 512^2 x 5-level
+airfoil (the headline case) against the oracle.  The oracle cannot afford
+its own preprocess at these sizes, so both sides use the device factor,
+after a spot check of columns of A against the oracle's a_times."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def solver_pair(m, L, len_, offx, offy, body, dt, re):
+    from ibfs_amd import geometry, hip
+
+    delta = len_ / m
+    if body == "circle":
+        xb, yb, bd = geometry.circle(delta, 1.0, 0.0, 0.0)
+    else:
+        xb, yb, bd = geometry.naca4(delta, aoa_deg=20.0)
+    a = dict(m=m, n=m, mgridlev=L, dt=dt, re=re, len_=len_, offx=offx, offy=offy, xb=xb, yb=yb, bdyno=bd)
+    return orc.Oracle(**a), hip.Solver(**a)
+
+
+@pytest.mark.parametrize("m", [200, 256, 320, 384, 512, 1024])
+def test_transforms_at_production_sizes(m, rng):
+    o, g = solver_pair(m, 3, 4.0, 2.0, 2.0, "circle", 0.004, 100.0)
+    x = rng.standard_normal(o.nw)
+    assert relerr(g.dst(x), o.dst(x)) < 1e-13
+    assert relerr(g.ctci(x), o.ctci(x)) < 1e-12
+    assert relerr(g.ainv(x), o.ainv(x)) < 1e-12
+
+
+def run_pair(o, g, nsteps, spot_cols):
+    cm, cv = g.build_cholesky()
+    A_cols = cm.reshape(o.nf, o.nf).T  # [i, j]: upper triangle holds A
+    for j in spot_cols:
+        col = o.a_column(j)
+        # rows i <= j of column j are stored in the upper triangle
+        assert relerr(A_cols[: j + 1, j], col[: j + 1]) < TOL
+    o.set_cholesky(cm, cv)
+    oh = np.array([np.concatenate([f, c, [s]]) for f, c, s in (o.advance() for _ in range(nsteps))])
+    gh = g.advance(nsteps)
+    st = g.get_state()
+    for k in range(o.L):
+        w = slice(k * o.nw, (k + 1) * o.nw)
+        qq = slice(k * o.nq, (k + 1) * o.nq)
+        assert relerr(st["omega"][w], o.omega[w]) < TOL, f"omega level {k + 1}"
+        assert relerr(st["q"][qq], o.q[qq]) < TOL, f"q level {k + 1}"
+    assert relerr(st["fb"], o.fb) < TOL
+    fscale = np.max(np.abs(oh[:, :2]))
+    assert np.max(np.abs(gh[:, :2] - oh[:, :2])) < TOL * fscale
+
+
+def test_cylinder_200_three_levels():
+    """The CPU reference case (200^2, 3 levels, Re 100, dt 0.004): 4 steps."""
+    o, g = solver_pair(200, 3, 4.0, 2.0, 2.0, "circle", 0.004, 100.0)
+    assert o.nb == 157 and g.nb == 157
+    run_pair(o, g, 4, [0, 78, 313])
+
+
+def test_airfoil_512_five_levels():
+    """The headline case (512^2, 5 levels, NACA0012 at 20 deg, Re 300): 2 steps."""
+    o, g = solver_pair(512, 5, 4.0, 1.0, 2.0, "naca", 0.002, 300.0)
+    assert g.nb == 261
+    run_pair(o, g, 2, [0, 130, 521])
