@@ -1,0 +1,521 @@
+// ibfs_engine_t.hpp -- Engine<T>: one group of equal-size cases on the device.
+#pragma once
+#include "ibfs_engine.hpp"
+
+namespace ibfs {
+
+// Per-stage twiddles of a size-specialised pass: stage s, butterfly index
+// k < ns, factor r = 1..R-1 holds exp(-2 pi i k r / (ns R)).
+template <typename T>
+bool make_spec_twiddles(int n2, DevBuf<cplx<T>>& tw, hipStream_t st) {
+  static const int kSizes[] = {400, 512, 640, 768, 1024, 2048};
+  if (std::find(std::begin(kSizes), std::end(kSizes), n2) == std::end(kSizes)) return false;
+  const spec::Sched sc = spec::make_sched(n2);
+  std::vector<cplx<T>> t(std::max(sc.ntw, 1));
+  for (int s = 0; s < sc.nst; s++) {
+    const int R = sc.rad[s], ns = sc.ns[s];
+    for (int k = 0; k < ns; k++)
+      for (int r = 1; r < R; r++) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) / (long double)(ns * R);
+        t[sc.toff[s] + k * (R - 1) + (r - 1)] = {(T)cosl(a), (T)sinl(a)};
+      }
+  }
+  tw.alloc(t.size());
+  tw.upload(t.data(), t.size(), st);
+  return true;
+}
+
+template <typename T>
+void make_plan(int n2, FftPlan<T>& plan, DevBuf<cplx<T>>& tw, hipStream_t s) {
+  std::vector<int> rad;
+  if (!factor_235(n2, rad) || n2 > kEmax)
+    throw ArgError{1, "grid size not supported: 2*m and 2*n must factor into 2, 3, 5 and be <= " +
+                          std::to_string(kEmax)};
+  plan.n2 = n2;
+  plan.nst = (int)rad.size();
+  for (int i = 0; i < kMaxStages; i++) plan.rad[i] = i < plan.nst ? rad[i] : 1;
+  std::vector<cplx<T>> t(n2);
+  for (int k = 0; k < n2; k++) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n2;
+    t[k] = {(T)cosl(a), (T)sinl(a)};
+  }
+  tw.alloc(n2);
+  tw.upload(t.data(), n2, s);
+  plan.tw = tw.p;
+}
+
+template <typename T>
+struct Engine : GroupBase {
+  hipStream_t st = nullptr;
+  Geo geo{};
+  T norm = T(1);
+  FftPlan<T> pm{}, pn{};
+  DevBuf<cplx<T>> tw_m, tw_n, stw_m, stw_n;
+  bool spec_m = false, spec_n = false;
+
+  DevBuf<int> Lc;
+  DevBuf<T> con1, con2, vfac;
+  DevBuf<T> laminv, lam1n, lam1i;
+  DevBuf<T> omega, rhs_old, q, q0, S, T1, T2, T3, RR, lastbc;
+
+  DevBuf<int> face_id, face_ptr, face_pt, frc_ptr, frc_face, vtx_id, vtx_face, bdyno;
+  DevBuf<double> face_w, frc_w, wsum;
+  DevBuf<CaseSm> csm;
+  std::vector<CaseSm> hcs;
+  Smear sm{};
+
+  DevBuf<double> fb, motion_in, motion, rhsf, ytmp, rtmp, Linv, Asym;
+  DevBuf<double> partial, diag;
+  DevBuf<int> counter;
+  int diag_cap = 0, nchunk = 64;
+  int nf_tot = 0, nf_max = 0, nvtx_max = 0;
+  long mat_tot = 0;
+  bool refine = false;
+
+  long fbs() const { return (long)Lmax * nw; }
+  long qbs() const { return (long)Lmax * nq; }
+  bool is_double() const override { return sizeof(T) == 8; }
+
+  Engine(std::vector<CaseHost> cases, hipStream_t stream) {
+    st = stream;
+    cs = std::move(cases);
+    B = (int)cs.size();
+    m = cs[0].d.m;
+    n = cs[0].d.n;
+    Lmax = 0;
+    for (auto& c : cs) Lmax = std::max(Lmax, c.L);
+    nw = (m - 1) * (n - 1);
+    nq = (m + 1) * n + (n + 1) * m;
+    nbc = 2 * (m + 1) + 2 * (n + 1);
+    geo = Geo{m, n, Lmax, m - 1, n - 1, nw, nq, nbc, 0, n + 1, 2 * (n + 1), 2 * (n + 1) + m + 1};
+    norm = T(cs[0].normalize);
+    refine = sizeof(T) == 4;
+    make_plan<T>(2 * m, pm, tw_m, st);
+    make_plan<T>(2 * n, pn, tw_n, st);
+    spec_m = make_spec_twiddles<T>(2 * m, stw_m, st);
+    spec_n = make_spec_twiddles<T>(2 * n, stw_n, st);
+
+    // per-case scalars and eigenvalue tables (src/myfft.f90:42-69)
+    std::vector<int> hL(B);
+    std::vector<T> hc1((size_t)B * Lmax, T(0)), hc2(hc1), hvf(hc1);
+    std::vector<T> hlaminv(nw), hl1n((size_t)B * Lmax * nw, T(0)), hl1i(hl1n);
+    std::vector<double> lam(nw);
+    for (int j = 1; j <= n - 1; j++)
+      for (int i = 1; i <= m - 1; i++) {
+        const size_t e = (size_t)(i - 1) + (size_t)(j - 1) * (m - 1);
+        lam[e] = -2.0 * (std::cos(kPiRef * (double)(float)i / (double)(float)m) +
+                         std::cos(kPiRef * (double)(float)j / (double)(float)n) - 2.0);
+        hlaminv[e] = T(1.0 / lam[e] / cs[0].normalize);
+      }
+    for (int b = 0; b < B; b++) {
+      const CaseHost& c = cs[b];
+      hL[b] = c.L;
+      const double del2 = c.delta * c.delta;
+      for (int k = 0; k < c.L; k++) {
+        hc1[b * Lmax + k] = T(c.con1[k]);
+        hc2[b * Lmax + k] = T(c.con2[k]);
+        hvf[b * Lmax + k] = T(c.vfac[k]);
+        const double del22 = del2 * std::pow(4.0, k);
+        const size_t base = ((size_t)b * Lmax + k) * nw;
+        for (int e = 0; e < nw; e++) {
+          hl1n[base + e] = T(1.0 - 0.5 * c.d.dt * lam[e] / del22 / c.d.re);
+          hl1i[base + e] = T(1.0 / (1.0 + 0.5 * c.d.dt * lam[e] / del22 / c.d.re));
+        }
+      }
+    }
+    upvec(Lc, hL, st);
+    upvec(con1, hc1, st);
+    upvec(con2, hc2, st);
+    upvec(vfac, hvf, st);
+    upvec(laminv, hlaminv, st);
+    upvec(lam1n, hl1n, st);
+    upvec(lam1i, hl1i, st);
+
+    // smearing tables, concatenated
+    std::vector<int> fid, fptr, fpt, rptr, rface, vid, vface, bd;
+    std::vector<double> fw, rw, ws;
+    hcs.resize(B);
+    long loff = 0;
+    int foff = 0, hof = 0;
+    hoff.resize(B);
+    for (int b = 0; b < B; b++) {
+      const CaseHost& c = cs[b];
+      CaseSm& h = hcs[b];
+      h.nact = (int)c.face_id.size();
+      h.fbase = (int)fid.size();
+      h.pbase = (int)fptr.size();
+      const int ebase = (int)fpt.size();
+      fid.insert(fid.end(), c.face_id.begin(), c.face_id.end());
+      for (int v : c.face_ptr) fptr.push_back(v + ebase);
+      fpt.insert(fpt.end(), c.face_pt.begin(), c.face_pt.end());
+      fw.insert(fw.end(), c.face_w.begin(), c.face_w.end());
+      h.nf = c.nf;
+      h.rbase = (int)rptr.size();
+      const int rebase = (int)rface.size();
+      for (int v : c.frc_ptr) rptr.push_back(v + rebase);
+      rface.insert(rface.end(), c.frc_face.begin(), c.frc_face.end());
+      rw.insert(rw.end(), c.frc_w.begin(), c.frc_w.end());
+      h.nvtx = (int)c.vtx_id.size();
+      h.vbase = (int)vid.size();
+      vid.insert(vid.end(), c.vtx_id.begin(), c.vtx_id.end());
+      vface.insert(vface.end(), c.vtx_face.begin(), c.vtx_face.end());
+      h.foff = foff;
+      h.nb = c.d.nb;
+      h.loff = loff;
+      h.nbody = c.nbody;
+      h.bmin = c.bmin;
+      h.ncol = c.ncol;
+      h.hoff = hof;
+      hoff[b] = hof;
+      h.dt = c.d.dt;
+      h.delta = c.delta;
+      ws.insert(ws.end(), c.wsum.begin(), c.wsum.begin() + c.nf);
+      bd.insert(bd.end(), c.d.bdyno.begin(), c.d.bdyno.end());
+      foff += c.nf;
+      loff += (long)c.nf * c.nf;
+      hof += c.ncol;
+      nf_max = std::max(nf_max, c.nf);
+      nvtx_max = std::max(nvtx_max, h.nvtx);
+    }
+    nf_tot = foff;
+    mat_tot = loff;
+    rowlen = hof;
+    upvec(face_id, fid, st);
+    upvec(face_ptr, fptr, st);
+    upvec(face_pt, fpt, st);
+    upvec(face_w, fw, st);
+    upvec(frc_ptr, rptr, st);
+    upvec(frc_face, rface, st);
+    upvec(frc_w, rw, st);
+    upvec(vtx_id, vid, st);
+    upvec(vtx_face, vface, st);
+    upvec(wsum, ws, st);
+    upvec(bdyno, bd, st);
+    upvec(csm, hcs, st);
+    sm = Smear{face_id.p, face_ptr.p, face_pt.p, face_w.p, frc_ptr.p, frc_face.p, frc_w.p, vtx_id.p, vtx_face.p,
+               csm.p};
+
+    // state and initial condition (src/variables.f90:54-67)
+    const size_t W = (size_t)B * Lmax * nw, Q = (size_t)B * Lmax * nq;
+    omega.alloc(W); omega.zero(st);
+    rhs_old.alloc(W); rhs_old.zero(st);
+    S.alloc(W); S.zero(st);
+    q.alloc(Q); q.zero(st);
+    q0.alloc(Q);
+    std::vector<T> hq0(Q, T(0));
+    for (int b = 0; b < B; b++) {
+      const CaseHost& c = cs[b];
+      if (!c.d.unif_flow) continue;
+      for (int k = 0; k < c.L; k++) {
+        const T v = T(c.delta * std::pow(2.0, k));
+        const size_t base = ((size_t)b * Lmax + k) * nq;
+        for (size_t e = 0; e < (size_t)(m + 1) * n; e++) hq0[base + e] = v;
+      }
+    }
+    q0.upload(hq0.data(), Q, st);
+    T1.alloc((size_t)B * 2 * nw);
+    T2.alloc((size_t)B * nw);
+    T3.alloc((size_t)B * nw);
+    RR.alloc((size_t)B * nw); RR.zero(st);
+    lastbc.alloc((size_t)B * Lmax * nbc);
+    const size_t nff = std::max(nf_tot, 1);
+    fb.alloc(nff); fb.zero(st);
+    motion_in.alloc(nff); motion_in.zero(st);
+    motion.alloc(nff); motion.zero(st);
+    rhsf.alloc(nff);
+    ytmp.alloc(nff);
+    rtmp.alloc(nff);
+    Linv.alloc(std::max<long>(mat_tot, 1));
+    if (refine) Asym.alloc(std::max<long>(mat_tot, 1));
+    partial.alloc((size_t)B * Lmax * nchunk);
+    counter.alloc(B);
+    counter.zero(st);
+    ensure_diag(1);
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  // ------------------------------------------------------------------ passes
+  static constexpr int spec_p(int n2, int axis) {
+    return axis == 0 ? (n2 >= 1024 ? 1 : (n2 >= 512 ? 2 : 4)) : (n2 >= 2048 ? 1 : (n2 >= 1024 ? 2 : 4));
+  }
+
+  // Launch one pass over nseq sequences along the other axis; mult = 2 when
+  // a workgroup's P FFTs take 2P sequences of one field, 1 when they pair
+  // two fields (X + iY).
+  template <int AXIS, class Op>
+  void launch(const Op& op, int nseq, int mult, int batch) {
+    const int n2 = AXIS == 0 ? pm.n2 : pn.n2;
+    const bool sp = AXIS == 0 ? spec_m : spec_n;
+    const cplx<T>* stw = AXIS == 0 ? stw_m.p : stw_n.p;
+    if (sp) {
+      switch (n2) {
+#define IBFS_SPEC_CASE(NN)                                                                             \
+  case NN: {                                                                                           \
+    constexpr int P = spec_p(NN, AXIS);                                                                \
+    const int groups = (nseq + mult * P - 1) / (mult * P);                                             \
+    hipLaunchKernelGGL((spec::dst_pass_n<T, NN, P, AXIS, Op>), dim3(groups, batch), dim3(P * NN / 4), \
+                       0, st, op, stw);                                                                \
+    return;                                                                                            \
+  }
+        IBFS_SPEC_CASE(400)
+        IBFS_SPEC_CASE(512)
+        IBFS_SPEC_CASE(640)
+        IBFS_SPEC_CASE(768)
+        IBFS_SPEC_CASE(1024)
+        IBFS_SPEC_CASE(2048)
+#undef IBFS_SPEC_CASE
+        default: break;
+      }
+    }
+    const FftPlan<T>& plan = AXIS == 0 ? pm : pn;
+    const int P = kEmax / n2;
+    const int groups = (nseq + mult * P - 1) / (mult * P);
+    hipLaunchKernelGGL((dst_pass<T, kEmax, AXIS, Op>), dim3(groups, batch), dim3(kThreads),
+                       kEmax * sizeof(cplx<T>), st, op, plan);
+  }
+
+  template <class Src, class Dst>
+  void row_pass(const Src& src, const Dst& dst, int batch, const int* gate = nullptr, int lev = 0) {
+    launch<0>(RowOp<T, Src, Dst>{geo, src, dst, gate, lev}, geo.nm1, 2, batch);
+  }
+  template <class Src, class Dst>
+  void col_pass(const Src& src, const Dst& dst, int batch, const int* gate = nullptr, int lev = 0) {
+    launch<1>(ColOp<T, Src, Dst>{geo, src, dst, gate, lev}, geo.mm1, 2, batch);
+  }
+
+  // out = S * diag(tab) * S (src): ctci (tab = laminv) or ainv (tab = the
+  // level-1 lam1i of each case, divided by 4mn).  t1/t2: scratch, stride nw.
+  template <class Src, class Dst>
+  void ctci_like(const Src& src, bool ainv_form, const Dst& dst, T* t1, T* t2, int batch, const int* gate, int lev,
+                 const T* lam1i_l1 = nullptr, long tbs = 0) {
+    row_pass(src, DstField<T>{t1, nw}, batch, gate, lev);
+    if (ainv_form)
+      col_pass(SrcField<T>{t1, nw}, DstScaleDiv<T>{t2, lam1i_l1, norm, nw, tbs}, batch, gate, lev);
+    else
+      col_pass(SrcField<T>{t1, nw}, DstScale<T>{t2, laminv.p, nw}, batch, gate, lev);
+    row_pass(SrcField<T>{t2, nw}, DstField<T>{t1, nw}, batch, gate, lev);
+    col_pass(SrcField<T>{t1, nw}, dst, batch, gate, lev);
+  }
+
+  // vort2flux (src/operators.f90:191-240) on a batch of multi-level fields om
+  // (stride Lmax*nw); streamfunctions go to s (same layout); when vel is set,
+  // curl(s) is written on every level (stride Lmax*nq).
+  void vort2flux(T* om, T* s, T* vel, int batch, const int* gate, T* t1, T* t2) {
+    const long bs = fbs();
+    const int wcnt = (m / 2 - 1) * (n / 2 - 1);
+    for (int lev = 1; lev < Lmax; lev++)
+      hipLaunchKernelGGL(k_coarsify<T>, dim3((wcnt + 255) / 256, batch), dim3(256), 0, st, geo,
+                         om + (long)lev * nw, om + (long)(lev - 1) * nw, bs, gate, lev);
+    for (int lev = Lmax - 1; lev >= 0; lev--) {
+      T* sc = (lev + 1 < Lmax) ? s + (long)(lev + 1) * nw : nullptr;
+      ctci_like(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, false, DstField<T>{s + (long)lev * nw, bs}, t1,
+                t2, batch, gate, lev);
+      if (vel)
+        hipLaunchKernelGGL(k_curl<T>, dim3((nq + 255) / 256, batch), dim3(256), 0, st, geo, s + (long)lev * nw,
+                           sc, vel + (long)lev * nq, bs, qbs(), gate, lev);
+    }
+  }
+
+  // --------------------------------------------------------------- one step
+  void enqueue_step(int first) override {
+    const long FB = fbs(), QB = qbs(), BB = (long)Lmax * nbc;
+    hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, omega.p, lastbc.p,
+                       Lc.p);
+    // STEP 1 (src/operators.f90:103-134), level slots Lmax..1
+    for (int lev = Lmax - 1; lev >= 0; lev--) {
+      Step1RowOp<T> r1{geo,
+                       omega.p + (long)lev * nw,
+                       lev + 1 < Lmax ? omega.p + (long)(lev + 1) * nw : nullptr,
+                       q.p + (long)lev * nq,
+                       q0.p + (long)lev * nq,
+                       lastbc.p + (long)lev * nbc,
+                       rhs_old.p + (long)lev * nw,
+                       T1.p,
+                       con1.p,
+                       con2.p,
+                       vfac.p,
+                       Lc.p,
+                       FB,
+                       QB,
+                       BB,
+                       norm,
+                       lev,
+                       first};
+      launch<0>(r1, geo.nm1, 1, B);
+      Step1ColOp<T> c1{geo, T1.p, lam1n.p, lam1i.p, T2.p, Lc.p, lev};
+      launch<1>(c1, geo.mm1, 1, B);
+      row_pass(SrcField<T>{T2.p, nw}, DstField<T>{T3.p, nw}, B, Lc.p, lev);
+      col_pass(SrcField<T>{T3.p, nw}, DstField<T>{omega.p + (long)lev * nw, FB}, B, Lc.p, lev);
+    }
+    // STEP 2 (src/operators.f90:142-171); motion is prepared per call
+    vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
+    hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf_max + 255) / 256, B), dim3(256), 0, st, geo, sm, S.p,
+                       Lmax > 1 ? S.p + nw : nullptr, FB, motion.p, rhsf.p, 0L, Lc.p, -1);
+    const dim3 g4((nf_max + 3) / 4, B);
+    hipLaunchKernelGGL(k_trmv_lower, g4, dim3(256), 0, st, sm, Linv.p, rhsf.p, ytmp.p);
+    hipLaunchKernelGGL(k_trmv_upper_t, g4, dim3(256), 0, st, sm, Linv.p, ytmp.p, fb.p, 0);
+    if (refine) {  // one FP64 refinement sweep: fb += cholsl(rhsf - A fb)
+      hipLaunchKernelGGL(k_symv_residual, g4, dim3(256), 0, st, sm, Asym.p, rhsf.p, fb.p, rtmp.p);
+      hipLaunchKernelGGL(k_trmv_lower, g4, dim3(256), 0, st, sm, Linv.p, rtmp.p, ytmp.p);
+      hipLaunchKernelGGL(k_trmv_upper_t, g4, dim3(256), 0, st, sm, Linv.p, ytmp.p, fb.p, 1);
+    }
+    // STEP 3 (src/operators.f90:174)
+    hipLaunchKernelGGL((k_rotreg<T, double>), dim3((nvtx_max + 255) / 256, B), dim3(256), 0, st, sm, fb.p, 0L, RR.p,
+                       (long)nw, -1);
+    ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, FB);
+    // STEP 4 (src/operators.f90:177)
+    vort2flux(omega.p, S.p, q.p, B, Lc.p, T2.p, T3.p);
+    // diagnostics (src/operators.f90:184-186)
+    hipLaunchKernelGGL(k_circ_partial<T>, dim3(nchunk, Lmax, B), dim3(kThreads), 0, st, geo, omega.p, partial.p,
+                       nchunk, Lc.p);
+    hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(256), 0, st, geo, sm, q.p, QB, motion.p, fb.p, wsum.p, bdyno.p,
+                       partial.p, nchunk, Lc.p, diag.p, rowlen, counter.p, diag_cap);
+  }
+
+  void enqueue_motion() override {
+    hipLaunchKernelGGL(k_motion<T>, dim3((nf_max + 255) / 256, B), dim3(256), 0, st, sm, q0.p, qbs(), motion_in.p,
+                       motion.p);
+  }
+  void set_motion(int c, const double* mo) override {
+    if (mo) motion_in.upload(mo, cs[c].nf, st, hcs[c].foff);
+    else if (cs[c].nf) HIPCHK(hipMemsetAsync(motion_in.p + hcs[c].foff, 0, sizeof(double) * cs[c].nf, st));
+  }
+
+  void ensure_diag(int cap) override {
+    if (cap <= diag_cap) return;
+    diag.alloc((size_t)cap * std::max(rowlen, 1));
+    diag_cap = cap;
+  }
+  void reset_counters() override { counter.zero(st); }
+  void download_diag(double* dst, int nsteps, int ldrow, int col0) override {
+    std::vector<double> h((size_t)nsteps * rowlen);
+    diag.download(h.data(), h.size(), st);
+    HIPCHK(hipStreamSynchronize(st));
+    for (int s = 0; s < nsteps; s++)
+      std::memcpy(dst + (size_t)s * ldrow + col0, h.data() + (size_t)s * rowlen, sizeof(double) * rowlen);
+  }
+
+  // ----------------------------------------------------------------- state
+  template <typename Src, typename Dst>
+  static void convert(const Src* a, Dst* b, size_t n) {
+    for (size_t i = 0; i < n; i++) b[i] = Dst(a[i]);
+  }
+  void put(DevBuf<T>& d, size_t off, const double* h, size_t cnt) {
+    if (!h || !cnt) return;
+    if (sizeof(T) == 8) {
+      d.upload(reinterpret_cast<const T*>(h), cnt, st, off);
+      HIPCHK(hipStreamSynchronize(st));
+    } else {
+      std::vector<T> t(cnt);
+      convert(h, t.data(), cnt);
+      d.upload(t.data(), cnt, st, off);
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  }
+  void get(const DevBuf<T>& d, size_t off, double* h, size_t cnt) {
+    if (!h || !cnt) return;
+    if (sizeof(T) == 8) {
+      d.download(reinterpret_cast<T*>(h), cnt, st, off);
+      HIPCHK(hipStreamSynchronize(st));
+    } else {
+      std::vector<T> t(cnt);
+      d.download(t.data(), cnt, st, off);
+      HIPCHK(hipStreamSynchronize(st));
+      convert(t.data(), h, cnt);
+    }
+  }
+  void set_state(int c, const double* om, const double* ro, const double* qq, const double* qq0,
+                 const double* f) override {
+    const size_t W = (size_t)cs[c].L * nw, Q = (size_t)cs[c].L * nq;
+    put(omega, (size_t)c * fbs(), om, W);
+    put(rhs_old, (size_t)c * fbs(), ro, W);
+    put(q, (size_t)c * qbs(), qq, Q);
+    put(q0, (size_t)c * qbs(), qq0, Q);
+    if (f && cs[c].nf) {
+      fb.upload(f, cs[c].nf, st, hcs[c].foff);
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  }
+  void get_state(int c, double* om, double* ro, double* qq, double* qq0, double* f) override {
+    const size_t W = (size_t)cs[c].L * nw, Q = (size_t)cs[c].L * nq;
+    get(omega, (size_t)c * fbs(), om, W);
+    get(rhs_old, (size_t)c * fbs(), ro, W);
+    get(q, (size_t)c * qbs(), qq, Q);
+    get(q0, (size_t)c * qbs(), qq0, Q);
+    if (f && cs[c].nf) {
+      fb.download(f, cs[c].nf, st, hcs[c].foff);
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  }
+
+  // ---------------------------------------------------------- force system
+  // Li = L^{-1} of the factor of case c (and, for the refinement sweep, the
+  // symmetric matrix the factor was built from: the upper triangle of the
+  // ibfs.chd matrix, src/operators.f90:779).
+  void upload_factor(int c) override {
+    CaseHost& h = cs[c];
+    const int nf = h.nf;
+    if (nf == 0) { h.have_chol = true; return; }
+    std::vector<double> Li((size_t)nf * nf, 0.0), col(nf);
+    const double* a = h.cholmat.data();
+    auto Lf = [&](int i, int k) -> double { return i == k ? h.cholvec[i] : a[(size_t)i + (size_t)k * nf]; };
+    for (int j = 0; j < nf; j++) {
+      std::fill(col.begin(), col.end(), 0.0);
+      col[j] = 1.0 / Lf(j, j);
+      for (int i = j + 1; i < nf; i++) {
+        double s = 0.0;
+        for (int k = j; k < i; k++) s += Lf(i, k) * col[k];
+        col[i] = -s / Lf(i, i);
+      }
+      std::copy(col.begin(), col.end(), Li.begin() + (size_t)j * nf);
+    }
+    Linv.upload(Li.data(), Li.size(), st, hcs[c].loff);
+    if (refine) {
+      std::vector<double> As((size_t)nf * nf);
+      for (int j = 0; j < nf; j++)
+        for (int i = 0; i < nf; i++) As[i + (size_t)j * nf] = i <= j ? a[i + (size_t)j * nf] : a[j + (size_t)i * nf];
+      Asym.upload(As.data(), As.size(), st, hcs[c].loff);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    h.have_chol = true;
+  }
+
+  // a_times (src/operators.f90:444-462) for the unit vectors of case c,
+  // batched over columns; A (nf x nf, column-major) written to acol (device).
+  void build_columns(int c, double* acol) {
+    const int nf = cs[c].nf, L = cs[c].L;
+    const int Bc = std::max(1, std::min(nf, 64));
+    const long bsL = fbs();
+    DevBuf<double> F;
+    DevBuf<T> RRb, t1, t2, VB, SB;
+    DevBuf<int> gate;
+    F.alloc((size_t)Bc * nf);
+    RRb.alloc((size_t)Bc * nw);
+    t1.alloc((size_t)Bc * nw);
+    t2.alloc((size_t)Bc * nw);
+    VB.alloc((size_t)Bc * bsL);
+    SB.alloc((size_t)Bc * bsL);
+    RRb.zero(st);
+    VB.zero(st);
+    std::vector<int> hg(Bc, L);
+    upvec(gate, hg, st);
+    std::vector<double> hF((size_t)Bc * nf);
+    const T* l1i = lam1i.p + (long)c * bsL;  // level 1 of case c
+    for (int c0 = 0; c0 < nf; c0 += Bc) {
+      const int bb = std::min(Bc, nf - c0);
+      std::fill(hF.begin(), hF.end(), 0.0);
+      for (int b = 0; b < bb; b++) hF[(size_t)b * nf + c0 + b] = 1.0;
+      F.upload(hF.data(), (size_t)bb * nf, st);
+      hipLaunchKernelGGL((k_rotreg<T, double>), dim3((hcs[c].nvtx + 255) / 256, bb), dim3(256), 0, st, sm, F.p,
+                         (long)nf, RRb.p, (long)nw, c);
+      ctci_like(SrcField<T>{RRb.p, nw}, true, DstField<T>{VB.p, bsL}, t1.p, t2.p, bb, nullptr, 0, l1i, 0);
+      vort2flux(VB.p, SB.p, nullptr, bb, gate.p, t1.p, t2.p);
+      hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf + 255) / 256, bb), dim3(256), 0, st, geo, sm, SB.p,
+                         Lmax > 1 ? SB.p + nw : nullptr, bsL, (const double*)nullptr, acol + (long)c0 * nf,
+                         (long)nf, gate.p, c);
+      HIPCHK(hipStreamSynchronize(st));  // hF is reused
+    }
+  }
+};
+
+}  // namespace ibfs

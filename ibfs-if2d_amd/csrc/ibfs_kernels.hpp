@@ -1,4 +1,5 @@
-// ibfs_kernels.hpp -- device side of one multi-domain IBFS time step.
+// ibfs_kernels.hpp -- device side of one multi-domain IBFS time step for a
+// group of B independent cases that share the grid size m x n.
 //
 // Index conventions (reference src/grid.f90:119-134, src/variables.f90:46):
 //   vertex field  w[ii + jj*(m-1)], ii = i-2, jj = j-2 for the reference's
@@ -8,7 +9,12 @@
 //   edge vector   the reference's 1-based position pos lives at pos-1, with
 //                 segment bases left = 0, right = n+1, bottom = 2(n+1),
 //                 top = 2(n+1)+m+1 (src/grid.f90:73-76).
-// Levels are stored one after another (level k at offset (k-1)*size).
+// Multi-level fields of case b are stored level after level in Lmax slots:
+// field(b, lev) = base + (b*Lmax + lev)*size.  Case b uses levels
+// 0..Lc[b]-1; a kernel launched for level slot lev skips cases with
+// lev >= Lc[b], and level Lc[b]-1 is that case's coarsest (zero far field).
+// The element type T is double (plain form) or float (mixed form); the force
+// system, smearing weights and all reductions stay in double.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -17,8 +23,8 @@
 namespace ibfs {
 
 struct Geo {
-  int m, n, L;
-  int mm1, nm1;   // m-1, n-1
+  int m, n, Lmax;
+  int mm1, nm1;  // m-1, n-1
   int nw, nq, nbc;
   int left, right, bottom, top;
 };
@@ -26,6 +32,9 @@ struct Geo {
 __device__ __forceinline__ int uix(const Geo& g, int i, int j) { return (i - 1) + (j - 1) * (g.m + 1); }
 __device__ __forceinline__ int vix(const Geo& g, int i, int j) { return (g.m + 1) * g.n + (i - 1) + (j - 1) * g.m; }
 __device__ __forceinline__ int wix(const Geo& g, int i, int j) { return (i - 2) + (j - 2) * g.mm1; }
+
+__device__ __forceinline__ bool active(const int* Lc, int b, int lev) { return Lc == nullptr || lev < Lc[b]; }
+__device__ __forceinline__ bool has_coarser(const int* Lc, int b, int lev) { return lev + 1 < Lc[b]; }
 
 // get_bc (src/operators.f90:336-367) evaluated at one 1-based edge position.
 // r is the coarser level's vertex field; the reference's assumed-shape dummy
@@ -52,11 +61,10 @@ __device__ __forceinline__ T getbc_at(const Geo& g, const T* __restrict__ r, int
 }
 
 // apply_bc (src/operators.f90:370-390) contribution at vertex (i,j) for an
-// edge vector given as a get_bc of the coarser field rc (rc == nullptr: 0).
+// edge vector given as get_bc(rc, gfac) of the coarser field rc.
 // Reference order: bottom/top first (i loop), then left/right (j loop).
 template <typename T>
 __device__ __forceinline__ T applybc_add(const Geo& g, T acc, int i, int j, const T* __restrict__ rc, T gfac, T fac) {
-  if (rc == nullptr) return acc;
   if (j == 2) acc = acc + fac * getbc_at<T>(g, rc, g.bottom + i, gfac);
   if (j == g.n) acc = acc + fac * getbc_at<T>(g, rc, g.top + i, gfac);
   if (i == 2) acc = acc + fac * getbc_at<T>(g, rc, g.left + j, gfac);
@@ -81,7 +89,6 @@ __device__ __forceinline__ T rot_nonlinear(const Geo& g, int i, int j, const T* 
                                            const T* __restrict__ q, const T* __restrict__ q0,
                                            const T* __restrict__ bc) {
   const int m = g.m, n = g.n;
-  // vorticity with the edge values of bc outside the interior
   auto OmU = [&](int ii, int jj) -> T {  // along j at column ii (2..m)
     if (jj == 1) return bc[g.bottom + ii - 1];
     if (jj == n + 1) return bc[g.top + ii - 1];
@@ -110,9 +117,8 @@ __device__ __forceinline__ T rot_nonlinear(const Geo& g, int i, int j, const T* 
 }
 
 // ---------------------------------------------------------------------------
-// pass operators
+// pass operators (see dst_core.hpp for the load/store protocol)
 // ---------------------------------------------------------------------------
-// plain field source/destination (batch stride bs)
 template <typename T>
 struct SrcField {
   const T* a; long bs;
@@ -120,13 +126,13 @@ struct SrcField {
 };
 
 // omega_k + apply_bc(get_bc(s_{k+1}, 1), 1): the vort of vort2flux
-// (src/operators.f90:230-234).  sc == nullptr on the coarsest level.
+// (src/operators.f90:230-234); no edge term on a case's coarsest level.
 template <typename T>
 struct SrcVortBC {
-  const T* om; const T* sc; long bs;
+  const T* om; const T* sc; long bs; const int* Lc; int lev;
   __device__ T operator()(const Geo& g, int b, int ii, int jj) const {
     T v = om[b * bs + ii + jj * g.mm1];
-    if (sc == nullptr) return v;
+    if (sc == nullptr || !has_coarser(Lc, b, lev)) return v;
     const int i = ii + 2, j = jj + 2;
     if (i == 2 || i == g.m || j == 2 || j == g.n) v = applybc_add<T>(g, v, i, j, sc + b * bs, T(1), T(1));
     return v;
@@ -138,7 +144,7 @@ struct DstField {
   T* a; long bs;
   __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const { a[b * bs + ii + jj * g.mm1] = v; }
 };
-// a = tab * v (ctci: laminv * in, src/myfft.f90:106)
+// a = tab * v (ctci: laminv * in, src/myfft.f90:106); tab shared by the group
 template <typename T>
 struct DstScale {
   T* a; const T* tab; long bs;
@@ -147,13 +153,14 @@ struct DstScale {
     a[b * bs + e] = tab[e] * v;
   }
 };
-// a = tab * v / norm (ainv: lam1i(:,:,1) * in / normalize, src/myfft.f90:121)
+// a = tab * v / norm (ainv: lam1i(:,:,1) * in / normalize, src/myfft.f90:121);
+// tab is the level-1 table of case b (stride tbs)
 template <typename T>
 struct DstScaleDiv {
-  T* a; const T* tab; T norm; long bs;
+  T* a; const T* tab; T norm; long bs; long tbs;
   __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
     const int e = ii + jj * g.mm1;
-    a[b * bs + e] = tab[e] * v / norm;
+    a[b * bs + e] = tab[b * tbs + e] * v / norm;
   }
 };
 // a = a - v (projection, src/operators.f90:174)
@@ -161,20 +168,22 @@ template <typename T>
 struct DstSub {
   T* a; long bs;
   __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
-    const int e = b * bs + ii + jj * g.mm1;
+    const long e = b * bs + ii + jj * g.mm1;
     a[e] = a[e] - v;
   }
 };
 
-// single-field row pass: rows jj; FFT p pairs rows 2Pg+p and 2Pg+P+p
+// single-field row pass: rows jj; FFT p pairs rows 2Pg+p and 2Pg+P+p.
+// Cases whose level count is <= lev are skipped (Lc == nullptr: none).
 template <typename T, class Src, class Dst>
 struct RowOp {
-  Geo g; Src src; Dst dst;
+  Geo g; Src src; Dst dst; const int* Lc; int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int row = gr * 2 * P + s;
-    return row < g.nm1 ? src(g, b, idx, row) : T(0);
+    return (row < g.nm1 && active(Lc, b, lev)) ? src(g, b, idx, row) : T(0);
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
+    if (!active(Lc, b, lev)) return;
     const int r0 = gr * 2 * P + p, r1 = r0 + P;
     if (r0 < g.nm1) dst(g, b, k - 1, r0, ya);
     if (r1 < g.nm1) dst(g, b, k - 1, r1, yb);
@@ -183,22 +192,23 @@ struct RowOp {
 // single-field column pass: columns ii
 template <typename T, class Src, class Dst>
 struct ColOp {
-  Geo g; Src src; Dst dst;
+  Geo g; Src src; Dst dst; const int* Lc; int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int col = gr * 2 * P + s;
-    return col < g.mm1 ? src(g, b, col, idx) : T(0);
+    return (col < g.mm1 && active(Lc, b, lev)) ? src(g, b, col, idx) : T(0);
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
+    if (!active(Lc, b, lev)) return;
     const int c0 = gr * 2 * P + p, c1 = c0 + P;
     if (c0 < g.mm1) dst(g, b, c0, k - 1, ya);
     if (c1 < g.mm1) dst(g, b, c1, k - 1, yb);
   }
 };
 
-// Step 1, first pass at level k: X = con1*rhs + con2*rhs_old + rhsbc with
-// rhs = rot(nonlinear(omega_k, q_k, q0_k, lastbc_k)) and Y = omega_k / (4mn),
-// rows of both transformed together (FFT p = X row + i Y row).
-// rhs_old_k <- rhs.  src/operators.f90:110-133.
+// Step 1, first pass at level slot lev: X = con1*rhs + con2*rhs_old + rhsbc
+// with rhs = rot(nonlinear(omega_k, q_k, q0_k, lastbc_k)) and
+// Y = omega_k / (4mn), rows of both transformed together (FFT p = X row +
+// i Y row); rhs_old_k <- rhs.  src/operators.f90:110-133.
 // Y is pre-divided by the normalisation 4mn so that both halves of a packed
 // FFT have the scale of the final result: an FFT's rounding is relative to
 // the larger half, and packing omega at full scale against X (which carries
@@ -206,36 +216,41 @@ struct ColOp {
 template <typename T>
 struct Step1RowOp {
   Geo g;
-  const T* om;      // omega_k (old)
-  const T* q; const T* q0;
-  const T* lastbc;  // edge vector of level k (get_bc of old omega_{k+1}, x0.25)
-  const T* omc;     // new omega_{k+1} (nullptr on the coarsest level)
-  T* rhs_old;
-  T* tx; T* ty;     // row-transformed X and Y
-  T con1, con2, vfac, norm;
-  int first;        // itime == 1: explicit Euler start
+  const T* om;      // omega at level lev (old)        stride fbs
+  const T* omc;     // omega at level lev+1 (new)      stride fbs; nullptr if lev+1 == Lmax
+  const T* q; const T* q0;  //                         stride qbs
+  const T* lastbc;  // get_bc of old omega_{lev+1}, x0.25  stride bbs
+  T* rhs_old;       //                                 stride fbs
+  T* t1;            // row-transformed X at t1 + b*2nw, Y at t1 + b*2nw + nw
+  const T* con1; const T* con2; const T* vfac;  // [b*Lmax + lev]
+  const int* Lc;
+  long fbs, qbs, bbs;
+  T norm;
+  int lev, first;   // first: itime == 1, explicit Euler start
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int row = gr * P + (s < P ? s : s - P);
-    if (row >= g.nm1) return T(0);
+    if (row >= g.nm1 || !active(Lc, b, lev)) return T(0);
     const int e = idx + row * g.mm1;
-    if (s >= P) return om[e] / norm;
+    if (s >= P) return om[b * fbs + e] / norm;
     const int i = idx + 2, j = row + 2;
-    const T rhs = rot_nonlinear<T>(g, i, j, om, q, q0, lastbc);
-    const T ro = first ? rhs : rhs_old[e];
-    rhs_old[e] = rhs;
+    const T rhs = rot_nonlinear<T>(g, i, j, om + b * fbs, q + b * qbs, q0 + b * qbs, lastbc + b * bbs);
+    T* ro = rhs_old + b * fbs + e;
+    const T rold = first ? rhs : *ro;
+    *ro = rhs;
+    const int c = b * g.Lmax + lev;
     T bcs = T(0);
     if (i == 2 || i == g.m || j == 2 || j == g.n) {
-      bcs = applybc_vec<T>(g, bcs, i, j, lastbc, vfac);
-      bcs = applybc_add<T>(g, bcs, i, j, omc, T(0.25), vfac);
+      bcs = applybc_vec<T>(g, bcs, i, j, lastbc + b * bbs, vfac[c]);
+      if (omc != nullptr && has_coarser(Lc, b, lev)) bcs = applybc_add<T>(g, bcs, i, j, omc + b * fbs, T(0.25), vfac[c]);
     }
-    return con1 * rhs + con2 * ro + bcs;
+    return con1[c] * rhs + con2[c] * rold + bcs;
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     const int row = gr * P + p;
-    if (row >= g.nm1) return;
-    const int e = (k - 1) + row * g.mm1;
-    tx[e] = ya;
-    ty[e] = yb;
+    if (row >= g.nm1 || !active(Lc, b, lev)) return;
+    const long e = (long)b * 2 * g.nw + (k - 1) + row * g.mm1;
+    t1[e] = ya;
+    t1[e + g.nw] = yb;
   }
 };
 
@@ -245,44 +260,52 @@ struct Step1RowOp {
 template <typename T>
 struct Step1ColOp {
   Geo g;
-  const T* tx; const T* ty;
-  const T* lam1n; const T* lam1i;  // level-k tables (lam1n = lam1 * 4mn)
-  T* z;
+  const T* t1;                     // as written by Step1RowOp
+  const T* lam1n; const T* lam1i;  // level lev of case b at (b*Lmax + lev)*nw
+  T* z;                            // z + b*nw
+  const int* Lc;
+  int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int col = gr * P + (s < P ? s : s - P);
-    if (col >= g.mm1) return T(0);
-    const int e = col + idx * g.mm1;
-    return s < P ? tx[e] : ty[e];
+    if (col >= g.mm1 || !active(Lc, b, lev)) return T(0);
+    const long e = (long)b * 2 * g.nw + col + idx * g.mm1;
+    return s < P ? t1[e] : t1[e + g.nw];
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     const int col = gr * P + p;
-    if (col >= g.mm1) return;
+    if (col >= g.mm1 || !active(Lc, b, lev)) return;
     const int e = col + (k - 1) * g.mm1;
-    z[e] = lam1i[e] * (ya + lam1n[e] * yb);
+    const long t = ((long)b * g.Lmax + lev) * g.nw + e;
+    z[(long)b * g.nw + e] = lam1i[t] * (ya + lam1n[t] * yb);
   }
 };
 
 // ---------------------------------------------------------------------------
-// small kernels
+// small kernels (grid.y = case unless stated)
 // ---------------------------------------------------------------------------
-// lastbc(:,k) = get_bc(omega(:,:,k+1), 0.25) for k < L; 0 for k = L
-// (src/operators.f90:97-100).  grid: ceil(nbc*L/256)
+// lastbc(:,k) = get_bc(omega(:,:,k+1), 0.25) below a case's coarsest level,
+// 0 on it (src/operators.f90:97-100).  grid.x over Lmax*nbc
 template <typename T>
-__global__ void k_lastbc(Geo g, const T* __restrict__ om, T* __restrict__ lastbc) {
+__global__ void k_lastbc(Geo g, const T* __restrict__ om, T* __restrict__ lastbc, const int* __restrict__ Lc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.nbc * g.L) return;
+  if (t >= g.nbc * g.Lmax) return;
+  const int b = blockIdx.y;
   const int lev = t / g.nbc, p = t - lev * g.nbc;
-  lastbc[t] = (lev < g.L - 1) ? getbc_at<T>(g, om + (long)(lev + 1) * g.nw, p + 1, T(0.25)) : T(0);
+  const long fb = (long)b * g.Lmax;
+  T v = T(0);
+  if (has_coarser(Lc, b, lev)) v = getbc_at<T>(g, om + (fb + lev + 1) * g.nw, p + 1, T(0.25));
+  lastbc[(fb + lev) * g.nbc + p] = v;
 }
 
-// coarsify (src/operators.f90:307-333), in place on level k from level k-1.
-// grid: ceil(((m/2-1)(n/2-1))/256) x batch
+// coarsify (src/operators.f90:307-333) of level lev from level lev-1, in
+// place.  grid.x over (m/2-1)(n/2-1)
 template <typename T>
-__global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ fine, long bs) {
+__global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ fine, long bs,
+                           const int* __restrict__ Lc, int lev) {
   const int wi = g.m / 2 - 1, wj = g.n / 2 - 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= wi * wj) return;
   const int b = blockIdx.y;
+  if (t >= wi * wj || !active(Lc, b, lev)) return;
   const int i = g.m / 4 + 2 + t % wi, j = g.n / 4 + 2 + t / wi;
   const int ii = 2 * i - g.m / 2 - 1, jj = 2 * j - g.n / 2 - 1;
   const T* r = fine + b * bs;
@@ -318,17 +341,19 @@ __device__ __forceinline__ T curl_face(const Geo& g, int f, const T* __restrict_
   }
 }
 
-// vel = curl(s, get_bc(sc)) over all faces.  grid: ceil(nq/256) x batch
+// vel = curl(s_lev, get_bc(s_{lev+1})) over all faces (zero edges on a case's
+// coarsest level).  grid.x over nq; s/sc stride bs, vel stride vbs
 template <typename T>
 __global__ void k_curl(Geo g, const T* __restrict__ s, const T* __restrict__ sc, T* __restrict__ vel, long bs,
-                       long vbs) {
+                       long vbs, const int* __restrict__ Lc, int lev) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= g.nq) return;
   const int b = blockIdx.y;
-  vel[b * vbs + f] = curl_face<T>(g, f, s + b * bs, sc ? sc + b * bs : nullptr);
+  if (f >= g.nq || !active(Lc, b, lev)) return;
+  const T* c = (sc != nullptr && (Lc == nullptr || has_coarser(Lc, b, lev))) ? sc + b * bs : nullptr;
+  vel[b * vbs + f] = curl_face<T>(g, f, s + b * bs, c);
 }
 
-// rot(x): Nq -> W (src/operators.f90:291-304).  grid: ceil(nw/256)
+// rot(x): Nq -> W (src/operators.f90:291-304).  grid.x over nw
 template <typename T>
 __global__ void k_rot(Geo g, const T* __restrict__ x, T* __restrict__ r) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -348,142 +373,188 @@ __global__ void k_rot_nonlinear(Geo g, const T* __restrict__ om, const T* __rest
 }
 
 // ---------------------------------------------------------------------------
-// smearing (E / E^T) on compact tables
+// smearing (E / E^T) on compact per-case tables
 // ---------------------------------------------------------------------------
-// Active faces: face_id[a] (0-based face), entries face_ptr[a]..face_ptr[a+1]
-// of (face_pt, face_w): the force index (0-based, x or y component) and the
-// weight, in increasing IB-point order (the ELL order of src/grid.f90:197-228).
-// Force gather: for force f, frc_ptr[f]..frc_ptr[f+1] of (frc_face, frc_w) in
-// increasing face order (the accumulation order of regT, src/operators.f90:541-545).
-// Active vertices: vtx_id[v] (0-based W index) and vtx_face[4v..4v+3] = active
-// face slots of v(i,j), v(i-1,j), u(i,j), u(i,j-1) (-1: face has no weight).
+// Active faces of case b: face_id[fbase+a] (0-based face of that case's q),
+// entries face_ptr[pbase+a]..face_ptr[pbase+a+1] of (face_pt, face_w): the
+// case-local force index (x or y component) and the weight, in increasing
+// IB-point order (the ELL order of src/grid.f90:197-228).
+// Force gather: force f of case b has entries frc_ptr[rbase+f]..+1 of
+// (frc_face, frc_w) in increasing face order (the accumulation order of
+// regT, src/operators.f90:541-545).
+// Active vertices: vtx_id[vbase+v] (0-based W index) and
+// vtx_face[4(vbase+v)..+3] = active-face slots (case-local) of v(i,j),
+// v(i-1,j), u(i,j), u(i,j-1) (-1: face has no weight).
+struct CaseSm {
+  int nact, fbase, pbase;  // faces
+  int nf, rbase;           // forces
+  int nvtx, vbase;         // vertices
+  int foff;                // offset of the case's force vectors
+  int nb;                  // IB points
+  long loff;               // offset of the case's Nf x Nf matrices
+  int nbody, bmin, ncol, hoff;  // diagnostics row layout
+  double dt, delta;
+};
+
 struct Smear {
-  int nact;            // active faces
   const int* face_id;
   const int* face_ptr;
   const int* face_pt;
   const double* face_w;
-  int nf;
   const int* frc_ptr;
   const int* frc_face;
   const double* frc_w;
-  int nvtx;
   const int* vtx_id;
   const int* vtx_face;
+  const CaseSm* cs;
 };
 
-template <typename T>
-__device__ __forceinline__ T reg_face(const Smear& sm, int a, const T* __restrict__ h0) {
-  T s = T(0);
-  for (int e = sm.face_ptr[a]; e < sm.face_ptr[a + 1]; e++) s = s + T(sm.face_w[e]) * h0[sm.face_pt[e]];
+template <typename H>
+__device__ __forceinline__ double reg_face(const Smear& sm, const CaseSm& c, int a, const H* __restrict__ h0) {
+  double s = 0.0;
+  for (int e = sm.face_ptr[c.pbase + a]; e < sm.face_ptr[c.pbase + a + 1]; e++)
+    s = s + sm.face_w[e] * double(h0[sm.face_pt[e]]);
   return s;
 }
 
 // rr = rot(reg(h0)) on the active vertices only; all other entries of rr are
-// never written and stay zero.  grid: ceil(nvtx/256) x batch
-template <typename T>
-__global__ void k_rotreg(Smear sm, const T* __restrict__ h0, long hbs, T* __restrict__ rr, long bs) {
+// never written and stay zero.  h0 of case b at h0 + foff (or b*hbs when
+// hbs > 0, for batches of force vectors of one case).  grid.x over max nvtx
+template <typename T, typename H>
+__global__ void k_rotreg(Smear sm, const H* __restrict__ h0, long hbs, T* __restrict__ rr, long bs, int one_case) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= sm.nvtx) return;
   const int b = blockIdx.y;
-  const T* h = h0 + b * hbs;
-  T fv[4];
+  const CaseSm c = sm.cs[one_case >= 0 ? one_case : b];
+  if (v >= c.nvtx) return;
+  const H* h = hbs > 0 ? h0 + b * hbs : h0 + c.foff;
+  double fv[4];
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const int a = sm.vtx_face[4 * v + c];
-    fv[c] = a >= 0 ? reg_face<T>(sm, a, h) : T(0);
+  for (int k = 0; k < 4; k++) {
+    const int a = sm.vtx_face[4 * (c.vbase + v) + k];
+    fv[k] = a >= 0 ? reg_face<H>(sm, c, a, h) : 0.0;
   }
-  rr[b * bs + sm.vtx_id[v]] = fv[0] - fv[1] - fv[2] + fv[3];
+  rr[b * bs + sm.vtx_id[c.vbase + v]] = T(fv[0] - fv[1] - fv[2] + fv[3]);
 }
 
-// h = reg(h0) on all faces (operator test entry): zero, then active faces
+// h = reg(h0) on the active faces of case 0 (operator test entry)
 template <typename T>
-__global__ void k_reg_active(Smear sm, const T* __restrict__ h0, T* __restrict__ h) {
+__global__ void k_reg_active(Smear sm, const double* __restrict__ h0, T* __restrict__ h) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= sm.nact) return;
-  h[sm.face_id[a]] = reg_face<T>(sm, a, h0);
+  const CaseSm c = sm.cs[0];
+  if (a >= c.nact) return;
+  h[sm.face_id[c.fbase + a]] = T(reg_face<double>(sm, c, a, h0));
 }
 
-// h0 = regT(h)  (grid: ceil(nf/256))
+// h0 = regT(h) for case 0 (operator test entry)
 template <typename T>
-__global__ void k_regT(Smear sm, const T* __restrict__ h, T* __restrict__ h0) {
+__global__ void k_regT(Smear sm, const T* __restrict__ h, double* __restrict__ h0) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= sm.nf) return;
-  T s = T(0);
-  for (int e = sm.frc_ptr[f]; e < sm.frc_ptr[f + 1]; e++) s = s + T(sm.frc_w[e]) * h[sm.frc_face[e]];
+  const CaseSm c = sm.cs[0];
+  if (f >= c.nf) return;
+  double s = 0.0;
+  for (int e = sm.frc_ptr[c.rbase + f]; e < sm.frc_ptr[c.rbase + f + 1]; e++)
+    s = s + sm.frc_w[e] * double(h[sm.frc_face[e]]);
   h0[f] = s;
 }
 
-// out[f] = regT(curl(s1, get_bc(s2)))[f] - sub[f]  (sub may be null), batched:
-// the right-hand side of the force system (src/operators.f90:163-164) and
-// the columns of a_times (src/operators.f90:459-461).
+// out = regT(curl(s1, get_bc(s2))) - sub (sub may be null): the right-hand
+// side of the force system (src/operators.f90:163-164), and the columns of
+// a_times (src/operators.f90:459-461) for batches of one case.  s2 is used
+// only for cases with more than one level.  grid.x over max nf.
 template <typename T>
 __global__ void k_regT_curl(Geo g, Smear sm, const T* __restrict__ s1, const T* __restrict__ s2, long bs,
-                            const double* __restrict__ sub, double* __restrict__ out, long obs) {
+                            const double* __restrict__ sub, double* __restrict__ out, long obs,
+                            const int* __restrict__ Lc, int one_case) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= sm.nf) return;
   const int b = blockIdx.y;
-  T s = T(0);
-  for (int e = sm.frc_ptr[f]; e < sm.frc_ptr[f + 1]; e++)
-    s = s + T(sm.frc_w[e]) * curl_face<T>(g, sm.frc_face[e], s1 + b * bs, s2 ? s2 + b * bs : nullptr);
-  double r = double(s);
-  if (sub) r = r - sub[f];
-  out[b * obs + f] = r;
+  const int ci = one_case >= 0 ? one_case : b;
+  const CaseSm c = sm.cs[ci];
+  if (f >= c.nf) return;
+  const T* sc = (s2 != nullptr && Lc[b] > 1) ? s2 + b * bs : nullptr;  // Lc indexed by batch entry
+  double s = 0.0;
+  for (int e = sm.frc_ptr[c.rbase + f]; e < sm.frc_ptr[c.rbase + f + 1]; e++)
+    s = s + sm.frc_w[e] * double(curl_face<T>(g, sm.frc_face[e], s1 + b * bs, sc));
+  const long o = obs > 0 ? b * obs + f : c.foff + f;
+  if (sub) s = s - sub[c.foff + f];
+  out[o] = s;
 }
 
-// motion_eff = motion_in - regT(q0_1) (src/operators.f90:155)
+// motion_eff = motion_in - regT(q0_1) (src/operators.f90:155), per case
 template <typename T>
-__global__ void k_motion(Smear sm, const T* __restrict__ q0, const double* __restrict__ motion_in,
+__global__ void k_motion(Smear sm, const T* __restrict__ q0, long qbs, const double* __restrict__ motion_in,
                          double* __restrict__ motion) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= sm.nf) return;
-  T s = T(0);
-  for (int e = sm.frc_ptr[f]; e < sm.frc_ptr[f + 1]; e++) s = s + T(sm.frc_w[e]) * q0[sm.frc_face[e]];
-  motion[f] = (motion_in ? motion_in[f] : 0.0) - double(s);
+  const int b = blockIdx.y;
+  const CaseSm c = sm.cs[b];
+  if (f >= c.nf) return;
+  double s = 0.0;
+  for (int e = sm.frc_ptr[c.rbase + f]; e < sm.frc_ptr[c.rbase + f + 1]; e++)
+    s = s + sm.frc_w[e] * double(q0[b * qbs + sm.frc_face[e]]);
+  motion[c.foff + f] = motion_in[c.foff + f] - s;
 }
 
 // ---------------------------------------------------------------------------
 // force solve: fb = cholsl(rhsf) as two triangular products with the
-// explicit inverse Li = L^{-1} of the Cholesky factor (column-major, lower).
-// y = Li r  then  x = Li^T y.  One wave per 64 output rows.
+// explicit inverse Li = L^{-1} of the Cholesky factor (column-major, lower):
+// y = Li r, then x = Li^T y.  One wave per output row, 4 rows per block.
 // ---------------------------------------------------------------------------
-// y[i] = sum_{k<=i} Li(i,k) r[k]      grid: ceil(nf/4) blocks of 256 (4 rows per block, one wave each)
-__global__ void k_trmv_lower(int nf, const double* __restrict__ Li, const double* __restrict__ r,
+__global__ void k_trmv_lower(Smear sm, const double* __restrict__ Li, const double* __restrict__ r,
                              double* __restrict__ y) {
+  const CaseSm c = sm.cs[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= nf) return;
+  if (i >= c.nf) return;
+  const double* L = Li + c.loff;
+  const double* rr = r + c.foff;
   double s = 0.0;
-  for (int k = lane; k <= i; k += 64) s += Li[i + (long)k * nf] * r[k];
+  for (int k = lane; k <= i; k += 64) s += L[i + (long)k * c.nf] * rr[k];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) y[i] = s;
+  if (lane == 0) y[c.foff + i] = s;
 }
-// x[i] = sum_{k>=i} Li(k,i) y[k]   (column i of Li, contiguous)
-__global__ void k_trmv_upper_t(int nf, const double* __restrict__ Li, const double* __restrict__ y,
-                               double* __restrict__ x) {
+// x[i] (+)= sum_{k>=i} Li(k,i) y[k]   (column i of Li, contiguous)
+__global__ void k_trmv_upper_t(Smear sm, const double* __restrict__ Li, const double* __restrict__ y,
+                               double* __restrict__ x, int accumulate) {
+  const CaseSm c = sm.cs[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= nf) return;
+  if (i >= c.nf) return;
+  const double* col = Li + c.loff + (long)i * c.nf;
+  const double* yy = y + c.foff;
   double s = 0.0;
-  const double* col = Li + (long)i * nf;
-  for (int k = i + lane; k < nf; k += 64) s += col[k] * y[k];
+  for (int k = i + lane; k < c.nf; k += 64) s += col[k] * yy[k];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) x[i] = s;
+  if (lane == 0) x[c.foff + i] = accumulate ? x[c.foff + i] + s : s;
+}
+// refinement residual r = b - A x with the symmetric A the factor was built
+// from (upper triangle of the ibfs.chd matrix, stored here as a full matrix)
+__global__ void k_symv_residual(Smear sm, const double* __restrict__ A, const double* __restrict__ bvec,
+                                const double* __restrict__ x, double* __restrict__ r) {
+  const CaseSm c = sm.cs[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= c.nf) return;
+  const double* Ai = A + c.loff;
+  double s = 0.0;
+  for (int k = lane; k < c.nf; k += 64) s += Ai[i + (long)k * c.nf] * x[c.foff + k];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) r[c.foff + i] = bvec[c.foff + i] - s;
 }
 
 // ---------------------------------------------------------------------------
 // diagnostics
 // ---------------------------------------------------------------------------
 // per-level partial sums of omega (write_circ, src/operators.f90:675-684):
-// grid (nchunk, L); partial[lev*nchunk + c]
+// grid (nchunk, Lmax, B); partial[((b*Lmax)+lev)*nchunk + c]
 template <typename T>
-__global__ void k_circ_partial(Geo g, const T* __restrict__ om, double* __restrict__ partial, int nchunk) {
+__global__ void k_circ_partial(Geo g, const T* __restrict__ om, double* __restrict__ partial, int nchunk,
+                               const int* __restrict__ Lc) {
   __shared__ double red[kThreads];
-  const int lev = blockIdx.y, c = blockIdx.x;
+  const int lev = blockIdx.y, c = blockIdx.x, b = blockIdx.z;
+  if (!active(Lc, b, lev)) return;
   const long per = (g.nw + nchunk - 1) / nchunk;
   const long lo = c * per, hi = min((long)g.nw, lo + per);
   double s = 0.0;
-  const T* w = om + (long)lev * g.nw;
+  const T* w = om + ((long)b * g.Lmax + lev) * g.nw;
   for (long e = lo + threadIdx.x; e < hi; e += kThreads) s += double(w[e]);
   red[threadIdx.x] = s;
   __syncthreads();
@@ -491,74 +562,75 @@ __global__ void k_circ_partial(Geo g, const T* __restrict__ om, double* __restri
     if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[lev * nchunk + c] = red[0];
+  if (threadIdx.x == 0) partial[((long)b * g.Lmax + lev) * nchunk + c] = red[0];
 }
 
-// One workgroup: slip, body forces and circulation of this step.
-// out = [fx(bmin..bmax), fy(bmin..bmax), circ(1..L), slip]
-// force: write_force (src/operators.f90:583-636) as 2*delta * sum over the
-// faces of reg(fb/dt restricted to the body); slip: write_slip
+// One workgroup per case: slip, body forces and circulation of this step.
+// Row of case b: [fx(bmin..bmax), fy(bmin..bmax), circ(1..L), slip] at
+// out_base + slot*rowlen + hoff.  force: write_force (src/operators.f90:
+// 583-636) = 2*delta * sum over faces of reg(fb/dt restricted to the body),
+// evaluated as 2*delta * sum_p wsum(p) fb(p)/dt with wsum(p) the sum of the
+// point's weights on u (x) or v (y) faces; slip: write_slip
 // (src/operators.f90:564) = max |regT(q_1) - motion|.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_diag(Geo g, Smear sm, const T* __restrict__ q1,
-                                               const double* __restrict__ motion, const double* __restrict__ fb,
-                                               const int* __restrict__ bdyno, int nb, int bmin, int nbody,
-                                               const double* __restrict__ partial, int nchunk, double dt,
-                                               double delta, double* __restrict__ out_base,
-                                               int* __restrict__ counter, int capacity) {
-  __shared__ double red[1024];
-  const int tid = threadIdx.x;
-  const int slot = counter ? *counter : 0;
-  double* out = out_base + (long)(slot < capacity ? slot : capacity - 1) * (2 * nbody + g.L + 1);
+__global__ __launch_bounds__(256) void k_diag(Geo g, Smear sm, const T* __restrict__ q, long qbs,
+                                              const double* __restrict__ motion, const double* __restrict__ fb,
+                                              const double* __restrict__ wsum, const int* __restrict__ bdyno,
+                                              const double* __restrict__ partial, int nchunk,
+                                              const int* __restrict__ Lc, double* __restrict__ out_base, int rowlen,
+                                              int* __restrict__ counter, int capacity) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x, b = blockIdx.x;
+  const CaseSm c = sm.cs[b];
+  const int L = Lc[b];
+  const int slot0 = counter[b];
+  const int slot = slot0 < capacity ? slot0 : capacity - 1;
+  double* out = out_base + (long)slot * rowlen + c.hoff;
+  const T* q1 = q + b * qbs;
   // slip
   double mx = 0.0;
-  for (int f = tid; f < sm.nf; f += 1024) {
-    T s = T(0);
-    for (int e = sm.frc_ptr[f]; e < sm.frc_ptr[f + 1]; e++) s = s + T(sm.frc_w[e]) * q1[sm.frc_face[e]];
-    mx = fmax(mx, fabs(double(s) - motion[f]));
+  for (int f = tid; f < c.nf; f += 256) {
+    double s = 0.0;
+    for (int e = sm.frc_ptr[c.rbase + f]; e < sm.frc_ptr[c.rbase + f + 1]; e++)
+      s = s + sm.frc_w[e] * double(q1[sm.frc_face[e]]);
+    mx = fmax(mx, fabs(s - motion[c.foff + f]));
   }
   red[tid] = mx;
   __syncthreads();
-  for (int off = 512; off > 0; off >>= 1) {
+  for (int off = 128; off > 0; off >>= 1) {
     if (tid < off) red[tid] = fmax(red[tid], red[tid + off]);
     __syncthreads();
   }
-  if (tid == 0) out[2 * nbody + g.L] = red[0];
+  if (tid == 0) out[2 * c.nbody + L] = red[0];
   __syncthreads();
-  // forces per body and component
-  const int nu = (g.m + 1) * g.n;
-  for (int bd = 0; bd < nbody; bd++) {
+  // forces, one body and component at a time
+  const int pbase = c.foff / 2;
+  for (int bd = 0; bd < c.nbody; bd++)
     for (int comp = 0; comp < 2; comp++) {
       double s = 0.0;
-      for (int a = tid; a < sm.nact; a += 1024) {
-        const bool isu = sm.face_id[a] < nu;
-        if (isu != (comp == 0)) continue;
-        double fv = 0.0;
-        for (int e = sm.face_ptr[a]; e < sm.face_ptr[a + 1]; e++) {
-          const int f = sm.face_pt[e];
-          const int pt = f < nb ? f : f - nb;
-          if (bdyno[pt] == bmin + bd) fv = fv + sm.face_w[e] * (fb[f] / dt);
+      for (int p = tid; p < c.nb; p += 256)
+        if (bdyno[pbase + p] == c.bmin + bd) {
+          const int f = comp * c.nb + p;
+          s += wsum[c.foff + f] * (fb[c.foff + f] / c.dt);
         }
-        s += fv;
-      }
       red[tid] = s;
       __syncthreads();
-      for (int off = 512; off > 0; off >>= 1) {
+      for (int off = 128; off > 0; off >>= 1) {
         if (tid < off) red[tid] += red[tid + off];
         __syncthreads();
       }
-      if (tid == 0) out[comp * nbody + bd] = red[0] * 2.0 * delta;
+      if (tid == 0) out[comp * c.nbody + bd] = red[0] * 2.0 * c.delta;
       __syncthreads();
     }
-  }
   // circulation
-  if (tid < g.L) {
+  if (tid < L) {
     double s = 0.0;
-    for (int c = 0; c < nchunk; c++) s += partial[tid * nchunk + c];
-    out[2 * nbody + tid] = s;
+    const double* pp = partial + ((long)b * g.Lmax + tid) * nchunk;
+    for (int k = 0; k < nchunk; k++) s += pp[k];
+    out[2 * c.nbody + tid] = s;
   }
   __syncthreads();
-  if (tid == 0 && counter) *counter = slot + 1;
+  if (tid == 0) counter[b] = slot0 + 1;
 }
 
 }  // namespace ibfs

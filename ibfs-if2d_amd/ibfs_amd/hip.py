@@ -59,6 +59,16 @@ def _load():
     L.ibfs_last_error.restype = C.c_char_p
     L.ibfs_last_error.argtypes = [C.c_void_p]
     L.ibfs_destroy.argtypes = [C.c_void_p]
+    _IP = C.POINTER(C.c_int)
+    L.ibfs_ensemble_create.argtypes = [C.c_int, _IP, _IP, _IP, _IP, _D, _D, _D, _D, _D, _D, _D, _IP, C.c_int,
+                                       C.c_int, C.POINTER(C.c_void_p)]
+    L.ibfs_ensemble_info.argtypes = [C.c_void_p, _IP, _IP, _IP]
+    L.ibfs_case_dims.argtypes = [C.c_void_p, C.c_int, _IP]
+    L.ibfs_case_set_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D, _D]
+    L.ibfs_case_get_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D, _D]
+    L.ibfs_case_set_cholesky.argtypes = [C.c_void_p, C.c_int, _D, _D]
+    L.ibfs_case_build_cholesky.argtypes = [C.c_void_p, C.c_int, _D, _D]
+    L.ibfs_ensemble_build_cholesky.argtypes = [C.c_void_p]
     return L
 
 
@@ -224,3 +234,95 @@ class Solver:
         vel = np.zeros(self.nq * max(nlev, 1))
         self._chk(_lib.ibfs_op_vort2flux(self._h, _p(om), _p(vel), nlev), "ibfs_op_vort2flux")
         return om, vel[: self.nq * nlev]
+
+
+class Ensemble:
+    """Independent cases advanced in lockstep by one handle (cases of equal
+    grid size share kernel launches; one HIP graph per ensemble step).
+
+    cases: list of dicts with the Solver keyword arguments (m, n, mgridlev,
+    dt, re, len_, offx, offy, xb, yb, bdyno)."""
+
+    def __init__(self, cases, unif_flow=True, precision=PLAIN_FP64):
+        nc = len(cases)
+        ia = lambda key: (C.c_int * nc)(*[int(c[key]) for c in cases])
+        da = lambda key: (C.c_double * nc)(*[float(c[key]) for c in cases])
+        nb = (C.c_int * nc)(*[len(c["xb"]) for c in cases])
+        xb = np.ascontiguousarray(np.concatenate([np.asarray(c["xb"], float) for c in cases]))
+        yb = np.ascontiguousarray(np.concatenate([np.asarray(c["yb"], float) for c in cases]))
+        bd = np.ascontiguousarray(np.concatenate(
+            [np.zeros(len(c["xb"]), np.int32) if c.get("bdyno") is None else np.asarray(c["bdyno"], np.int32)
+             for c in cases]).astype(np.int32))
+        h = C.c_void_p()
+        rc = _lib.ibfs_ensemble_create(nc, ia("m"), ia("n"), ia("mgridlev"), nb, da("dt"), da("re"), da("len_"),
+                                       da("offx"), da("offy"), _p(xb), _p(yb), bd.ctypes.data_as(_I),
+                                       1 if unif_flow else 0, precision, C.byref(h))
+        self._h = h
+        if rc:
+            msg = _lib.ibfs_last_error(h).decode() if h.value else "invalid arguments"
+            self.close()
+            raise IBFSError(f"ibfs_ensemble_create failed ({rc}): {msg}")
+        self.ncases = nc
+        rl = C.c_int()
+        col0 = (C.c_int * nc)()
+        _lib.ibfs_ensemble_info(h, None, C.byref(rl), col0)
+        self.rowlen = rl.value
+        self.col0 = list(col0)
+        self.dims = []
+        for c in range(nc):
+            d = (C.c_int * 12)()
+            _lib.ibfs_case_dims(h, c, d)
+            self.dims.append(dict(zip(("m", "n", "L", "nb", "nf", "nq", "nw", "nbc", "nbody", "bmin", "bmax",
+                                       "nsmear"), list(d))))
+        self.itime = 0
+
+    close = Solver.close
+    __del__ = Solver.__del__
+    _chk = Solver._chk
+
+    def build_cholesky(self):
+        self._chk(_lib.ibfs_ensemble_build_cholesky(self._h), "ibfs_ensemble_build_cholesky")
+
+    def build_case_cholesky(self, c):
+        d = self.dims[c]
+        cm = np.zeros(d["nf"] * d["nf"])
+        cv = np.zeros(d["nf"])
+        self._chk(_lib.ibfs_case_build_cholesky(self._h, c, _p(cm), _p(cv)), "ibfs_case_build_cholesky")
+        return cm, cv
+
+    def set_case_cholesky(self, c, cholmat, cholvec):
+        self._chk(_lib.ibfs_case_set_cholesky(self._h, c, _p(_f64(cholmat)), _p(_f64(cholvec))),
+                  "ibfs_case_set_cholesky")
+
+    def advance(self, nsteps=1, motion=None):
+        it = C.c_int(self.itime)
+        hist = np.zeros(nsteps * self.rowlen)
+        self._chk(_lib.ibfs_advance(self._h, C.byref(it), nsteps, _p(_f64(motion)), _p(hist)), "ibfs_advance")
+        self.itime = it.value
+        return hist.reshape(nsteps, self.rowlen)
+
+    def advance_timed(self, nsteps, motion=None):
+        it = C.c_int(self.itime)
+        self._chk(_lib.ibfs_advance(self._h, C.byref(it), nsteps, _p(_f64(motion)), None), "ibfs_advance")
+        self.itime = it.value
+        ms = C.c_double()
+        _lib.ibfs_last_advance_ms(self._h, C.byref(ms))
+        return ms.value
+
+    def case_rows(self, hist, c):
+        d = self.dims[c]
+        ncol = 2 * d["nbody"] + d["L"] + 1
+        return hist[:, self.col0[c]:self.col0[c] + ncol]
+
+    def get_state(self, c):
+        d = self.dims[c]
+        W, Q = d["nw"] * d["L"], d["nq"] * d["L"]
+        out = {"omega": np.zeros(W), "rhs_old": np.zeros(W), "q": np.zeros(Q), "q0": np.zeros(Q),
+               "fb": np.zeros(d["nf"])}
+        self._chk(_lib.ibfs_case_get_state(self._h, c, *[_p(out[k]) for k in ("omega", "rhs_old", "q", "q0", "fb")]),
+                  "ibfs_case_get_state")
+        return out
+
+    def set_state(self, c, omega=None, rhs_old=None, q=None, q0=None, fb=None):
+        a = [_f64(v) for v in (omega, rhs_old, q, q0, fb)]
+        self._chk(_lib.ibfs_case_set_state(self._h, c, *[_p(v) for v in a]), "ibfs_case_set_state")

@@ -100,6 +100,31 @@ int ibfs_step(ibfs_handle* h, int* itime, const double* motion, double* fb, doub
  * graph. */
 int ibfs_advance(ibfs_handle* h, int* itime, int nsteps, const double* motion, double* hist);
 
+/* ---------------------------------------------------------------------------
+ * Ensembles (no counterpart in the reference, which runs one case per
+ * process): ncases independent stationary-body cases advanced in lockstep by
+ * one handle.  Cases may differ in grid size, levels, Re, dt, domain and
+ * body; cases of equal m x n are batched into the same kernel launches, and
+ * one HIP graph replays the whole ensemble step.  Per-case arrays are
+ * indexed by case; xb, yb, bdyno are the cases' point lists concatenated
+ * (nb[c] entries each).  The motion vector of ibfs_advance is the cases'
+ * Nf vectors concatenated; a history row holds every case's row
+ * [fx.., fy.., circ(1..L), slip] at the column given by col0[c] of
+ * ibfs_ensemble_info.  The single-case calls above address case 0.
+ * ------------------------------------------------------------------------- */
+int ibfs_ensemble_create(int ncases, const int* m, const int* n, const int* mgridlev, const int* nb,
+                         const double* dt, const double* re, const double* len, const double* offsetx,
+                         const double* offsety, const double* xb, const double* yb, const int* bdyno, int unif_flow,
+                         int precision, ibfs_handle** out);
+int ibfs_ensemble_info(const ibfs_handle* h, int* ncases, int* rowlen, int* col0);
+int ibfs_case_dims(const ibfs_handle* h, int c, int* dims);
+int ibfs_case_set_state(ibfs_handle* h, int c, const double* omega, const double* rhs_old, const double* q,
+                        const double* q0, const double* fb);
+int ibfs_case_get_state(ibfs_handle* h, int c, double* omega, double* rhs_old, double* q, double* q0, double* fb);
+int ibfs_case_set_cholesky(ibfs_handle* h, int c, const double* cholmat, const double* cholvec);
+int ibfs_case_build_cholesky(ibfs_handle* h, int c, double* cholmat, double* cholvec);
+int ibfs_ensemble_build_cholesky(ibfs_handle* h);
+
 /* Operator entry points, one per reference module procedure; used by the
  * parity tests.  Each copies its inputs in, runs the device kernels on the
  * handle's stream, and copies the result out; none touches the state. */

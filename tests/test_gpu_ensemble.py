@@ -1,0 +1,101 @@
+"""Ensembles and the mixed-precision form.
+
+* An ensemble runs each case with the same kernels and arithmetic as a
+  single-case handle, so every case must match its own single-case run bit
+  for bit (cases of different sizes, level counts, Re, dt and bodies mixed).
+* The mixed form (FP32 fields and transforms, FP64 force solve, reductions
+  and one refinement sweep) is measured against the FP64 form; the bounds
+  are the ones the chosen form is held to (force histories <= 1e-5 relative
+  to the force scale) plus per-field bounds recorded with their values."""
+import numpy as np
+import pytest
+
+from conftest import case_args, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def ens_cases():
+    a = case_args("cyl32")
+    b = case_args("cyl32")
+    b.update(mgridlev=2, re=250.0, dt=0.015)
+    c = case_args("naca48x40")
+    d = case_args("cyl40r5")
+    e = case_args("cyl32")
+    e.update(offx=1.5, offy=2.25, re=60.0)
+    return [a, b, c, d, e]
+
+
+def test_ensemble_matches_single_case_runs():
+    from ibfs_amd import hip
+
+    cases = ens_cases()
+    ens = hip.Ensemble(cases)
+    assert ens.ncases == 5
+    ens.build_cholesky()
+    singles = []
+    for c in cases:
+        s = hip.Solver(**c)
+        s.build_cholesky()
+        singles.append(s)
+    hist = np.concatenate([ens.advance(4), ens.advance(7)])
+    for i, s in enumerate(singles):
+        sh = s.advance(11)
+        assert np.array_equal(ens.case_rows(hist, i), sh), f"case {i} history"
+        es, ss = ens.get_state(i), s.get_state()
+        for key in ("omega", "q", "rhs_old", "fb"):
+            assert np.array_equal(es[key], ss[key]), f"case {i} {key}"
+
+
+def test_ensemble_factor_equals_single_factor():
+    from ibfs_amd import hip
+
+    cases = ens_cases()[:3]
+    ens = hip.Ensemble(cases)
+    for i, c in enumerate(cases):
+        cm, cv = ens.build_case_cholesky(i)
+        s = hip.Solver(**c)
+        cm2, cv2 = s.build_cholesky()
+        assert np.array_equal(cm, cm2) and np.array_equal(cv, cv2)
+
+
+def mixed_vs_fp64(args, nsteps):
+    from ibfs_amd import hip
+
+    g64 = hip.Solver(**args)
+    g32 = hip.Solver(**args, precision=hip.MIXED)
+    cm, cv = g64.build_cholesky()
+    g32.set_cholesky(cm, cv)
+    h64 = g64.advance(nsteps)
+    h32 = g32.advance(nsteps)
+    s64, s32 = g64.get_state(), g32.get_state()
+    nb = g64.nbody
+    fscale = np.max(np.abs(h64[:, : 2 * nb]))
+    ferr = np.max(np.abs(h32[:, : 2 * nb] - h64[:, : 2 * nb])) / fscale
+    errs = {"force": ferr}
+    for k in range(g64.L):
+        w = slice(k * g64.nw, (k + 1) * g64.nw)
+        qq = slice(k * g64.nq, (k + 1) * g64.nq)
+        errs[f"omega{k + 1}"] = relerr(s32["omega"][w], s64["omega"][w])
+        errs[f"q{k + 1}"] = relerr(s32["q"][qq], s64["q"][qq])
+    errs["fb"] = relerr(s32["fb"], s64["fb"])
+    return errs
+
+
+@pytest.mark.parametrize("name", ["cyl32", "naca48x40", "cyl40r5"])
+def test_mixed_form_small(name):
+    errs = mixed_vs_fp64(case_args(name), 50)
+    print(name, {k: f"{v:.2e}" for k, v in errs.items()})
+    assert errs["force"] < 1e-5
+    assert max(v for k, v in errs.items() if k.startswith("omega")) < 1e-4
+
+
+def test_mixed_form_headline_case():
+    """512^2 x 5 levels NACA0012 at 20 deg, Re 300: 100 steps."""
+    from ibfs_amd import geometry
+
+    xb, yb, bd = geometry.naca4(4.0 / 512, aoa_deg=20.0)
+    args = dict(m=512, n=512, mgridlev=5, dt=0.002, re=300.0, len_=4.0, offx=1.0, offy=2.0, xb=xb, yb=yb, bdyno=bd)
+    errs = mixed_vs_fp64(args, 100)
+    print("naca512", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert errs["force"] < 1e-5
