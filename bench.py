@@ -7,10 +7,13 @@ about the quarter chord, IB spacing = grid spacing -> 261 points), Re = 300,
 dt = 0.002, impulsive start from rest (synthetic: no data files).  One step
 is one call of the reference's advance (src/operators.f90:62-188).
 
-Prints one JSON line: steps/s of the plain FP64 form, the roofline of the
-dominant kernel (a 1-D DST-I pass, timed live with HIP events on the
-library's stream) and a CPU baseline (the oracle's FP64 step, single
-thread, on a bounded sample of the same workload).
+Prints one JSON line.  ``value`` is steps/s of the plain FP64 form (the
+drop-in).  ``chosen`` reports the mixed FP32/FP64 form for one case and for
+a batch of independent copies of the case in one graph per step (aggregate
+case-steps/s), with its force error against the FP64 run.  ``roofline`` is
+the dominant kernel (a 1-D DST-I pass, timed live with HIP events on the
+library's stream); ``cpu_baseline`` is the oracle's FP64 step, single
+thread, on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -27,13 +30,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ibfs-if2d_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-WORKLOAD = dict(m=512, n=512, mgridlev=5, dt=0.002, re=300.0, len_=4.0, offx=1.0, offy=2.0)
-
-
-def workload_geometry(m=512, len_=4.0):
-    from ibfs_amd import geometry
-
-    return geometry.naca4(len_ / m, aoa_deg=20.0, x_le=0.0)
 
 
 def step_bytes(m, n, L, nf, s=8):
@@ -46,11 +42,14 @@ def step_bytes(m, n, L, nf, s=8):
     return s * (2 * (7 * L + 2) * W + L * (2 * Nq + 2 * W) + (2 * L - 1) * W + (L + 1) * Nq) + 8 * nf * (nf - 1)
 
 
-def cpu_baseline(xb, yb, bd, cm, cv, budget_s=15.0):
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(case, cm, cv, budget_s=15.0):
     import oracle as orc
 
-    w = WORKLOAD
-    o = orc.Oracle(w["m"], w["n"], w["mgridlev"], w["dt"], w["re"], w["len_"], w["offx"], w["offy"], xb, yb, bd)
+    o = orc.Oracle(**case)
     o.set_cholesky(cm, cv)
     o.advance()  # itime 1 (explicit Euler start), untimed
     n = 0
@@ -69,7 +68,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=16, help="copies of the case in the batched chosen form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-chosen", action="store_true", help="skip the chosen-form legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -77,50 +78,83 @@ def main(argv=None):
               "one per GPU with no communication; see DESIGN.md)", file=sys.stderr)
         return 2
 
-    from ibfs_amd import hip
+    from ibfs_amd import cases, hip
 
-    w = WORKLOAD
-    xb, yb, bd = workload_geometry(w["m"], w["len_"])
-    g = hip.Solver(w["m"], w["n"], w["mgridlev"], w["dt"], w["re"], w["len_"], w["offx"], w["offy"], xb, yb, bd)
+    case = cases.naca_512()
+    K, W0 = args.steps, max(args.warmup, 1)
+
+    # ---- plain FP64, one case
+    g = hip.Solver(**case)
     t0 = time.perf_counter()
     cm, cv = g.build_cholesky()
     t_pre = time.perf_counter() - t0
-    print(f"# preprocess (device a_times x {g.nf} + choldc): {t_pre:.3f} s", file=sys.stderr, flush=True)
-
-    g.advance(max(args.warmup, 1))
+    log(f"preprocess (device a_times x {g.nf} + choldc): {t_pre:.3f} s")
+    h64 = g.advance(W0)
     t0 = time.perf_counter()
-    dev_ms = g.advance_timed(args.steps, hist=True)
+    dev_ms = g.advance_timed(K)
     wall = time.perf_counter() - t0
-    steps_s = args.steps / wall
-    print(f"# {args.steps} steps: wall {wall * 1e3:.1f} ms, device {dev_ms:.1f} ms", file=sys.stderr, flush=True)
+    steps_s = K / wall
+    log(f"plain FP64: {K} steps wall {wall * 1e3:.1f} ms (device {dev_ms:.1f} ms) -> {steps_s:.1f} steps/s")
 
-    # dominant kernel: 1-D DST-I pass over one W field (read W + write W doubles)
-    W = g.nw
-    pass_bytes = 2 * 8 * W
+    # ---- dominant kernel: 1-D DST-I pass over one W field (read + write W doubles)
+    pass_bytes = 2 * 8 * g.nw
     t_row = g.time_dst_pass(0, 200)
     t_col = g.time_dst_pass(1, 200)
     t_pass = max(t_row, t_col)
     achieved = pass_bytes / (t_pass * 1e-3) / 1e9
     B = step_bytes(g.m, g.n, g.L, g.nf)
-    print(f"# DST pass: row {t_row * 1e3:.2f} us, column {t_col * 1e3:.2f} us; step bytes {B / 1e6:.1f} MB "
-          f"-> whole-step {B * steps_s / 1e9:.1f} GB/s", file=sys.stderr, flush=True)
+    log(f"DST pass: row {t_row * 1e3:.2f} us, column {t_col * 1e3:.2f} us; step bytes {B / 1e6:.1f} MB -> "
+        f"whole-step {B * steps_s / 1e9:.1f} GB/s")
+
+    chosen = None
+    if not args.no_chosen:
+        # ---- mixed form, one case: force error against the FP64 run over the warm-up window
+        gm = hip.Solver(**case, precision=hip.MIXED)
+        gm.set_cholesky(cm, cv)
+        hm = gm.advance(W0)
+        nb = g.nbody
+        ferr = float(np.max(np.abs(hm[:, :2 * nb] - h64[:, :2 * nb])) / np.max(np.abs(h64[:, :2 * nb])))
+        t0 = time.perf_counter()
+        gm.advance_timed(K)
+        wm = time.perf_counter() - t0
+        mixed_s = K / wm
+        log(f"mixed FP32/FP64, 1 case: {mixed_s:.1f} steps/s; force error vs FP64 over {W0} steps {ferr:.2e}")
+        del gm
+        # ---- mixed form, batched: args.batch independent copies in one graph per step
+        res = {}
+        for prec, name in ((hip.MIXED, "mixed"), (hip.PLAIN_FP64, "fp64")):
+            ens = hip.Ensemble([case] * args.batch, precision=prec)
+            for c in range(args.batch):
+                ens.set_case_cholesky(c, cm, cv)
+            ens.advance(W0)
+            kb = max(20, K // 4)
+            t0 = time.perf_counter()
+            ens.advance_timed(kb)
+            wb = time.perf_counter() - t0
+            res[name] = args.batch * kb / wb
+            log(f"{name}, batch of {args.batch}: {res[name]:.1f} case-steps/s ({kb} steps, {wb * 1e3:.1f} ms)")
+            del ens
+        chosen = {"mixed_one_case_steps_s": mixed_s, "batched_mixed_case_steps_s": res["mixed"],
+                  "batched_fp64_case_steps_s": res["fp64"], "batch": args.batch,
+                  "speedup_vs_plain": res["mixed"] / steps_s, "force_err_vs_fp64": ferr,
+                  "force_err_window_steps": W0}
 
     cpu = None
     if not args.no_cpu:
-        v, n = cpu_baseline(xb, yb, bd, cm, cv, args.cpu_seconds)
+        v, n = cpu_baseline(case, cm, cv, args.cpu_seconds)
         cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
                "sample": f"{n} steps (itime 2..{n + 1}) of the same 512^2 x 5 NACA0012 case, oracle FP64 step "
                          f"(own radix-4/2/3/5 FFT DST-I, gcc -O3), 1 thread of {os.cpu_count()} "
                          f"({platform.processor() or platform.machine()})"}
-        print(f"# CPU baseline: {v:.3f} steps/s over {n} steps -> GPU/CPU {steps_s / v:.1f}x", file=sys.stderr)
+        log(f"CPU baseline: {v:.3f} steps/s over {n} steps -> GPU/CPU {steps_s / v:.1f}x")
 
     out = {
         "metric": "time steps/s at 512^2 x 5 levels (plain FP64)",
         "value": steps_s,
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": K,
+        "warmup": W0,
         "ms_per_step": 1e3 / steps_s,
         "higher_is_better": True,
         "scaling": "weak",
@@ -137,6 +171,7 @@ def main(argv=None):
         "step_roofline": {"bytes_per_step": B, "achieved_GBs": B * steps_s / 1e9,
                           "frac": B * steps_s / 1e9 / HBM_PEAK_GBS},
         "preprocess_s": t_pre,
+        "chosen": chosen,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
