@@ -105,6 +105,13 @@ def main(argv=None):
     B = step_bytes(g.m, g.n, g.L, g.nf)
     log(f"DST pass: row {t_row * 1e3:.2f} us, column {t_col * 1e3:.2f} us; step bytes {B / 1e6:.1f} MB -> "
         f"whole-step {B * steps_s / 1e9:.1f} GB/s")
+    # streaming copies: one of the pass's size (the counter calibration point)
+    # and a large one (the measured copy bandwidth of this GPU)
+    t_copy_w = g.time_copy(g.nw, 200)
+    n_big = 1 << 25  # 32 Mi doubles = 256 MiB each way
+    t_copy_big = g.time_copy(n_big, 20)
+    copy_gbs = 2 * 8 * n_big / (t_copy_big * 1e-3) / 1e9
+    log(f"copy of W doubles {t_copy_w * 1e3:.2f} us; 256 MiB copy {copy_gbs:.0f} GB/s")
 
     chosen = None
     if not args.no_chosen:
@@ -167,7 +174,9 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "dst_pass (1-D DST-I over a 511x511 FP64 field)",
-                     "launch_us": t_pass * 1e3, "bytes_per_launch": pass_bytes},
+                     "launch_us": t_pass * 1e3, "bytes_per_launch": pass_bytes,
+                     "row_pass_us": t_row * 1e3, "col_pass_us": t_col * 1e3,
+                     "copy_same_size_us": t_copy_w * 1e3, "measured_copy_GBs": copy_gbs},
         "step_roofline": {"bytes_per_step": B, "achieved_GBs": B * steps_s / 1e9,
                           "frac": B * steps_s / 1e9 / HBM_PEAK_GBS},
         "preprocess_s": t_pre,

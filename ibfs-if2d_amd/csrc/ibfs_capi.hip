@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 
 #include "../../include/ibfs.h"
 #include "ibfs_engine_t.hpp"
@@ -35,6 +36,7 @@ struct ibfs_handle {
   int rowlen = 0;
   int ncases = 0;
   int diag_cap = 0;
+  bool use_graph = true;  // IBFS_NO_GRAPH=1: eager launches (for counter-collection profiling)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 
@@ -182,6 +184,8 @@ int create(std::vector<CaseDesc> cases, int precision, ibfs_handle** out) {
     }
     h->rowlen = col0;
     h->diag_cap = 1;
+    const char* ng = std::getenv("IBFS_NO_GRAPH");
+    h->use_graph = !(ng && ng[0] == '1');
   });
 }
 
@@ -334,8 +338,12 @@ int ibfs_advance(ibfs_handle* h, int* itime, int nsteps, const double* motion, d
       if (it == 1) {
         h->enqueue_all(1);
       } else {
-        h->capture();
-        HIPCHK(hipGraphLaunch(h->gexec, s));
+        if (h->use_graph) {
+          h->capture();
+          HIPCHK(hipGraphLaunch(h->gexec, s));
+        } else {
+          h->enqueue_all(0);
+        }
       }
       *itime = it;
     }
@@ -379,6 +387,32 @@ int ibfs_last_advance_ms(const ibfs_handle* h, double* ms) {
   const int nw = E->nw, nq = E->nq;                                                  \
   (void)nq;                                                                          \
   (void)st
+
+namespace {
+// plain streaming copy of n doubles (the calibration kernel for HBM counters)
+__global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+}  // namespace
+
+int ibfs_time_copy(ibfs_handle* h, long n, int reps, double* ms_per_launch) {
+  if (!h || !ms_per_launch || reps < 1 || n < 1) return IBFS_EINVAL;
+  return guarded(h, [&] {
+    hipStream_t st = h->stream;
+    DevBuf<double> a, b;
+    a.alloc(n); b.alloc(n);
+    a.zero(st);
+    const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, a.p, b.p, n);
+    HIPCHK(hipEventRecord(h->ev0, st));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, a.p, b.p, n);
+    HIPCHK(hipEventRecord(h->ev1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *ms_per_launch = ms / reps;
+  });
+}
 
 int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch) {
   if (!h || !ms_per_launch || reps < 1 || (axis != 0 && axis != 1)) return IBFS_EINVAL;

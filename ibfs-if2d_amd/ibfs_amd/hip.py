@@ -56,6 +56,7 @@ def _load():
     L.ibfs_op_vort2flux.argtypes = [C.c_void_p, _D, _D, C.c_int]
     L.ibfs_last_advance_ms.argtypes = [C.c_void_p, _D]
     L.ibfs_time_dst_pass.argtypes = [C.c_void_p, C.c_int, C.c_int, _D]
+    L.ibfs_time_copy.argtypes = [C.c_void_p, C.c_long, C.c_int, _D]
     L.ibfs_last_error.restype = C.c_char_p
     L.ibfs_last_error.argtypes = [C.c_void_p]
     L.ibfs_destroy.argtypes = [C.c_void_p]
@@ -177,6 +178,12 @@ class Solver:
         """Mean device time (ms) of one 1-D DST-I pass over a W field."""
         ms = C.c_double()
         self._chk(_lib.ibfs_time_dst_pass(self._h, axis, reps, C.byref(ms)), "ibfs_time_dst_pass")
+        return ms.value
+
+    def time_copy(self, n, reps=200):
+        """Mean device time (ms) of a streaming copy of n doubles."""
+        ms = C.c_double()
+        self._chk(_lib.ibfs_time_copy(self._h, int(n), reps, C.byref(ms)), "ibfs_time_copy")
         return ms.value
 
     def split(self, row):

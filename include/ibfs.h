@@ -154,6 +154,11 @@ int ibfs_last_advance_ms(const ibfs_handle* h, double* ms);
  * 1 = along j (columns). */
 int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch);
 
+/* Measurement only (no counterpart in the reference): a plain streaming copy
+ * of n doubles, launched reps times; the reference point that calibrates
+ * HBM byte counters (FETCH_SIZE / WRITE_SIZE) against known traffic. */
+int ibfs_time_copy(ibfs_handle* h, long n, int reps, double* ms_per_launch);
+
 const char* ibfs_last_error(const ibfs_handle* h);
 int ibfs_destroy(ibfs_handle* h);
 
