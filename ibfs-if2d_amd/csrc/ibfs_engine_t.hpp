@@ -71,6 +71,7 @@ struct Engine : GroupBase {
   int nf_tot = 0, nf_max = 0, nvtx_max = 0;
   long mat_tot = 0;
   bool refine = false;
+  Engine* base = nullptr;  // set on a tangent-linear engine: the state it is linearised about
 
   long fbs() const { return (long)Lmax * nw; }
   long qbs() const { return (long)Lmax * nq; }
@@ -319,6 +320,9 @@ struct Engine : GroupBase {
   // --------------------------------------------------------------- one step
   void enqueue_step(int first) override {
     const long FB = fbs(), QB = qbs(), BB = (long)Lmax * nbc;
+    if (base)  // tangent step: the edge values of the base state before its own step
+      hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, base->omega.p,
+                         base->lastbc.p, Lc.p);
     hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, omega.p, lastbc.p,
                        Lc.p);
     // STEP 1 (src/operators.f90:103-134), level slots Lmax..1
@@ -341,6 +345,12 @@ struct Engine : GroupBase {
                        norm,
                        lev,
                        first};
+      if (base) {
+        r1.bom = base->omega.p + (long)lev * nw;
+        r1.bq = base->q.p + (long)lev * nq;
+        r1.bq0 = base->q0.p + (long)lev * nq;
+        r1.blb = base->lastbc.p + (long)lev * nbc;
+      }
       launch<0>(r1, geo.nm1, 1, B);
       Step1ColOp<T> c1{geo, T1.p, lam1n.p, lam1i.p, T2.p, Lc.p, lev};
       launch<1>(c1, geo.mm1, 1, B);

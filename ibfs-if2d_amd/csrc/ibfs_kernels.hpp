@@ -227,13 +227,22 @@ struct Step1RowOp {
   long fbs, qbs, bbs;
   T norm;
   int lev, first;   // first: itime == 1, explicit Euler start
+  // Tangent-linear step: base state at level lev (same strides).  When bom
+  // is set, (om, q, lastbc) are perturbations, q0 is zero, and rhs is the
+  // linearised term rot(nl(d omega, q+q0, d bc)) + rot(nl(omega, d q, bc)).
+  const T* bom = nullptr;
+  const T* bq = nullptr;
+  const T* bq0 = nullptr;
+  const T* blb = nullptr;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int row = gr * P + (s < P ? s : s - P);
     if (row >= g.nm1 || !active(Lc, b, lev)) return T(0);
     const int e = idx + row * g.mm1;
     if (s >= P) return om[b * fbs + e] / norm;
     const int i = idx + 2, j = row + 2;
-    const T rhs = rot_nonlinear<T>(g, i, j, om + b * fbs, q + b * qbs, q0 + b * qbs, lastbc + b * bbs);
+    T rhs = rot_nonlinear<T>(g, i, j, om + b * fbs, bom ? bq + b * qbs : q + b * qbs,
+                             bom ? bq0 + b * qbs : q0 + b * qbs, lastbc + b * bbs);
+    if (bom) rhs = rhs + rot_nonlinear<T>(g, i, j, bom + b * fbs, q + b * qbs, q0 + b * qbs, blb + b * bbs);
     T* ro = rhs_old + b * fbs + e;
     const T rold = first ? rhs : *ro;
     *ro = rhs;

@@ -70,6 +70,10 @@ def _load():
     L.ibfs_case_set_cholesky.argtypes = [C.c_void_p, C.c_int, _D, _D]
     L.ibfs_case_build_cholesky.argtypes = [C.c_void_p, C.c_int, _D, _D]
     L.ibfs_ensemble_build_cholesky.argtypes = [C.c_void_p]
+    L.ibfs_objective.argtypes = [C.c_void_p, _IP, C.c_int, C.c_int, _D, C.c_double, C.c_double, _D]
+    L.ibfs_tangent_linear.argtypes = [C.c_void_p, _IP, C.c_int, C.c_int, _D, _D, C.c_double, C.c_double, _D, _D]
+    L.ibfs_adjoint_gradient.argtypes = [C.c_void_p, _IP, C.c_int, C.c_int, C.c_int, _D, C.c_double, C.c_double,
+                                        _D, _D]
     return L
 
 
@@ -180,6 +184,51 @@ class Solver:
         self._chk(_lib.ibfs_time_dst_pass(self._h, axis, reps, C.byref(ms)), "ibfs_time_dst_pass")
         return ms.value
 
+    # ---- sensitivities (mean drag/lift over a window w.r.t. the surface velocity u_B) ----
+    def _ncases(self):
+        return getattr(self, "ncases", 1)
+
+    def _nf_total(self):
+        return sum(d["nf"] for d in self.dims) if hasattr(self, "dims") else self.nf
+
+    def objective(self, nspin, nwin, ub=None, cd=1.0, cl=0.0):
+        it = C.c_int(self.itime)
+        J = np.zeros(self._ncases())
+        self._chk(_lib.ibfs_objective(self._h, C.byref(it), nspin, nwin, _p(_f64(ub)), cd, cl, _p(J)),
+                  "ibfs_objective")
+        self.itime = it.value
+        return J
+
+    def tangent(self, nspin, nwin, ub, dub, cd=1.0, cl=0.0):
+        it = C.c_int(self.itime)
+        J = np.zeros(self._ncases())
+        dJ = np.zeros(self._ncases())
+        self._chk(_lib.ibfs_tangent_linear(self._h, C.byref(it), nspin, nwin, _p(_f64(ub)), _p(_f64(dub)), cd, cl,
+                                           _p(J), _p(dJ)), "ibfs_tangent_linear")
+        self.itime = it.value
+        return J, dJ
+
+    def adjoint(self, nspin, nwin, ckpt, ub, cd=1.0, cl=0.0):
+        it = C.c_int(self.itime)
+        J = np.zeros(self._ncases())
+        g = np.zeros(self._nf_total())
+        self._chk(_lib.ibfs_adjoint_gradient(self._h, C.byref(it), nspin, nwin, ckpt, _p(_f64(ub)), cd, cl, _p(J),
+                                             _p(g)), "ibfs_adjoint_gradient")
+        self.itime = it.value
+        return J, g
+
+    def reset(self):
+        """Back to rest at itime 0 (zero vorticity, fluxes, forces)."""
+        z = lambda n: np.zeros(n)
+        if hasattr(self, "dims"):
+            for c, d in enumerate(self.dims):
+                self.set_state(c, omega=z(d["nw"] * d["L"]), rhs_old=z(d["nw"] * d["L"]), q=z(d["nq"] * d["L"]),
+                               fb=z(d["nf"]))
+        else:
+            self.set_state(omega=z(self.nw * self.L), rhs_old=z(self.nw * self.L), q=z(self.nq * self.L),
+                           fb=z(self.nf))
+        self.itime = 0
+
     def time_copy(self, n, reps=200):
         """Mean device time (ms) of a streaming copy of n doubles."""
         ms = C.c_double()
@@ -286,6 +335,12 @@ class Ensemble:
     close = Solver.close
     __del__ = Solver.__del__
     _chk = Solver._chk
+    _ncases = Solver._ncases
+    _nf_total = Solver._nf_total
+    objective = Solver.objective
+    tangent = Solver.tangent
+    adjoint = Solver.adjoint
+    reset = Solver.reset
 
     def build_cholesky(self):
         self._chk(_lib.ibfs_ensemble_build_cholesky(self._h), "ibfs_ensemble_build_cholesky")

@@ -125,6 +125,30 @@ int ibfs_case_set_cholesky(ibfs_handle* h, int c, const double* cholmat, const d
 int ibfs_case_build_cholesky(ibfs_handle* h, int c, double* cholmat, double* cholvec);
 int ibfs_ensemble_build_cholesky(ibfs_handle* h);
 
+/* ---------------------------------------------------------------------------
+ * Sensitivities (no counterpart in the reference).  Objective of every case:
+ *   J = (1/nwin) * sum over steps itime+nspin+1 .. itime+nspin+nwin of
+ *       (cd * sum over bodies of the x force + cl * sum of the y force)
+ * with the forces of write_force (src/operators.f90:583-636), after
+ * nspin + nwin steps from the handle's current state.  The control is the
+ * prescribed surface velocity u_B at every IB point (Nf values per case,
+ * x components then y components, concatenated over cases), active in every
+ * step as motion = delta * u_B (src/grid.f90:573-574).  All three advance
+ * the handle's state and *itime by nspin + nwin steps.  J, dJ: one value per
+ * case.  Tangent and adjoint runs need an FP64 handle whose cases share one
+ * grid size.
+ * ------------------------------------------------------------------------- */
+/* forward only: J */
+int ibfs_objective(ibfs_handle* h, int* itime, int nspin, int nwin, const double* ub, double cd, double cl,
+                   double* J);
+/* FP64 tangent-linear model along du_B: J and dJ = dJ/du_B . du_B */
+int ibfs_tangent_linear(ibfs_handle* h, int* itime, int nspin, int nwin, const double* ub, const double* dub,
+                        double cd, double cl, double* J, double* dJ);
+/* discrete adjoint: J and grad = dJ/du_B (concatenated per case), with the
+ * forward states checkpointed every ckpt steps and recomputed per segment */
+int ibfs_adjoint_gradient(ibfs_handle* h, int* itime, int nspin, int nwin, int ckpt, const double* ub, double cd,
+                          double cl, double* J, double* grad);
+
 /* Operator entry points, one per reference module procedure; used by the
  * parity tests.  Each copies its inputs in, runs the device kernels on the
  * handle's stream, and copies the result out; none touches the state. */
