@@ -72,6 +72,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-chosen", action="store_true", help="skip the chosen-form legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--adjoint-spin", type=int, default=2000, help="spin-up steps of the adjoint leg (0 with "
+                    "--adjoint-window 0 skips it)")
+    ap.add_argument("--adjoint-window", type=int, default=400)
+    ap.add_argument("--ensemble", type=int, default=1, help="run the 64-case uneven ensemble leg (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent cases shard "
@@ -146,6 +150,48 @@ def main(argv=None):
                   "speedup_vs_plain": res["mixed"] / steps_s, "force_err_vs_fp64": ferr,
                   "force_err_window_steps": W0}
 
+    adjoint = None
+    if args.adjoint_spin + args.adjoint_window > 0:
+        # ---- discrete adjoint: mean drag over the window after the spin-up,
+        # gradient w.r.t. u_B ~ N(0, 0.05^2) at the 188 points of the 384^2 x 4 cylinder
+        ac = cases.cylinder_384_adjoint()
+        sa = hip.Solver(**ac)
+        sa.build_cholesky()
+        ub = np.random.default_rng(0).normal(0.0, 0.05, sa.nf)
+        t0 = time.perf_counter()
+        Ja, ga = sa.adjoint(args.adjoint_spin, args.adjoint_window, 50, ub, 1.0, 0.0)
+        ta = time.perf_counter() - t0
+        sa.reset()
+        t0 = time.perf_counter()
+        sa.objective(args.adjoint_spin, args.adjoint_window, ub, 1.0, 0.0)
+        tf = time.perf_counter() - t0
+        adjoint = {"grads_per_s": 1.0 / ta, "seconds_per_gradient": ta, "forward_only_s": tf,
+                   "case": "384^2 x 4 levels cylinder (188 IB points), Re 150, dt 0.003",
+                   "spin_up_steps": args.adjoint_spin, "window_steps": args.adjoint_window, "checkpoint_every": 50,
+                   "parameters": int(sa.nf), "J_mean_drag": float(Ja[0]), "grad_norm": float(np.linalg.norm(ga))}
+        log(f"adjoint: {ta:.2f} s per gradient ({args.adjoint_spin}+{args.adjoint_window} steps, {sa.nf} params; "
+            f"forward alone {tf:.2f} s), J = {Ja[0]:.6f}")
+        del sa
+
+    ensemble = None
+    if args.ensemble:
+        # ---- 64 uneven cases (sizes 256..512, 3..5 levels, cylinders and airfoils), mixed form
+        cs = cases.ensemble_64()
+        ens = hip.Ensemble(cs, precision=hip.MIXED)
+        t0 = time.perf_counter()
+        ens.build_cholesky()
+        tb = time.perf_counter() - t0
+        ens.advance(W0)
+        kb = 100
+        t0 = time.perf_counter()
+        ens.advance_timed(kb)
+        we = time.perf_counter() - t0
+        ensemble = {"cases": len(cs), "case_steps_per_s": len(cs) * kb / we, "ms_per_ensemble_step": 1e3 * we / kb,
+                    "factor_build_s": tb, "precision": "mixed FP32/FP64"}
+        log(f"ensemble of {len(cs)} uneven cases: {len(cs) * kb / we:.0f} case-steps/s ({1e3 * we / kb:.2f} ms per "
+            f"ensemble step; factors {tb:.1f} s)")
+        del ens
+
     cpu = None
     if not args.no_cpu:
         v, n = cpu_baseline(case, cm, cv, args.cpu_seconds)
@@ -181,6 +227,8 @@ def main(argv=None):
                           "frac": B * steps_s / 1e9 / HBM_PEAK_GBS},
         "preprocess_s": t_pre,
         "chosen": chosen,
+        "adjoint": adjoint,
+        "ensemble64": ensemble,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
