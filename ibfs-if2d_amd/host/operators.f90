@@ -11,6 +11,10 @@ MODULE operators
   USE variables
   IMPLICIT NONE
 
+  ! force.dat, circ.dat and slip.dat stay open for the whole run on units of
+  ! their own (the reference reopens them every step on units 100 and 106,
+  ! which it also uses for the restart and Cholesky files)
+  INTEGER, PARAMETER :: uforce = 201, ucirc = 202, uslip = 206
   LOGICAL :: files_open = .FALSE.
   LOGICAL :: have_factor = .FALSE.
   REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE :: motion, hist
@@ -51,9 +55,9 @@ CONTAINS
 
   SUBROUTINE open_outputs(first_it)
     INTEGER :: first_it
-    CALL open_one(100, 'output/force.dat', first_it)
-    CALL open_one(101, 'output/circ.dat', first_it)
-    CALL open_one(106, 'output/slip.dat', first_it)
+    CALL open_one(uforce, 'output/force.dat', first_it)
+    CALL open_one(ucirc, 'output/circ.dat', first_it)
+    CALL open_one(uslip, 'output/slip.dat', first_it)
     files_open = .TRUE.
   END SUBROUTINE open_outputs
 
@@ -75,7 +79,7 @@ CONTAINS
 
   SUBROUTINE close_outputs
     IF (files_open) THEN
-       CLOSE(100); CLOSE(101); CLOSE(106)
+       CLOSE(uforce); CLOSE(ucirc); CLOSE(uslip)
        files_open = .FALSE.
     END IF
   END SUBROUTINE close_outputs
@@ -119,9 +123,9 @@ CONTAINS
 
     DO s = 1, nsteps
        k = (s-1)*ncol
-       WRITE(100,*) itime+s, (hist(k+b), b=1,nbody), (hist(k+nbody+b), b=1,nbody)
-       WRITE(101,*) itime+s, (hist(k+2*nbody+b), b=1,mgridlev)
-       WRITE(106,*) itime+s, hist(k+ncol)
+       WRITE(uforce,*) itime+s, (hist(k+b), b=1,nbody), (hist(k+nbody+b), b=1,nbody)
+       WRITE(ucirc,*) itime+s, (hist(k+2*nbody+b), b=1,mgridlev)
+       WRITE(uslip,*) itime+s, hist(k+ncol)
     END DO
     itime = it_c
     WRITE(*,*) "* itime =", itime, " SLIP =", hist((nsteps-1)*ncol+ncol)

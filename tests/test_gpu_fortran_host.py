@@ -1,0 +1,127 @@
+"""The Fortran 90 host (ibfs-if2d_amd/host) as a drop-in for the reference
+program: same input/ibfs.inp namelist, same advance loop, same output files
+(output/force.dat, circ.dat, slip.dat, restart ibfsNNNNNNN.var, input/ibfs.chd),
+with the step done by the gfx950 library.  Its outputs are compared with the
+oracle run of the same case, and a restart from a saved state continues the
+run exactly."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ibfs-if2d_amd", "host", "ibfs_gpu")
+
+NML = """ &READ_PARAMETERS ISTART = {istart}, ISTOP = {istop}, ISAVE = 5, M = 32,
+ N = 32, DT = 0.02, RE = 100., CGTOL = 1.E-7, CG_MAX_ITER = 3000,
+ UNIF_FLOW = T, LEN = 4.0, OFFSETX = 2.0, OFFSETY = 2.0, MGRIDLEV = 3,
+ MAKE_VORTEX = F, STATIONARY = T /
+"""
+
+
+def run_host(workdir, istart, istop, nblock=1):
+    os.makedirs(os.path.join(workdir, "input"), exist_ok=True)
+    os.makedirs(os.path.join(workdir, "output"), exist_ok=True)
+    with open(os.path.join(workdir, "input", "ibfs.inp"), "w") as f:
+        f.write(NML.format(istart=istart, istop=istop))
+    with open(os.path.join(workdir, "input", "ibfs_gpu.inp"), "w") as f:
+        f.write(f" &IBFS_GPU PRECISION = 0, NBLOCK = {nblock} /\n")
+    r = subprocess.run([EXE], cwd=workdir, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def read_table(path):
+    rows = []
+    with open(path) as f:
+        for line in f:
+            if line.strip():
+                rows.append([float(v.replace("D", "E")) for v in line.split()])
+    return np.array(rows)
+
+
+def read_record(path):
+    """One Fortran unformatted sequential record of float64."""
+    with open(path, "rb") as f:
+        n = struct.unpack("<i", f.read(4))[0]
+        data = np.frombuffer(f.read(n), dtype=np.float64)
+        assert struct.unpack("<i", f.read(4))[0] == n
+    return data
+
+
+@pytest.fixture(scope="module")
+def host_run(tmp_path_factory):
+    if not os.path.exists(EXE):
+        pytest.fail("Fortran host not built (run __graft_entry__.build())")
+    d = str(tmp_path_factory.mktemp("host"))
+    r = run_host(d, 0, 10)
+    return d, r
+
+
+def oracle_run(d, nsteps):
+    from ibfs_amd import geometry
+
+    xb, yb, bd = geometry.read_geom(os.path.join(d, "input", "geom.inp"))
+    o = orc.Oracle(32, 32, 3, 0.02, 100.0, 4.0, 2.0, 2.0, xb, yb, bd)
+    chd = read_record(os.path.join(d, "input", "ibfs.chd"))
+    nf = o.nf
+    o.set_cholesky(chd[: nf * nf], chd[nf * nf:])
+    rows = [o.advance() for _ in range(nsteps)]
+    return o, rows
+
+
+def test_host_outputs_match_oracle(host_run):
+    d, _ = host_run
+    force = read_table(os.path.join(d, "output", "force.dat"))
+    circ = read_table(os.path.join(d, "output", "circ.dat"))
+    slip = read_table(os.path.join(d, "output", "slip.dat"))
+    assert force.shape == (10, 3) and circ.shape == (10, 4) and slip.shape == (10, 2)
+    assert np.array_equal(force[:, 0], np.arange(1, 11))
+    o, rows = oracle_run(d, 10)
+    of = np.array([r[0] for r in rows])
+    # list-directed output keeps ~15-17 significant digits
+    assert np.max(np.abs(force[:, 1:] - of)) < 1e-11 * np.max(np.abs(of))
+    oc = np.array([r[1] for r in rows])
+    assert np.max(np.abs(circ[:, 1:] - oc)) < 1e-11 * np.sum(np.abs(o.omega))
+    osl = np.array([r[2] for r in rows])
+    assert np.max(np.abs(slip[:, 1] - osl)) < 1e-11 * o.delta
+    # restart file at istop: omega, fb, rhs_old, q, q0 (src/variables.f90:152)
+    var = read_record(os.path.join(d, "output", "ibfs0000010.var"))
+    W, Q = o.nw * o.L, o.nq * o.L
+    assert var.size == 2 * W + o.nf + 2 * Q
+    assert relerr(var[:W], o.omega) < 1e-11
+    assert relerr(var[W:W + o.nf], o.fb) < 1e-11
+    assert relerr(var[W + o.nf:2 * W + o.nf], o.rhs_old) < 1e-10
+    assert relerr(var[2 * W + o.nf:2 * W + o.nf + Q], o.q) < 1e-11
+
+
+def test_host_restart_continues_the_run(host_run, tmp_path):
+    d, _ = host_run
+    d2 = str(tmp_path)
+    shutil.copytree(os.path.join(d, "input"), os.path.join(d2, "input"))
+    os.makedirs(os.path.join(d2, "output"))
+    shutil.copy(os.path.join(d, "output", "ibfs0000005.var"), os.path.join(d2, "output"))
+    run_host(d2, 5, 10)
+    a = read_record(os.path.join(d, "output", "ibfs0000010.var"))
+    b = read_record(os.path.join(d2, "output", "ibfs0000010.var"))
+    assert np.array_equal(a, b)
+    f1 = read_table(os.path.join(d, "output", "force.dat"))
+    f2 = read_table(os.path.join(d2, "output", "force.dat"))
+    assert np.array_equal(f1[5:], f2)
+
+
+def test_host_blocked_calls_write_the_same_files(host_run, tmp_path):
+    d, _ = host_run
+    d3 = str(tmp_path)
+    shutil.copytree(os.path.join(d, "input"), os.path.join(d3, "input"))
+    run_host(d3, 0, 10, nblock=4)
+    for fn in ("force.dat", "circ.dat", "slip.dat"):
+        assert np.array_equal(read_table(os.path.join(d, "output", fn)), read_table(os.path.join(d3, "output", fn)))
