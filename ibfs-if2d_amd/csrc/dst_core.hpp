@@ -205,6 +205,66 @@ __global__ __launch_bounds__(kThreads) void dst_pass(Op op, FftPlan<T> plan) {
   }
 }
 
+// Fused column pair: DST-I along j, a pointwise spectral operation, and a
+// second DST-I along j, with one HBM round trip.  For a separable 2-D
+// transform S = Sx Sy, the sequence Sx . Sy . diag(tab) . Sy . Sx keeps the
+// middle three factors inside one column, so S diag(tab) S costs a row pass,
+// this pass and a row pass.  The operator adds
+//     void mid(int b, int g, int p, int P, int k, T ya, T yb, T& va, T& vb)
+// which maps the first transform's outputs of FFT p at k (as store would
+// receive them) to the inputs of the second transform at the same place.
+template <typename T, class Op>
+__device__ __forceinline__ void col2_mid(const Op& op, cplx<T>* z, int b, int g, int P, int n2, int t, int nt) {
+  const int N = n2 / 2, len = N - 1;
+  for (int e = t; e < P * len; e += nt) {
+    const int k = e / P + 1, p = e - (k - 1) * P;
+    const cplx<T> Z = z[p * n2 + k];
+    T va, vb;
+    op.mid(b, g, p, P, k, -Z.im, Z.re, va, vb);
+    z[p * n2 + k] = {va, vb};
+    z[p * n2 + n2 - k] = {-va, -vb};
+  }
+  for (int p = t; p < P; p += nt) {
+    z[p * n2] = {T(0), T(0)};
+    z[p * n2 + N] = {T(0), T(0)};
+  }
+  __syncthreads();
+}
+
+template <typename T, int EMAX, class Op>
+__global__ __launch_bounds__(kThreads) void dst_col2(Op op, FftPlan<T> plan) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* z = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int n2 = plan.n2;
+  const int N = n2 / 2;
+  const int len = N - 1;
+  const int P = EMAX / n2;
+  const int b = blockIdx.y;
+  const int g = blockIdx.x;
+  for (int p = threadIdx.x; p < P; p += kThreads) {
+    z[p * n2] = {T(0), T(0)};
+    z[p * n2 + N] = {T(0), T(0)};
+  }
+  const int tot = 2 * P * len;
+  for (int e = threadIdx.x; e < tot; e += kThreads) {
+    const int idx = e / (2 * P), s = e - idx * (2 * P);
+    const T x = op.load(b, g, s, P, idx);
+    const int p = s < P ? s : s - P;
+    const int h = s < P ? 0 : 1;
+    reinterpret_cast<T*>(z + p * n2 + idx + 1)[h] = x;
+    reinterpret_cast<T*>(z + p * n2 + n2 - idx - 1)[h] = -x;
+  }
+  __syncthreads();
+  fft_run<T, EMAX>(z, P, plan);
+  col2_mid<T, Op>(op, z, b, g, P, n2, threadIdx.x, kThreads);
+  fft_run<T, EMAX>(z, P, plan);
+  for (int e = threadIdx.x; e < P * len; e += kThreads) {
+    const int k = e / P + 1, p = e - (k - 1) * P;
+    const cplx<T> Z = z[p * n2 + k];
+    op.store(b, g, p, P, k, -Z.im, Z.re);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Size-specialised pass (the sizes of the production cases).  N2 = 2N is a
 // compile-time constant, so the radix schedule, every index division and the
@@ -246,14 +306,52 @@ constexpr Sched make_sched(int n2) {
   return s;
 }
 
+// Twiddles of one thread, all stages, held in registers: stage s with
+// ns > 1 owns NBT(s) * (R(s)-1) factors starting at off(s), loaded once at
+// kernel start so that no stage waits on a memory load.
+template <int N2>
+struct TwLayout {
+  static constexpr Sched sc = make_sched(N2);
+  static constexpr int TPF = N2 / 4;
+  static constexpr int nbt(int s) { return (N2 / sc.rad[s] + TPF - 1) / TPF; }
+  static constexpr int off(int s) {
+    int o = 0;
+    for (int q = 0; q < s; q++)
+      if (sc.ns[q] > 1) o += nbt(q) * (sc.rad[q] - 1);
+    return o;
+  }
+  static constexpr int total() { return off(sc.nst) > 0 ? off(sc.nst) : 1; }
+};
+
+template <typename T, int N2, int S>
+__device__ __forceinline__ void tw_load(cplx<T>* wr, const cplx<T>* __restrict__ tw, int lt) {
+  constexpr Sched sc = make_sched(N2);
+  if constexpr (S < sc.nst) {
+    constexpr int R = sc.rad[S], NS = sc.ns[S], NR = N2 / R, TPF = N2 / 4;
+    constexpr int NBT = (NR + TPF - 1) / TPF;
+    if constexpr (NS > 1) {
+      constexpr int O = TwLayout<N2>::off(S);
+#pragma unroll
+      for (int i = 0; i < NBT; i++) {
+        const int j = lt + i * TPF;
+        const int k = (j < NR ? j : 0) % NS;
+#pragma unroll
+        for (int r = 1; r < R; r++) wr[O + i * (R - 1) + r - 1] = tw[sc.toff[S] + k * (R - 1) + r - 1];
+      }
+    }
+    tw_load<T, N2, S + 1>(wr, tw, lt);
+  }
+}
+
 template <typename T, int N2, int P, int S>
-__device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* __restrict__ tw, int t) {
+__device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* wr, int t) {
   constexpr Sched sc = make_sched(N2);
   constexpr int R = sc.rad[S];
   constexpr int NS = sc.ns[S];
   constexpr int NR = N2 / R;       // butterflies per FFT
   constexpr int TPF = N2 / 4;
   constexpr int NBT = (NR + TPF - 1) / TPF;
+  constexpr int O = TwLayout<N2>::off(S);
   const int p = t / TPF, lt = t % TPF;
   cplx<T>* zp = z + p * N2;
   cplx<T> v[NBT][R];
@@ -264,10 +362,8 @@ __device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* __
 #pragma unroll
       for (int r = 0; r < R; r++) v[i][r] = zp[j + r * NR];
       if constexpr (NS > 1) {
-        const int k = j % NS;
-        const cplx<T>* w = tw + sc.toff[S] + k * (R - 1);
 #pragma unroll
-        for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], w[r - 1]);
+        for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], wr[O + i * (R - 1) + r - 1]);
       }
     }
   }
@@ -287,11 +383,42 @@ __device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* __
 }
 
 template <typename T, int N2, int P, int S>
-__device__ __forceinline__ void run(cplx<T>* z, const cplx<T>* tw, int t) {
+__device__ __forceinline__ void run(cplx<T>* z, const cplx<T>* wr, int t) {
   constexpr Sched sc = make_sched(N2);
   if constexpr (S < sc.nst) {
-    stage<T, N2, P, S>(z, tw, t);
-    run<T, N2, P, S + 1>(z, tw, t);
+    stage<T, N2, P, S>(z, wr, t);
+    run<T, N2, P, S + 1>(z, wr, t);
+  }
+}
+
+// Loads of one pass: every thread first issues all of its NL loads (values
+// kept in registers), then writes them into the odd extensions, so the loads
+// share one memory round trip.
+template <typename T, int N2, int P, int AXIS, class Op>
+__device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g, int t) {
+  constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
+  constexpr int TOT = 2 * P * LEN, NL = (TOT + NT - 1) / NT;
+  T x[NL];
+#pragma unroll
+  for (int q = 0; q < NL; q++) {
+    const int e = t + q * NT;
+    int s = 0, idx = 0;
+    if (AXIS == 0) { s = e / LEN; idx = e - s * LEN; }
+    else { idx = e / (2 * P); s = e - idx * (2 * P); }
+    x[q] = (NL * NT == TOT || e < TOT) ? op.load(b, g, s, P, idx) : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < NL; q++) {
+    const int e = t + q * NT;
+    if (NL * NT == TOT || e < TOT) {
+      int s, idx;
+      if (AXIS == 0) { s = e / LEN; idx = e - s * LEN; }
+      else { idx = e / (2 * P); s = e - idx * (2 * P); }
+      const int p = s < P ? s : s - P;
+      const int h = s < P ? 0 : 1;
+      reinterpret_cast<T*>(z + p * N2 + idx + 1)[h] = x[q];
+      reinterpret_cast<T*>(z + p * N2 + N2 - idx - 1)[h] = -x[q];
+    }
   }
 }
 
@@ -303,28 +430,48 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
   const int b = blockIdx.y;
   int g = blockIdx.x;
   if (AXIS == 1 && (gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  cplx<T> wr[TwLayout<N2>::total()];
+  tw_load<T, N2, 0>(wr, tw, t % (N2 / 4));
   if (t < P) {
     z[t * N2] = {T(0), T(0)};
     z[t * N2 + N] = {T(0), T(0)};
   }
-#pragma unroll 4
-  for (int e = t; e < 2 * P * LEN; e += NT) {
-    int s, idx;
-    if (AXIS == 0) { s = e / LEN; idx = e - s * LEN; }
-    else { idx = e / (2 * P); s = e - idx * (2 * P); }
-    const T x = op.load(b, g, s, P, idx);
-    const int p = s < P ? s : s - P;
-    const int h = s < P ? 0 : 1;
-    reinterpret_cast<T*>(z + p * N2 + idx + 1)[h] = x;
-    reinterpret_cast<T*>(z + p * N2 + N2 - idx - 1)[h] = -x;
-  }
+  load_seqs<T, N2, P, AXIS, Op>(op, z, b, g, t);
   __syncthreads();
-  run<T, N2, P, 0>(z, tw, t);
+  run<T, N2, P, 0>(z, wr, t);
 #pragma unroll 4
   for (int e = t; e < P * LEN; e += NT) {
     int p, k;
     if (AXIS == 0) { p = e / LEN; k = e - p * LEN + 1; }
     else { k = e / P + 1; p = e - (k - 1) * P; }
+    const cplx<T> Z = z[p * N2 + k];
+    op.store(b, g, p, P, k, -Z.im, Z.re);
+  }
+}
+
+// fused column pair (see dst_col2) at a compile-time size
+template <typename T, int N2, int P, class Op>
+__global__ __launch_bounds__(P * N2 / 4) void dst_col2_n(Op op, const cplx<T>* __restrict__ tw) {
+  constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
+  __shared__ cplx<T> z[P * N2];
+  const int t = threadIdx.x;
+  const int b = blockIdx.y;
+  int g = blockIdx.x;
+  if ((gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  cplx<T> wr[TwLayout<N2>::total()];
+  tw_load<T, N2, 0>(wr, tw, t % (N2 / 4));
+  if (t < P) {
+    z[t * N2] = {T(0), T(0)};
+    z[t * N2 + N] = {T(0), T(0)};
+  }
+  load_seqs<T, N2, P, 1, Op>(op, z, b, g, t);
+  __syncthreads();
+  run<T, N2, P, 0>(z, wr, t);
+  col2_mid<T, Op>(op, z, b, g, P, N2, t, NT);
+  run<T, N2, P, 0>(z, wr, t);
+#pragma unroll 4
+  for (int e = t; e < P * LEN; e += NT) {
+    const int k = e / P + 1, p = e - (k - 1) * P;
     const cplx<T> Z = z[p * N2 + k];
     op.store(b, g, p, P, k, -Z.im, Z.re);
   }

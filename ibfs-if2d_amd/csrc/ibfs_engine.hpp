@@ -102,6 +102,9 @@ struct CaseHost {
   std::vector<double> con1, con2, vfac;
   // compact smearing tables (case-local indices)
   std::vector<int> face_id, face_ptr, face_pt, frc_ptr, frc_face, vtx_id, vtx_face;
+  // per force entry: curl(s) at an interior face is s[frc_ia] - s[frc_ib]
+  // (vertex indices); -1 marks a face on the domain edge (get_bc terms)
+  std::vector<int> frc_ia, frc_ib;
   std::vector<double> face_w, frc_w, wsum;
   // force system
   bool have_chol = false;
@@ -157,12 +160,29 @@ inline void build_smear(CaseHost& c) {
     c.face_ptr.push_back((int)c.face_pt.size());
     c.nsmear = std::max(c.nsmear, (int)kv.second.size());
   }
-  c.frc_ptr.assign(1, 0); c.frc_face.clear(); c.frc_w.clear();
+  c.frc_ptr.assign(1, 0); c.frc_face.clear(); c.frc_w.clear(); c.frc_ia.clear(); c.frc_ib.clear();
+  const int nu = (m + 1) * n;
+  auto w_at = [&](int i, int j) { return (i - 2) + (j - 2) * (m - 1); };
+  // the two vertices of curl at an interior face (src/operators.f90:254-286)
+  auto curl_pair = [&](int face, int& a, int& b) {
+    a = b = -1;
+    if (face < nu) {  // u(i,j) = s(i,j+1) - s(i,j)
+      const int i = face % (m + 1) + 1, j = face / (m + 1) + 1;
+      if (i >= 2 && i <= m && j >= 2 && j <= n - 1) { a = w_at(i, j + 1); b = w_at(i, j); }
+    } else {          // v(i,j) = s(i,j) - s(i+1,j)
+      const int fv = face - nu, i = fv % m + 1, j = fv / m + 1;
+      if (i >= 2 && i <= m - 1 && j >= 2 && j <= n) { a = w_at(i, j); b = w_at(i + 1, j); }
+    }
+  };
   c.wsum.assign(std::max(c.nf, 1), 0.0);
   for (int f = 0; f < c.nf; f++) {
     for (auto& e : frc[f]) {
       c.frc_face.push_back(e.first);
       c.frc_w.push_back(e.second);
+      int a, b;
+      curl_pair(e.first, a, b);
+      c.frc_ia.push_back(a);
+      c.frc_ib.push_back(b);
       c.wsum[f] += e.second;
     }
     c.frc_ptr.push_back((int)c.frc_face.size());
@@ -175,7 +195,6 @@ inline void build_smear(CaseHost& c) {
     if (it == vtx.end()) it = vtx.emplace(w, std::array<int, 4>{-1, -1, -1, -1}).first;
     it->second[k] = a;
   };
-  const int nu = (m + 1) * n;
   for (auto& kv : slot_of_face) {
     const int face = kv.first, a = kv.second;
     if (face < nu) {

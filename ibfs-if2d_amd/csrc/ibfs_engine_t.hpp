@@ -58,7 +58,7 @@ struct Engine : GroupBase {
   DevBuf<T> laminv, lam1n, lam1i;
   DevBuf<T> omega, rhs_old, q, q0, S, T1, T2, T3, RR, lastbc;
 
-  DevBuf<int> face_id, face_ptr, face_pt, frc_ptr, frc_face, vtx_id, vtx_face, bdyno;
+  DevBuf<int> face_id, face_ptr, face_pt, frc_ptr, frc_face, frc_ia, frc_ib, vtx_id, vtx_face, bdyno;
   DevBuf<double> face_w, frc_w, wsum;
   DevBuf<CaseSm> csm;
   std::vector<CaseSm> hcs;
@@ -66,10 +66,11 @@ struct Engine : GroupBase {
 
   DevBuf<double> fb, motion_in, motion, rhsf, ytmp, rtmp, Linv, Asym;
   DevBuf<double> partial, diag;
+  DevBuf<double> Ainv, Gm;  // A^-1 (Nf x Nf per case) and rot(reg(A^-1)) (nvtx x Nf per case)
   DevBuf<int> counter;
   int diag_cap = 0, nchunk = 64;
   int nf_tot = 0, nf_max = 0, nvtx_max = 0;
-  long mat_tot = 0;
+  long mat_tot = 0, gmat_tot = 0;
   bool refine = false;
   Engine* base = nullptr;  // set on a tangent-linear engine: the state it is linearised about
 
@@ -133,10 +134,10 @@ struct Engine : GroupBase {
     upvec(lam1i, hl1i, st);
 
     // smearing tables, concatenated
-    std::vector<int> fid, fptr, fpt, rptr, rface, vid, vface, bd;
+    std::vector<int> fid, fptr, fpt, rptr, rface, ria, rib, vid, vface, bd;
     std::vector<double> fw, rw, ws;
     hcs.resize(B);
-    long loff = 0;
+    long loff = 0, goff = 0;
     int foff = 0, hof = 0;
     hoff.resize(B);
     for (int b = 0; b < B; b++) {
@@ -155,6 +156,8 @@ struct Engine : GroupBase {
       const int rebase = (int)rface.size();
       for (int v : c.frc_ptr) rptr.push_back(v + rebase);
       rface.insert(rface.end(), c.frc_face.begin(), c.frc_face.end());
+      ria.insert(ria.end(), c.frc_ia.begin(), c.frc_ia.end());
+      rib.insert(rib.end(), c.frc_ib.begin(), c.frc_ib.end());
       rw.insert(rw.end(), c.frc_w.begin(), c.frc_w.end());
       h.nvtx = (int)c.vtx_id.size();
       h.vbase = (int)vid.size();
@@ -163,6 +166,7 @@ struct Engine : GroupBase {
       h.foff = foff;
       h.nb = c.d.nb;
       h.loff = loff;
+      h.goff = goff;
       h.nbody = c.nbody;
       h.bmin = c.bmin;
       h.ncol = c.ncol;
@@ -174,12 +178,14 @@ struct Engine : GroupBase {
       bd.insert(bd.end(), c.d.bdyno.begin(), c.d.bdyno.end());
       foff += c.nf;
       loff += (long)c.nf * c.nf;
+      goff += (long)h.nvtx * c.nf;
       hof += c.ncol;
       nf_max = std::max(nf_max, c.nf);
       nvtx_max = std::max(nvtx_max, h.nvtx);
     }
     nf_tot = foff;
     mat_tot = loff;
+    gmat_tot = goff;
     rowlen = hof;
     upvec(face_id, fid, st);
     upvec(face_ptr, fptr, st);
@@ -188,13 +194,15 @@ struct Engine : GroupBase {
     upvec(frc_ptr, rptr, st);
     upvec(frc_face, rface, st);
     upvec(frc_w, rw, st);
+    upvec(frc_ia, ria, st);
+    upvec(frc_ib, rib, st);
     upvec(vtx_id, vid, st);
     upvec(vtx_face, vface, st);
     upvec(wsum, ws, st);
     upvec(bdyno, bd, st);
     upvec(csm, hcs, st);
-    sm = Smear{face_id.p, face_ptr.p, face_pt.p, face_w.p, frc_ptr.p, frc_face.p, frc_w.p, vtx_id.p, vtx_face.p,
-               csm.p};
+    sm = Smear{face_id.p, face_ptr.p, face_pt.p, face_w.p, frc_ptr.p, frc_face.p, frc_w.p, frc_ia.p, frc_ib.p,
+               vtx_id.p,   vtx_face.p, csm.p};
 
     // state and initial condition (src/variables.f90:54-67)
     const size_t W = (size_t)B * Lmax * nw, Q = (size_t)B * Lmax * nq;
@@ -227,6 +235,8 @@ struct Engine : GroupBase {
     ytmp.alloc(nff);
     rtmp.alloc(nff);
     Linv.alloc(std::max<long>(mat_tot, 1));
+    Ainv.alloc(std::max<long>(mat_tot, 1));
+    Gm.alloc(std::max<long>(gmat_tot, 1));
     if (refine) Asym.alloc(std::max<long>(mat_tot, 1));
     partial.alloc((size_t)B * Lmax * nchunk);
     counter.alloc(B);
@@ -275,6 +285,36 @@ struct Engine : GroupBase {
                        kEmax * sizeof(cplx<T>), st, op, plan);
   }
 
+  // fused column pair (dst_col2) over nseq columns
+  template <class Op>
+  void launch_col2(const Op& op, int nseq, int mult, int batch) {
+    const int n2 = pn.n2;
+    if (spec_n) {
+      switch (n2) {
+#define IBFS_SPEC_CASE(NN)                                                                                 \
+  case NN: {                                                                                               \
+    constexpr int P = spec_p(NN, 1);                                                                       \
+    const int groups = (nseq + mult * P - 1) / (mult * P);                                                 \
+    hipLaunchKernelGGL((spec::dst_col2_n<T, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op, \
+                       stw_n.p);                                                                           \
+    return;                                                                                                \
+  }
+        IBFS_SPEC_CASE(400)
+        IBFS_SPEC_CASE(512)
+        IBFS_SPEC_CASE(640)
+        IBFS_SPEC_CASE(768)
+        IBFS_SPEC_CASE(1024)
+        IBFS_SPEC_CASE(2048)
+#undef IBFS_SPEC_CASE
+        default: break;
+      }
+    }
+    const int P = kEmax / n2;
+    const int groups = (nseq + mult * P - 1) / (mult * P);
+    hipLaunchKernelGGL((dst_col2<T, kEmax, Op>), dim3(groups, batch), dim3(kThreads), kEmax * sizeof(cplx<T>), st,
+                       op, pn);
+  }
+
   template <class Src, class Dst>
   void row_pass(const Src& src, const Dst& dst, int batch, const int* gate = nullptr, int lev = 0) {
     launch<0>(RowOp<T, Src, Dst>{geo, src, dst, gate, lev}, geo.nm1, 2, batch);
@@ -285,17 +325,23 @@ struct Engine : GroupBase {
   }
 
   // out = S * diag(tab) * S (src): ctci (tab = laminv) or ainv (tab = the
-  // level-1 lam1i of each case, divided by 4mn).  t1/t2: scratch, stride nw.
+  // level-1 lam1i of each case, divided by 4mn), as Sx . (Sy tab Sy) . Sx:
+  // a row pass, the fused column pair and a row pass.  t1/t2: scratch,
+  // stride nw; dst may alias either.
   template <class Src, class Dst>
   void ctci_like(const Src& src, bool ainv_form, const Dst& dst, T* t1, T* t2, int batch, const int* gate, int lev,
                  const T* lam1i_l1 = nullptr, long tbs = 0) {
     row_pass(src, DstField<T>{t1, nw}, batch, gate, lev);
     if (ainv_form)
-      col_pass(SrcField<T>{t1, nw}, DstScaleDiv<T>{t2, lam1i_l1, norm, nw, tbs}, batch, gate, lev);
+      launch_col2(Col2Op<T, SrcField<T>, MidScaleDiv<T>, DstField<T>>{geo, SrcField<T>{t1, nw},
+                                                                       MidScaleDiv<T>{lam1i_l1, norm, tbs},
+                                                                       DstField<T>{t2, nw}, gate, lev},
+                  geo.mm1, 2, batch);
     else
-      col_pass(SrcField<T>{t1, nw}, DstScale<T>{t2, laminv.p, nw}, batch, gate, lev);
-    row_pass(SrcField<T>{t2, nw}, DstField<T>{t1, nw}, batch, gate, lev);
-    col_pass(SrcField<T>{t1, nw}, dst, batch, gate, lev);
+      launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{geo, SrcField<T>{t1, nw}, MidScale<T>{laminv.p},
+                                                                    DstField<T>{t2, nw}, gate, lev},
+                  geo.mm1, 2, batch);
+    row_pass(SrcField<T>{t2, nw}, dst, batch, gate, lev);
   }
 
   // vort2flux (src/operators.f90:191-240) on a batch of multi-level fields om
@@ -311,10 +357,10 @@ struct Engine : GroupBase {
       T* sc = (lev + 1 < Lmax) ? s + (long)(lev + 1) * nw : nullptr;
       ctci_like(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, false, DstField<T>{s + (long)lev * nw, bs}, t1,
                 t2, batch, gate, lev);
-      if (vel)
-        hipLaunchKernelGGL(k_curl<T>, dim3((nq + 255) / 256, batch), dim3(256), 0, st, geo, s + (long)lev * nw,
-                           sc, vel + (long)lev * nq, bs, qbs(), gate, lev);
     }
+    if (vel)
+      hipLaunchKernelGGL(k_curl_levels<T>, dim3((nq + 255) / 256, Lmax, batch), dim3(256), 0, st, geo, s, vel, bs,
+                         qbs(), gate);
   }
 
   // --------------------------------------------------------------- one step
@@ -352,33 +398,37 @@ struct Engine : GroupBase {
         r1.blb = base->lastbc.p + (long)lev * nbc;
       }
       launch<0>(r1, geo.nm1, 1, B);
-      Step1ColOp<T> c1{geo, T1.p, lam1n.p, lam1i.p, T2.p, Lc.p, lev};
-      launch<1>(c1, geo.mm1, 1, B);
-      row_pass(SrcField<T>{T2.p, nw}, DstField<T>{T3.p, nw}, B, Lc.p, lev);
-      col_pass(SrcField<T>{T3.p, nw}, DstField<T>{omega.p + (long)lev * nw, FB}, B, Lc.p, lev);
+      launch_col2(Step1Col2Op<T, DstField<T>>{geo, T1.p, lam1n.p, lam1i.p, DstField<T>{T2.p, nw}, Lc.p, lev},
+                  geo.mm1, 1, B);
+      row_pass(SrcField<T>{T2.p, nw}, DstField<T>{omega.p + (long)lev * nw, FB}, B, Lc.p, lev);
     }
-    // STEP 2 (src/operators.f90:142-171); motion is prepared per call
+    // STEP 2 (src/operators.f90:142-171); motion is prepared per call.  The
+    // force kernel also writes the step-3 source RR = rot(reg(fb)).
     vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
-    hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf_max + 255) / 256, B), dim3(256), 0, st, geo, sm, S.p,
-                       Lmax > 1 ? S.p + nw : nullptr, FB, motion.p, rhsf.p, 0L, Lc.p, -1);
-    const dim3 g4((nf_max + 3) / 4, B);
-    hipLaunchKernelGGL(k_trmv_lower, g4, dim3(256), 0, st, sm, Linv.p, rhsf.p, ytmp.p);
-    hipLaunchKernelGGL(k_trmv_upper_t, g4, dim3(256), 0, st, sm, Linv.p, ytmp.p, fb.p, 0);
-    if (refine) {  // one FP64 refinement sweep: fb += cholsl(rhsf - A fb)
-      hipLaunchKernelGGL(k_symv_residual, g4, dim3(256), 0, st, sm, Asym.p, rhsf.p, fb.p, rtmp.p);
-      hipLaunchKernelGGL(k_trmv_lower, g4, dim3(256), 0, st, sm, Linv.p, rtmp.p, ytmp.p);
-      hipLaunchKernelGGL(k_trmv_upper_t, g4, dim3(256), 0, st, sm, Linv.p, ytmp.p, fb.p, 1);
+    const T* S2 = Lmax > 1 ? S.p + nw : nullptr;
+    const size_t lds = sizeof(double) * 2 * std::max(nf_max, 1);
+    const dim3 gall(std::max(1, (nf_max + nvtx_max + kForceRows - 1) / kForceRows), B);
+    if (!refine) {
+      hipLaunchKernelGGL(k_force_solve<T>, gall, dim3(kForceThreads), lds, st, geo, sm, S.p, S2, FB, motion.p, Lc.p, Ainv.p,
+                         Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 0);
+    } else {  // one FP64 refinement sweep: fb += A^-1 (rhsf - A fb)
+      const dim3 gf(std::max(1, (nf_max + kForceRows - 1) / kForceRows), B);
+      hipLaunchKernelGGL(k_force_solve<T>, gf, dim3(kForceThreads), lds, st, geo, sm, S.p, S2, FB, motion.p, Lc.p, Ainv.p,
+                         Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 1);
+      hipLaunchKernelGGL(k_symv_residual, dim3((nf_max + 3) / 4, B), dim3(256), 0, st, sm, Asym.p, rhsf.p, fb.p,
+                         rtmp.p);
+      hipLaunchKernelGGL(k_force_solve<T>, gall, dim3(kForceThreads), lds, st, geo, sm, S.p, S2, FB, motion.p, Lc.p, Ainv.p,
+                         Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)rtmp.p, 2);
     }
     // STEP 3 (src/operators.f90:174)
-    hipLaunchKernelGGL((k_rotreg<T, double>), dim3((nvtx_max + 255) / 256, B), dim3(256), 0, st, sm, fb.p, 0L, RR.p,
-                       (long)nw, -1);
     ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, FB);
-    // STEP 4 (src/operators.f90:177)
-    vort2flux(omega.p, S.p, q.p, B, Lc.p, T2.p, T3.p);
-    // diagnostics (src/operators.f90:184-186)
-    hipLaunchKernelGGL(k_circ_partial<T>, dim3(nchunk, Lmax, B), dim3(kThreads), 0, st, geo, omega.p, partial.p,
-                       nchunk, Lc.p);
-    hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(256), 0, st, geo, sm, q.p, QB, motion.p, fb.p, wsum.p, bdyno.p,
+    // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
+    vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
+    // fluxes and diagnostics (src/operators.f90:184-186)
+    const int ncurl = (nq + 255) / 256 * Lmax;
+    hipLaunchKernelGGL(k_tail<T>, dim3(ncurl + nchunk * Lmax, B), dim3(256), 0, st, geo, S.p, q.p, omega.p, FB, QB,
+                       partial.p, nchunk, Lc.p);
+    hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, sm, q.p, QB, motion.p, fb.p, wsum.p, bdyno.p,
                        partial.p, nchunk, Lc.p, diag.p, rowlen, counter.p, diag_cap);
   }
 
@@ -480,6 +530,11 @@ struct Engine : GroupBase {
       std::copy(col.begin(), col.end(), Li.begin() + (size_t)j * nf);
     }
     Linv.upload(Li.data(), Li.size(), st, hcs[c].loff);
+    const long nn = (long)nf * nf, ng = (long)hcs[c].nvtx * nf;
+    hipLaunchKernelGGL(k_make_ainv, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, sm, c, Linv.p, Ainv.p);
+    if (ng > 0)
+      hipLaunchKernelGGL(k_make_g, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, sm, c, Ainv.p, Gm.p);
+    HIPCHK(hipGetLastError());
     if (refine) {
       std::vector<double> As((size_t)nf * nf);
       for (int j = 0; j < nf; j++)

@@ -144,25 +144,6 @@ struct DstField {
   T* a; long bs;
   __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const { a[b * bs + ii + jj * g.mm1] = v; }
 };
-// a = tab * v (ctci: laminv * in, src/myfft.f90:106); tab shared by the group
-template <typename T>
-struct DstScale {
-  T* a; const T* tab; long bs;
-  __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
-    const int e = ii + jj * g.mm1;
-    a[b * bs + e] = tab[e] * v;
-  }
-};
-// a = tab * v / norm (ainv: lam1i(:,:,1) * in / normalize, src/myfft.f90:121);
-// tab is the level-1 table of case b (stride tbs)
-template <typename T>
-struct DstScaleDiv {
-  T* a; const T* tab; T norm; long bs; long tbs;
-  __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
-    const int e = ii + jj * g.mm1;
-    a[b * bs + e] = tab[b * tbs + e] * v / norm;
-  }
-};
 // a = a - v (projection, src/operators.f90:174)
 template <typename T>
 struct DstSub {
@@ -196,6 +177,44 @@ struct ColOp {
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int col = gr * 2 * P + s;
     return (col < g.mm1 && active(Lc, b, lev)) ? src(g, b, col, idx) : T(0);
+  }
+  __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
+    if (!active(Lc, b, lev)) return;
+    const int c0 = gr * 2 * P + p, c1 = c0 + P;
+    if (c0 < g.mm1) dst(g, b, c0, k - 1, ya);
+    if (c1 < g.mm1) dst(g, b, c1, k - 1, yb);
+  }
+};
+
+// spectral factors of the fused column pair (value at column ii, row jj)
+// tab * v: ctci's laminv (src/myfft.f90:106), shared by the group
+template <typename T>
+struct MidScale {
+  const T* tab;
+  __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const { return tab[ii + jj * g.mm1] * v; }
+};
+// tab * v / norm: ainv's lam1i(:,:,1) / normalize (src/myfft.f90:121), per case
+template <typename T>
+struct MidScaleDiv {
+  const T* tab; T norm; long tbs;
+  __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const {
+    return tab[b * tbs + ii + jj * g.mm1] * v / norm;
+  }
+};
+
+// fused column pair: columns ii, Sy . diag(mid) . Sy
+template <typename T, class Src, class Mid, class Dst>
+struct Col2Op {
+  Geo g; Src src; Mid md; Dst dst; const int* Lc; int lev;
+  __device__ T load(int b, int gr, int s, int P, int idx) const {
+    const int col = gr * 2 * P + s;
+    return (col < g.mm1 && active(Lc, b, lev)) ? src(g, b, col, idx) : T(0);
+  }
+  __device__ void mid(int b, int gr, int p, int P, int k, T ya, T yb, T& va, T& vb) const {
+    const int c0 = gr * 2 * P + p, c1 = c0 + P;
+    const bool on = active(Lc, b, lev);
+    va = (on && c0 < g.mm1) ? md(g, b, c0, k - 1, ya) : T(0);
+    vb = (on && c1 < g.mm1) ? md(g, b, c1, k - 1, yb) : T(0);
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     if (!active(Lc, b, lev)) return;
@@ -263,15 +282,19 @@ struct Step1RowOp {
   }
 };
 
-// Step 1, second pass: columns of X~ and Y~ together, then
-// Z = lam1i_k * (X^ + lam1_k * S(omega)) (src/operators.f90:126-130), where
-// lam1_k * S(omega) = lam1n_k * S(omega / 4mn) with lam1n = lam1 * 4mn.
-template <typename T>
-struct Step1ColOp {
+// Step 1, second and third transforms as a fused column pair: the columns
+// of X~ and Y~ (FFT p = X column + i Y column, as written by Step1RowOp),
+// the CN combine Z = lam1i_k * (X^ + lam1_k * S(omega))
+// (src/operators.f90:126-130) with lam1_k * S(omega) = lam1n_k * S(omega/4mn),
+// lam1n = lam1 * 4mn, then Sy of the combined column (FFT p = Z column + i 0).
+// Tables of level lev of case b at (b*Lmax + lev)*nw.  P columns per
+// workgroup.
+template <typename T, class Dst>
+struct Step1Col2Op {
   Geo g;
-  const T* t1;                     // as written by Step1RowOp
-  const T* lam1n; const T* lam1i;  // level lev of case b at (b*Lmax + lev)*nw
-  T* z;                            // z + b*nw
+  const T* t1;
+  const T* lam1n; const T* lam1i;
+  Dst dst;
   const int* Lc;
   int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
@@ -280,12 +303,20 @@ struct Step1ColOp {
     const long e = (long)b * 2 * g.nw + col + idx * g.mm1;
     return s < P ? t1[e] : t1[e + g.nw];
   }
+  __device__ void mid(int b, int gr, int p, int P, int k, T ya, T yb, T& va, T& vb) const {
+    const int col = gr * P + p;
+    vb = T(0);
+    if (col >= g.mm1 || !active(Lc, b, lev)) {
+      va = T(0);
+      return;
+    }
+    const long t = ((long)b * g.Lmax + lev) * g.nw + col + (k - 1) * g.mm1;
+    va = lam1i[t] * (ya + lam1n[t] * yb);
+  }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     const int col = gr * P + p;
     if (col >= g.mm1 || !active(Lc, b, lev)) return;
-    const int e = col + (k - 1) * g.mm1;
-    const long t = ((long)b * g.Lmax + lev) * g.nw + e;
-    z[(long)b * g.nw + e] = lam1i[t] * (ya + lam1n[t] * yb);
+    dst(g, b, col, k - 1, ya);
   }
 };
 
@@ -362,6 +393,20 @@ __global__ void k_curl(Geo g, const T* __restrict__ s, const T* __restrict__ sc,
   vel[b * vbs + f] = curl_face<T>(g, f, s + b * bs, c);
 }
 
+// k_curl on every level at once, once all streamfunctions are final:
+// grid (nq / 256, Lmax, batch); s stride bs per case, nw per level; vel
+// stride vbs per case, nq per level.
+template <typename T>
+__global__ void k_curl_levels(Geo g, const T* __restrict__ s, T* __restrict__ vel, long bs, long vbs,
+                              const int* __restrict__ Lc) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lev = blockIdx.y, b = blockIdx.z;
+  if (f >= g.nq || !active(Lc, b, lev)) return;
+  const T* sl = s + b * bs + (long)lev * g.nw;
+  const bool coarser = lev + 1 < g.Lmax && (Lc == nullptr || has_coarser(Lc, b, lev));
+  vel[b * vbs + (long)lev * g.nq + f] = curl_face<T>(g, f, sl, coarser ? sl + g.nw : nullptr);
+}
+
 // rot(x): Nq -> W (src/operators.f90:291-304).  grid.x over nw
 template <typename T>
 __global__ void k_rot(Geo g, const T* __restrict__ x, T* __restrict__ r) {
@@ -401,6 +446,7 @@ struct CaseSm {
   int foff;                // offset of the case's force vectors
   int nb;                  // IB points
   long loff;               // offset of the case's Nf x Nf matrices
+  long goff;               // offset of the case's nvtx x Nf matrix rot(reg(A^-1))
   int nbody, bmin, ncol, hoff;  // diagnostics row layout
   double dt, delta;
 };
@@ -413,16 +459,112 @@ struct Smear {
   const int* frc_ptr;
   const int* frc_face;
   const double* frc_w;
+  const int* frc_ia;  // per force entry: curl(s) = s[ia] - s[ib] at an interior face, ia = -1 on the edge
+  const int* frc_ib;
   const int* vtx_id;
   const int* vtx_face;
   const CaseSm* cs;
 };
 
+// s = sum over entries e = lo..hi-1, in order, of w[e] * value(idx[e]).
+// The entries are taken K at a time: all K index/weight loads are issued,
+// then all K value loads, then the K products are added in entry order, so
+// a list of up to K entries costs three dependent memory latencies instead
+// of one per entry.  Loads past hi are not issued.
+template <int K, class F>
+__device__ __forceinline__ double gather_sum(int lo, int hi, const int* __restrict__ idx,
+                                             const double* __restrict__ w, F value) {
+  double s = 0.0;
+  for (int base = lo; base < hi; base += K) {
+    int id[K];
+    double ww[K], v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const bool ok = base + k < hi;
+      id[k] = ok ? idx[base + k] : 0;
+      ww[k] = ok ? w[base + k] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = (base + k < hi) ? value(id[k]) : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (base + k < hi) s = s + ww[k] * v[k];
+  }
+  return s;
+}
+// wave64 and workgroup (256 threads = 4 waves) reductions, fixed order
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;  // complete in lane 0
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+// workgroup-wide sum / max (blockDim.x a multiple of 64, at most 1024),
+// returned to every thread; waves are combined in a fixed order
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = red[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) s += red[w];
+  return s;
+}
+__device__ __forceinline__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = red[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) s = fmax(s, red[w]);
+  return s;
+}
+constexpr int kFrcK = 9;   // entries of one force: at most 3 x 3 faces of the delta support
+constexpr int kFaceK = 6;  // entries of one face: the IB points whose support covers it
+
 template <typename H>
 __device__ __forceinline__ double reg_face(const Smear& sm, const CaseSm& c, int a, const H* __restrict__ h0) {
+  return gather_sum<kFaceK>(sm.face_ptr[c.pbase + a], sm.face_ptr[c.pbase + a + 1], sm.face_pt, sm.face_w,
+                            [&](int p) { return double(h0[p]); });
+}
+
+// regT(curl(s1, get_bc(sc))) - sub for force f of case c (sub may be null)
+template <typename T>
+__device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, const CaseSm& c, int f,
+                                               const T* __restrict__ s1, const T* __restrict__ sc,
+                                               const double* __restrict__ sub) {
+  // Interior faces take curl = s1[ia] - s1[ib] from the table, so the K
+  // entries' index, weight and field loads are each issued together; faces
+  // on the domain edge (ia < 0, only for bodies touching the edge) fall back
+  // to curl_face with its get_bc terms.  Same values as curl_face.
+  const int lo = sm.frc_ptr[c.rbase + f], hi = sm.frc_ptr[c.rbase + f + 1];
   double s = 0.0;
-  for (int e = sm.face_ptr[c.pbase + a]; e < sm.face_ptr[c.pbase + a + 1]; e++)
-    s = s + sm.face_w[e] * double(h0[sm.face_pt[e]]);
+  for (int base = lo; base < hi; base += kFrcK) {
+    int ia[kFrcK], ib[kFrcK];
+    double ww[kFrcK];
+    T va[kFrcK], vb[kFrcK];
+#pragma unroll
+    for (int k = 0; k < kFrcK; k++) {
+      const bool ok = base + k < hi;
+      ia[k] = ok ? sm.frc_ia[base + k] : 0;
+      ib[k] = ok ? sm.frc_ib[base + k] : 0;
+      ww[k] = ok ? sm.frc_w[base + k] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kFrcK; k++) {
+      va[k] = s1[ia[k] < 0 ? 0 : ia[k]];
+      vb[k] = s1[ib[k] < 0 ? 0 : ib[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kFrcK; k++) {
+      if (base + k >= hi) break;
+      const double v = ia[k] >= 0 ? double(va[k] - vb[k]) : double(curl_face<T>(g, sm.frc_face[base + k], s1, sc));
+      s = s + ww[k] * v;
+    }
+  }
+  if (sub) s = s - sub[c.foff + f];
   return s;
 }
 
@@ -460,10 +602,8 @@ __global__ void k_regT(Smear sm, const T* __restrict__ h, double* __restrict__ h
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const CaseSm c = sm.cs[0];
   if (f >= c.nf) return;
-  double s = 0.0;
-  for (int e = sm.frc_ptr[c.rbase + f]; e < sm.frc_ptr[c.rbase + f + 1]; e++)
-    s = s + sm.frc_w[e] * double(h[sm.frc_face[e]]);
-  h0[f] = s;
+  h0[f] = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f], sm.frc_ptr[c.rbase + f + 1], sm.frc_face, sm.frc_w,
+                            [&](int face) { return double(h[face]); });
 }
 
 // out = regT(curl(s1, get_bc(s2))) - sub (sub may be null): the right-hand
@@ -480,12 +620,8 @@ __global__ void k_regT_curl(Geo g, Smear sm, const T* __restrict__ s1, const T* 
   const CaseSm c = sm.cs[ci];
   if (f >= c.nf) return;
   const T* sc = (s2 != nullptr && Lc[b] > 1) ? s2 + b * bs : nullptr;  // Lc indexed by batch entry
-  double s = 0.0;
-  for (int e = sm.frc_ptr[c.rbase + f]; e < sm.frc_ptr[c.rbase + f + 1]; e++)
-    s = s + sm.frc_w[e] * double(curl_face<T>(g, sm.frc_face[e], s1 + b * bs, sc));
   const long o = obs > 0 ? b * obs + f : c.foff + f;
-  if (sub) s = s - sub[c.foff + f];
-  out[o] = s;
+  out[o] = regT_curl_at<T>(g, sm, c, f, s1 + b * bs, sc, sub);
 }
 
 // motion_eff = motion_in - regT(q0_1) (src/operators.f90:155), per case
@@ -496,9 +632,9 @@ __global__ void k_motion(Smear sm, const T* __restrict__ q0, long qbs, const dou
   const int b = blockIdx.y;
   const CaseSm c = sm.cs[b];
   if (f >= c.nf) return;
-  double s = 0.0;
-  for (int e = sm.frc_ptr[c.rbase + f]; e < sm.frc_ptr[c.rbase + f + 1]; e++)
-    s = s + sm.frc_w[e] * double(q0[b * qbs + sm.frc_face[e]]);
+  const T* q0b = q0 + b * qbs;
+  const double s = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f], sm.frc_ptr[c.rbase + f + 1], sm.frc_face, sm.frc_w,
+                                     [&](int face) { return double(q0b[face]); });
   motion[c.foff + f] = motion_in[c.foff + f] - s;
 }
 
@@ -535,110 +671,239 @@ __global__ void k_trmv_upper_t(Smear sm, const double* __restrict__ Li, const do
   if (lane == 0) x[c.foff + i] = accumulate ? x[c.foff + i] + s : s;
 }
 // refinement residual r = b - A x with the symmetric A the factor was built
-// from (upper triangle of the ibfs.chd matrix, stored here as a full matrix)
+// from (upper triangle of the ibfs.chd matrix, stored here as a full
+// matrix, so row i is read as the contiguous column i)
 __global__ void k_symv_residual(Smear sm, const double* __restrict__ A, const double* __restrict__ bvec,
                                 const double* __restrict__ x, double* __restrict__ r) {
   const CaseSm c = sm.cs[blockIdx.y];
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= c.nf) return;
-  const double* Ai = A + c.loff;
+  const double* Ai = A + c.loff + (long)i * c.nf;
   double s = 0.0;
-  for (int k = lane; k < c.nf; k += 64) s += Ai[i + (long)k * c.nf] * x[c.foff + k];
+  for (int k = lane; k < c.nf; k += 64) s += Ai[k] * x[c.foff + k];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) r[c.foff + i] = bvec[c.foff + i] - s;
 }
 
-// ---------------------------------------------------------------------------
-// diagnostics
-// ---------------------------------------------------------------------------
-// per-level partial sums of omega (write_circ, src/operators.f90:675-684):
-// grid (nchunk, Lmax, B); partial[((b*Lmax)+lev)*nchunk + c]
-template <typename T>
-__global__ void k_circ_partial(Geo g, const T* __restrict__ om, double* __restrict__ partial, int nchunk,
-                               const int* __restrict__ Lc) {
-  __shared__ double red[kThreads];
-  const int lev = blockIdx.y, c = blockIdx.x, b = blockIdx.z;
-  if (!active(Lc, b, lev)) return;
-  const long per = (g.nw + nchunk - 1) / nchunk;
-  const long lo = c * per, hi = min((long)g.nw, lo + per);
+// A^-1 = Li^T Li of case ci (column-major, both triangles written from one
+// product, so the stored matrix is exactly symmetric).  grid over nf^2
+__global__ void k_make_ainv(Smear sm, int ci, const double* __restrict__ Li, double* __restrict__ Ai) {
+  const CaseSm c = sm.cs[ci];
+  const int nf = c.nf;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nf * nf) return;
+  const int i = (int)(t % nf), j = (int)(t / nf);
+  if (i > j) return;
+  const double* ci_col = Li + c.loff + (long)i * nf;  // Li(k, i), zero for k < i
+  const double* cj_col = Li + c.loff + (long)j * nf;
   double s = 0.0;
-  const T* w = om + ((long)b * g.Lmax + lev) * g.nw;
-  for (long e = lo + threadIdx.x; e < hi; e += kThreads) s += double(w[e]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = kThreads / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[((long)b * g.Lmax + lev) * nchunk + c] = red[0];
+  for (int k = j; k < nf; k++) s += ci_col[k] * cj_col[k];
+  Ai[c.loff + i + (long)j * nf] = s;
+  Ai[c.loff + j + (long)i * nf] = s;
 }
 
-// One workgroup per case: slip, body forces and circulation of this step.
-// Row of case b: [fx(bmin..bmax), fy(bmin..bmax), circ(1..L), slip] at
-// out_base + slot*rowlen + hoff.  force: write_force (src/operators.f90:
-// 583-636) = 2*delta * sum over faces of reg(fb/dt restricted to the body),
-// evaluated as 2*delta * sum_p wsum(p) fb(p)/dt with wsum(p) the sum of the
-// point's weights on u (x) or v (y) faces; slip: write_slip
-// (src/operators.f90:564) = max |regT(q_1) - motion|.
+// G = rot(reg(A^-1)) of case ci on its active vertices: G[v*nf + j] is the
+// step-3 source at vertex v for the unit force j (row-major, nvtx x nf).
+// grid over nvtx*nf
+__global__ void k_make_g(Smear sm, int ci, const double* __restrict__ Ai, double* __restrict__ G) {
+  const CaseSm c = sm.cs[ci];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)c.nvtx * c.nf) return;
+  const int j = (int)(t % c.nf), v = (int)(t / c.nf);
+  const double* col = Ai + c.loff + (long)j * c.nf;
+  double fv[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int a = sm.vtx_face[4 * (c.vbase + v) + k];
+    fv[k] = a >= 0 ? reg_face<double>(sm, c, a, col) : 0.0;
+  }
+  G[c.goff + (long)v * c.nf + j] = fv[0] - fv[1] - fv[2] + fv[3];
+}
+
+// Force system of the step, one launch.  Every workgroup of case b
+// evaluates the whole right-hand side rhsf = regT(curl(s_1, get_bc(s_2))) -
+// motion (src/operators.f90:163-164) into LDS and multiplies kForceRows
+// rows of [A^-1; G] with it: rows < nf give fb = A^-1 rhsf (the cholsl of
+// src/operators.f90:166, with the explicit inverse of the factored matrix),
+// the others give the step-3 source rot(reg(fb)) (src/operators.f90:174) at
+// the active vertices; every other entry of rr stays zero.
+//   mode 0: as above.
+//   mode 1: rows < nf only; rhsf is also written out (first solve of the
+//           refined form).
+//   mode 2: the refinement sweep with the residual res = rhsf - A fb:
+//           fb += A^-1 res and rr = G (rhsf + res).
+// Latency: the right-hand side takes two forces per thread and pass, whose
+// gathers are independent; each wave holds kRowsPerWave rows and walks the
+// columns once, so the row loads of one column block are all in flight
+// together.
+// grid (ceil(max rows / kForceRows), B), 256 threads, LDS 2*nf_max doubles.
+constexpr int kForceThreads = 512;
+constexpr int kRowsPerWave = 2;
+constexpr int kForceRows = kForceThreads / 64 * kRowsPerWave;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag(Geo g, Smear sm, const T* __restrict__ q, long qbs,
+__global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, const T* __restrict__ s1,
+                                                     const T* __restrict__ s2, long bs,
+                                                     const double* __restrict__ motion, const int* __restrict__ Lc,
+                                                     const double* __restrict__ Ainv, const double* __restrict__ G,
+                                                     double* __restrict__ fb, T* __restrict__ rr, long rbs,
+                                                     double* __restrict__ rhsf, const double* __restrict__ res,
+                                                     int mode) {
+  extern __shared__ double vec[];
+  const int b = blockIdx.y, tid = threadIdx.x;
+  const CaseSm c = sm.cs[b];
+  const int nf = c.nf;
+  const int nrows = mode == 1 ? nf : nf + c.nvtx;
+  const int row0 = blockIdx.x * kForceRows;
+  if (row0 >= nrows) return;
+  const int NT = blockDim.x;
+  if (mode < 2) {
+    const T* s1b = s1 + b * bs;
+    const T* sc = (s2 != nullptr && Lc[b] > 1) ? s2 + b * bs : nullptr;
+    for (int f0 = tid; f0 < nf; f0 += 2 * NT) {
+      const int f1 = f0 + NT;
+      const double v0 = regT_curl_at<T>(g, sm, c, f0, s1b, sc, motion);
+      const double v1 = f1 < nf ? regT_curl_at<T>(g, sm, c, f1, s1b, sc, motion) : 0.0;
+      vec[f0] = v0;
+      if (f1 < nf) vec[f1] = v1;
+      if (mode == 1 && blockIdx.x == 0) {
+        rhsf[c.foff + f0] = v0;
+        if (f1 < nf) rhsf[c.foff + f1] = v1;
+      }
+    }
+  } else {
+    for (int f = tid; f < nf; f += NT) {
+      const double r = res[c.foff + f];
+      vec[f] = r;
+      vec[nf + f] = rhsf[c.foff + f] + r;
+    }
+  }
+  __syncthreads();
+  const int lane = tid & 63;
+  const int rbase = row0 + (tid >> 6) * kRowsPerWave;
+  const double* Mr[kRowsPerWave];
+  int xo[kRowsPerWave];
+  double acc[kRowsPerWave];
+#pragma unroll
+  for (int q = 0; q < kRowsPerWave; q++) {
+    const int r = min(rbase + q, nrows - 1);  // rows past the end repeat the last row and are not stored
+    const bool frow = r < nf;
+    Mr[q] = frow ? Ainv + c.loff + (long)r * nf : G + c.goff + (long)(r - nf) * nf;
+    xo[q] = (!frow && mode == 2) ? nf : 0;
+    acc[q] = 0.0;
+  }
+  for (int k = lane; k < nf; k += 64) {
+#pragma unroll
+    for (int q = 0; q < kRowsPerWave; q++) acc[q] += Mr[q][k] * vec[xo[q] + k];
+  }
+#pragma unroll
+  for (int q = 0; q < kRowsPerWave; q++) {
+    const double s = wave_sum(acc[q]);
+    const int r = rbase + q;
+    if (lane == 0 && r < nrows) {
+      if (r < nf) fb[c.foff + r] = mode == 2 ? fb[c.foff + r] + s : s;
+      else rr[b * rbs + sm.vtx_id[c.vbase + r - nf]] = T(s);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tail of the step: fluxes, circulation and diagnostics
+// ---------------------------------------------------------------------------
+// One launch after the last streamfunction of the step is final:
+//  * workgroups x < ncurl = ceil(nq/256)*Lmax: q = curl(s) on level
+//    x / ceil(nq/256) (src/operators.f90:236-238, as k_curl_levels);
+//  * the next nchunk*Lmax: partial sums of omega for write_circ
+//    (src/operators.f90:675-684), chunk (x-ncurl) % nchunk of level
+//    (x-ncurl) / nchunk, at partial[((b*Lmax)+lev)*nchunk + chunk].
+// grid (ncurl + nchunk*Lmax, B), 256 threads.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T* __restrict__ q,
+                                              const T* __restrict__ om, long bs, long qbs,
+                                              double* __restrict__ partial, int nchunk, const int* __restrict__ Lc) {
+  __shared__ double red[4];
+  const int tid = threadIdx.x, b = blockIdx.y;
+  const int ncb = (g.nq + 255) / 256, ncurl = ncb * g.Lmax;
+  const int x = blockIdx.x;
+  if (x < ncurl) {
+    const int lev = x / ncb, f = (x - lev * ncb) * 256 + tid;
+    if (f >= g.nq || !active(Lc, b, lev)) return;
+    const T* sl = s + b * bs + (long)lev * g.nw;
+    q[b * qbs + (long)lev * g.nq + f] = curl_face<T>(g, f, sl, has_coarser(Lc, b, lev) ? sl + g.nw : nullptr);
+    return;
+  }
+  const int xc = x - ncurl, lev = xc / nchunk, ch = xc - lev * nchunk;
+  if (!active(Lc, b, lev)) return;
+  const long per = (g.nw + nchunk - 1) / nchunk;
+  const long lo = ch * per, hi = min((long)g.nw, lo + per);
+  const T* w = om + b * bs + (long)lev * g.nw;
+  double acc = 0.0;
+  for (long e = lo + tid; e < hi; e += 256) acc += double(w[e]);
+  acc = block_sum(acc, red);
+  if (tid == 0) partial[((long)b * g.Lmax + lev) * nchunk + ch] = acc;
+}
+
+// One workgroup per case: the diagnostics row of the step,
+// [fx(bmin..bmax), fy(bmin..bmax), circ(1..L), slip] at
+// out_base + slot*rowlen + hoff, slot = counter[b] (then incremented):
+//   force: write_force (src/operators.f90:583-636) = 2*delta * sum over
+//          faces of reg(fb/dt restricted to the body), evaluated as
+//          2*delta * sum_p wsum(p) fb(p)/dt with wsum(p) the sum of the
+//          point's weights on u (x) or v (y) faces;
+//   circ:  the sum of the level's partial sums from k_tail;
+//   slip:  write_slip (src/operators.f90:564) = max |regT(q_1) - motion|.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_diag(Geo g, Smear sm, const T* __restrict__ q, long qbs,
                                               const double* __restrict__ motion, const double* __restrict__ fb,
                                               const double* __restrict__ wsum, const int* __restrict__ bdyno,
                                               const double* __restrict__ partial, int nchunk,
                                               const int* __restrict__ Lc, double* __restrict__ out_base, int rowlen,
                                               int* __restrict__ counter, int capacity) {
-  __shared__ double red[256];
-  const int tid = threadIdx.x, b = blockIdx.x;
+  __shared__ double red[16];
+  const int tid = threadIdx.x, b = blockIdx.x, lane = tid & 63, NT = blockDim.x;
   const CaseSm c = sm.cs[b];
   const int L = Lc[b];
   const int slot0 = counter[b];
   const int slot = slot0 < capacity ? slot0 : capacity - 1;
   double* out = out_base + (long)slot * rowlen + c.hoff;
+  // circulation: wave w sums the partials of level w
+  for (int lv = tid >> 6; lv < L; lv += NT >> 6) {
+    const double* pp = partial + ((long)b * g.Lmax + lv) * nchunk;
+    double acc = 0.0;
+    for (int k = lane; k < nchunk; k += 64) acc += pp[k];
+    acc = wave_sum(acc);
+    if (lane == 0) out[2 * c.nbody + lv] = acc;
+  }
+  // slip, two forces per thread and pass
   const T* q1 = q + b * qbs;
-  // slip
+  auto qv = [&](int face) { return double(q1[face]); };
   double mx = 0.0;
-  for (int f = tid; f < c.nf; f += 256) {
-    double s = 0.0;
-    for (int e = sm.frc_ptr[c.rbase + f]; e < sm.frc_ptr[c.rbase + f + 1]; e++)
-      s = s + sm.frc_w[e] * double(q1[sm.frc_face[e]]);
-    mx = fmax(mx, fabs(s - motion[c.foff + f]));
+  for (int f0 = tid; f0 < c.nf; f0 += 2 * NT) {
+    const int f1 = f0 + NT;
+    const double e0 = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f0], sm.frc_ptr[c.rbase + f0 + 1], sm.frc_face,
+                                        sm.frc_w, qv) - motion[c.foff + f0];
+    double e1 = 0.0;
+    if (f1 < c.nf)
+      e1 = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f1], sm.frc_ptr[c.rbase + f1 + 1], sm.frc_face, sm.frc_w, qv) -
+           motion[c.foff + f1];
+    mx = fmax(mx, fmax(fabs(e0), fabs(e1)));
   }
-  red[tid] = mx;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (tid < off) red[tid] = fmax(red[tid], red[tid + off]);
-    __syncthreads();
-  }
-  if (tid == 0) out[2 * c.nbody + L] = red[0];
-  __syncthreads();
+  mx = block_max(mx, red);
+  if (tid == 0) out[2 * c.nbody + L] = mx;
   // forces, one body and component at a time
   const int pbase = c.foff / 2;
   for (int bd = 0; bd < c.nbody; bd++)
     for (int comp = 0; comp < 2; comp++) {
-      double s = 0.0;
-      for (int p = tid; p < c.nb; p += 256)
+      double acc = 0.0;
+      for (int p = tid; p < c.nb; p += NT)
         if (bdyno[pbase + p] == c.bmin + bd) {
           const int f = comp * c.nb + p;
-          s += wsum[c.foff + f] * (fb[c.foff + f] / c.dt);
+          acc += wsum[c.foff + f] * (fb[c.foff + f] / c.dt);
         }
-      red[tid] = s;
-      __syncthreads();
-      for (int off = 128; off > 0; off >>= 1) {
-        if (tid < off) red[tid] += red[tid + off];
-        __syncthreads();
-      }
-      if (tid == 0) out[comp * c.nbody + bd] = red[0] * 2.0 * c.delta;
-      __syncthreads();
+      acc = block_sum(acc, red);
+      if (tid == 0) out[comp * c.nbody + bd] = acc * 2.0 * c.delta;
     }
-  // circulation
-  if (tid < L) {
-    double s = 0.0;
-    const double* pp = partial + ((long)b * g.Lmax + tid) * nchunk;
-    for (int k = 0; k < nchunk; k++) s += pp[k];
-    out[2 * c.nbody + tid] = s;
-  }
-  __syncthreads();
   if (tid == 0) counter[b] = slot0 + 1;
 }
 
