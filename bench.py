@@ -11,9 +11,9 @@ Prints one JSON line.  ``value`` is steps/s of the plain FP64 form (the
 drop-in).  ``chosen`` reports the mixed FP32/FP64 form for one case and for
 a batch of independent copies of the case in one graph per step (aggregate
 case-steps/s), with its force error against the FP64 run.  ``roofline`` is
-the dominant kernel (a 1-D DST-I pass, timed live with HIP events on the
-library's stream); ``cpu_baseline`` is the oracle's FP64 step, single
-thread, on a bounded sample of the same workload.
+the step's dominant kernel (the fused column pair of ctci, timed live with
+HIP events on the library's stream); ``cpu_baseline`` is the oracle's FP64
+step, single thread, on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -30,6 +30,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ibfs-if2d_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# HBM bytes per launch of the fused column pair at 512^2 (FP64): FETCH_SIZE
+# and WRITE_SIZE per dispatch from two rocprofv3 --pmc runs, each scaled by
+# what the counter reads for a copy kernel of the same size
+# (profiles/dst_col2_512x5_fp64_hbm_counters.md: 2630.8 KiB x 1.980 fetched +
+# 2135.8 KiB x 1.000 written)
+PAIR_TRAFFIC_BYTES = int((2630.8 * 1.980 + 2135.8 * 1.000) * 1024)
 
 
 def step_bytes(m, n, L, nf, s=8):
@@ -100,15 +106,17 @@ def main(argv=None):
     steps_s = K / wall
     log(f"plain FP64: {K} steps wall {wall * 1e3:.1f} ms (device {dev_ms:.1f} ms) -> {steps_s:.1f} steps/s")
 
-    # ---- dominant kernel: 1-D DST-I pass over one W field (read + write W doubles)
-    pass_bytes = 2 * 8 * g.nw
+    # ---- dominant kernel of the step: the fused column pair of ctci (DST-I
+    # along j, times laminv, DST-I along j), 10 of the step's launches; per
+    # launch it reads the W field and the W table laminv and writes W doubles
+    pair_bytes = 3 * 8 * g.nw
+    t_pair = g.time_dst_pass(2, 200)
     t_row = g.time_dst_pass(0, 200)
     t_col = g.time_dst_pass(1, 200)
-    t_pass = max(t_row, t_col)
-    achieved = pass_bytes / (t_pass * 1e-3) / 1e9
+    achieved = pair_bytes / (t_pair * 1e-3) / 1e9
     B = step_bytes(g.m, g.n, g.L, g.nf)
-    log(f"DST pass: row {t_row * 1e3:.2f} us, column {t_col * 1e3:.2f} us; step bytes {B / 1e6:.1f} MB -> "
-        f"whole-step {B * steps_s / 1e9:.1f} GB/s")
+    log(f"column pair {t_pair * 1e3:.2f} us ({achieved:.0f} GB/s); DST pass: row {t_row * 1e3:.2f} us, column "
+        f"{t_col * 1e3:.2f} us; step bytes {B / 1e6:.1f} MB -> whole-step {B * steps_s / 1e9:.1f} GB/s")
     # streaming copies: one of the pass's size (the counter calibration point)
     # and a large one (the measured copy bandwidth of this GPU)
     t_copy_w = g.time_copy(g.nw, 200)
@@ -218,9 +226,9 @@ def main(argv=None):
                                "impulsive start", "m": g.m, "n": g.n, "levels": g.L, "nb": g.nb,
                    "precision": "plain FP64", "parallelism": "single GPU, one case"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "dst_pass (1-D DST-I over a 511x511 FP64 field)",
-                     "launch_us": t_pass * 1e3, "bytes_per_launch": pass_bytes,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": PAIR_TRAFFIC_BYTES,
+                     "kernel": "dst_col2_n (fused column pair of ctci over a 511x511 FP64 field)",
+                     "launch_us": t_pair * 1e3, "bytes_per_launch": pair_bytes,
                      "row_pass_us": t_row * 1e3, "col_pass_us": t_col * 1e3,
                      "copy_same_size_us": t_copy_w * 1e3, "measured_copy_GBs": copy_gbs},
         "step_roofline": {"bytes_per_step": B, "achieved_GBs": B * steps_s / 1e9,

@@ -589,7 +589,7 @@ int ibfs_time_copy(ibfs_handle* h, long n, int reps, double* ms_per_launch) {
 }
 
 int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch) {
-  if (!h || !ms_per_launch || reps < 1 || (axis != 0 && axis != 1)) return IBFS_EINVAL;
+  if (!h || !ms_per_launch || reps < 1 || axis < 0 || axis > 2) return IBFS_EINVAL;
   return guarded(h, [&] {
     NEED64();
     DevBuf<double> a, b;
@@ -597,7 +597,12 @@ int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch
     a.zero(st);
     auto launch = [&] {
       if (axis == 0) E->row_pass(SrcField<double>{a.p, 0}, DstField<double>{b.p, 0}, 1);
-      else E->col_pass(SrcField<double>{a.p, 0}, DstField<double>{b.p, 0}, 1);
+      else if (axis == 1) E->col_pass(SrcField<double>{a.p, 0}, DstField<double>{b.p, 0}, 1);
+      else
+        E->launch_col2(Col2Op<double, SrcField<double>, MidScale<double>, DstField<double>>{
+                           E->geo, SrcField<double>{a.p, 0}, MidScale<double>{E->laminv.p}, DstField<double>{b.p, 0},
+                           nullptr, 0},
+                       E->geo.mm1, 2, 1);
     };
     launch();
     HIPCHK(hipEventRecord(h->ev0, st));

@@ -179,7 +179,9 @@ class Solver:
         return ms.value
 
     def time_dst_pass(self, axis, reps=200):
-        """Mean device time (ms) of one 1-D DST-I pass over a W field."""
+        """Mean device time (ms) of one pass over a W field: axis 0 = DST-I
+        along i, 1 = along j, 2 = the fused column pair of ctci (along j,
+        times laminv, along j)."""
         ms = C.c_double()
         self._chk(_lib.ibfs_time_dst_pass(self._h, axis, reps, C.byref(ms)), "ibfs_time_dst_pass")
         return ms.value

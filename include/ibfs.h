@@ -175,7 +175,8 @@ int ibfs_last_advance_ms(const ibfs_handle* h, double* ms);
 /* Measurement only (no counterpart in the reference): launches one 1-D
  * DST-I pass over a W field reps times on the handle's stream and returns
  * the mean launch duration from HIP events.  axis 0 = along i (rows),
- * 1 = along j (columns). */
+ * 1 = along j (columns), 2 = the fused column pair of ctci (DST-I along j,
+ * times laminv, DST-I along j: one read and one write of the field). */
 int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch);
 
 /* Measurement only (no counterpart in the reference): a plain streaming copy
