@@ -151,10 +151,14 @@ def main(argv=None):
             ens.advance_timed(kb)
             wb = time.perf_counter() - t0
             res[name] = args.batch * kb / wb
-            log(f"{name}, batch of {args.batch}: {res[name]:.1f} case-steps/s ({kb} steps, {wb * 1e3:.1f} ms)")
+            res[name + "_tiles"] = ens.tile_shapes(0)
+            log(f"{name}, batch of {args.batch}: {res[name]:.1f} case-steps/s ({kb} steps, {wb * 1e3:.1f} ms); "
+                f"tile shapes {res[name + '_tiles']}")
             del ens
         chosen = {"mixed_one_case_steps_s": mixed_s, "batched_mixed_case_steps_s": res["mixed"],
                   "batched_fp64_case_steps_s": res["fp64"], "batch": args.batch,
+                  "tile_shapes": {"one_case_fp64": g.tile_shapes(0), "batched_mixed": res["mixed_tiles"],
+                                  "batched_fp64": res["fp64_tiles"]},
                   "speedup_vs_plain": res["mixed"] / steps_s, "force_err_vs_fp64": ferr,
                   "force_err_window_steps": W0}
 
@@ -195,7 +199,8 @@ def main(argv=None):
         ens.advance_timed(kb)
         we = time.perf_counter() - t0
         ensemble = {"cases": len(cs), "case_steps_per_s": len(cs) * kb / we, "ms_per_ensemble_step": 1e3 * we / kb,
-                    "factor_build_s": tb, "precision": "mixed FP32/FP64"}
+                    "factor_build_s": tb, "precision": "mixed FP32/FP64",
+                    "tile_shapes_per_group": [ens.tile_shapes(k) for k in range(ens.ngroups())]}
         log(f"ensemble of {len(cs)} uneven cases: {len(cs) * kb / we:.0f} case-steps/s ({1e3 * we / kb:.2f} ms per "
             f"ensemble step; factors {tb:.1f} s)")
         del ens

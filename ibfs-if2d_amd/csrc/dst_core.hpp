@@ -28,7 +28,9 @@
 //              position idx (0..N-2); 0 for sequences that do not exist
 //     void store(int b, int g, int p, int P, int k, T ya, T yb)
 //              outputs of FFT p (0..P-1) at k (1..N-1): ya is the transform
-//              of local sequence p, yb that of local sequence p+P
+//              of local sequence 2p, yb that of local sequence 2p+1
+// Sequences 2p and 2p+1 share FFT p whatever the number P of FFTs per
+// workgroup, so the results do not depend on P (the tile shape).
 // so that pointwise prologue/epilogue work (the AB2 combine, boundary
 // terms, eigenvalue scaling, the CN combine) runs inside the pass.  The row
 // pass (AXIS 0) walks loads/stores along idx/k, which is contiguous in
@@ -180,15 +182,15 @@ __global__ __launch_bounds__(kThreads) void dst_pass(Op op, FftPlan<T> plan) {
     z[p * n2] = {T(0), T(0)};
     z[p * n2 + N] = {T(0), T(0)};
   }
-  // load: local sequence s -> FFT (s mod P), real part if s < P else imaginary
+  // load: local sequence s -> FFT s/2, real part if s is even else imaginary
   const int tot = 2 * P * len;
   for (int e = threadIdx.x; e < tot; e += kThreads) {
     int s, idx;
     if (AXIS == 0) { s = e / len; idx = e - s * len; }
     else { idx = e / (2 * P); s = e - idx * (2 * P); }
     const T x = op.load(b, g, s, P, idx);
-    const int p = s < P ? s : s - P;
-    const int h = s < P ? 0 : 1;
+    const int p = s >> 1;
+    const int h = s & 1;
     reinterpret_cast<T*>(z + p * n2 + idx + 1)[h] = x;
     reinterpret_cast<T*>(z + p * n2 + n2 - idx - 1)[h] = -x;
   }
@@ -249,8 +251,8 @@ __global__ __launch_bounds__(kThreads) void dst_col2(Op op, FftPlan<T> plan) {
   for (int e = threadIdx.x; e < tot; e += kThreads) {
     const int idx = e / (2 * P), s = e - idx * (2 * P);
     const T x = op.load(b, g, s, P, idx);
-    const int p = s < P ? s : s - P;
-    const int h = s < P ? 0 : 1;
+    const int p = s >> 1;
+    const int h = s & 1;
     reinterpret_cast<T*>(z + p * n2 + idx + 1)[h] = x;
     reinterpret_cast<T*>(z + p * n2 + n2 - idx - 1)[h] = -x;
   }
@@ -414,8 +416,8 @@ __device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g
       int s, idx;
       if (AXIS == 0) { s = e / LEN; idx = e - s * LEN; }
       else { idx = e / (2 * P); s = e - idx * (2 * P); }
-      const int p = s < P ? s : s - P;
-      const int h = s < P ? 0 : 1;
+      const int p = s >> 1;
+      const int h = s & 1;
       reinterpret_cast<T*>(z + p * N2 + idx + 1)[h] = x[q];
       reinterpret_cast<T*>(z + p * N2 + N2 - idx - 1)[h] = -x[q];
     }

@@ -213,10 +213,10 @@ template <typename T>
 struct PairRowOp {
   Geo g; const T* x; const T* y; T* tx; T* ty; const int* Lc; int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
-    const int row = gr * P + (s < P ? s : s - P);
+    const int row = gr * P + (s >> 1);  // even s: x, odd s: y
     if (row >= g.nm1 || !active(Lc, b, lev)) return T(0);
     const long e = (long)b * g.nw + idx + row * g.mm1;
-    return s < P ? x[e] : y[e];
+    return (s & 1) ? y[e] : x[e];
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     const int row = gr * P + p;
@@ -230,10 +230,10 @@ template <typename T>
 struct PairColOp {
   Geo g; const T* tx; const T* ty; T* ax; T* ay; long ays; T yscale; const int* Lc; int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
-    const int col = gr * P + (s < P ? s : s - P);
+    const int col = gr * P + (s >> 1);  // even s: tx, odd s: ty
     if (col >= g.mm1 || !active(Lc, b, lev)) return T(0);
     const long e = (long)b * g.nw + col + idx * g.mm1;
-    return s < P ? tx[e] : ty[e];
+    return (s & 1) ? ty[e] : tx[e];
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     const int col = gr * P + p;

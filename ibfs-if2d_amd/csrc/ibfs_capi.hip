@@ -588,6 +588,14 @@ int ibfs_time_copy(ibfs_handle* h, long n, int reps, double* ms_per_launch) {
   });
 }
 
+int ibfs_tile_shapes(const ibfs_handle* h, int group, int* ngroups, int* shapes) {
+  if (!h || group < 0) return IBFS_EINVAL;
+  if (ngroups) *ngroups = (int)h->groups.size();
+  if (group >= (int)h->groups.size()) return IBFS_EINVAL;
+  if (shapes) h->groups[group]->tile_shapes(shapes);
+  return IBFS_OK;
+}
+
 int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch) {
   if (!h || !ms_per_launch || reps < 1 || axis < 0 || axis > 2) return IBFS_EINVAL;
   return guarded(h, [&] {

@@ -12,9 +12,11 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -263,6 +265,9 @@ struct GroupBase {
   virtual void ensure_diag(int cap) = 0;
   virtual void download_diag(double* dst, int nsteps, int ldrow, int col0) = 0;
   virtual bool is_double() const = 0;
+  // FFTs per workgroup chosen by the tuner: row, column, column pair,
+  // step-1 row, step-1 column pair (0: generic pass)
+  virtual void tile_shapes(int* out) const = 0;
 };
 
 }  // namespace ibfs

@@ -243,76 +243,202 @@ struct Engine : GroupBase {
     counter.zero(st);
     ensure_diag(1);
     HIPCHK(hipStreamSynchronize(st));
+    tune();
+  }
+  void tile_shapes(int* out) const override {
+    for (int k = 0; k < kKinds; k++) out[k] = ptile[k];
   }
 
   // ------------------------------------------------------------------ passes
-  static constexpr int spec_p(int n2, int axis) {
-    return axis == 0 ? (n2 >= 1024 ? 1 : (n2 >= 512 ? 2 : 4)) : (n2 >= 2048 ? 1 : (n2 >= 1024 ? 2 : 4));
+  // Tile shapes: FFTs per workgroup (P) of the size-specialised kernels, per
+  // kind of pass, chosen at construction by timing every candidate on this
+  // group's batch size (tune()).  Results do not depend on P.
+  enum Kind { kRow = 0, kCol = 1, kCol2 = 2, kStep1Row = 3, kStep1Col2 = 4, kKinds = 5 };
+  int ptile[kKinds] = {1, 1, 1, 1, 1};
+  static constexpr bool p_fits(int n2, int P) {
+    return P * n2 / 4 <= 1024 && (long)P * n2 * (long)sizeof(cplx<T>) <= 65536;
+  }
+
+  template <int NN, int P, int AXIS, class Op>
+  void launch_spec(const Op& op, const cplx<T>* stw, int nseq, int mult, int batch) {
+    if constexpr (p_fits(NN, P)) {
+      const int groups = (nseq + mult * P - 1) / (mult * P);
+      hipLaunchKernelGGL((spec::dst_pass_n<T, NN, P, AXIS, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
+                         stw);
+    }
+  }
+  template <int NN, int P, class Op>
+  void launch_spec_col2(const Op& op, int nseq, int mult, int batch) {
+    if constexpr (p_fits(NN, P)) {
+      const int groups = (nseq + mult * P - 1) / (mult * P);
+      hipLaunchKernelGGL((spec::dst_col2_n<T, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
+                         stw_n.p);
+    }
   }
 
   // Launch one pass over nseq sequences along the other axis; mult = 2 when
   // a workgroup's P FFTs take 2P sequences of one field, 1 when they pair
-  // two fields (X + iY).
+  // two fields (X + iY).  kind selects the tile shape.
   template <int AXIS, class Op>
-  void launch(const Op& op, int nseq, int mult, int batch) {
+  void launch(const Op& op, int nseq, int mult, int batch, int kind = AXIS) {
     const int n2 = AXIS == 0 ? pm.n2 : pn.n2;
     const bool sp = AXIS == 0 ? spec_m : spec_n;
     const cplx<T>* stw = AXIS == 0 ? stw_m.p : stw_n.p;
+    const int P = ptile[kind];
     if (sp) {
-      switch (n2) {
-#define IBFS_SPEC_CASE(NN)                                                                             \
-  case NN: {                                                                                           \
-    constexpr int P = spec_p(NN, AXIS);                                                                \
-    const int groups = (nseq + mult * P - 1) / (mult * P);                                             \
-    hipLaunchKernelGGL((spec::dst_pass_n<T, NN, P, AXIS, Op>), dim3(groups, batch), dim3(P * NN / 4), \
-                       0, st, op, stw);                                                                \
-    return;                                                                                            \
-  }
-        IBFS_SPEC_CASE(400)
-        IBFS_SPEC_CASE(512)
-        IBFS_SPEC_CASE(640)
-        IBFS_SPEC_CASE(768)
-        IBFS_SPEC_CASE(1024)
-        IBFS_SPEC_CASE(2048)
+      switch (n2 * 8 + P) {
+#define IBFS_SPEC_CASE(NN, PP)                                 \
+  case NN * 8 + PP:                                            \
+    launch_spec<NN, PP, AXIS, Op>(op, stw, nseq, mult, batch); \
+    return;
+#define IBFS_SPEC_SIZE(NN) IBFS_SPEC_CASE(NN, 1) IBFS_SPEC_CASE(NN, 2) IBFS_SPEC_CASE(NN, 4)
+        IBFS_SPEC_SIZE(400)
+        IBFS_SPEC_SIZE(512)
+        IBFS_SPEC_SIZE(640)
+        IBFS_SPEC_SIZE(768)
+        IBFS_SPEC_SIZE(1024)
+        IBFS_SPEC_SIZE(2048)
+#undef IBFS_SPEC_SIZE
 #undef IBFS_SPEC_CASE
         default: break;
       }
     }
     const FftPlan<T>& plan = AXIS == 0 ? pm : pn;
-    const int P = kEmax / n2;
-    const int groups = (nseq + mult * P - 1) / (mult * P);
+    const int Pg = kEmax / n2;
+    const int groups = (nseq + mult * Pg - 1) / (mult * Pg);
     hipLaunchKernelGGL((dst_pass<T, kEmax, AXIS, Op>), dim3(groups, batch), dim3(kThreads),
                        kEmax * sizeof(cplx<T>), st, op, plan);
   }
 
   // fused column pair (dst_col2) over nseq columns
   template <class Op>
-  void launch_col2(const Op& op, int nseq, int mult, int batch) {
+  void launch_col2(const Op& op, int nseq, int mult, int batch, int kind = kCol2) {
     const int n2 = pn.n2;
+    const int P = ptile[kind];
     if (spec_n) {
-      switch (n2) {
-#define IBFS_SPEC_CASE(NN)                                                                                 \
-  case NN: {                                                                                               \
-    constexpr int P = spec_p(NN, 1);                                                                       \
-    const int groups = (nseq + mult * P - 1) / (mult * P);                                                 \
-    hipLaunchKernelGGL((spec::dst_col2_n<T, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op, \
-                       stw_n.p);                                                                           \
-    return;                                                                                                \
-  }
-        IBFS_SPEC_CASE(400)
-        IBFS_SPEC_CASE(512)
-        IBFS_SPEC_CASE(640)
-        IBFS_SPEC_CASE(768)
-        IBFS_SPEC_CASE(1024)
-        IBFS_SPEC_CASE(2048)
+      switch (n2 * 8 + P) {
+#define IBFS_SPEC_CASE(NN, PP)                           \
+  case NN * 8 + PP:                                      \
+    launch_spec_col2<NN, PP, Op>(op, nseq, mult, batch); \
+    return;
+#define IBFS_SPEC_SIZE(NN) IBFS_SPEC_CASE(NN, 1) IBFS_SPEC_CASE(NN, 2) IBFS_SPEC_CASE(NN, 4)
+        IBFS_SPEC_SIZE(400)
+        IBFS_SPEC_SIZE(512)
+        IBFS_SPEC_SIZE(640)
+        IBFS_SPEC_SIZE(768)
+        IBFS_SPEC_SIZE(1024)
+        IBFS_SPEC_SIZE(2048)
+#undef IBFS_SPEC_SIZE
 #undef IBFS_SPEC_CASE
         default: break;
       }
     }
-    const int P = kEmax / n2;
-    const int groups = (nseq + mult * P - 1) / (mult * P);
+    const int Pg = kEmax / n2;
+    const int groups = (nseq + mult * Pg - 1) / (mult * Pg);
     hipLaunchKernelGGL((dst_col2<T, kEmax, Op>), dim3(groups, batch), dim3(kThreads), kEmax * sizeof(cplx<T>), st,
                        op, pn);
+  }
+
+  // ------------------------------------------------------------ autotuner
+  // For every kind of pass, launches a representative pass over this
+  // group's batch (zero inputs in scratch buffers, the state untouched) with
+  // each tile shape that fits (P = 1, 2, 4 FFTs per workgroup), times 20
+  // launches with HIP events in three rounds that interleave the candidates
+  // (so clock ramp-up and noise hit all of them), and keeps the fastest.  The choice is cached
+  // per (kind, m, n, element size, batch).  IBFS_TILE=P forces the largest
+  // fitting value <= P instead.
+  void tune() {
+    static std::mutex mu;
+    static std::map<std::array<int, 5>, int> cache;
+    const char* force = std::getenv("IBFS_TILE");
+    const int fp = force ? std::atoi(force) : 0;
+    DevBuf<T> qz, bz;
+    bool scratch = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    for (int k = 0; k < kKinds; k++) {
+      const bool rowtype = k == kRow || k == kStep1Row;
+      const int n2 = rowtype ? pm.n2 : pn.n2;
+      if (!(rowtype ? spec_m : spec_n)) continue;  // generic pass: P fixed by its LDS budget
+      std::vector<int> cand;
+      for (int P : {1, 2, 4})
+        if (p_fits(n2, P)) cand.push_back(P);
+      if (fp > 0) {
+        int best = cand[0];
+        for (int P : cand)
+          if (P <= fp) best = P;
+        ptile[k] = best;
+        continue;
+      }
+      const std::array<int, 5> key{k, m, n, (int)sizeof(T), B};
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+          ptile[k] = it->second;
+          continue;
+        }
+      }
+      if (!scratch) {
+        qz.alloc((size_t)B * nq);
+        qz.zero(st);
+        bz.alloc((size_t)B * nbc);
+        bz.zero(st);
+        T2.zero(st);
+        T3.zero(st);
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        scratch = true;
+      }
+      std::vector<float> tmin(cand.size(), 1e30f);
+      for (int round = 0; round < 3; round++)  // rounds interleave the candidates
+        for (size_t ci = 0; ci < cand.size(); ci++) {
+        ptile[k] = cand[ci];
+        auto run = [&] {
+          switch (k) {
+            case kRow: row_pass(SrcField<T>{T2.p, nw}, DstField<T>{T3.p, nw}, B); break;
+            case kCol: col_pass(SrcField<T>{T2.p, nw}, DstField<T>{T3.p, nw}, B); break;
+            case kCol2:
+              launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{geo, SrcField<T>{T2.p, nw},
+                                                                            MidScale<T>{laminv.p},
+                                                                            DstField<T>{T3.p, nw}, nullptr, 0},
+                          geo.mm1, 2, B);
+              break;
+            case kStep1Row:
+              launch<0>(Step1RowOp<T>{geo, T2.p, nullptr, qz.p, qz.p, bz.p, T3.p, T1.p, con1.p, con2.p, vfac.p,
+                                      Lc.p, (long)nw, (long)nq, (long)nbc, norm, 0, 0},
+                        geo.nm1, 1, B, kStep1Row);
+              break;
+            default:
+              launch_col2(Step1Col2Op<T, DstField<T>>{geo, T1.p, lam1n.p, lam1i.p, DstField<T>{T2.p, nw}, Lc.p, 0},
+                          geo.mm1, 1, B, kStep1Col2);
+              break;
+          }
+        };
+        run();
+        run();
+        HIPCHK(hipEventRecord(e0, st));
+        constexpr int reps = 20;
+        for (int r = 0; r < reps; r++) run();
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipStreamSynchronize(st));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        tmin[ci] = std::min(tmin[ci], ms);
+      }
+      int best = cand[0];
+      float best_ms = tmin[0];
+      for (size_t ci = 1; ci < cand.size(); ci++)
+        if (tmin[ci] < best_ms) {
+          best_ms = tmin[ci];
+          best = cand[ci];
+        }
+      ptile[k] = best;
+      std::lock_guard<std::mutex> lk(mu);
+      cache[key] = best;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    HIPCHK(hipStreamSynchronize(st));
   }
 
   template <class Src, class Dst>
@@ -397,9 +523,9 @@ struct Engine : GroupBase {
         r1.bq0 = base->q0.p + (long)lev * nq;
         r1.blb = base->lastbc.p + (long)lev * nbc;
       }
-      launch<0>(r1, geo.nm1, 1, B);
+      launch<0>(r1, geo.nm1, 1, B, kStep1Row);
       launch_col2(Step1Col2Op<T, DstField<T>>{geo, T1.p, lam1n.p, lam1i.p, DstField<T>{T2.p, nw}, Lc.p, lev},
-                  geo.mm1, 1, B);
+                  geo.mm1, 1, B, kStep1Col2);
       row_pass(SrcField<T>{T2.p, nw}, DstField<T>{omega.p + (long)lev * nw, FB}, B, Lc.p, lev);
     }
     // STEP 2 (src/operators.f90:142-171); motion is prepared per call.  The

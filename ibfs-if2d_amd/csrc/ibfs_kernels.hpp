@@ -154,7 +154,7 @@ struct DstSub {
   }
 };
 
-// single-field row pass: rows jj; FFT p pairs rows 2Pg+p and 2Pg+P+p.
+// single-field row pass: rows jj; FFT p pairs rows 2Pg+2p and 2Pg+2p+1.
 // Cases whose level count is <= lev are skipped (Lc == nullptr: none).
 template <typename T, class Src, class Dst>
 struct RowOp {
@@ -165,7 +165,7 @@ struct RowOp {
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     if (!active(Lc, b, lev)) return;
-    const int r0 = gr * 2 * P + p, r1 = r0 + P;
+    const int r0 = gr * 2 * P + 2 * p, r1 = r0 + 1;
     if (r0 < g.nm1) dst(g, b, k - 1, r0, ya);
     if (r1 < g.nm1) dst(g, b, k - 1, r1, yb);
   }
@@ -180,7 +180,7 @@ struct ColOp {
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     if (!active(Lc, b, lev)) return;
-    const int c0 = gr * 2 * P + p, c1 = c0 + P;
+    const int c0 = gr * 2 * P + 2 * p, c1 = c0 + 1;
     if (c0 < g.mm1) dst(g, b, c0, k - 1, ya);
     if (c1 < g.mm1) dst(g, b, c1, k - 1, yb);
   }
@@ -211,14 +211,14 @@ struct Col2Op {
     return (col < g.mm1 && active(Lc, b, lev)) ? src(g, b, col, idx) : T(0);
   }
   __device__ void mid(int b, int gr, int p, int P, int k, T ya, T yb, T& va, T& vb) const {
-    const int c0 = gr * 2 * P + p, c1 = c0 + P;
+    const int c0 = gr * 2 * P + 2 * p, c1 = c0 + 1;
     const bool on = active(Lc, b, lev);
     va = (on && c0 < g.mm1) ? md(g, b, c0, k - 1, ya) : T(0);
     vb = (on && c1 < g.mm1) ? md(g, b, c1, k - 1, yb) : T(0);
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     if (!active(Lc, b, lev)) return;
-    const int c0 = gr * 2 * P + p, c1 = c0 + P;
+    const int c0 = gr * 2 * P + 2 * p, c1 = c0 + 1;
     if (c0 < g.mm1) dst(g, b, c0, k - 1, ya);
     if (c1 < g.mm1) dst(g, b, c1, k - 1, yb);
   }
@@ -254,10 +254,10 @@ struct Step1RowOp {
   const T* bq0 = nullptr;
   const T* blb = nullptr;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
-    const int row = gr * P + (s < P ? s : s - P);
+    const int row = gr * P + (s >> 1);  // even s: X row, odd s: Y row
     if (row >= g.nm1 || !active(Lc, b, lev)) return T(0);
     const int e = idx + row * g.mm1;
-    if (s >= P) return om[b * fbs + e] / norm;
+    if (s & 1) return om[b * fbs + e] / norm;
     const int i = idx + 2, j = row + 2;
     T rhs = rot_nonlinear<T>(g, i, j, om + b * fbs, bom ? bq + b * qbs : q + b * qbs,
                              bom ? bq0 + b * qbs : q0 + b * qbs, lastbc + b * bbs);
@@ -298,10 +298,10 @@ struct Step1Col2Op {
   const int* Lc;
   int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
-    const int col = gr * P + (s < P ? s : s - P);
+    const int col = gr * P + (s >> 1);  // even s: X column, odd s: Y column
     if (col >= g.mm1 || !active(Lc, b, lev)) return T(0);
     const long e = (long)b * 2 * g.nw + col + idx * g.mm1;
-    return s < P ? t1[e] : t1[e + g.nw];
+    return (s & 1) ? t1[e + g.nw] : t1[e];
   }
   __device__ void mid(int b, int gr, int p, int P, int k, T ya, T yb, T& va, T& vb) const {
     const int col = gr * P + p;

@@ -57,6 +57,7 @@ def _load():
     L.ibfs_last_advance_ms.argtypes = [C.c_void_p, _D]
     L.ibfs_time_dst_pass.argtypes = [C.c_void_p, C.c_int, C.c_int, _D]
     L.ibfs_time_copy.argtypes = [C.c_void_p, C.c_long, C.c_int, _D]
+    L.ibfs_tile_shapes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ibfs_last_error.restype = C.c_char_p
     L.ibfs_last_error.argtypes = [C.c_void_p]
     L.ibfs_destroy.argtypes = [C.c_void_p]
@@ -231,6 +232,21 @@ class Solver:
                            fb=z(self.nf))
         self.itime = 0
 
+    def ngroups(self):
+        """Number of groups (cases of one grid size) of the handle."""
+        ng = C.c_int()
+        self._chk(_lib.ibfs_tile_shapes(self._h, 0, C.byref(ng), None), "ibfs_tile_shapes")
+        return ng.value
+
+    def tile_shapes(self, group=0):
+        """FFTs per workgroup the tuner chose for one group of the handle:
+        {kind: P} for the row pass, column pass, column pair, step-1 row
+        pass and step-1 column pair (0: generic pass)."""
+        out = (C.c_int * 5)()
+        ng = C.c_int()
+        self._chk(_lib.ibfs_tile_shapes(self._h, group, C.byref(ng), out), "ibfs_tile_shapes")
+        return dict(zip(("row", "col", "col2", "step1_row", "step1_col2"), list(out)))
+
     def time_copy(self, n, reps=200):
         """Mean device time (ms) of a streaming copy of n doubles."""
         ms = C.c_double()
@@ -337,6 +353,8 @@ class Ensemble:
     close = Solver.close
     __del__ = Solver.__del__
     _chk = Solver._chk
+    tile_shapes = Solver.tile_shapes
+    ngroups = Solver.ngroups
     _ncases = Solver._ncases
     _nf_total = Solver._nf_total
     objective = Solver.objective

@@ -172,6 +172,16 @@ int ibfs_op_cholsl(ibfs_handle* h, const double* b, double* x);                 
  * the handle's stream), for measurement. */
 int ibfs_last_advance_ms(const ibfs_handle* h, double* ms);
 
+/* Tile shapes (no counterpart in the reference): the FFTs per workgroup that
+ * the autotuner chose for group `group` of the handle (groups = cases of one
+ * grid size), for the row pass, column pass, fused column pair, step-1 row
+ * pass and step-1 column pair (0: the generic pass of a size without a
+ * specialised kernel).  *ngroups (may be NULL) receives the group count.
+ * The tuner times every fitting shape on the group's batch when the handle
+ * is created; IBFS_TILE=P in the environment forces P instead.  Results do
+ * not depend on the tile shape. */
+int ibfs_tile_shapes(const ibfs_handle* h, int group, int* ngroups, int* shapes);
+
 /* Measurement only (no counterpart in the reference): launches one 1-D
  * DST-I pass over a W field reps times on the handle's stream and returns
  * the mean launch duration from HIP events.  axis 0 = along i (rows),

@@ -69,3 +69,30 @@ def test_airfoil_512_five_levels():
     o, g = solver_pair(512, 5, 4.0, 1.0, 2.0, "naca", 0.002, 300.0)
     assert g.nb == 261
     run_pair(o, g, 2, [0, 130, 521])
+
+
+def test_tile_shapes_do_not_change_results(monkeypatch):
+    """The tuner's choice of FFTs per workgroup (1, 2 or 4) must not change a
+    single bit: sequences 2p and 2p+1 share FFT p whatever the shape.  Forced
+    shapes and the tuned one on the 200^2 x 3 cylinder (FFT length 400, all
+    three shapes fit) give identical states after 3 steps."""
+    from ibfs_amd import hip
+
+    runs = {}
+    for tile in ("1", "2", "4", None):
+        if tile is None:
+            monkeypatch.delenv("IBFS_TILE", raising=False)
+        else:
+            monkeypatch.setenv("IBFS_TILE", tile)
+        _, g = solver_pair(200, 3, 4.0, 2.0, 2.0, "circle", 0.004, 100.0)
+        shapes = g.tile_shapes()
+        if tile is not None:
+            assert set(shapes.values()) == {int(tile)}
+        g.build_cholesky()
+        g.advance(3)
+        runs[tile] = (shapes, g.get_state())
+    base = runs["1"][1]
+    for tile, (shapes, st) in runs.items():
+        print("tile", tile, shapes)
+        for k in base:
+            assert np.array_equal(st[k], base[k]), k
