@@ -185,14 +185,6 @@ struct SrcNoCentre {
     return a[b * bs + ii + jj * g.mm1];
   }
 };
-template <typename T>
-struct DstAdd {
-  T* a; long bs; T scale;
-  __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
-    const long e = b * bs + ii + jj * g.mm1;
-    a[e] = a[e] + scale * v;
-  }
-};
 // column-pass epilogue of T = S(aw*): X' = lam1i*T, Y' = lam1n*lam1i*T
 template <typename T>
 struct DstCNSplit {

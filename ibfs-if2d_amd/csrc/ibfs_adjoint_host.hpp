@@ -1,5 +1,6 @@
 // ibfs_adjoint_host.hpp -- launch schedule of the discrete adjoint step and
-// of its checkpointed reverse sweep, on an FP64 group of cases.
+// of its checkpointed reverse sweep, on a group of cases: field adjoints in
+// the group's element type T, force-system adjoints in FP64.
 //
 // Objective: J = (1/nwin) sum over the last nwin of nspin+nwin steps of
 // (cd * sum of body x forces + cl * sum of body y forces), i.e. the mean
@@ -34,15 +35,16 @@ __global__ void k_zero_centre(Geo g, T* __restrict__ w, long bs, const int* __re
   w[b * bs + wix(g, i, j)] = T(0);
 }
 
+template <typename T>
 struct AdjointRunner {
-  Engine<double>& E;
+  Engine<T>& E;
   hipStream_t st;
   int B, Lmax, nw, nq, nbc;
   long FB, QB, BB;
-  DevBuf<double> AW, AWO, ARO, AQ, LB, AN, AX, TK, ABC, ALB, ANEW, URR, U, T1, T2, T3, T4, T5;
+  DevBuf<T> AW, AWO, ARO, AQ, LB, AN, AX, TK, ABC, ALB, ANEW, URR, U, T1, T2, T3, T4, T5;
   DevBuf<double> AFB, Z, Wv, Y, AU;
 
-  explicit AdjointRunner(Engine<double>& e) : E(e), st(e.st) {
+  explicit AdjointRunner(Engine<T>& e) : E(e), st(e.st) {
     B = E.B; Lmax = E.Lmax; nw = E.nw; nq = E.nq; nbc = E.nbc;
     FB = (long)Lmax * nw; QB = (long)Lmax * nq; BB = (long)Lmax * nbc;
     const size_t W = (size_t)B * FB, Q = (size_t)B * QB, C = (size_t)B * BB, F = (size_t)B * nw;
@@ -67,44 +69,44 @@ struct AdjointRunner {
   // level stride QB) or only the level-1 vertex source urr (the step-2 solve,
   // whose fluxes feed regT); then the reverse of its in-place coarsify
   // chain.  The adjoint of the vorticity accumulates into AW with sign.
-  void mp_reverse(const double* aq, const double* urr, double sign) {
+  void mp_reverse(const T* aq, const T* urr, T sign) {
     const int* Lc = E.Lc.p;
     for (int k = 0; k < Lmax; k++) {
-      DstAddCopy<double> dst{AW.p + (long)k * nw, FB, sign, TK.p};
+      DstAddCopy<T> dst{AW.p + (long)k * nw, FB, sign, TK.p};
       if (urr && k == 0) {
-        E.ctci_like(SrcField<double>{urr, nw}, false, dst, T1.p, T2.p, B, Lc, k);
+        E.ctci_like(SrcField<T>{urr, nw}, false, dst, T1.p, T2.p, B, Lc, k);
       } else {
-        SrcAdjS<double> src{aq ? aq + (long)k * nq : nullptr, QB, k > 0 ? ABC.p : nullptr, (long)nbc};
+        SrcAdjS<T> src{aq ? aq + (long)k * nq : nullptr, QB, k > 0 ? ABC.p : nullptr, (long)nbc};
         E.ctci_like(src, false, dst, T1.p, T2.p, B, Lc, k);
       }
       if (k + 1 < Lmax)
-        hipLaunchKernelGGL(k_bc_adj<double>, grid(nbc), dim3(256), 0, st, E.geo, aq ? aq + (long)k * nq : nullptr,
-                           QB, TK.p, (long)nw, 1.0, ABC.p, (long)nbc, Lc, k);
+        hipLaunchKernelGGL(k_bc_adj<T>, grid(nbc), dim3(256), 0, st, E.geo, aq ? aq + (long)k * nq : nullptr, QB,
+                           TK.p, (long)nw, T(1), ABC.p, (long)nbc, Lc, k);
     }
     const int wcnt = (E.m / 2 - 1) * (E.n / 2 - 1);
     for (int k = Lmax - 1; k >= 1; k--) {
-      hipLaunchKernelGGL(k_coarsify_adj<double>, grid((long)nw), dim3(256), 0, st, E.geo, AW.p + (long)k * nw,
+      hipLaunchKernelGGL(k_coarsify_adj<T>, grid((long)nw), dim3(256), 0, st, E.geo, AW.p + (long)k * nw,
                          AW.p + (long)(k - 1) * nw, FB, Lc, k);
-      hipLaunchKernelGGL(k_zero_centre<double>, grid(wcnt), dim3(256), 0, st, E.geo, AW.p + (long)k * nw, FB, Lc, k);
+      hipLaunchKernelGGL(k_zero_centre<T>, grid(wcnt), dim3(256), 0, st, E.geo, AW.p + (long)k * nw, FB, Lc, k);
     }
   }
 
   // One adjoint step.  In: AW, ARO, AQ = adjoints of the step's outputs
   // (omega, rhs_old, q); wold/qold = the forward state at the start of the
   // step.  Out: AW, ARO, AQ = adjoints of the step's inputs; AU += dJ/du_B.
-  void step(const double* wold, const double* qold, int first, bool in_window, double cd, double cl, double jscale) {
+  void step(const T* wold, const T* qold, int first, bool in_window, double cd, double cl, double jscale) {
     const int* Lc = E.Lc.p;
     const Geo& g = E.geo;
-    hipLaunchKernelGGL(k_lastbc<double>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, g, wold, LB.p, Lc);
+    hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, g, wold, LB.p, Lc);
     // STEP 4 reverse: fluxes and vorticity of the final vort2flux
-    mp_reverse(AQ.p, nullptr, 1.0);
+    mp_reverse(AQ.p, nullptr, T(1));
     // STEP 3 reverse: afb_total = afb_J - regT(curl(ainv(aw_1), 0)); z = A^-1 (regT(..) - afb_J) = -adrhsf
-    E.ctci_like(SrcField<double>{AW.p, FB}, true, DstField<double>{U.p, nw}, T1.p, T2.p, B, nullptr, 0, E.lam1i.p, FB);
+    E.ctci_like(SrcField<T>{AW.p, FB}, true, DstField<T>{U.p, nw}, T1.p, T2.p, B, nullptr, 0, E.lam1i.p, FB);
     if (in_window)
       hipLaunchKernelGGL(k_force_adj, grid(E.nf_max), dim3(256), 0, st, E.sm, E.wsum.p, cd, cl, jscale, AFB.p);
     else
       AFB.zero(st);
-    hipLaunchKernelGGL(k_regT_curl<double>, grid(E.nf_max), dim3(256), 0, st, g, E.sm, U.p, (const double*)nullptr,
+    hipLaunchKernelGGL(k_regT_curl<T>, grid(E.nf_max), dim3(256), 0, st, g, E.sm, U.p, (const T*)nullptr,
                        (long)nw, AFB.p, Wv.p, 0L, Lc, -1);
     const dim3 g4((E.nf_max + 3) / 4, B);
     hipLaunchKernelGGL(k_trmv_lower, g4, dim3(256), 0, st, E.sm, E.Linv.p, Wv.p, Y.p);
@@ -112,34 +114,33 @@ struct AdjointRunner {
     // dJ/du_B: rhsf = E q - (delta u_B - E q0), so au -= delta * adrhsf = delta * z
     hipLaunchKernelGGL(k_axpy_case, grid(E.nf_max), dim3(256), 0, st, E.sm, Z.p, 1.0, AU.p, 1);
     // STEP 2 reverse: aq*_1 = reg(adrhsf) = -reg(z); its level-1 vertex source is -rot(reg(z))
-    hipLaunchKernelGGL((k_rotreg<double, double>), dim3((E.nvtx_max + 255) / 256, B), dim3(256), 0, st, E.sm, Z.p,
+    hipLaunchKernelGGL((k_rotreg<T, double>), dim3((E.nvtx_max + 255) / 256, B), dim3(256), 0, st, E.sm, Z.p,
                        0L, URR.p, (long)nw, -1);
-    mp_reverse(nullptr, URR.p, -1.0);
+    mp_reverse(nullptr, URR.p, T(-1));
     // STEP 1 reverse, level by level (the forward ran coarse to fine)
     AWO.zero(st);
     for (int k = 0; k < Lmax; k++) {
-      E.row_pass(SrcField<double>{AW.p + (long)k * nw, FB}, DstField<double>{T1.p, nw}, B, Lc, k);
-      E.col_pass(SrcField<double>{T1.p, nw}, DstCNSplit<double>{T2.p, T3.p, E.lam1i.p, E.lam1n.p, FB, k, Lmax}, B,
-                 Lc, k);
-      E.launch<0>(PairRowOp<double>{g, T2.p, T3.p, T4.p, T5.p, Lc, k}, g.nm1, 1, B);
-      E.launch<1>(PairColOp<double>{g, T4.p, T5.p, AX.p, AWO.p + (long)k * nw, FB, 1.0 / (double)E.norm, Lc, k},
-                  g.mm1, 1, B);
-      hipLaunchKernelGGL(k_step1_adj_combine<double>, grid(nw), dim3(256), 0, st, g, AX.p, ARO.p + (long)k * nw, FB,
+      E.row_pass(SrcField<T>{AW.p + (long)k * nw, FB}, DstField<T>{T1.p, nw}, B, Lc, k);
+      E.col_pass(SrcField<T>{T1.p, nw}, DstCNSplit<T>{T2.p, T3.p, E.lam1i.p, E.lam1n.p, FB, k, Lmax}, B, Lc, k);
+      E.template launch<0>(PairRowOp<T>{g, T2.p, T3.p, T4.p, T5.p, Lc, k}, g.nm1, 1, B);
+      E.template launch<1>(PairColOp<T>{g, T4.p, T5.p, AX.p, AWO.p + (long)k * nw, FB, T(1) / E.norm, Lc, k},
+                           g.mm1, 1, B);
+      hipLaunchKernelGGL(k_step1_adj_combine<T>, grid(nw), dim3(256), 0, st, g, AX.p, ARO.p + (long)k * nw, FB,
                          AN.p, E.con1.p, E.con2.p, Lc, k, first);
-      hipLaunchKernelGGL(k_nl_adj_vertex<double>, grid(nw), dim3(256), 0, st, g, AN.p, qold + (long)k * nq,
+      hipLaunchKernelGGL(k_nl_adj_vertex<T>, grid(nw), dim3(256), 0, st, g, AN.p, qold + (long)k * nq,
                          E.q0.p + (long)k * nq, QB, AWO.p + (long)k * nw, FB, Lc, k);
-      hipLaunchKernelGGL(k_nl_adj_face<double>, grid(nq), dim3(256), 0, st, g, AN.p, wold + (long)k * nw, FB,
+      hipLaunchKernelGGL(k_nl_adj_face<T>, grid(nq), dim3(256), 0, st, g, AN.p, wold + (long)k * nw, FB,
                          LB.p + (long)k * nbc, BB, AQ.p + (long)k * nq, QB, Lc, k);
       if (k + 1 < Lmax) {
-        hipLaunchKernelGGL(k_nl_adj_bc<double>, grid(nbc), dim3(256), 0, st, g, AN.p, AX.p, qold + (long)k * nq,
+        hipLaunchKernelGGL(k_nl_adj_bc<T>, grid(nbc), dim3(256), 0, st, g, AN.p, AX.p, qold + (long)k * nq,
                            E.q0.p + (long)k * nq, QB, E.vfac.p, ALB.p + (long)k * nbc, BB, ANEW.p + (long)k * nbc,
                            Lc, k);
-        hipLaunchKernelGGL(k_gt_add<double>, grid(g.m + g.n + 4), dim3(256), 0, st, g, ANEW.p + (long)k * nbc, BB,
-                           0.25, AW.p + (long)(k + 1) * nw, FB, Lc, k + 1);
+        hipLaunchKernelGGL(k_gt_add<T>, grid(g.m + g.n + 4), dim3(256), 0, st, g, ANEW.p + (long)k * nbc, BB, T(0.25),
+                           AW.p + (long)(k + 1) * nw, FB, Lc, k + 1);
       }
     }
     for (int k = 0; k + 1 < Lmax; k++)
-      hipLaunchKernelGGL(k_gt_add<double>, grid(g.m + g.n + 4), dim3(256), 0, st, g, ALB.p + (long)k * nbc, BB, 0.25,
+      hipLaunchKernelGGL(k_gt_add<T>, grid(g.m + g.n + 4), dim3(256), 0, st, g, ALB.p + (long)k * nbc, BB, T(0.25),
                          AWO.p + (long)(k + 1) * nw, FB, Lc, k + 1);
     std::swap(AW.p, AWO.p);
   }

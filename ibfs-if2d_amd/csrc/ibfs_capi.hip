@@ -388,12 +388,122 @@ void accumulate_objective(ibfs_handle* h, const double* hist, int nrows, int it_
   }
 }
 
-Engine<double>* single_fp64_group(ibfs_handle* h) {
+// the one group of a handle whose cases share a grid size
+GroupBase* single_group(ibfs_handle* h) {
   if (h->groups.size() != 1) throw ArgError{IBFS_EINVAL, "tangent/adjoint runs need all cases of one grid size"};
-  Engine<double>* E = dynamic_cast<Engine<double>*>(h->groups[0].get());
-  if (!E) throw ArgError{IBFS_EINVAL, "tangent/adjoint runs need an FP64 handle"};
-  return E;
+  return h->groups[0].get();
 }
+
+// tangent-linear run of ibfs_tangent_linear on the single group E
+template <typename T>
+void tangent_impl(ibfs_handle* h, Engine<T>* E, int* itime, int nspin, int nwin, const double* ub, const double* dub,
+                  double cd, double cl, double* J, double* dJ) {
+  std::vector<CaseHost> tc = E->cs;
+  for (auto& c : tc) c.d.unif_flow = 0;  // the perturbation has no underlying potential flow
+  Engine<T> Tg(tc, h->stream);
+  Tg.base = E;
+  for (int c = 0; c < Tg.B; c++) Tg.upload_factor(c);
+  std::vector<double> mo = scaled_motion(h, ub), dmo = scaled_motion(h, dub);
+  const int N = nspin + nwin, it0 = *itime;
+  h->ensure_diag(N);
+  Tg.ensure_diag(N);
+  for (int c = 0; c < E->B; c++) {
+    E->set_motion(c, mo.data() + E->hcs[c].foff);
+    Tg.set_motion(c, dmo.data() + Tg.hcs[c].foff);
+  }
+  E->reset_counters();
+  Tg.reset_counters();
+  E->enqueue_motion();
+  Tg.enqueue_motion();
+  for (int k = 0; k < N; k++) {
+    const int first = (*itime + 1 == 1);
+    Tg.enqueue_step(first);  // linearised about the state before the base step
+    E->enqueue_step(first);
+    *itime += 1;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> hb((size_t)N * h->rowlen), ht((size_t)N * h->rowlen);
+  E->download_diag(hb.data(), N, h->rowlen, 0);
+  Tg.download_diag(ht.data(), N, h->rowlen, 0);
+  for (int c = 0; c < h->ncases; c++) J[c] = dJ[c] = 0.0;
+  accumulate_objective(h, hb.data(), N, it0 + 1, it0 + nspin, nwin, cd, cl, J);
+  accumulate_objective(h, ht.data(), N, it0 + 1, it0 + nspin, nwin, cd, cl, dJ);
+}
+
+// checkpointed adjoint run of ibfs_adjoint_gradient on the single group E
+template <typename T>
+void adjoint_impl(ibfs_handle* h, Engine<T>* E, int* itime, int nspin, int nwin, int ckpt, const double* ub,
+                  double cd, double cl, double* J, double* grad) {
+  // the transposes assume no smearing weight on the two outermost rings of
+  // faces of the finest level (bodies well inside the domain)
+  for (int c = 0; c < E->B; c++) {
+    const CaseHost& k = E->cs[c];
+    const int m = k.d.m, n = k.d.n, nu = (m + 1) * n;
+    for (int f : k.face_id) {
+      int i, j;
+      if (f < nu) { i = f % (m + 1) + 1; j = f / (m + 1) + 1; }
+      else { i = (f - nu) % m + 1; j = (f - nu) / m + 1; }
+      if (i < 3 || j < 3 || i > m - 2 || j > n - 2)
+        throw ArgError{IBFS_EINVAL, "adjoint: a body smears onto the outer rings of the finest grid"};
+    }
+  }
+  hipStream_t st = h->stream;
+  const int N = nspin + nwin, it0 = *itime;
+  const int nck = (N + ckpt - 1) / ckpt;
+  const size_t W = (size_t)E->B * E->fbs(), Q = (size_t)E->B * E->qbs();
+  DevBuf<T> CW, CRO, CQ, SW, SQ, FW, FRO, FQ;
+  CW.alloc(W * nck); CRO.alloc(W * nck); CQ.alloc(Q * nck);
+  SW.alloc(W * ckpt); SQ.alloc(Q * ckpt);
+  FW.alloc(W); FRO.alloc(W); FQ.alloc(Q);
+  auto d2d = [&](T* dst, const T* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+  };
+  // forward sweep with checkpoints
+  std::vector<double> mo = scaled_motion(h, ub);
+  for (int c = 0; c < h->ncases; c++) J[c] = 0.0;
+  std::vector<double> hist((size_t)ckpt * h->rowlen);
+  std::vector<int> ck_it(nck);
+  int it = it0;
+  for (int s = 0; s < nck; s++) {
+    d2d(CW.p + W * s, E->omega.p, W);
+    d2d(CRO.p + W * s, E->rhs_old.p, W);
+    d2d(CQ.p + Q * s, E->q.p, Q);
+    ck_it[s] = it;
+    const int ns = std::min(ckpt, N - s * ckpt);
+    const int itb = it;
+    advance_impl(h, &it, ns, mo.data(), hist.data());
+    accumulate_objective(h, hist.data(), ns, itb + 1, it0 + nspin, nwin, cd, cl, J);
+  }
+  d2d(FW.p, E->omega.p, W);
+  d2d(FRO.p, E->rhs_old.p, W);
+  d2d(FQ.p, E->q.p, Q);
+  // reverse sweep: recompute each segment's states, then step back through it
+  AdjointRunner<T> R(*E);
+  for (int s = nck - 1; s >= 0; s--) {
+    d2d(E->omega.p, CW.p + W * s, W);
+    d2d(E->rhs_old.p, CRO.p + W * s, W);
+    d2d(E->q.p, CQ.p + Q * s, Q);
+    const int ns = std::min(ckpt, N - s * ckpt);
+    for (int j = 0; j < ns; j++) {
+      d2d(SW.p + W * j, E->omega.p, W);
+      d2d(SQ.p + Q * j, E->q.p, Q);
+      E->enqueue_step(ck_it[s] + j + 1 == 1);
+    }
+    for (int j = ns - 1; j >= 0; j--) {
+      const int n = ck_it[s] + j + 1;  // itime of this step
+      R.step(SW.p + W * j, SQ.p + Q * j, n == 1, n > it0 + nspin, cd, cl, 1.0 / nwin);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  // back to the final forward state
+  d2d(E->omega.p, FW.p, W);
+  d2d(E->rhs_old.p, FRO.p, W);
+  d2d(E->q.p, FQ.p, Q);
+  R.AU.download(grad, E->nf_tot, st);
+  HIPCHK(hipStreamSynchronize(st));
+  *itime = it;
+}
+
 }  // namespace
 
 extern "C" {
@@ -416,37 +526,9 @@ int ibfs_tangent_linear(ibfs_handle* h, int* itime, int nspin, int nwin, const d
   if (!h || !itime || !J || !dJ || !dub || nspin < 0 || nwin < 1) return IBFS_EINVAL;
   return guarded(h, [&] {
     if (!h->all_factors()) throw ArgError{IBFS_ESTATE, "no Cholesky factor"};
-    Engine<double>* E = single_fp64_group(h);
-    std::vector<CaseHost> tc = E->cs;
-    for (auto& c : tc) c.d.unif_flow = 0;  // the perturbation has no underlying potential flow
-    Engine<double> Tg(tc, h->stream);
-    Tg.base = E;
-    for (int c = 0; c < Tg.B; c++) Tg.upload_factor(c);
-    std::vector<double> mo = scaled_motion(h, ub), dmo = scaled_motion(h, dub);
-    const int N = nspin + nwin, it0 = *itime;
-    h->ensure_diag(N);
-    Tg.ensure_diag(N);
-    for (int c = 0; c < E->B; c++) {
-      E->set_motion(c, mo.data() + E->hcs[c].foff);
-      Tg.set_motion(c, dmo.data() + Tg.hcs[c].foff);
-    }
-    E->reset_counters();
-    Tg.reset_counters();
-    E->enqueue_motion();
-    Tg.enqueue_motion();
-    for (int k = 0; k < N; k++) {
-      const int first = (*itime + 1 == 1);
-      Tg.enqueue_step(first);  // linearised about the state before the base step
-      E->enqueue_step(first);
-      *itime += 1;
-    }
-    HIPCHK(hipStreamSynchronize(h->stream));
-    std::vector<double> hb((size_t)N * h->rowlen), ht((size_t)N * h->rowlen);
-    E->download_diag(hb.data(), N, h->rowlen, 0);
-    Tg.download_diag(ht.data(), N, h->rowlen, 0);
-    for (int c = 0; c < h->ncases; c++) J[c] = dJ[c] = 0.0;
-    accumulate_objective(h, hb.data(), N, it0 + 1, it0 + nspin, nwin, cd, cl, J);
-    accumulate_objective(h, ht.data(), N, it0 + 1, it0 + nspin, nwin, cd, cl, dJ);
+    GroupBase* G = single_group(h);
+    if (auto* E = dynamic_cast<Engine<double>*>(G)) tangent_impl(h, E, itime, nspin, nwin, ub, dub, cd, cl, J, dJ);
+    else tangent_impl(h, dynamic_cast<Engine<float>*>(G), itime, nspin, nwin, ub, dub, cd, cl, J, dJ);
   });
 }
 
@@ -455,75 +537,9 @@ int ibfs_adjoint_gradient(ibfs_handle* h, int* itime, int nspin, int nwin, int c
   if (!h || !itime || !J || !grad || nspin < 0 || nwin < 1 || ckpt < 1) return IBFS_EINVAL;
   return guarded(h, [&] {
     if (!h->all_factors()) throw ArgError{IBFS_ESTATE, "no Cholesky factor"};
-    Engine<double>* E = single_fp64_group(h);
-    // the transposes assume no smearing weight on the two outermost rings of
-    // faces of the finest level (bodies well inside the domain)
-    for (int c = 0; c < E->B; c++) {
-      const CaseHost& k = E->cs[c];
-      const int m = k.d.m, n = k.d.n, nu = (m + 1) * n;
-      for (int f : k.face_id) {
-        int i, j;
-        if (f < nu) { i = f % (m + 1) + 1; j = f / (m + 1) + 1; }
-        else { i = (f - nu) % m + 1; j = (f - nu) / m + 1; }
-        if (i < 3 || j < 3 || i > m - 2 || j > n - 2)
-          throw ArgError{IBFS_EINVAL, "adjoint: a body smears onto the outer rings of the finest grid"};
-      }
-    }
-    hipStream_t st = h->stream;
-    const int N = nspin + nwin, it0 = *itime;
-    const int nck = (N + ckpt - 1) / ckpt;
-    const size_t W = (size_t)E->B * E->fbs(), Q = (size_t)E->B * E->qbs();
-    DevBuf<double> CW, CRO, CQ, SW, SQ, FW, FRO, FQ;
-    CW.alloc(W * nck); CRO.alloc(W * nck); CQ.alloc(Q * nck);
-    SW.alloc(W * ckpt); SQ.alloc(Q * ckpt);
-    FW.alloc(W); FRO.alloc(W); FQ.alloc(Q);
-    auto d2d = [&](double* dst, const double* src, size_t n) {
-      HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    };
-    // forward sweep with checkpoints
-    std::vector<double> mo = scaled_motion(h, ub);
-    for (int c = 0; c < h->ncases; c++) J[c] = 0.0;
-    std::vector<double> hist((size_t)ckpt * h->rowlen);
-    std::vector<int> ck_it(nck);
-    int it = it0;
-    for (int s = 0; s < nck; s++) {
-      d2d(CW.p + W * s, E->omega.p, W);
-      d2d(CRO.p + W * s, E->rhs_old.p, W);
-      d2d(CQ.p + Q * s, E->q.p, Q);
-      ck_it[s] = it;
-      const int ns = std::min(ckpt, N - s * ckpt);
-      const int itb = it;
-      advance_impl(h, &it, ns, mo.data(), hist.data());
-      accumulate_objective(h, hist.data(), ns, itb + 1, it0 + nspin, nwin, cd, cl, J);
-    }
-    d2d(FW.p, E->omega.p, W);
-    d2d(FRO.p, E->rhs_old.p, W);
-    d2d(FQ.p, E->q.p, Q);
-    // reverse sweep: recompute each segment's states, then step back through it
-    AdjointRunner R(*E);
-    for (int s = nck - 1; s >= 0; s--) {
-      d2d(E->omega.p, CW.p + W * s, W);
-      d2d(E->rhs_old.p, CRO.p + W * s, W);
-      d2d(E->q.p, CQ.p + Q * s, Q);
-      const int ns = std::min(ckpt, N - s * ckpt);
-      for (int j = 0; j < ns; j++) {
-        d2d(SW.p + W * j, E->omega.p, W);
-        d2d(SQ.p + Q * j, E->q.p, Q);
-        E->enqueue_step(ck_it[s] + j + 1 == 1);
-      }
-      for (int j = ns - 1; j >= 0; j--) {
-        const int n = ck_it[s] + j + 1;  // itime of this step
-        R.step(SW.p + W * j, SQ.p + Q * j, n == 1, n > it0 + nspin, cd, cl, 1.0 / nwin);
-      }
-      HIPCHK(hipStreamSynchronize(st));
-    }
-    // back to the final forward state
-    d2d(E->omega.p, FW.p, W);
-    d2d(E->rhs_old.p, FRO.p, W);
-    d2d(E->q.p, FQ.p, Q);
-    R.AU.download(grad, E->nf_tot, st);
-    HIPCHK(hipStreamSynchronize(st));
-    *itime = it;
+    GroupBase* G = single_group(h);
+    if (auto* E = dynamic_cast<Engine<double>*>(G)) adjoint_impl(h, E, itime, nspin, nwin, ckpt, ub, cd, cl, J, grad);
+    else adjoint_impl(h, dynamic_cast<Engine<float>*>(G), itime, nspin, nwin, ckpt, ub, cd, cl, J, grad);
   });
 }
 

@@ -404,13 +404,18 @@ struct Engine : GroupBase {
                           geo.mm1, 2, B);
               break;
             case kStep1Row:
-              launch<0>(Step1RowOp<T>{geo, T2.p, nullptr, qz.p, qz.p, bz.p, T3.p, T1.p, con1.p, con2.p, vfac.p,
-                                      Lc.p, (long)nw, (long)nq, (long)nbc, norm, 0, 0},
-                        geo.nm1, 1, B, kStep1Row);
+              launch<0>(RowOp<T, SrcStep1<T>, DstField<T>>{geo,
+                                                           SrcStep1<T>{T2.p, nullptr, qz.p, qz.p, bz.p, T3.p, con1.p,
+                                                                       con2.p, vfac.p, Lc.p, (long)nw, (long)nq,
+                                                                       (long)nbc, 0, 0},
+                                                           DstField<T>{T1.p, nw}, Lc.p, 0},
+                        geo.nm1, 2, B, kStep1Row);
               break;
             default:
-              launch_col2(Step1Col2Op<T, DstField<T>>{geo, T1.p, lam1n.p, lam1i.p, DstField<T>{T2.p, nw}, Lc.p, 0},
-                          geo.mm1, 1, B, kStep1Col2);
+              launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{geo, SrcField<T>{T1.p, nw},
+                                                                         MidTab<T>{lam1i.p, (long)Lmax * nw},
+                                                                         DstField<T>{T2.p, nw}, Lc.p, 0},
+                          geo.mm1, 2, B, kStep1Col2);
               break;
           }
         };
@@ -497,36 +502,37 @@ struct Engine : GroupBase {
                          base->lastbc.p, Lc.p);
     hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, omega.p, lastbc.p,
                        Lc.p);
-    // STEP 1 (src/operators.f90:103-134), level slots Lmax..1
+    // STEP 1 (src/operators.f90:103-134), level slots Lmax..1, in increment
+    // form: omega += S[lam1i * S(X + 2 vfac D0 omega)] (see SrcStep1)
     for (int lev = Lmax - 1; lev >= 0; lev--) {
-      Step1RowOp<T> r1{geo,
-                       omega.p + (long)lev * nw,
-                       lev + 1 < Lmax ? omega.p + (long)(lev + 1) * nw : nullptr,
-                       q.p + (long)lev * nq,
-                       q0.p + (long)lev * nq,
-                       lastbc.p + (long)lev * nbc,
-                       rhs_old.p + (long)lev * nw,
-                       T1.p,
-                       con1.p,
-                       con2.p,
-                       vfac.p,
-                       Lc.p,
-                       FB,
-                       QB,
-                       BB,
-                       norm,
-                       lev,
-                       first};
+      SrcStep1<T> s1{omega.p + (long)lev * nw,
+                     lev + 1 < Lmax ? omega.p + (long)(lev + 1) * nw : nullptr,
+                     q.p + (long)lev * nq,
+                     q0.p + (long)lev * nq,
+                     lastbc.p + (long)lev * nbc,
+                     rhs_old.p + (long)lev * nw,
+                     con1.p,
+                     con2.p,
+                     vfac.p,
+                     Lc.p,
+                     FB,
+                     QB,
+                     BB,
+                     lev,
+                     first};
       if (base) {
-        r1.bom = base->omega.p + (long)lev * nw;
-        r1.bq = base->q.p + (long)lev * nq;
-        r1.bq0 = base->q0.p + (long)lev * nq;
-        r1.blb = base->lastbc.p + (long)lev * nbc;
+        s1.bom = base->omega.p + (long)lev * nw;
+        s1.bq = base->q.p + (long)lev * nq;
+        s1.bq0 = base->q0.p + (long)lev * nq;
+        s1.blb = base->lastbc.p + (long)lev * nbc;
       }
-      launch<0>(r1, geo.nm1, 1, B, kStep1Row);
-      launch_col2(Step1Col2Op<T, DstField<T>>{geo, T1.p, lam1n.p, lam1i.p, DstField<T>{T2.p, nw}, Lc.p, lev},
-                  geo.mm1, 1, B, kStep1Col2);
-      row_pass(SrcField<T>{T2.p, nw}, DstField<T>{omega.p + (long)lev * nw, FB}, B, Lc.p, lev);
+      launch<0>(RowOp<T, SrcStep1<T>, DstField<T>>{geo, s1, DstField<T>{T1.p, nw}, Lc.p, lev}, geo.nm1, 2, B,
+                kStep1Row);
+      launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{geo, SrcField<T>{T1.p, nw},
+                                                                 MidTab<T>{lam1i.p + (long)lev * nw, (long)Lmax * nw},
+                                                                 DstField<T>{T2.p, nw}, Lc.p, lev},
+                  geo.mm1, 2, B, kStep1Col2);
+      row_pass(SrcField<T>{T2.p, nw}, DstAdd<T>{omega.p + (long)lev * nw, FB, T(1)}, B, Lc.p, lev);
     }
     // STEP 2 (src/operators.f90:142-171); motion is prepared per call.  The
     // force kernel also writes the step-3 source RR = rot(reg(fb)).

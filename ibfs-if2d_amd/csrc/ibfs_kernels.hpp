@@ -224,27 +224,29 @@ struct Col2Op {
   }
 };
 
-// Step 1, first pass at level slot lev: X = con1*rhs + con2*rhs_old + rhsbc
-// with rhs = rot(nonlinear(omega_k, q_k, q0_k, lastbc_k)) and
-// Y = omega_k / (4mn), rows of both transformed together (FFT p = X row +
-// i Y row); rhs_old_k <- rhs.  src/operators.f90:110-133.
-// Y is pre-divided by the normalisation 4mn so that both halves of a packed
-// FFT have the scale of the final result: an FFT's rounding is relative to
-// the larger half, and packing omega at full scale against X (which carries
-// the 1/(4mn) of con1/con2) would cost a factor 4mn in the accuracy of S(X).
+// Step 1 at level slot lev (src/operators.f90:103-134) in increment form.
+// The reference forms omega* = S[lam1i * (S X + lam1 * S omega)] with
+// X = con1*rhs + con2*rhs_old + rhsbc and rhs = rot(nonlinear(omega_k, q_k,
+// q0_k, lastbc_k)).  S S = 4mn I and (with a = dt/(2 Re del^2))
+// lam1 - 1/(4mn lam1i) = -2 a lam/(4mn), and S(-D0 w) = lam * S w for the
+// 5-point Laplacian D0 with zero edge values, so the same omega* is
+//   omega* = omega + S[lam1i * S(X + 2 vfac D0 omega)],   vfac = a/(4mn).
+// The transforms carry only the O(dt) increment: their rounding scales with
+// the increment, not with omega (with FP32 transforms the direct form puts
+// an error of order eps*|omega| into the slip that the force system then
+// removes, a 1e-4-relative bias of the mean forces at 1024^2), and one field
+// is transformed instead of two.
+// SrcStep1 gives X + 2 vfac D0 omega at (ii, jj) and stores rhs_old <- rhs.
 template <typename T>
-struct Step1RowOp {
-  Geo g;
+struct SrcStep1 {
   const T* om;      // omega at level lev (old)        stride fbs
   const T* omc;     // omega at level lev+1 (new)      stride fbs; nullptr if lev+1 == Lmax
   const T* q; const T* q0;  //                         stride qbs
   const T* lastbc;  // get_bc of old omega_{lev+1}, x0.25  stride bbs
   T* rhs_old;       //                                 stride fbs
-  T* t1;            // row-transformed X at t1 + b*2nw, Y at t1 + b*2nw + nw
   const T* con1; const T* con2; const T* vfac;  // [b*Lmax + lev]
   const int* Lc;
   long fbs, qbs, bbs;
-  T norm;
   int lev, first;   // first: itime == 1, explicit Euler start
   // Tangent-linear step: base state at level lev (same strides).  When bom
   // is set, (om, q, lastbc) are perturbations, q0 is zero, and rhs is the
@@ -253,14 +255,12 @@ struct Step1RowOp {
   const T* bq = nullptr;
   const T* bq0 = nullptr;
   const T* blb = nullptr;
-  __device__ T load(int b, int gr, int s, int P, int idx) const {
-    const int row = gr * P + (s >> 1);  // even s: X row, odd s: Y row
-    if (row >= g.nm1 || !active(Lc, b, lev)) return T(0);
-    const int e = idx + row * g.mm1;
-    if (s & 1) return om[b * fbs + e] / norm;
-    const int i = idx + 2, j = row + 2;
-    T rhs = rot_nonlinear<T>(g, i, j, om + b * fbs, bom ? bq + b * qbs : q + b * qbs,
-                             bom ? bq0 + b * qbs : q0 + b * qbs, lastbc + b * bbs);
+  __device__ T operator()(const Geo& g, int b, int ii, int jj) const {
+    const T* w = om + b * fbs;
+    const int e = ii + jj * g.mm1;
+    const int i = ii + 2, j = jj + 2;
+    T rhs = rot_nonlinear<T>(g, i, j, w, bom ? bq + b * qbs : q + b * qbs, bom ? bq0 + b * qbs : q0 + b * qbs,
+                             lastbc + b * bbs);
     if (bom) rhs = rhs + rot_nonlinear<T>(g, i, j, bom + b * fbs, q + b * qbs, q0 + b * qbs, blb + b * bbs);
     T* ro = rhs_old + b * fbs + e;
     const T rold = first ? rhs : *ro;
@@ -271,52 +271,24 @@ struct Step1RowOp {
       bcs = applybc_vec<T>(g, bcs, i, j, lastbc + b * bbs, vfac[c]);
       if (omc != nullptr && has_coarser(Lc, b, lev)) bcs = applybc_add<T>(g, bcs, i, j, omc + b * fbs, T(0.25), vfac[c]);
     }
-    return con1[c] * rhs + con2[c] * rold + bcs;
-  }
-  __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
-    const int row = gr * P + p;
-    if (row >= g.nm1 || !active(Lc, b, lev)) return;
-    const long e = (long)b * 2 * g.nw + (k - 1) + row * g.mm1;
-    t1[e] = ya;
-    t1[e + g.nw] = yb;
+    const T lap = (ii > 0 ? w[e - 1] : T(0)) + (ii + 1 < g.mm1 ? w[e + 1] : T(0)) + (jj > 0 ? w[e - g.mm1] : T(0)) +
+                  (jj + 1 < g.nm1 ? w[e + g.mm1] : T(0)) - T(4) * w[e];
+    return con1[c] * rhs + con2[c] * rold + bcs + T(2) * vfac[c] * lap;
   }
 };
-
-// Step 1, second and third transforms as a fused column pair: the columns
-// of X~ and Y~ (FFT p = X column + i Y column, as written by Step1RowOp),
-// the CN combine Z = lam1i_k * (X^ + lam1_k * S(omega))
-// (src/operators.f90:126-130) with lam1_k * S(omega) = lam1n_k * S(omega/4mn),
-// lam1n = lam1 * 4mn, then Sy of the combined column (FFT p = Z column + i 0).
-// Tables of level lev of case b at (b*Lmax + lev)*nw.  P columns per
-// workgroup.
-template <typename T, class Dst>
-struct Step1Col2Op {
-  Geo g;
-  const T* t1;
-  const T* lam1n; const T* lam1i;
-  Dst dst;
-  const int* Lc;
-  int lev;
-  __device__ T load(int b, int gr, int s, int P, int idx) const {
-    const int col = gr * P + (s >> 1);  // even s: X column, odd s: Y column
-    if (col >= g.mm1 || !active(Lc, b, lev)) return T(0);
-    const long e = (long)b * 2 * g.nw + col + idx * g.mm1;
-    return (s & 1) ? t1[e + g.nw] : t1[e];
-  }
-  __device__ void mid(int b, int gr, int p, int P, int k, T ya, T yb, T& va, T& vb) const {
-    const int col = gr * P + p;
-    vb = T(0);
-    if (col >= g.mm1 || !active(Lc, b, lev)) {
-      va = T(0);
-      return;
-    }
-    const long t = ((long)b * g.Lmax + lev) * g.nw + col + (k - 1) * g.mm1;
-    va = lam1i[t] * (ya + lam1n[t] * yb);
-  }
-  __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
-    const int col = gr * P + p;
-    if (col >= g.mm1 || !active(Lc, b, lev)) return;
-    dst(g, b, col, k - 1, ya);
+// tab * v with a per-case table (step 1's lam1i of level lev)
+template <typename T>
+struct MidTab {
+  const T* tab; long tbs;
+  __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const { return tab[b * tbs + ii + jj * g.mm1] * v; }
+};
+// a = a + scale * v (the step-1 increment; adjoint accumulations)
+template <typename T>
+struct DstAdd {
+  T* a; long bs; T scale;
+  __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
+    const long e = b * bs + ii + jj * g.mm1;
+    a[e] = a[e] + scale * v;
   }
 };
 

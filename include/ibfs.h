@@ -135,8 +135,9 @@ int ibfs_ensemble_build_cholesky(ibfs_handle* h);
  * x components then y components, concatenated over cases), active in every
  * step as motion = delta * u_B (src/grid.f90:573-574).  All three advance
  * the handle's state and *itime by nspin + nwin steps.  J, dJ: one value per
- * case.  Tangent and adjoint runs need an FP64 handle whose cases share one
- * grid size.
+ * case.  Tangent and adjoint runs need a handle whose cases share one grid
+ * size; on a mixed handle they run with FP32 fields and the FP64 force system
+ * (the FP64 handle is the reference for their gradients).
  * ------------------------------------------------------------------------- */
 /* forward only: J */
 int ibfs_objective(ibfs_handle* h, int* itime, int nspin, int nwin, const double* ub, double cd, double cl,
