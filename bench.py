@@ -10,7 +10,7 @@ is one call of the reference's advance (src/operators.f90:62-188).
 Prints one JSON line.  ``value`` is steps/s of the plain FP64 form (the
 drop-in).  ``chosen`` reports the mixed FP32/FP64 form for one case and for
 a batch of independent copies of the case in one graph per step (aggregate
-case-steps/s), with its force error against the FP64 run.  ``roofline`` is
+case-steps/s), with its per-field error against the FP64 run.  ``roofline`` is
 the step's dominant kernel (the fused column pair of ctci, timed live with
 HIP events on the library's stream); ``cpu_baseline`` is the oracle's FP64
 step, single thread, on a bounded sample of the same workload.
@@ -40,12 +40,15 @@ PAIR_TRAFFIC_BYTES = int((2630.8 * 1.980 + 2135.8 * 1.000) * 1024)
 
 def step_bytes(m, n, L, nf, s=8):
     """Algorithmic HBM bytes of one step: every 2-D DST-I reads and writes one
-    W field (7L+2 per step), step 1 reads q, q0 and reads/writes rhs_old per
-    level, coarsify/projection reads, curl writes, and the Cholesky factor
-    read (fixed formula, independent of how the kernels are fused)."""
+    W field (6L+2 per step: two per level in step 1, which transforms only
+    the increment, two per level in each vort2flux, two in the projection),
+    step 1 reads q, q0, reads/writes rhs_old and reads omega twice (stencil,
+    increment) per level, coarsify/projection reads, curl writes, and the
+    Cholesky factor read (fixed formula, independent of how the kernels are
+    fused)."""
     W = (m - 1) * (n - 1)
     Nq = (m + 1) * n + (n + 1) * m
-    return s * (2 * (7 * L + 2) * W + L * (2 * Nq + 2 * W) + (2 * L - 1) * W + (L + 1) * Nq) + 8 * nf * (nf - 1)
+    return s * (2 * (6 * L + 2) * W + L * (2 * Nq + 4 * W) + (2 * L - 1) * W + (L + 1) * Nq) + 8 * nf * (nf - 1)
 
 
 def log(msg):
@@ -82,6 +85,9 @@ def main(argv=None):
                     "--adjoint-window 0 skips it)")
     ap.add_argument("--adjoint-window", type=int, default=400)
     ap.add_argument("--ensemble", type=int, default=1, help="run the 64-case uneven ensemble leg (0: skip)")
+    ap.add_argument("--config5", type=int, default=1, help="run the batched 1024^2 x 6 tandem adjoint leg (0: skip)")
+    ap.add_argument("--config5-spin", type=int, default=2000)
+    ap.add_argument("--config5-window", type=int, default=400)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent cases shard "
@@ -137,7 +143,35 @@ def main(argv=None):
         gm.advance_timed(K)
         wm = time.perf_counter() - t0
         mixed_s = K / wm
-        log(f"mixed FP32/FP64, 1 case: {mixed_s:.1f} steps/s; force error vs FP64 over {W0} steps {ferr:.2e}")
+        # per-field error against the FP64 run: both forms have taken W0 + K
+        # steps from the same start; Wf more give the window of the mean forces
+        Wf = 500
+        hf, hmf = g.advance(Wf), gm.advance(Wf)
+        a, b = g.get_state(), gm.get_state()
+
+        def rel_levels(key, per):
+            return [float(np.max(np.abs(b[key][k * per:(k + 1) * per] - a[key][k * per:(k + 1) * per]))
+                          / np.max(np.abs(a[key][k * per:(k + 1) * per]))) for k in range(g.L)]
+
+        F64, Fm = hf[:, :2 * nb], hmf[:, :2 * nb]
+        mean_rel = lambda cols: float(abs(Fm[:, cols].sum(1).mean() - F64[:, cols].sum(1).mean())
+                                      / abs(F64[:, cols].sum(1).mean()))
+        field_err = {"after_steps": W0 + K + Wf, "omega_per_level": rel_levels("omega", g.nw),
+                     "q_per_level": rel_levels("q", g.nq),
+                     "fb": float(np.max(np.abs(b["fb"] - a["fb"])) / np.max(np.abs(a["fb"]))),
+                     "force_history_window": float(np.max(np.abs(Fm - F64)) / np.max(np.abs(F64))),
+                     "mean_drag_window": mean_rel(slice(0, nb)), "mean_lift_window": mean_rel(slice(nb, 2 * nb)),
+                     "circulation": float(np.max(np.abs(hmf[:, 2 * nb:2 * nb + g.L] - hf[:, 2 * nb:2 * nb + g.L]))
+                                          / np.max(np.abs(hf[:, 2 * nb:2 * nb + g.L]))),
+                     "window_steps": Wf,
+                     "note": "max |mixed - fp64| / max |fp64| per field (per level for omega and q); mean forces "
+                             "over the last window_steps steps, relative"}
+        ferr = max(ferr, field_err["force_history_window"], field_err["mean_drag_window"],
+                   field_err["mean_lift_window"])
+        log(f"mixed FP32/FP64, 1 case: {mixed_s:.1f} steps/s; force error vs FP64 {ferr:.2e} (worst of the first "
+            f"{W0} steps, the last {Wf} steps' history and their mean drag/lift); omega per level "
+            + " ".join(f"{x:.1e}" for x in field_err["omega_per_level"]) + "; q per level "
+            + " ".join(f"{x:.1e}" for x in field_err["q_per_level"]))
         del gm
         # ---- mixed form, batched: args.batch independent copies in one graph per step
         res = {}
@@ -160,7 +194,7 @@ def main(argv=None):
                   "tile_shapes": {"one_case_fp64": g.tile_shapes(0), "batched_mixed": res["mixed_tiles"],
                                   "batched_fp64": res["fp64_tiles"]},
                   "speedup_vs_plain": res["mixed"] / steps_s, "force_err_vs_fp64": ferr,
-                  "force_err_window_steps": W0}
+                  "field_err_vs_fp64": field_err}
 
     adjoint = None
     if args.adjoint_spin + args.adjoint_window > 0:
@@ -205,6 +239,49 @@ def main(argv=None):
             f"ensemble step; factors {tb:.1f} s)")
         del ens
 
+    config5 = None
+    if args.config5:
+        # ---- batched adjoint: 8 copies of the 1024^2 x 6 tandem-cylinder case
+        # (628 IB points, Re 200, dt 0.0015), each with its own seeded u_B,
+        # objective mean lift + mean drag, mixed form, checkpoints every 100
+        # steps; gradient error against the FP64 adjoint of case 0
+        tc = cases.tandem_1024()
+        s64 = hip.Solver(**tc)
+        cm5, cv5 = s64.build_cholesky()
+        nf5 = s64.nf
+        rng = np.random.default_rng(5)
+        ubs = [rng.normal(0.0, 0.05, nf5) for _ in range(8)]
+        sp5, wn5 = args.config5_spin, args.config5_window
+        ens = hip.Ensemble([tc] * 8, precision=hip.MIXED)
+        for k in range(8):
+            ens.set_case_cholesky(k, cm5, cv5)
+        ens.advance(1)  # graph capture and first-step set-up outside the timings
+        ens.reset()
+        t0 = time.perf_counter()
+        ens.objective(sp5, wn5, np.concatenate(ubs), 1.0, 1.0)
+        tf5 = time.perf_counter() - t0
+        ens.reset()
+        t0 = time.perf_counter()
+        J5, g5 = ens.adjoint(sp5, wn5, 100, np.concatenate(ubs), 1.0, 1.0)
+        ta5 = time.perf_counter() - t0
+        del ens
+        t0 = time.perf_counter()
+        J64, g64 = s64.adjoint(sp5, wn5, 100, ubs[0], 1.0, 1.0)
+        t64 = time.perf_counter() - t0
+        gerr = float(np.linalg.norm(g5[:nf5] - g64) / np.linalg.norm(g64))
+        config5 = {"case": "8 x (1024^2 x 6 levels, tandem cylinders D=1 gap 3D, 628 IB points, Re 200, dt 0.0015)",
+                   "spin_up_steps": sp5, "window_steps": wn5, "checkpoint_every": 100, "parameters_per_case": nf5,
+                   "objective": "mean lift + mean drag", "precision": "mixed FP32/FP64",
+                   "forward_case_steps_per_s": 8 * (sp5 + wn5) / tf5,
+                   "adjoint_s": ta5, "gradients_per_s": 8 / ta5,
+                   "adjoint_case_steps_per_s": 8 * (sp5 + wn5) / ta5,
+                   "fp64_single_adjoint_s": t64,
+                   "grad_err_vs_fp64_case0": gerr, "J_err_vs_fp64_case0": float(abs(J5[0] - J64[0]) / abs(J64[0]))}
+        log(f"config 5: 8 mixed adjoints of 1024^2 x 6 tandem ({sp5}+{wn5} steps): {ta5:.1f} s -> "
+            f"{8 / ta5:.3f} gradients/s; forward {8 * (sp5 + wn5) / tf5:.0f} case-steps/s; FP64 single adjoint "
+            f"{t64:.1f} s; gradient error vs FP64 {gerr:.2e}, J error {config5['J_err_vs_fp64_case0']:.2e}")
+        del s64
+
     cpu = None
     if not args.no_cpu:
         v, n = cpu_baseline(case, cm, cv, args.cpu_seconds)
@@ -242,6 +319,7 @@ def main(argv=None):
         "chosen": chosen,
         "adjoint": adjoint,
         "ensemble64": ensemble,
+        "tandem1024_adjoint_batch8": config5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

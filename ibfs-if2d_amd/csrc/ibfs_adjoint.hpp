@@ -185,57 +185,25 @@ struct SrcNoCentre {
     return a[b * bs + ii + jj * g.mm1];
   }
 };
-// column-pass epilogue of T = S(aw*): X' = lam1i*T, Y' = lam1n*lam1i*T
+// Transpose of step 1 in increment form.  The forward step is
+// omega* = omega + S lam1i S (X + 2 vfac D0 omega) (D0: 5-point Laplacian
+// with zero edge values, symmetric; S and diag(lam1i) symmetric), so with
+// aX = S lam1i S aw* (the adjoint of X, one transform pair):
+// aw_old += aw* + 2 vfac D0 aX.  grid.x over nw, grid.y = case
 template <typename T>
-struct DstCNSplit {
-  T* x; T* y; const T* lam1i; const T* lam1n; long tbs; int lev, Lmax;
-  __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
-    const int e = ii + jj * g.mm1;
-    const long t = ((long)b * Lmax + lev) * g.nw + e;
-    const T u = lam1i[t] * v;
-    x[(long)b * g.nw + e] = u;
-    y[(long)b * g.nw + e] = lam1n[t] * u;
-  }
-};
-
-// Packed pair pass: X + iY with X, Y two fields of stride nw (second pass
-// of a two-field transform); the column pass writes S X to ax and adds
-// yscale * S Y into ay (stride ays).
-template <typename T>
-struct PairRowOp {
-  Geo g; const T* x; const T* y; T* tx; T* ty; const int* Lc; int lev;
-  __device__ T load(int b, int gr, int s, int P, int idx) const {
-    const int row = gr * P + (s >> 1);  // even s: x, odd s: y
-    if (row >= g.nm1 || !active(Lc, b, lev)) return T(0);
-    const long e = (long)b * g.nw + idx + row * g.mm1;
-    return (s & 1) ? y[e] : x[e];
-  }
-  __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
-    const int row = gr * P + p;
-    if (row >= g.nm1 || !active(Lc, b, lev)) return;
-    const long e = (long)b * g.nw + (k - 1) + row * g.mm1;
-    tx[e] = ya;
-    ty[e] = yb;
-  }
-};
-template <typename T>
-struct PairColOp {
-  Geo g; const T* tx; const T* ty; T* ax; T* ay; long ays; T yscale; const int* Lc; int lev;
-  __device__ T load(int b, int gr, int s, int P, int idx) const {
-    const int col = gr * P + (s >> 1);  // even s: tx, odd s: ty
-    if (col >= g.mm1 || !active(Lc, b, lev)) return T(0);
-    const long e = (long)b * g.nw + col + idx * g.mm1;
-    return (s & 1) ? ty[e] : tx[e];
-  }
-  __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
-    const int col = gr * P + p;
-    if (col >= g.mm1 || !active(Lc, b, lev)) return;
-    const int e = col + (k - 1) * g.mm1;
-    ax[(long)b * g.nw + e] = ya;
-    T* d = ay + b * ays + e;
-    *d = *d + yscale * yb;
-  }
-};
+__global__ void k_step1_adj_incr(Geo g, const T* __restrict__ aws, long fbs, const T* __restrict__ aX,
+                                 T* __restrict__ awo, const T* __restrict__ vfac, const int* __restrict__ Lc,
+                                 int lev) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (e >= g.nw || !active(Lc, b, lev)) return;
+  const int ii = e % g.mm1, jj = e / g.mm1;
+  const T* x = aX + (long)b * g.nw;
+  const T lap = (ii > 0 ? x[e - 1] : T(0)) + (ii + 1 < g.mm1 ? x[e + 1] : T(0)) + (jj > 0 ? x[e - g.mm1] : T(0)) +
+                (jj + 1 < g.nm1 ? x[e + g.mm1] : T(0)) - T(4) * x[e];
+  T* d = awo + b * fbs + e;
+  *d = *d + aws[b * fbs + e] + T(2) * vfac[b * g.Lmax + lev] * lap;
+}
 
 // ------------------------------------------------------ nonlinear adjoint
 // Face adjoint of N = rot(fq): afq = curl(aN, 0) on the faces the forward

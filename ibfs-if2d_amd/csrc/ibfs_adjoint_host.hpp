@@ -41,7 +41,7 @@ struct AdjointRunner {
   hipStream_t st;
   int B, Lmax, nw, nq, nbc;
   long FB, QB, BB;
-  DevBuf<T> AW, AWO, ARO, AQ, LB, AN, AX, TK, ABC, ALB, ANEW, URR, U, T1, T2, T3, T4, T5;
+  DevBuf<T> AW, AWO, ARO, AQ, LB, AN, AX, TK, ABC, ALB, ANEW, URR, U, T1, T2;
   DevBuf<double> AFB, Z, Wv, Y, AU;
 
   explicit AdjointRunner(Engine<T>& e) : E(e), st(e.st) {
@@ -49,8 +49,7 @@ struct AdjointRunner {
     FB = (long)Lmax * nw; QB = (long)Lmax * nq; BB = (long)Lmax * nbc;
     const size_t W = (size_t)B * FB, Q = (size_t)B * QB, C = (size_t)B * BB, F = (size_t)B * nw;
     AW.alloc(W); AWO.alloc(W); ARO.alloc(W); AQ.alloc(Q); LB.alloc(C); ALB.alloc(C); ANEW.alloc(C);
-    AN.alloc(F); AX.alloc(F); TK.alloc(F); URR.alloc(F); U.alloc(F); T1.alloc(F); T2.alloc(F); T3.alloc(F);
-    T4.alloc(F); T5.alloc(F);
+    AN.alloc(F); AX.alloc(F); TK.alloc(F); URR.alloc(F); U.alloc(F); T1.alloc(F); T2.alloc(F);
     ABC.alloc((size_t)B * nbc);
     const size_t nf = std::max(E.nf_tot, 1);
     AFB.alloc(nf); Z.alloc(nf); Wv.alloc(nf); Y.alloc(nf); AU.alloc(nf);
@@ -120,11 +119,15 @@ struct AdjointRunner {
     // STEP 1 reverse, level by level (the forward ran coarse to fine)
     AWO.zero(st);
     for (int k = 0; k < Lmax; k++) {
+      // aX = S lam1i S aw*, then aw_old += aw* + 2 vfac D0 aX (k_step1_adj_incr)
       E.row_pass(SrcField<T>{AW.p + (long)k * nw, FB}, DstField<T>{T1.p, nw}, B, Lc, k);
-      E.col_pass(SrcField<T>{T1.p, nw}, DstCNSplit<T>{T2.p, T3.p, E.lam1i.p, E.lam1n.p, FB, k, Lmax}, B, Lc, k);
-      E.template launch<0>(PairRowOp<T>{g, T2.p, T3.p, T4.p, T5.p, Lc, k}, g.nm1, 1, B);
-      E.template launch<1>(PairColOp<T>{g, T4.p, T5.p, AX.p, AWO.p + (long)k * nw, FB, T(1) / E.norm, Lc, k},
-                           g.mm1, 1, B);
+      E.launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{g, SrcField<T>{T1.p, nw},
+                                                                   MidTab<T>{E.lam1i.p + (long)k * nw, (long)Lmax * nw},
+                                                                   DstField<T>{T2.p, nw}, Lc, k},
+                    g.mm1, 2, B, Engine<T>::kStep1Col2);
+      E.row_pass(SrcField<T>{T2.p, nw}, DstField<T>{AX.p, nw}, B, Lc, k);
+      hipLaunchKernelGGL(k_step1_adj_incr<T>, grid(nw), dim3(256), 0, st, g, AW.p + (long)k * nw, FB, AX.p,
+                         AWO.p + (long)k * nw, E.vfac.p, Lc, k);
       hipLaunchKernelGGL(k_step1_adj_combine<T>, grid(nw), dim3(256), 0, st, g, AX.p, ARO.p + (long)k * nw, FB,
                          AN.p, E.con1.p, E.con2.p, Lc, k, first);
       hipLaunchKernelGGL(k_nl_adj_vertex<T>, grid(nw), dim3(256), 0, st, g, AN.p, qold + (long)k * nq,

@@ -55,7 +55,7 @@ struct Engine : GroupBase {
 
   DevBuf<int> Lc;
   DevBuf<T> con1, con2, vfac;
-  DevBuf<T> laminv, lam1n, lam1i;
+  DevBuf<T> laminv, lam1i;
   DevBuf<T> omega, rhs_old, q, q0, S, T1, T2, T3, RR, lastbc;
 
   DevBuf<int> face_id, face_ptr, face_pt, frc_ptr, frc_face, frc_ia, frc_ib, vtx_id, vtx_face, bdyno;
@@ -100,7 +100,7 @@ struct Engine : GroupBase {
     // per-case scalars and eigenvalue tables (src/myfft.f90:42-69)
     std::vector<int> hL(B);
     std::vector<T> hc1((size_t)B * Lmax, T(0)), hc2(hc1), hvf(hc1);
-    std::vector<T> hlaminv(nw), hl1n((size_t)B * Lmax * nw, T(0)), hl1i(hl1n);
+    std::vector<T> hlaminv(nw), hl1i((size_t)B * Lmax * nw, T(0));
     std::vector<double> lam(nw);
     for (int j = 1; j <= n - 1; j++)
       for (int i = 1; i <= m - 1; i++) {
@@ -120,7 +120,6 @@ struct Engine : GroupBase {
         const double del22 = del2 * std::pow(4.0, k);
         const size_t base = ((size_t)b * Lmax + k) * nw;
         for (int e = 0; e < nw; e++) {
-          hl1n[base + e] = T(1.0 - 0.5 * c.d.dt * lam[e] / del22 / c.d.re);
           hl1i[base + e] = T(1.0 / (1.0 + 0.5 * c.d.dt * lam[e] / del22 / c.d.re));
         }
       }
@@ -130,7 +129,6 @@ struct Engine : GroupBase {
     upvec(con2, hc2, st);
     upvec(vfac, hvf, st);
     upvec(laminv, hlaminv, st);
-    upvec(lam1n, hl1n, st);
     upvec(lam1i, hl1i, st);
 
     // smearing tables, concatenated
