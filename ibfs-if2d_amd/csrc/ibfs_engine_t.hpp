@@ -44,14 +44,40 @@ void make_plan(int n2, FftPlan<T>& plan, DevBuf<cplx<T>>& tw, hipStream_t s) {
   plan.tw = tw.p;
 }
 
+// Plans and twiddles of both axes in element type V.
+template <typename V>
+struct FftSet {
+  FftPlan<V> pm{}, pn{};
+  DevBuf<cplx<V>> tw_m, tw_n, stw_m, stw_n;
+  bool spec_m = false, spec_n = false;
+  void init(int m, int n, hipStream_t st) {
+    make_plan<V>(2 * m, pm, tw_m, st);
+    make_plan<V>(2 * n, pn, tw_n, st);
+    spec_m = make_spec_twiddles<V>(2 * m, stw_m, st);
+    spec_n = make_spec_twiddles<V>(2 * n, stw_n, st);
+  }
+};
+
+// element type of a pass operator (what its load returns)
+template <class Op>
+using op_value_t = std::decay_t<decltype(std::declval<const Op&>().load(0, 0, 0, 1, 0))>;
+
 template <typename T>
 struct Engine : GroupBase {
   hipStream_t st = nullptr;
   Geo geo{};
   T norm = T(1);
-  FftPlan<T> pm{}, pn{};
-  DevBuf<cplx<T>> tw_m, tw_n, stw_m, stw_n;
-  bool spec_m = false, spec_n = false;
+  FftSet<T> fs;
+  // An FP32 engine solves the step-2 Poisson problems (the streamfunction
+  // whose curl is the right-hand side of the force system) in FP64: with
+  // FP32 transforms there, their rounding, of order eps*|psi|, reaches the
+  // slip at the body, and the force that removes it carries per-step noise
+  // of ~1e-4 of the force scale (mean forces are unaffected).  Step 4 solves
+  // the finest level in FP64 too, so that the stored fluxes there (the
+  // difference of nearby streamfunction values) keep FP32 relative accuracy.
+  static constexpr bool kSolve64 = sizeof(T) == 4;
+  FftSet<double> fs64;
+  DevBuf<double> laminv64, S64, U1, U2;
 
   DevBuf<int> Lc;
   DevBuf<T> con1, con2, vfac;
@@ -92,15 +118,14 @@ struct Engine : GroupBase {
     geo = Geo{m, n, Lmax, m - 1, n - 1, nw, nq, nbc, 0, n + 1, 2 * (n + 1), 2 * (n + 1) + m + 1};
     norm = T(cs[0].normalize);
     refine = sizeof(T) == 4;
-    make_plan<T>(2 * m, pm, tw_m, st);
-    make_plan<T>(2 * n, pn, tw_n, st);
-    spec_m = make_spec_twiddles<T>(2 * m, stw_m, st);
-    spec_n = make_spec_twiddles<T>(2 * n, stw_n, st);
+    fs.init(m, n, st);
+    if constexpr (kSolve64) fs64.init(m, n, st);
 
     // per-case scalars and eigenvalue tables (src/myfft.f90:42-69)
     std::vector<int> hL(B);
     std::vector<T> hc1((size_t)B * Lmax, T(0)), hc2(hc1), hvf(hc1);
     std::vector<T> hlaminv(nw), hl1i((size_t)B * Lmax * nw, T(0));
+    std::vector<double> hlaminv64(nw);
     std::vector<double> lam(nw);
     for (int j = 1; j <= n - 1; j++)
       for (int i = 1; i <= m - 1; i++) {
@@ -108,6 +133,7 @@ struct Engine : GroupBase {
         lam[e] = -2.0 * (std::cos(kPiRef * (double)(float)i / (double)(float)m) +
                          std::cos(kPiRef * (double)(float)j / (double)(float)n) - 2.0);
         hlaminv[e] = T(1.0 / lam[e] / cs[0].normalize);
+        hlaminv64[e] = 1.0 / lam[e] / cs[0].normalize;
       }
     for (int b = 0; b < B; b++) {
       const CaseHost& c = cs[b];
@@ -129,6 +155,7 @@ struct Engine : GroupBase {
     upvec(con2, hc2, st);
     upvec(vfac, hvf, st);
     upvec(laminv, hlaminv, st);
+    if constexpr (kSolve64) upvec(laminv64, hlaminv64, st);
     upvec(lam1i, hl1i, st);
 
     // smearing tables, concatenated
@@ -220,7 +247,13 @@ struct Engine : GroupBase {
       }
     }
     q0.upload(hq0.data(), Q, st);
-    T1.alloc((size_t)B * 2 * nw);
+    T1.alloc((size_t)B * nw);
+    if constexpr (kSolve64) {
+      S64.alloc(W);
+      S64.zero(st);
+      U1.alloc((size_t)B * nw);
+      U2.alloc((size_t)B * nw);
+    }
     T2.alloc((size_t)B * nw);
     T3.alloc((size_t)B * nw);
     RR.alloc((size_t)B * nw); RR.zero(st);
@@ -253,24 +286,38 @@ struct Engine : GroupBase {
   // group's batch size (tune()).  Results do not depend on P.
   enum Kind { kRow = 0, kCol = 1, kCol2 = 2, kStep1Row = 3, kStep1Col2 = 4, kKinds = 5 };
   int ptile[kKinds] = {1, 1, 1, 1, 1};
+  template <typename V = T>
   static constexpr bool p_fits(int n2, int P) {
-    return P * n2 / 4 <= 1024 && (long)P * n2 * (long)sizeof(cplx<T>) <= 65536;
+    return P * n2 / 4 <= 1024 && (long)P * n2 * (long)sizeof(cplx<V>) <= 65536;
+  }
+  // the transforms of element type V (the FP64 set of an FP32 engine serves
+  // its FP64 step-2 solves); passes of the other type use P = 1
+  template <typename V>
+  const FftSet<V>& fset() const {
+    if constexpr (std::is_same_v<V, T>) return fs;
+    else return fs64;
+  }
+  template <typename V>
+  int tile(int kind) const {
+    return std::is_same_v<V, T> ? ptile[kind] : 1;
   }
 
   template <int NN, int P, int AXIS, class Op>
-  void launch_spec(const Op& op, const cplx<T>* stw, int nseq, int mult, int batch) {
-    if constexpr (p_fits(NN, P)) {
+  void launch_spec(const Op& op, const cplx<op_value_t<Op>>* stw, int nseq, int mult, int batch) {
+    using V = op_value_t<Op>;
+    if constexpr (p_fits<V>(NN, P)) {
       const int groups = (nseq + mult * P - 1) / (mult * P);
-      hipLaunchKernelGGL((spec::dst_pass_n<T, NN, P, AXIS, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
+      hipLaunchKernelGGL((spec::dst_pass_n<V, NN, P, AXIS, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
                          stw);
     }
   }
   template <int NN, int P, class Op>
   void launch_spec_col2(const Op& op, int nseq, int mult, int batch) {
-    if constexpr (p_fits(NN, P)) {
+    using V = op_value_t<Op>;
+    if constexpr (p_fits<V>(NN, P)) {
       const int groups = (nseq + mult * P - 1) / (mult * P);
-      hipLaunchKernelGGL((spec::dst_col2_n<T, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
-                         stw_n.p);
+      hipLaunchKernelGGL((spec::dst_col2_n<V, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
+                         fset<V>().stw_n.p);
     }
   }
 
@@ -279,10 +326,12 @@ struct Engine : GroupBase {
   // two fields (X + iY).  kind selects the tile shape.
   template <int AXIS, class Op>
   void launch(const Op& op, int nseq, int mult, int batch, int kind = AXIS) {
-    const int n2 = AXIS == 0 ? pm.n2 : pn.n2;
-    const bool sp = AXIS == 0 ? spec_m : spec_n;
-    const cplx<T>* stw = AXIS == 0 ? stw_m.p : stw_n.p;
-    const int P = ptile[kind];
+    using V = op_value_t<Op>;
+    const FftSet<V>& F = fset<V>();
+    const int n2 = AXIS == 0 ? F.pm.n2 : F.pn.n2;
+    const bool sp = AXIS == 0 ? F.spec_m : F.spec_n;
+    const cplx<V>* stw = AXIS == 0 ? F.stw_m.p : F.stw_n.p;
+    const int P = tile<V>(kind);
     if (sp) {
       switch (n2 * 8 + P) {
 #define IBFS_SPEC_CASE(NN, PP)                                 \
@@ -301,19 +350,21 @@ struct Engine : GroupBase {
         default: break;
       }
     }
-    const FftPlan<T>& plan = AXIS == 0 ? pm : pn;
+    const FftPlan<V>& plan = AXIS == 0 ? F.pm : F.pn;
     const int Pg = kEmax / n2;
     const int groups = (nseq + mult * Pg - 1) / (mult * Pg);
-    hipLaunchKernelGGL((dst_pass<T, kEmax, AXIS, Op>), dim3(groups, batch), dim3(kThreads),
-                       kEmax * sizeof(cplx<T>), st, op, plan);
+    hipLaunchKernelGGL((dst_pass<V, kEmax, AXIS, Op>), dim3(groups, batch), dim3(kThreads),
+                       kEmax * sizeof(cplx<V>), st, op, plan);
   }
 
   // fused column pair (dst_col2) over nseq columns
   template <class Op>
   void launch_col2(const Op& op, int nseq, int mult, int batch, int kind = kCol2) {
-    const int n2 = pn.n2;
-    const int P = ptile[kind];
-    if (spec_n) {
+    using V = op_value_t<Op>;
+    const FftSet<V>& F = fset<V>();
+    const int n2 = F.pn.n2;
+    const int P = tile<V>(kind);
+    if (F.spec_n) {
       switch (n2 * 8 + P) {
 #define IBFS_SPEC_CASE(NN, PP)                           \
   case NN * 8 + PP:                                      \
@@ -333,8 +384,8 @@ struct Engine : GroupBase {
     }
     const int Pg = kEmax / n2;
     const int groups = (nseq + mult * Pg - 1) / (mult * Pg);
-    hipLaunchKernelGGL((dst_col2<T, kEmax, Op>), dim3(groups, batch), dim3(kThreads), kEmax * sizeof(cplx<T>), st,
-                       op, pn);
+    hipLaunchKernelGGL((dst_col2<V, kEmax, Op>), dim3(groups, batch), dim3(kThreads), kEmax * sizeof(cplx<V>), st,
+                       op, F.pn);
   }
 
   // ------------------------------------------------------------ autotuner
@@ -355,11 +406,11 @@ struct Engine : GroupBase {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     for (int k = 0; k < kKinds; k++) {
       const bool rowtype = k == kRow || k == kStep1Row;
-      const int n2 = rowtype ? pm.n2 : pn.n2;
-      if (!(rowtype ? spec_m : spec_n)) continue;  // generic pass: P fixed by its LDS budget
+      const int n2 = rowtype ? fs.pm.n2 : fs.pn.n2;
+      if (!(rowtype ? fs.spec_m : fs.spec_n)) continue;  // generic pass: P fixed by its LDS budget
       std::vector<int> cand;
       for (int P : {1, 2, 4})
-        if (p_fits(n2, P)) cand.push_back(P);
+        if (p_fits<T>(n2, P)) cand.push_back(P);
       if (fp > 0) {
         int best = cand[0];
         for (int P : cand)
@@ -473,15 +524,20 @@ struct Engine : GroupBase {
     row_pass(SrcField<T>{t2, nw}, dst, batch, gate, lev);
   }
 
+  // the in-place coarsify chain of vort2flux (src/operators.f90:224-226)
+  void coarsify(T* om, int batch, const int* gate) {
+    const int wcnt = (m / 2 - 1) * (n / 2 - 1);
+    for (int lev = 1; lev < Lmax; lev++)
+      hipLaunchKernelGGL(k_coarsify<T>, dim3((wcnt + 255) / 256, batch), dim3(256), 0, st, geo,
+                         om + (long)lev * nw, om + (long)(lev - 1) * nw, fbs(), gate, lev);
+  }
+
   // vort2flux (src/operators.f90:191-240) on a batch of multi-level fields om
   // (stride Lmax*nw); streamfunctions go to s (same layout); when vel is set,
   // curl(s) is written on every level (stride Lmax*nq).
   void vort2flux(T* om, T* s, T* vel, int batch, const int* gate, T* t1, T* t2) {
     const long bs = fbs();
-    const int wcnt = (m / 2 - 1) * (n / 2 - 1);
-    for (int lev = 1; lev < Lmax; lev++)
-      hipLaunchKernelGGL(k_coarsify<T>, dim3((wcnt + 255) / 256, batch), dim3(256), 0, st, geo,
-                         om + (long)lev * nw, om + (long)(lev - 1) * nw, bs, gate, lev);
+    coarsify(om, batch, gate);
     for (int lev = Lmax - 1; lev >= 0; lev--) {
       T* sc = (lev + 1 < Lmax) ? s + (long)(lev + 1) * nw : nullptr;
       ctci_like(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, false, DstField<T>{s + (long)lev * nw, bs}, t1,
@@ -490,6 +546,46 @@ struct Engine : GroupBase {
     if (vel)
       hipLaunchKernelGGL(k_curl_levels<T>, dim3((nq + 255) / 256, Lmax, batch), dim3(256), 0, st, geo, s, vel, bs,
                          qbs(), gate);
+  }
+
+  // ctci of level lev in FP64 from the FP32 vorticity om and the coarser
+  // level's streamfunction sc (element type TC), into S64 level lev: the
+  // three passes of ctci_like in FP64 with the FP64 laminv.
+  template <typename TC>
+  void ctci64(const T* om, const TC* sc, int batch, const int* gate, int lev) {
+    const long bs = fbs();
+    launch<0>(RowOp<double, SrcVortBC<T, double, TC>, DstField<double>>{
+                  geo, SrcVortBC<T, double, TC>{om + (long)lev * nw, sc, bs, gate, lev}, DstField<double>{U1.p, nw},
+                  gate, lev},
+              geo.nm1, 2, batch);
+    launch_col2(Col2Op<double, SrcField<double>, MidScale<double>, DstField<double>>{
+                    geo, SrcField<double>{U1.p, nw}, MidScale<double>{laminv64.p}, DstField<double>{U2.p, nw}, gate,
+                    lev},
+                geo.mm1, 2, batch);
+    launch<0>(RowOp<double, SrcField<double>, DstField<double>>{geo, SrcField<double>{U2.p, nw},
+                                                                DstField<double>{S64.p + (long)lev * nw, bs}, gate,
+                                                                lev},
+              geo.nm1, 2, batch);
+  }
+  // vort2flux streamfunctions of an FP32 engine with every level solved in
+  // FP64 into S64 (step 2: they feed the force system)
+  void vort2flux64(T* om, int batch, const int* gate) {
+    coarsify(om, batch, gate);
+    for (int lev = Lmax - 1; lev >= 0; lev--)
+      ctci64<double>(om, (lev + 1 < Lmax) ? S64.p + (long)(lev + 1) * nw : nullptr, batch, gate, lev);
+  }
+  // vort2flux streamfunctions of an FP32 engine with the coarse levels in
+  // FP32 into S and the finest level in FP64 into S64 (step 4: the stored
+  // fluxes of the finest level come from the FP64 solve)
+  void vort2flux_fine64(T* om, int batch, const int* gate) {
+    const long bs = fbs();
+    coarsify(om, batch, gate);
+    for (int lev = Lmax - 1; lev >= 1; lev--) {
+      T* sc = (lev + 1 < Lmax) ? S.p + (long)(lev + 1) * nw : nullptr;
+      ctci_like(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, false, DstField<T>{S.p + (long)lev * nw, bs},
+                T2.p, T3.p, batch, gate, lev);
+    }
+    ctci64<T>(om, Lmax > 1 ? S.p + nw : nullptr, batch, gate, 0);
   }
 
   // --------------------------------------------------------------- one step
@@ -534,30 +630,40 @@ struct Engine : GroupBase {
     }
     // STEP 2 (src/operators.f90:142-171); motion is prepared per call.  The
     // force kernel also writes the step-3 source RR = rot(reg(fb)).
-    vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
-    const T* S2 = Lmax > 1 ? S.p + nw : nullptr;
     const size_t lds = sizeof(double) * 2 * std::max(nf_max, 1);
     const dim3 gall(std::max(1, (nf_max + nvtx_max + kForceRows - 1) / kForceRows), B);
-    if (!refine) {
-      hipLaunchKernelGGL(k_force_solve<T>, gall, dim3(kForceThreads), lds, st, geo, sm, S.p, S2, FB, motion.p, Lc.p, Ainv.p,
-                         Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 0);
-    } else {  // one FP64 refinement sweep: fb += A^-1 (rhsf - A fb)
-      const dim3 gf(std::max(1, (nf_max + kForceRows - 1) / kForceRows), B);
-      hipLaunchKernelGGL(k_force_solve<T>, gf, dim3(kForceThreads), lds, st, geo, sm, S.p, S2, FB, motion.p, Lc.p, Ainv.p,
-                         Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 1);
-      hipLaunchKernelGGL(k_symv_residual, dim3((nf_max + 3) / 4, B), dim3(256), 0, st, sm, Asym.p, rhsf.p, fb.p,
-                         rtmp.p);
-      hipLaunchKernelGGL(k_force_solve<T>, gall, dim3(kForceThreads), lds, st, geo, sm, S.p, S2, FB, motion.p, Lc.p, Ainv.p,
-                         Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)rtmp.p, 2);
+    const auto force = [&](auto s1) {
+      using TS = std::remove_const_t<std::remove_pointer_t<decltype(s1)>>;
+      const TS* s2 = Lmax > 1 ? s1 + nw : nullptr;
+      if (!refine) {
+        hipLaunchKernelGGL((k_force_solve<T, TS>), gall, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
+                           Lc.p, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 0);
+      } else {  // one FP64 refinement sweep: fb += A^-1 (rhsf - A fb)
+        const dim3 gf(std::max(1, (nf_max + kForceRows - 1) / kForceRows), B);
+        hipLaunchKernelGGL((k_force_solve<T, TS>), gf, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
+                           Lc.p, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 1);
+        hipLaunchKernelGGL(k_symv_residual, dim3((nf_max + 3) / 4, B), dim3(256), 0, st, sm, Asym.p, rhsf.p, fb.p,
+                           rtmp.p);
+        hipLaunchKernelGGL((k_force_solve<T, TS>), gall, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
+                           Lc.p, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)rtmp.p, 2);
+      }
+    };
+    if constexpr (kSolve64) {
+      vort2flux64(omega.p, B, Lc.p);
+      force((const double*)S64.p);
+    } else {
+      vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
+      force((const T*)S.p);
     }
     // STEP 3 (src/operators.f90:174)
     ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, FB);
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
-    vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
+    if constexpr (kSolve64) vort2flux_fine64(omega.p, B, Lc.p);
+    else vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
     // fluxes and diagnostics (src/operators.f90:184-186)
     const int ncurl = (nq + 255) / 256 * Lmax;
     hipLaunchKernelGGL(k_tail<T>, dim3(ncurl + nchunk * Lmax, B), dim3(256), 0, st, geo, S.p, q.p, omega.p, FB, QB,
-                       partial.p, nchunk, Lc.p);
+                       partial.p, nchunk, Lc.p, kSolve64 ? (const double*)S64.p : nullptr);
     hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, sm, q.p, QB, motion.p, fb.p, wsum.p, bdyno.p,
                        partial.p, nchunk, Lc.p, diag.p, rowlen, counter.p, diag_cap);
   }

@@ -127,14 +127,20 @@ struct SrcField {
 
 // omega_k + apply_bc(get_bc(s_{k+1}, 1), 1): the vort of vort2flux
 // (src/operators.f90:230-234); no edge term on a case's coarsest level.
-template <typename T>
+// TS: element type of the result (FP64 for the FP64 solves of an FP32
+// engine, whose omega is FP32); TC: element type of the coarser level's
+// streamfunction sc.
+template <typename T, typename TS = T, typename TC = TS>
 struct SrcVortBC {
-  const T* om; const T* sc; long bs; const int* Lc; int lev;
-  __device__ T operator()(const Geo& g, int b, int ii, int jj) const {
-    T v = om[b * bs + ii + jj * g.mm1];
+  const T* om; const TC* sc; long bs; const int* Lc; int lev;
+  __device__ TS operator()(const Geo& g, int b, int ii, int jj) const {
+    TS v = TS(om[b * bs + ii + jj * g.mm1]);
     if (sc == nullptr || !has_coarser(Lc, b, lev)) return v;
     const int i = ii + 2, j = jj + 2;
-    if (i == 2 || i == g.m || j == 2 || j == g.n) v = applybc_add<T>(g, v, i, j, sc + b * bs, T(1), T(1));
+    if (i == 2 || i == g.m || j == 2 || j == g.n) {
+      if constexpr (std::is_same_v<TS, TC>) v = applybc_add<TS>(g, v, i, j, sc + b * bs, TS(1), TS(1));
+      else v = v + TS(applybc_add<TC>(g, TC(0), i, j, sc + b * bs, TC(1), TC(1)));
+    }
     return v;
   }
 };
@@ -328,12 +334,13 @@ __global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ 
 }
 
 // curl face value (src/operators.f90:243-288) for 0-based face f; the edge
-// vector is get_bc(sc, 1) (sc == nullptr: zero edge values).
-template <typename T>
-__device__ __forceinline__ T curl_face(const Geo& g, int f, const T* __restrict__ s, const T* __restrict__ sc) {
+// vector is get_bc(sc, 1) (sc == nullptr: zero edge values), sc of element
+// type TC.
+template <typename T, typename TC = T>
+__device__ __forceinline__ T curl_face(const Geo& g, int f, const T* __restrict__ s, const TC* __restrict__ sc) {
   const int m = g.m, n = g.n;
   auto X = [&](int i, int j) -> T { return s[wix(g, i, j)]; };
-  auto B = [&](int pos) -> T { return sc ? getbc_at<T>(g, sc, pos, T(1)) : T(0); };
+  auto B = [&](int pos) -> T { return sc ? T(getbc_at<TC>(g, sc, pos, TC(1))) : T(0); };
   const int nu = (m + 1) * n;
   if (f < nu) {
     const int i = f % (m + 1) + 1, j = f / (m + 1) + 1;
@@ -714,9 +721,9 @@ constexpr int kForceThreads = 512;
 constexpr int kRowsPerWave = 2;
 constexpr int kForceRows = kForceThreads / 64 * kRowsPerWave;
 
-template <typename T>
-__global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, const T* __restrict__ s1,
-                                                     const T* __restrict__ s2, long bs,
+template <typename T, typename TS = T>
+__global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, const TS* __restrict__ s1,
+                                                     const TS* __restrict__ s2, long bs,
                                                      const double* __restrict__ motion, const int* __restrict__ Lc,
                                                      const double* __restrict__ Ainv, const double* __restrict__ G,
                                                      double* __restrict__ fb, T* __restrict__ rr, long rbs,
@@ -731,12 +738,12 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
   if (row0 >= nrows) return;
   const int NT = blockDim.x;
   if (mode < 2) {
-    const T* s1b = s1 + b * bs;
-    const T* sc = (s2 != nullptr && Lc[b] > 1) ? s2 + b * bs : nullptr;
+    const TS* s1b = s1 + b * bs;
+    const TS* sc = (s2 != nullptr && Lc[b] > 1) ? s2 + b * bs : nullptr;
     for (int f0 = tid; f0 < nf; f0 += 2 * NT) {
       const int f1 = f0 + NT;
-      const double v0 = regT_curl_at<T>(g, sm, c, f0, s1b, sc, motion);
-      const double v1 = f1 < nf ? regT_curl_at<T>(g, sm, c, f1, s1b, sc, motion) : 0.0;
+      const double v0 = regT_curl_at<TS>(g, sm, c, f0, s1b, sc, motion);
+      const double v1 = f1 < nf ? regT_curl_at<TS>(g, sm, c, f1, s1b, sc, motion) : 0.0;
       vec[f0] = v0;
       if (f1 < nf) vec[f1] = v1;
       if (mode == 1 && blockIdx.x == 0) {
@@ -785,7 +792,8 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
 // ---------------------------------------------------------------------------
 // One launch after the last streamfunction of the step is final:
 //  * workgroups x < ncurl = ceil(nq/256)*Lmax: q = curl(s) on level
-//    x / ceil(nq/256) (src/operators.f90:236-238, as k_curl_levels);
+//    x / ceil(nq/256) (src/operators.f90:236-238, as k_curl_levels); with
+//    s1d set, the finest level's streamfunction is s1d (FP64);
 //  * the next nchunk*Lmax: partial sums of omega for write_circ
 //    (src/operators.f90:675-684), chunk (x-ncurl) % nchunk of level
 //    (x-ncurl) / nchunk, at partial[((b*Lmax)+lev)*nchunk + chunk].
@@ -793,7 +801,8 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
 template <typename T>
 __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T* __restrict__ q,
                                               const T* __restrict__ om, long bs, long qbs,
-                                              double* __restrict__ partial, int nchunk, const int* __restrict__ Lc) {
+                                              double* __restrict__ partial, int nchunk, const int* __restrict__ Lc,
+                                              const double* __restrict__ s1d) {
   __shared__ double red[4];
   const int tid = threadIdx.x, b = blockIdx.y;
   const int ncb = (g.nq + 255) / 256, ncurl = ncb * g.Lmax;
@@ -802,7 +811,11 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
     const int lev = x / ncb, f = (x - lev * ncb) * 256 + tid;
     if (f >= g.nq || !active(Lc, b, lev)) return;
     const T* sl = s + b * bs + (long)lev * g.nw;
-    q[b * qbs + (long)lev * g.nq + f] = curl_face<T>(g, f, sl, has_coarser(Lc, b, lev) ? sl + g.nw : nullptr);
+    const T* sc = has_coarser(Lc, b, lev) ? sl + g.nw : nullptr;
+    if (lev == 0 && s1d != nullptr)  // finest level solved in FP64
+      q[b * qbs + f] = T(curl_face<double, T>(g, f, s1d + b * bs, sc));
+    else
+      q[b * qbs + (long)lev * g.nq + f] = curl_face<T>(g, f, sl, sc);
     return;
   }
   const int xc = x - ncurl, lev = xc / nchunk, ch = xc - lev * nchunk;
