@@ -72,6 +72,38 @@ def cpu_baseline(case, cm, cv, budget_s=15.0):
     return n / el, n
 
 
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense matrix peaks
+
+
+def dense_vs_fft(hip, cases, sizes=(128, 200, 256, 320, 384, 512)):
+    """One DST-I pass over an (m-1)^2 field, single case: the FFT pass (the
+    tuned shape the step uses) against the dense product with the DST-I
+    matrix on the matrix cores (best of ks = 1, 2, 4, 8 waves per output
+    tile), along i and along j, in FP64 and FP32 (the mixed form's
+    element type).  flops of the dense pass: 2 (m-1)^3."""
+    out = {}
+    for prec, name in ((hip.PLAIN_FP64, "fp64"), (hip.MIXED, "fp32")):
+        rows = {}
+        for m in sizes:
+            s = hip.Solver(**cases.cylinder_grid(m), precision=prec)
+            r = {}
+            for axis, ax in ((0, "row"), (1, "col")):
+                fft = s.time_dst_pass(axis, 200) * 1e3
+                best = min((s.time_dst_gemm(axis, ks, 200) * 1e3, ks) for ks in (1, 2, 4, 8))
+                flops = 2.0 * (m - 1) ** 3
+                r[ax] = {"fft_us": fft, "mfma_us": best[0], "mfma_ks": best[1],
+                         "mfma_tflops": flops / (best[0] * 1e-6) / 1e12,
+                         "mfma_util": flops / (best[0] * 1e-6) / 1e12 / MFMA_PEAK_TFLOPS[name],
+                         "faster": "mfma" if best[0] < fft else "fft"}
+            rows[str(m)] = r
+            log(f"dense vs FFT {name} m={m}: row FFT {r['row']['fft_us']:.2f} us / MFMA {r['row']['mfma_us']:.2f} us "
+                f"(ks {r['row']['mfma_ks']}, {100 * r['row']['mfma_util']:.1f}% of peak); col FFT "
+                f"{r['col']['fft_us']:.2f} / MFMA {r['col']['mfma_us']:.2f} us")
+            del s
+        out[name] = rows
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +120,8 @@ def main(argv=None):
     ap.add_argument("--config5", type=int, default=1, help="run the batched 1024^2 x 6 tandem adjoint leg (0: skip)")
     ap.add_argument("--config5-spin", type=int, default=2000)
     ap.add_argument("--config5-window", type=int, default=400)
+    ap.add_argument("--dense-sweep", type=int, default=1, help="time the dense matrix-core DST-I passes against "
+                    "the FFT passes at every production size (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent cases shard "
@@ -130,6 +164,10 @@ def main(argv=None):
     t_copy_big = g.time_copy(n_big, 20)
     copy_gbs = 2 * 8 * n_big / (t_copy_big * 1e-3) / 1e9
     log(f"copy of W doubles {t_copy_w * 1e3:.2f} us; 256 MiB copy {copy_gbs:.0f} GB/s")
+
+    dense = None
+    if args.dense_sweep:
+        dense = dense_vs_fft(hip, cases)
 
     chosen = None
     if not args.no_chosen:
@@ -321,6 +359,7 @@ def main(argv=None):
         "ensemble64": ensemble,
         "tandem1024_adjoint_batch8": config5,
         "cpu_baseline": cpu,
+        "dst_dense_mfma_vs_fft": dense,
     }
     print(json.dumps(out))
     return 0

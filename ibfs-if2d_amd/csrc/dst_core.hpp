@@ -39,7 +39,30 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace ibfs {
+
+// Optional operator hooks used by the size-specialised passes to issue
+// every global load of a workgroup before its first FFT stage:
+//   mid_coef(b, g, p, P, k, ca, cb) / mid_apply(..., ya, yb, ca, cb, va, vb):
+//       mid split into the table reads and the arithmetic;
+//   kStorePre, store_pre(b, g, p, P, k, oa, ob) / store_put(..., ya, yb, oa, ob):
+//       a read-modify-write store split into the read and the write.
+template <class Op, class = void>
+struct op_has_mid_coef : std::false_type {};
+template <class Op>
+struct op_has_mid_coef<Op, std::void_t<decltype(&Op::mid_coef)>> : std::true_type {};
+template <class Op, class = void>
+struct op_store_pre : std::false_type {};
+template <class Op>
+struct op_store_pre<Op, std::void_t<decltype(Op::kStorePre)>> : std::bool_constant<Op::kStorePre> {};
+//   kLoadPost, load_aux(b, g, s, P, idx, aux) / load_post(..., aux):
+//       a loader with a side store split into the loads and the store.
+template <class Op, class = void>
+struct op_load_post : std::false_type {};
+template <class Op>
+struct op_load_post<Op, std::void_t<decltype(Op::kLoadPost)>> : std::bool_constant<Op::kLoadPost> {};
 
 constexpr int kThreads = 256;
 constexpr int kMaxStages = 16;
@@ -274,10 +297,11 @@ __global__ __launch_bounds__(kThreads) void dst_col2(Op op, FftPlan<T> plan) {
 // TPF = N2/4 threads (one radix-4 butterfly per thread per stage); a
 // workgroup holds P FFTs.  Twiddles come from a per-stage table laid out
 // [stage][k][r-1] so that a butterfly reads its R-1 factors contiguously.
-// For the column pass the workgroup order is made XCD-aware: consecutive
-// workgroups are dispatched round-robin over the 8 XCDs, so group g is
-// remapped to give every XCD one contiguous range of columns and the column
-// segments shared by neighbouring groups stay in that XCD's L2.
+// The workgroup order is made XCD-aware: consecutive workgroups are
+// dispatched round-robin over the 8 XCDs, so group g is remapped to give
+// every XCD one contiguous range of sequences.  Column segments shared by
+// neighbouring column groups, and the neighbouring rows that the stencil
+// loaders of row passes read, then stay in that XCD's L2.
 // ---------------------------------------------------------------------------
 namespace spec {
 
@@ -400,14 +424,16 @@ template <typename T, int N2, int P, int AXIS, class Op>
 __device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g, int t) {
   constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
   constexpr int TOT = 2 * P * LEN, NL = (TOT + NT - 1) / NT;
-  T x[NL];
+  constexpr bool kPost = op_load_post<Op>::value;
+  T x[NL], aux[kPost ? NL : 1];
 #pragma unroll
   for (int q = 0; q < NL; q++) {
     const int e = t + q * NT;
     int s = 0, idx = 0;
     if (AXIS == 0) { s = e / LEN; idx = e - s * LEN; }
     else { idx = e / (2 * P); s = e - idx * (2 * P); }
-    x[q] = (NL * NT == TOT || e < TOT) ? op.load(b, g, s, P, idx) : T(0);
+    if constexpr (kPost) x[q] = (NL * NT == TOT || e < TOT) ? op.load_aux(b, g, s, P, idx, aux[q]) : T(0);
+    else x[q] = (NL * NT == TOT || e < TOT) ? op.load(b, g, s, P, idx) : T(0);
   }
 #pragma unroll
   for (int q = 0; q < NL; q++) {
@@ -420,6 +446,7 @@ __device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g
       const int h = s & 1;
       reinterpret_cast<T*>(z + p * N2 + idx + 1)[h] = x[q];
       reinterpret_cast<T*>(z + p * N2 + N2 - idx - 1)[h] = -x[q];
+      if constexpr (kPost) op.load_post(b, g, s, P, idx, aux[q]);
     }
   }
 }
@@ -427,11 +454,31 @@ __device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g
 template <typename T, int N2, int P, int AXIS, class Op>
 __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* __restrict__ tw) {
   constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
+  constexpr int NQ = (P * LEN + NT - 1) / NT;  // outputs (FFT, k) per thread
+  constexpr bool kPre = op_store_pre<Op>::value;
   __shared__ cplx<T> z[P * N2];
   const int t = threadIdx.x;
   const int b = blockIdx.y;
   int g = blockIdx.x;
-  if (AXIS == 1 && (gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  if ((gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  auto out_at = [&](int e, int& p, int& k) {
+    if (AXIS == 0) { p = e / LEN; k = e - p * LEN + 1; }
+    else { k = e / P + 1; p = e - (k - 1) * P; }
+  };
+  // old values of a read-modify-write destination, read with the inputs
+  T oa[kPre ? NQ : 1], ob[kPre ? NQ : 1];
+  if constexpr (kPre) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int e = t + q * NT;
+      oa[q] = ob[q] = T(0);
+      if (e < P * LEN) {
+        int p, k;
+        out_at(e, p, k);
+        op.store_pre(b, g, p, P, k, oa[q], ob[q]);
+      }
+    }
+  }
   cplx<T> wr[TwLayout<N2>::total()];
   tw_load<T, N2, 0>(wr, tw, t % (N2 / 4));
   if (t < P) {
@@ -441,13 +488,16 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
   load_seqs<T, N2, P, AXIS, Op>(op, z, b, g, t);
   __syncthreads();
   run<T, N2, P, 0>(z, wr, t);
-#pragma unroll 4
-  for (int e = t; e < P * LEN; e += NT) {
-    int p, k;
-    if (AXIS == 0) { p = e / LEN; k = e - p * LEN + 1; }
-    else { k = e / P + 1; p = e - (k - 1) * P; }
-    const cplx<T> Z = z[p * N2 + k];
-    op.store(b, g, p, P, k, -Z.im, Z.re);
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int e = t + q * NT;
+    if (e < P * LEN) {
+      int p, k;
+      out_at(e, p, k);
+      const cplx<T> Z = z[p * N2 + k];
+      if constexpr (kPre) op.store_put(b, g, p, P, k, -Z.im, Z.re, oa[q], ob[q]);
+      else op.store(b, g, p, P, k, -Z.im, Z.re);
+    }
   }
 }
 
@@ -455,11 +505,26 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
 template <typename T, int N2, int P, class Op>
 __global__ __launch_bounds__(P * N2 / 4) void dst_col2_n(Op op, const cplx<T>* __restrict__ tw) {
   constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
+  constexpr int NQ = (P * LEN + NT - 1) / NT;  // (FFT, k) places per thread
+  constexpr bool kCoef = op_has_mid_coef<Op>::value;
   __shared__ cplx<T> z[P * N2];
   const int t = threadIdx.x;
   const int b = blockIdx.y;
   int g = blockIdx.x;
   if ((gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  // the spectral table entries of this thread's places, read with the inputs
+  T ca[kCoef ? NQ : 1], cb[kCoef ? NQ : 1];
+  if constexpr (kCoef) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int e = t + q * NT;
+      ca[q] = cb[q] = T(0);
+      if (e < P * LEN) {
+        const int k = e / P + 1, p = e - (k - 1) * P;
+        op.mid_coef(b, g, p, P, k, ca[q], cb[q]);
+      }
+    }
+  }
   cplx<T> wr[TwLayout<N2>::total()];
   tw_load<T, N2, 0>(wr, tw, t % (N2 / 4));
   if (t < P) {
@@ -469,13 +534,36 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_col2_n(Op op, const cplx<T>* _
   load_seqs<T, N2, P, 1, Op>(op, z, b, g, t);
   __syncthreads();
   run<T, N2, P, 0>(z, wr, t);
-  col2_mid<T, Op>(op, z, b, g, P, N2, t, NT);
+  if constexpr (kCoef) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int e = t + q * NT;
+      if (e < P * LEN) {
+        const int k = e / P + 1, p = e - (k - 1) * P;
+        const cplx<T> Z = z[p * N2 + k];
+        T va, vb;
+        op.mid_apply(b, g, p, P, k, -Z.im, Z.re, ca[q], cb[q], va, vb);
+        z[p * N2 + k] = {va, vb};
+        z[p * N2 + N2 - k] = {-va, -vb};
+      }
+    }
+    if (t < P) {
+      z[t * N2] = {T(0), T(0)};
+      z[t * N2 + N] = {T(0), T(0)};
+    }
+    __syncthreads();
+  } else {
+    col2_mid<T, Op>(op, z, b, g, P, N2, t, NT);
+  }
   run<T, N2, P, 0>(z, wr, t);
-#pragma unroll 4
-  for (int e = t; e < P * LEN; e += NT) {
-    const int k = e / P + 1, p = e - (k - 1) * P;
-    const cplx<T> Z = z[p * N2 + k];
-    op.store(b, g, p, P, k, -Z.im, Z.re);
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int e = t + q * NT;
+    if (e < P * LEN) {
+      const int k = e / P + 1, p = e - (k - 1) * P;
+      const cplx<T> Z = z[p * N2 + k];
+      op.store(b, g, p, P, k, -Z.im, Z.re);
+    }
   }
 }
 

@@ -8,6 +8,7 @@
 #include <cstdlib>
 
 #include "../../include/ibfs.h"
+#include "dst_gemm.hpp"
 #include "ibfs_adjoint_host.hpp"
 #include "ibfs_engine_t.hpp"
 
@@ -264,6 +265,7 @@ int ibfs_case_set_state(ibfs_handle* h, int c, const double* omega, const double
   return guarded(h, [&] {
     if (c < 0 || c >= h->ncases) throw ArgError{IBFS_EINVAL, "case index out of range"};
     h->gr(c).set_state(h->where[c].second, omega, rhs_old, q, q0, fb);
+    if (q0) h->drop_graph();  // the step's kernels depend on the form of q0 (freestream or general)
   });
 }
 int ibfs_case_get_state(ibfs_handle* h, int c, double* omega, double* rhs_old, double* q, double* q0, double* fb) {
@@ -612,30 +614,121 @@ int ibfs_tile_shapes(const ibfs_handle* h, int group, int* ngroups, int* shapes)
   return IBFS_OK;
 }
 
+extern "C++" {
+namespace {
+// mean duration (ms) of reps launches of f on the handle's stream
+template <class F>
+double time_launches(ibfs_handle* h, int reps, F&& f) {
+  f();
+  HIPCHK(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < reps; r++) f();
+  HIPCHK(hipEventRecord(h->ev1, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  return ms / reps;
+}
+
+// one FFT pass over a zero W field of group 0, in the engine's element type
+template <typename T>
+double time_fft_pass(ibfs_handle* h, Engine<T>* E, int axis, int reps) {
+  DevBuf<T> a, b;
+  a.alloc(E->nw); b.alloc(E->nw);
+  a.zero(h->stream);
+  return time_launches(h, reps, [&] {
+    if (axis == 0) E->row_pass(SrcField<T>{a.p, 0}, DstField<T>{b.p, 0}, 1);
+    else if (axis == 1) E->col_pass(SrcField<T>{a.p, 0}, DstField<T>{b.p, 0}, 1);
+    else
+      E->launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{E->geo, SrcField<T>{a.p, 0},
+                                                                        MidScale<T>{E->laminv.p}, DstField<T>{b.p, 0},
+                                                                        nullptr, 0},
+                     E->geo.mm1, 2, 1);
+  });
+}
+
+// DST-I matrix S(a, b) = 2 sin(pi (a+1)(b+1) / N), a, b < N-1, with the
+// argument reduced exactly modulo 2N
+template <typename T>
+std::vector<T> dst_matrix(int N) {
+  const int L = N - 1;
+  std::vector<T> S((size_t)L * L);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int a = 0; a < L; a++)
+    for (int b = 0; b < L; b++) {
+      const long r = ((long)(a + 1) * (b + 1)) % (2L * N);
+      S[(size_t)a * L + b] = T(2.0L * sinl(pi * (long double)r / (long double)N));
+    }
+  return S;
+}
+
+// a dense DST-I pass (Y = X S_m along i, Y = S_n X along j) of src into dst
+template <typename T>
+void gemm_pass(hipStream_t st, const Geo& g, int axis, int ks, const T* S, const T* src, T* dst) {
+  const int M = g.nm1, N = g.mm1, K = axis == 0 ? g.mm1 : g.nm1;
+  const dim3 grid((N + 15) / 16, (M + 15) / 16, 1);
+  const T* A = axis == 0 ? src : S;
+  const T* B = axis == 0 ? S : src;
+  const int lda = axis == 0 ? g.mm1 : g.nm1, ldb = axis == 0 ? g.mm1 : g.mm1;
+  switch (ks) {
+    case 1: hipLaunchKernelGGL((k_gemm16<T, 1>), grid, dim3(64), 0, st, A, lda, 0L, B, ldb, 0L, dst, g.mm1, 0L, M, N, K); break;
+    case 2: hipLaunchKernelGGL((k_gemm16<T, 2>), grid, dim3(128), 0, st, A, lda, 0L, B, ldb, 0L, dst, g.mm1, 0L, M, N, K); break;
+    case 4: hipLaunchKernelGGL((k_gemm16<T, 4>), grid, dim3(256), 0, st, A, lda, 0L, B, ldb, 0L, dst, g.mm1, 0L, M, N, K); break;
+    default: hipLaunchKernelGGL((k_gemm16<T, 8>), grid, dim3(512), 0, st, A, lda, 0L, B, ldb, 0L, dst, g.mm1, 0L, M, N, K); break;
+  }
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+double time_gemm_pass(ibfs_handle* h, const Geo& g, int axis, int ks, int reps) {
+  const int L = axis == 0 ? g.mm1 : g.nm1;
+  const std::vector<T> hs = dst_matrix<T>(L + 1);
+  DevBuf<T> S, a, b;
+  S.alloc(hs.size());
+  S.upload(hs.data(), hs.size(), h->stream);
+  a.alloc(g.nw); b.alloc(g.nw);
+  a.zero(h->stream);
+  return time_launches(h, reps, [&] { gemm_pass<T>(h->stream, g, axis, ks, S.p, a.p, b.p); });
+}
+}  // namespace
+}  // extern "C++"
+
 int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch) {
   if (!h || !ms_per_launch || reps < 1 || axis < 0 || axis > 2) return IBFS_EINVAL;
   return guarded(h, [&] {
+    if (auto* E = dynamic_cast<Engine<double>*>(h->groups[0].get())) *ms_per_launch = time_fft_pass(h, E, axis, reps);
+    else if (auto* F = dynamic_cast<Engine<float>*>(h->groups[0].get()))
+      *ms_per_launch = time_fft_pass(h, F, axis, reps);
+    else throw ArgError{IBFS_EINVAL, "no engine"};
+  });
+}
+
+int ibfs_time_dst_gemm(ibfs_handle* h, int axis, int ks, int reps, double* ms_per_launch) {
+  if (!h || !ms_per_launch || reps < 1 || axis < 0 || axis > 1 || (ks != 1 && ks != 2 && ks != 4 && ks != 8))
+    return IBFS_EINVAL;
+  return guarded(h, [&] {
+    if (auto* E = dynamic_cast<Engine<double>*>(h->groups[0].get()))
+      *ms_per_launch = time_gemm_pass<double>(h, E->geo, axis, ks, reps);
+    else if (auto* F = dynamic_cast<Engine<float>*>(h->groups[0].get()))
+      *ms_per_launch = time_gemm_pass<float>(h, F->geo, axis, ks, reps);
+    else throw ArgError{IBFS_EINVAL, "no engine"};
+  });
+}
+
+int ibfs_op_dst_gemm(ibfs_handle* h, int ks, const double* in, double* out) {
+  if (ks != 1 && ks != 2 && ks != 4 && ks != 8) return IBFS_EINVAL;
+  return guarded(h, [&] {
     NEED64();
-    DevBuf<double> a, b;
+    const std::vector<double> hm = dst_matrix<double>(E->geo.m), hn = dst_matrix<double>(E->geo.n);
+    DevBuf<double> Sm, Sn, a, b;
+    Sm.alloc(hm.size()); Sn.alloc(hn.size());
+    Sm.upload(hm.data(), hm.size(), st);
+    Sn.upload(hn.data(), hn.size(), st);
     a.alloc(nw); b.alloc(nw);
-    a.zero(st);
-    auto launch = [&] {
-      if (axis == 0) E->row_pass(SrcField<double>{a.p, 0}, DstField<double>{b.p, 0}, 1);
-      else if (axis == 1) E->col_pass(SrcField<double>{a.p, 0}, DstField<double>{b.p, 0}, 1);
-      else
-        E->launch_col2(Col2Op<double, SrcField<double>, MidScale<double>, DstField<double>>{
-                           E->geo, SrcField<double>{a.p, 0}, MidScale<double>{E->laminv.p}, DstField<double>{b.p, 0},
-                           nullptr, 0},
-                       E->geo.mm1, 2, 1);
-    };
-    launch();
-    HIPCHK(hipEventRecord(h->ev0, st));
-    for (int r = 0; r < reps; r++) launch();
-    HIPCHK(hipEventRecord(h->ev1, st));
+    a.upload(in, nw, st);
+    gemm_pass<double>(st, E->geo, 0, ks, Sm.p, a.p, b.p);
+    gemm_pass<double>(st, E->geo, 1, ks, Sn.p, b.p, a.p);
+    a.download(out, nw, st);
     HIPCHK(hipStreamSynchronize(st));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    *ms_per_launch = ms / reps;
   });
 }
 

@@ -83,6 +83,17 @@ struct Engine : GroupBase {
   DevBuf<T> con1, con2, vfac;
   DevBuf<T> laminv, lam1i;
   DevBuf<T> omega, rhs_old, q, q0, S, T1, T2, T3, RR, lastbc;
+  // q0 of every case in the freestream form (src/variables.f90:59-63: one
+  // value on the u faces of a level, 0 on its v faces)?  Then the step-1
+  // loader takes the per-(case, level) value from q0c instead of loading q0.
+  DevBuf<T> q0c;
+  std::vector<T> hq0c;
+  std::vector<char> q0u;
+  bool q0_uniform() const {
+    for (char u : q0u)
+      if (!u) return false;
+    return true;
+  }
 
   DevBuf<int> face_id, face_ptr, face_pt, frc_ptr, frc_face, frc_ia, frc_ib, vtx_id, vtx_face, bdyno;
   DevBuf<double> face_w, frc_w, wsum;
@@ -247,6 +258,13 @@ struct Engine : GroupBase {
       }
     }
     q0.upload(hq0.data(), Q, st);
+    // the freestream form of q0, per case and level (u faces, v faces 0)
+    hq0c.assign((size_t)B * Lmax, T(0));
+    q0u.assign(B, 1);
+    for (int b = 0; b < B; b++)
+      for (int k = 0; k < Lmax; k++) hq0c[(size_t)b * Lmax + k] = hq0[((size_t)b * Lmax + k) * nq];
+    q0c.alloc(hq0c.size());
+    q0c.upload(hq0c.data(), hq0c.size(), st);
     T1.alloc((size_t)B * nw);
     if constexpr (kSolve64) {
       S64.alloc(W);
@@ -453,12 +471,13 @@ struct Engine : GroupBase {
                           geo.mm1, 2, B);
               break;
             case kStep1Row:
-              launch<0>(RowOp<T, SrcStep1<T>, DstField<T>>{geo,
-                                                           SrcStep1<T>{T2.p, nullptr, qz.p, qz.p, bz.p, T3.p, con1.p,
-                                                                       con2.p, vfac.p, Lc.p, (long)nw, (long)nq,
-                                                                       (long)nbc, 0, 0},
-                                                           DstField<T>{T1.p, nw}, Lc.p, 0},
-                        geo.nm1, 2, B, kStep1Row);
+              {  // the freestream form, as the step launches it
+                SrcStep1<T, true> s1f{T2.p, nullptr, qz.p, nullptr, bz.p, T3.p, con1.p, con2.p, vfac.p, Lc.p,
+                                      (long)nw, (long)nq, (long)nbc, 0, 0};
+                s1f.q0c = q0c.p;
+                launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geo, s1f, DstField<T>{T1.p, nw}, Lc.p, 0},
+                          geo.nm1, 2, B, kStep1Row);
+              }
               break;
             default:
               launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{geo, SrcField<T>{T1.p, nw},
@@ -525,11 +544,14 @@ struct Engine : GroupBase {
   }
 
   // the in-place coarsify chain of vort2flux (src/operators.f90:224-226)
+  // (two levels per launch, k_coarsify2)
   void coarsify(T* om, int batch, const int* gate) {
-    const int wcnt = (m / 2 - 1) * (n / 2 - 1);
-    for (int lev = 1; lev < Lmax; lev++)
-      hipLaunchKernelGGL(k_coarsify<T>, dim3((wcnt + 255) / 256, batch), dim3(256), 0, st, geo,
-                         om + (long)lev * nw, om + (long)(lev - 1) * nw, fbs(), gate, lev);
+    const int wcnt = (m / 2 - 1) * (n / 2 - 1), nblk = (wcnt + 255) / 256;
+    for (int lev = 1; lev < Lmax; lev += 2) {
+      const int nlev = lev + 1 < Lmax ? 2 : 1;
+      hipLaunchKernelGGL(k_coarsify2<T>, dim3(nblk * nlev, batch), dim3(256), 0, st, geo, om, fbs(), gate, lev,
+                         nblk);
+    }
   }
 
   // vort2flux (src/operators.f90:191-240) on a batch of multi-level fields om
@@ -620,8 +642,16 @@ struct Engine : GroupBase {
         s1.bq0 = base->q0.p + (long)lev * nq;
         s1.blb = base->lastbc.p + (long)lev * nbc;
       }
-      launch<0>(RowOp<T, SrcStep1<T>, DstField<T>>{geo, s1, DstField<T>{T1.p, nw}, Lc.p, lev}, geo.nm1, 2, B,
-                kStep1Row);
+      if (!base && q0_uniform()) {  // freestream q0: the loader reads no q0
+        SrcStep1<T, true> s1f{s1.om, s1.omc, s1.q, nullptr, s1.lastbc, s1.rhs_old, s1.con1, s1.con2, s1.vfac,
+                              s1.Lc, s1.fbs, s1.qbs, s1.bbs, s1.lev, s1.first};
+        s1f.q0c = q0c.p;
+        launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geo, s1f, DstField<T>{T1.p, nw}, Lc.p, lev}, geo.nm1, 2,
+                  B, kStep1Row);
+      } else {
+        launch<0>(RowOp<T, SrcStep1<T>, DstField<T>>{geo, s1, DstField<T>{T1.p, nw}, Lc.p, lev}, geo.nm1, 2, B,
+                  kStep1Row);
+      }
       launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{geo, SrcField<T>{T1.p, nw},
                                                                  MidTab<T>{lam1i.p + (long)lev * nw, (long)Lmax * nw},
                                                                  DstField<T>{T2.p, nw}, Lc.p, lev},
@@ -727,6 +757,20 @@ struct Engine : GroupBase {
     put(rhs_old, (size_t)c * fbs(), ro, W);
     put(q, (size_t)c * qbs(), qq, Q);
     put(q0, (size_t)c * qbs(), qq0, Q);
+    if (qq0) {  // is the new q0 in the freestream form (compared in the engine's element type)?
+      const size_t nu = (size_t)(m + 1) * n;
+      bool uni = true;
+      for (int k = 0; k < cs[c].L && uni; k++) {
+        const double* lv = qq0 + (size_t)k * nq;
+        const T u0 = T(lv[0]);
+        for (size_t e = 0; e < nu && uni; e++) uni = T(lv[e]) == u0;
+        for (size_t e = nu; e < (size_t)nq && uni; e++) uni = T(lv[e]) == T(0);
+        hq0c[(size_t)c * Lmax + k] = u0;
+      }
+      q0u[c] = uni;
+      q0c.upload(hq0c.data(), hq0c.size(), st);
+      HIPCHK(hipStreamSynchronize(st));
+    }
     if (f && cs[c].nf) {
       fb.upload(f, cs[c].nf, st, hcs[c].foff);
       HIPCHK(hipStreamSynchronize(st));

@@ -18,6 +18,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dst_core.hpp"
 
 namespace ibfs {
@@ -84,10 +86,13 @@ __device__ __forceinline__ T applybc_vec(const Geo& g, T acc, int i, int j, cons
 
 // rot(nonlinear(omega, q, q0, bc)) at vertex (i,j):
 // nonlinear src/operators.f90:465-510, rot src/operators.f90:291-304.
-template <typename T>
+// kFree: the freestream form of q0 (src/variables.f90:59-63), u0 on every u
+// face and 0 on every v face of the level (q0 is not read), with the same
+// additions as the array form, so the same result.
+template <typename T, bool kFree = false>
 __device__ __forceinline__ T rot_nonlinear(const Geo& g, int i, int j, const T* __restrict__ om,
                                            const T* __restrict__ q, const T* __restrict__ q0,
-                                           const T* __restrict__ bc) {
+                                           const T* __restrict__ bc, T u0 = T(0)) {
   const int m = g.m, n = g.n;
   auto OmU = [&](int ii, int jj) -> T {  // along j at column ii (2..m)
     if (jj == 1) return bc[g.bottom + ii - 1];
@@ -101,11 +106,13 @@ __device__ __forceinline__ T rot_nonlinear(const Geo& g, int i, int j, const T* 
   };
   auto vavg = [&](int ii, int jj) -> T {
     const int a = vix(g, ii, jj), b = vix(g, ii - 1, jj);
-    return T(0.5) * (q[a] + q[b] + q0[a] + q0[b]);
+    if constexpr (kFree) return T(0.5) * (q[a] + q[b] + T(0) + T(0));
+    else return T(0.5) * (q[a] + q[b] + q0[a] + q0[b]);
   };
   auto uavg = [&](int ii, int jj) -> T {
     const int a = uix(g, ii, jj), b = uix(g, ii, jj - 1);
-    return T(0.5) * (q[a] + q[b] + q0[a] + q0[b]);
+    if constexpr (kFree) return T(0.5) * (q[a] + q[b] + u0 + u0);
+    else return T(0.5) * (q[a] + q[b] + q0[a] + q0[b]);
   };
   auto fqu = [&](int ii, int jj) -> T {  // fq(u(ii,jj)), ii = 2..m, jj = 1..n
     return T(0.5) * (vavg(ii, jj + 1) * OmU(ii, jj + 1) + vavg(ii, jj) * OmU(ii, jj));
@@ -150,7 +157,10 @@ struct DstField {
   T* a; long bs;
   __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const { a[b * bs + ii + jj * g.mm1] = v; }
 };
-// a = a - v (projection, src/operators.f90:174)
+// a = a - v (projection, src/operators.f90:174).  Read-modify-write
+// destinations also offer pre (the old value, read by the size-specialised
+// passes together with their inputs) and put (the store given that value),
+// which give the same result as operator().
 template <typename T>
 struct DstSub {
   T* a; long bs;
@@ -158,7 +168,20 @@ struct DstSub {
     const long e = b * bs + ii + jj * g.mm1;
     a[e] = a[e] - v;
   }
+  __device__ T pre(const Geo& g, int b, int ii, int jj) const { return a[b * bs + ii + jj * g.mm1]; }
+  __device__ void put(const Geo& g, int b, int ii, int jj, T v, T old) const { a[b * bs + ii + jj * g.mm1] = old - v; }
 };
+
+// Dst::pre present?
+template <class D, class = void>
+struct dst_has_pre : std::false_type {};
+template <class D>
+struct dst_has_pre<D, std::void_t<decltype(&D::pre)>> : std::true_type {};
+// Src::post present (a loader with a side store)?
+template <class S, class = void>
+struct src_has_post : std::false_type {};
+template <class S>
+struct src_has_post<S, std::void_t<decltype(&S::post)>> : std::true_type {};
 
 // single-field row pass: rows jj; FFT p pairs rows 2Pg+2p and 2Pg+2p+1.
 // Cases whose level count is <= lev are skipped (Lc == nullptr: none).
@@ -169,11 +192,48 @@ struct RowOp {
     const int row = gr * 2 * P + s;
     return (row < g.nm1 && active(Lc, b, lev)) ? src(g, b, idx, row) : T(0);
   }
+  // loaders with a side store (Src::value/post): load without the store,
+  // then load_post once every load of the thread has been issued
+  static constexpr bool kLoadPost = src_has_post<Src>::value;
+  __device__ T load_aux(int b, int gr, int s, int P, int idx, T& aux) const {
+    aux = T(0);
+    if constexpr (kLoadPost) {
+      const int row = gr * 2 * P + s;
+      return (row < g.nm1 && active(Lc, b, lev)) ? src.value(g, b, idx, row, aux) : T(0);
+    } else {
+      return load(b, gr, s, P, idx);
+    }
+  }
+  __device__ void load_post(int b, int gr, int s, int P, int idx, T aux) const {
+    if constexpr (kLoadPost) {
+      const int row = gr * 2 * P + s;
+      if (row < g.nm1 && active(Lc, b, lev)) src.post(g, b, idx, row, aux);
+    }
+  }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     if (!active(Lc, b, lev)) return;
     const int r0 = gr * 2 * P + 2 * p, r1 = r0 + 1;
     if (r0 < g.nm1) dst(g, b, k - 1, r0, ya);
     if (r1 < g.nm1) dst(g, b, k - 1, r1, yb);
+  }
+  // prefetched form of store (only when Dst has pre/put)
+  static constexpr bool kStorePre = dst_has_pre<Dst>::value;
+  __device__ void store_pre(int b, int gr, int p, int P, int k, T& oa, T& ob) const {
+    oa = ob = T(0);
+    if constexpr (kStorePre) {
+      if (!active(Lc, b, lev)) return;
+      const int r0 = gr * 2 * P + 2 * p, r1 = r0 + 1;
+      if (r0 < g.nm1) oa = dst.pre(g, b, k - 1, r0);
+      if (r1 < g.nm1) ob = dst.pre(g, b, k - 1, r1);
+    }
+  }
+  __device__ void store_put(int b, int gr, int p, int P, int k, T ya, T yb, T oa, T ob) const {
+    if constexpr (kStorePre) {
+      if (!active(Lc, b, lev)) return;
+      const int r0 = gr * 2 * P + 2 * p, r1 = r0 + 1;
+      if (r0 < g.nm1) dst.put(g, b, k - 1, r0, ya, oa);
+      if (r1 < g.nm1) dst.put(g, b, k - 1, r1, yb, ob);
+    }
   }
 };
 // single-field column pass: columns ii
@@ -194,10 +254,15 @@ struct ColOp {
 
 // spectral factors of the fused column pair (value at column ii, row jj)
 // tab * v: ctci's laminv (src/myfft.f90:106), shared by the group
+// coef gives the table entry and apply the factor given that entry (the
+// size-specialised column pair reads the entries together with its inputs);
+// apply(coef(...), v) == operator()(..., v).
 template <typename T>
 struct MidScale {
   const T* tab;
   __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const { return tab[ii + jj * g.mm1] * v; }
+  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[ii + jj * g.mm1]; }
+  __device__ T apply(T c, T v) const { return c * v; }
 };
 // tab * v / norm: ainv's lam1i(:,:,1) / normalize (src/myfft.f90:121), per case
 template <typename T>
@@ -206,6 +271,8 @@ struct MidScaleDiv {
   __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const {
     return tab[b * tbs + ii + jj * g.mm1] * v / norm;
   }
+  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[b * tbs + ii + jj * g.mm1]; }
+  __device__ T apply(T c, T v) const { return c * v / norm; }
 };
 
 // fused column pair: columns ii, Sy . diag(mid) . Sy
@@ -221,6 +288,20 @@ struct Col2Op {
     const bool on = active(Lc, b, lev);
     va = (on && c0 < g.mm1) ? md(g, b, c0, k - 1, ya) : T(0);
     vb = (on && c1 < g.mm1) ? md(g, b, c1, k - 1, yb) : T(0);
+  }
+  // mid in two parts: the table entries of FFT p at k (read early), then
+  // the factors applied to the transform outputs
+  __device__ void mid_coef(int b, int gr, int p, int P, int k, T& ca, T& cb) const {
+    const int c0 = gr * 2 * P + 2 * p, c1 = c0 + 1;
+    const bool on = active(Lc, b, lev);
+    ca = (on && c0 < g.mm1) ? md.coef(g, b, c0, k - 1) : T(0);
+    cb = (on && c1 < g.mm1) ? md.coef(g, b, c1, k - 1) : T(0);
+  }
+  __device__ void mid_apply(int b, int gr, int p, int P, int k, T ya, T yb, T ca, T cb, T& va, T& vb) const {
+    const int c0 = gr * 2 * P + 2 * p, c1 = c0 + 1;
+    const bool on = active(Lc, b, lev);
+    va = (on && c0 < g.mm1) ? md.apply(ca, ya) : T(0);
+    vb = (on && c1 < g.mm1) ? md.apply(cb, yb) : T(0);
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     if (!active(Lc, b, lev)) return;
@@ -243,11 +324,11 @@ struct Col2Op {
 // removes, a 1e-4-relative bias of the mean forces at 1024^2), and one field
 // is transformed instead of two.
 // SrcStep1 gives X + 2 vfac D0 omega at (ii, jj) and stores rhs_old <- rhs.
-template <typename T>
+template <typename T, bool kFree = false>
 struct SrcStep1 {
   const T* om;      // omega at level lev (old)        stride fbs
   const T* omc;     // omega at level lev+1 (new)      stride fbs; nullptr if lev+1 == Lmax
-  const T* q; const T* q0;  //                         stride qbs
+  const T* q; const T* q0;  //                         stride qbs; not read when kFree
   const T* lastbc;  // get_bc of old omega_{lev+1}, x0.25  stride bbs
   T* rhs_old;       //                                 stride fbs
   const T* con1; const T* con2; const T* vfac;  // [b*Lmax + lev]
@@ -261,16 +342,28 @@ struct SrcStep1 {
   const T* bq = nullptr;
   const T* bq0 = nullptr;
   const T* blb = nullptr;
+  const T* q0c = nullptr;  // [b*Lmax + lev]: the u-face value of the freestream q0 (kFree)
   __device__ T operator()(const Geo& g, int b, int ii, int jj) const {
+    T rhs;
+    const T v = value(g, b, ii, jj, rhs);
+    post(g, b, ii, jj, rhs);
+    return v;
+  }
+  // value without the rhs_old store (rhs is returned for post): the
+  // size-specialised passes evaluate every element of a thread before any
+  // store, so that all the loads are in flight together
+  __device__ T value(const Geo& g, int b, int ii, int jj, T& rhs) const {
     const T* w = om + b * fbs;
     const int e = ii + jj * g.mm1;
     const int i = ii + 2, j = jj + 2;
-    T rhs = rot_nonlinear<T>(g, i, j, w, bom ? bq + b * qbs : q + b * qbs, bom ? bq0 + b * qbs : q0 + b * qbs,
+    if constexpr (kFree) {
+      rhs = rot_nonlinear<T, true>(g, i, j, w, q + b * qbs, nullptr, lastbc + b * bbs, q0c[b * g.Lmax + lev]);
+    } else {
+      rhs = rot_nonlinear<T>(g, i, j, w, bom ? bq + b * qbs : q + b * qbs, bom ? bq0 + b * qbs : q0 + b * qbs,
                              lastbc + b * bbs);
-    if (bom) rhs = rhs + rot_nonlinear<T>(g, i, j, bom + b * fbs, q + b * qbs, q0 + b * qbs, blb + b * bbs);
-    T* ro = rhs_old + b * fbs + e;
-    const T rold = first ? rhs : *ro;
-    *ro = rhs;
+      if (bom) rhs = rhs + rot_nonlinear<T>(g, i, j, bom + b * fbs, q + b * qbs, q0 + b * qbs, blb + b * bbs);
+    }
+    const T rold = first ? rhs : rhs_old[b * fbs + e];
     const int c = b * g.Lmax + lev;
     T bcs = T(0);
     if (i == 2 || i == g.m || j == 2 || j == g.n) {
@@ -281,12 +374,15 @@ struct SrcStep1 {
                   (jj + 1 < g.nm1 ? w[e + g.mm1] : T(0)) - T(4) * w[e];
     return con1[c] * rhs + con2[c] * rold + bcs + T(2) * vfac[c] * lap;
   }
+  __device__ void post(const Geo& g, int b, int ii, int jj, T rhs) const { rhs_old[b * fbs + ii + jj * g.mm1] = rhs; }
 };
 // tab * v with a per-case table (step 1's lam1i of level lev)
 template <typename T>
 struct MidTab {
   const T* tab; long tbs;
   __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const { return tab[b * tbs + ii + jj * g.mm1] * v; }
+  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[b * tbs + ii + jj * g.mm1]; }
+  __device__ T apply(T c, T v) const { return c * v; }
 };
 // a = a + scale * v (the step-1 increment; adjoint accumulations)
 template <typename T>
@@ -295,6 +391,10 @@ struct DstAdd {
   __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
     const long e = b * bs + ii + jj * g.mm1;
     a[e] = a[e] + scale * v;
+  }
+  __device__ T pre(const Geo& g, int b, int ii, int jj) const { return a[b * bs + ii + jj * g.mm1]; }
+  __device__ void put(const Geo& g, int b, int ii, int jj, T v, T old) const {
+    a[b * bs + ii + jj * g.mm1] = old + scale * v;
   }
 };
 
@@ -315,6 +415,15 @@ __global__ void k_lastbc(Geo g, const T* __restrict__ om, T* __restrict__ lastbc
   lastbc[(fb + lev) * g.nbc + p] = v;
 }
 
+// full-weighting restriction of the fine values F around fine vertex
+// (ii, jj), in the expression order of src/operators.f90:325-329 (the 0.5
+// and 0.25 products are exact, so FMA contraction cannot change the result)
+template <typename T, class F>
+__device__ __forceinline__ T restrict_at(const F& f, int ii, int jj) {
+  return f(ii, jj) + T(0.5) * (f(ii - 1, jj) + f(ii + 1, jj) + f(ii, jj - 1) + f(ii, jj + 1)) +
+         T(0.25) * (f(ii - 1, jj - 1) + f(ii - 1, jj + 1) + f(ii + 1, jj - 1) + f(ii + 1, jj + 1));
+}
+
 // coarsify (src/operators.f90:307-333) of level lev from level lev-1, in
 // place.  grid.x over (m/2-1)(n/2-1)
 template <typename T>
@@ -328,9 +437,42 @@ __global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ 
   const int ii = 2 * i - g.m / 2 - 1, jj = 2 * j - g.n / 2 - 1;
   const T* r = fine + b * bs;
   auto F = [&](int a, int c) -> T { return r[wix(g, a, c)]; };
-  coarse[b * bs + wix(g, i, j)] =
-      F(ii, jj) + T(0.5) * (F(ii - 1, jj) + F(ii + 1, jj) + F(ii, jj - 1) + F(ii, jj + 1)) +
-      T(0.25) * (F(ii - 1, jj - 1) + F(ii - 1, jj + 1) + F(ii + 1, jj - 1) + F(ii + 1, jj + 1));
+  coarse[b * bs + wix(g, i, j)] = restrict_at<T>(F, ii, jj);
+}
+
+// The same chain two levels per launch: workgroups x < nblk coarsify level
+// lev from lev-1 as k_coarsify; the next nblk coarsify level lev+1 from
+// level lev, where every value of level lev inside its own centre region is
+// recomputed from level lev-1 (restrict_at, the same expression, so the
+// result is bit-identical to two k_coarsify launches) rather than read, as
+// those entries are being overwritten by the first half of the grid.
+// grid.x = 2*nblk (nblk = ceil((m/2-1)(n/2-1)/256)) or nblk when
+// lev+1 == Lmax.
+template <typename T>
+__global__ void k_coarsify2(Geo g, T* __restrict__ om, long bs, const int* __restrict__ Lc, int lev, int nblk) {
+  const int wi = g.m / 2 - 1, wj = g.n / 2 - 1;
+  const int upper = blockIdx.x >= nblk;
+  const int t = (blockIdx.x - upper * nblk) * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  const int lc = lev + upper;  // level written
+  if (t >= wi * wj || !active(Lc, b, lc)) return;
+  const int i = g.m / 4 + 2 + t % wi, j = g.n / 4 + 2 + t / wi;
+  const int ii = 2 * i - g.m / 2 - 1, jj = 2 * j - g.n / 2 - 1;
+  const T* r1 = om + b * bs + (long)(lc - 1) * g.nw;  // level lc-1 as stored
+  auto F1 = [&](int a, int c) -> T { return r1[wix(g, a, c)]; };
+  if (!upper) {
+    om[b * bs + (long)lc * g.nw + wix(g, i, j)] = restrict_at<T>(F1, ii, jj);
+    return;
+  }
+  // level lev values: inside its centre, coarsified from level lev-1 here
+  const T* r0 = om + b * bs + (long)(lev - 1) * g.nw;
+  auto F0 = [&](int a, int c) -> T { return r0[wix(g, a, c)]; };
+  auto Fm = [&](int a, int c) -> T {
+    if (a >= g.m / 4 + 2 && a <= 3 * g.m / 4 && c >= g.n / 4 + 2 && c <= 3 * g.n / 4)
+      return restrict_at<T>(F0, 2 * a - g.m / 2 - 1, 2 * c - g.n / 2 - 1);
+    return F1(a, c);
+  };
+  om[b * bs + (long)lc * g.nw + wix(g, i, j)] = restrict_at<T>(Fm, ii, jj);
 }
 
 // curl face value (src/operators.f90:243-288) for 0-based face f; the edge
@@ -509,10 +651,11 @@ __device__ __forceinline__ double reg_face(const Smear& sm, const CaseSm& c, int
                             [&](int p) { return double(h0[p]); });
 }
 
-// regT(curl(s1, get_bc(sc))) - sub for force f of case c (sub may be null)
-template <typename T>
+// regT(curl(s1, get_bc(sc))) - sub for force f of case c (sub may be null);
+// sc may have another element type (the FP32 coarse level of a mixed case)
+template <typename T, typename TC = T>
 __device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, const CaseSm& c, int f,
-                                               const T* __restrict__ s1, const T* __restrict__ sc,
+                                               const T* __restrict__ s1, const TC* __restrict__ sc,
                                                const double* __restrict__ sub) {
   // Interior faces take curl = s1[ia] - s1[ib] from the table, so the K
   // entries' index, weight and field loads are each issued together; faces
@@ -539,7 +682,8 @@ __device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, co
 #pragma unroll
     for (int k = 0; k < kFrcK; k++) {
       if (base + k >= hi) break;
-      const double v = ia[k] >= 0 ? double(va[k] - vb[k]) : double(curl_face<T>(g, sm.frc_face[base + k], s1, sc));
+      const double v =
+          ia[k] >= 0 ? double(va[k] - vb[k]) : double(curl_face<T, TC>(g, sm.frc_face[base + k], s1, sc));
       s = s + ww[k] * v;
     }
   }

@@ -19,6 +19,15 @@ def cylinder_200():
     return dict(m=m, n=m, mgridlev=3, dt=0.004, re=100.0, len_=ln, offx=2.0, offy=2.0, xb=xb, yb=yb, bdyno=bd)
 
 
+def cylinder_grid(m: int, levels: int = 3):
+    """m x m cells, domain [-2,2]^2, circle d = 1 at IB spacing = grid
+    spacing, Re 100, dt = 0.25 * grid spacing (size sweeps)."""
+    ln = 4.0
+    xb, yb, bd = geometry.circle(ln / m, 1.0, 0.0, 0.0)
+    return dict(m=m, n=m, mgridlev=levels, dt=0.25 * ln / m, re=100.0, len_=ln, offx=2.0, offy=2.0, xb=xb, yb=yb,
+                bdyno=bd)
+
+
 def naca_512():
     """Headline case: 512^2 x 5 levels, domain [-1,3] x [-2,2], NACA0012,
     chord 1, 20 deg nose-up about the quarter chord, IB spacing = grid

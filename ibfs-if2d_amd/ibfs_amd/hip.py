@@ -56,6 +56,8 @@ def _load():
     L.ibfs_op_vort2flux.argtypes = [C.c_void_p, _D, _D, C.c_int]
     L.ibfs_last_advance_ms.argtypes = [C.c_void_p, _D]
     L.ibfs_time_dst_pass.argtypes = [C.c_void_p, C.c_int, C.c_int, _D]
+    L.ibfs_time_dst_gemm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _D]
+    L.ibfs_op_dst_gemm.argtypes = [C.c_void_p, C.c_int, _D, _D]
     L.ibfs_time_copy.argtypes = [C.c_void_p, C.c_long, C.c_int, _D]
     L.ibfs_tile_shapes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ibfs_last_error.restype = C.c_char_p
@@ -186,6 +188,20 @@ class Solver:
         ms = C.c_double()
         self._chk(_lib.ibfs_time_dst_pass(self._h, axis, reps, C.byref(ms)), "ibfs_time_dst_pass")
         return ms.value
+
+    def time_dst_gemm(self, axis, ks=4, reps=200):
+        """Mean device time (ms) of the dense matrix-core form of a DST-I pass
+        (axis 0 = along i, 1 = along j), ks waves splitting the inner
+        dimension; FP64 or FP32 as the handle's precision."""
+        ms = C.c_double()
+        self._chk(_lib.ibfs_time_dst_gemm(self._h, axis, ks, reps, C.byref(ms)), "ibfs_time_dst_gemm")
+        return ms.value
+
+    def op_dst_gemm(self, x, ks=4):
+        """2-D DST-I of a W field through the two dense matrix-core passes (FP64)."""
+        out = np.zeros(self.nw)
+        self._chk(_lib.ibfs_op_dst_gemm(self._h, ks, _p(_f64(x)), _p(out)), "ibfs_op_dst_gemm")
+        return out
 
     # ---- sensitivities (mean drag/lift over a window w.r.t. the surface velocity u_B) ----
     def _ncases(self):

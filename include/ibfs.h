@@ -190,6 +190,19 @@ int ibfs_tile_shapes(const ibfs_handle* h, int group, int* ngroups, int* shapes)
  * times laminv, DST-I along j: one read and one write of the field). */
 int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch);
 
+/* Measurement only (no counterpart in the reference): the dense form of a
+ * DST-I pass on the matrix cores, Y = X S_m (axis 0, along i) or Y = S_n X
+ * (axis 1, along j) with S the (N-1)x(N-1) DST-I matrix, one 16x16 output
+ * tile per workgroup and ks (1, 2, 4, 8) wave64s splitting the inner
+ * dimension (v_mfma_f64_16x16x4 on an FP64 handle, v_mfma_f32_16x16x4 on a
+ * mixed one).  Timed like ibfs_time_dst_pass (which follows the handle's
+ * precision too), so the two forms of the pass compare directly. */
+int ibfs_time_dst_gemm(ibfs_handle* h, int axis, int ks, int reps, double* ms_per_launch);
+
+/* Test entry (no counterpart in the reference): the 2-D DST-I of
+ * src/myfft.f90:83-94 computed with the two dense passes above (FP64). */
+int ibfs_op_dst_gemm(ibfs_handle* h, int ks, const double* in, double* out);
+
 /* Measurement only (no counterpart in the reference): a plain streaming copy
  * of n doubles, launched reps times; the reference point that calibrates
  * HBM byte counters (FETCH_SIZE / WRITE_SIZE) against known traffic. */

@@ -34,6 +34,8 @@ def test_transforms_at_production_sizes(m, rng):
     assert relerr(g.dst(x), o.dst(x)) < 1e-13
     assert relerr(g.ctci(x), o.ctci(x)) < 1e-12
     assert relerr(g.ainv(x), o.ainv(x)) < 1e-12
+    if m <= 512:
+        assert relerr(g.op_dst_gemm(x, 4), o.dst(x)) < 1e-13
 
 
 def run_pair(o, g, nsteps, spot_cols):
