@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per launch of the fused column pair at 512^2 (FP64): FETCH_SIZE
 # and WRITE_SIZE per dispatch from two rocprofv3 --pmc runs, each scaled by
 # what the counter reads for a copy kernel of the same size
-# (profiles/dst_col2_512x5_fp64_hbm_counters.md: 2630.8 KiB x 1.980 fetched +
+# (profiles/dst_col2_512x5_fp64_hbm_counters.md: 2632.8 KiB x 1.983 fetched +
 # 2135.8 KiB x 1.000 written)
-PAIR_TRAFFIC_BYTES = int((2630.8 * 1.980 + 2135.8 * 1.000) * 1024)
+PAIR_TRAFFIC_BYTES = int((2632.8 * 1.983 + 2135.8 * 1.000) * 1024)
 
 
 def step_bytes(m, n, L, nf, s=8):
@@ -95,6 +95,11 @@ def dense_vs_fft(hip, cases, sizes=(128, 200, 256, 320, 384, 512)):
                          "mfma_tflops": flops / (best[0] * 1e-6) / 1e12,
                          "mfma_util": flops / (best[0] * 1e-6) / 1e12 / MFMA_PEAK_TFLOPS[name],
                          "faster": "mfma" if best[0] < fft else "fft"}
+            # the step's column work is the fused pair (two transforms, one
+            # launch); its dense form is two column products
+            pair = s.time_dst_pass(2, 200) * 1e3
+            r["col_pair"] = {"fft_us": pair, "mfma_us": 2 * r["col"]["mfma_us"],
+                             "faster": "mfma" if 2 * r["col"]["mfma_us"] < pair else "fft"}
             rows[str(m)] = r
             log(f"dense vs FFT {name} m={m}: row FFT {r['row']['fft_us']:.2f} us / MFMA {r['row']['mfma_us']:.2f} us "
                 f"(ks {r['row']['mfma_ks']}, {100 * r['row']['mfma_util']:.1f}% of peak); col FFT "
@@ -124,8 +129,8 @@ def main(argv=None):
                     "the FFT passes at every production size (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent cases shard "
-              "one per GPU with no communication; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent cases would shard "
+              "one per GPU with no communication; one case does not shard)", file=sys.stderr)
         return 2
 
     from ibfs_amd import cases, hip
