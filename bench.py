@@ -125,6 +125,8 @@ def main(argv=None):
     ap.add_argument("--config5", type=int, default=1, help="run the batched 1024^2 x 6 tandem adjoint leg (0: skip)")
     ap.add_argument("--config5-spin", type=int, default=2000)
     ap.add_argument("--config5-window", type=int, default=400)
+    ap.add_argument("--cg-steps", type=int, default=20, help="steps of the stationary = F leg (CG force solve; "
+                    "0: skip)")
     ap.add_argument("--dense-sweep", type=int, default=1, help="time the dense matrix-core DST-I passes against "
                     "the FFT passes at every production size (0: skip)")
     args = ap.parse_args(argv)
@@ -169,6 +171,26 @@ def main(argv=None):
     t_copy_big = g.time_copy(n_big, 20)
     copy_gbs = 2 * 8 * n_big / (t_copy_big * 1e-3) / 1e9
     log(f"copy of W doubles {t_copy_w * 1e3:.2f} us; 256 MiB copy {copy_gbs:.0f} GB/s")
+
+    cg = None
+    if args.cg_steps:
+        # ---- stationary = F (the reference's default): cjgr on the device
+        # a_times every step, warm-started from the previous fb, cgtol 1e-7
+        gc = hip.Solver(**case)
+        gc.set_solver(False, cgtol=1e-7, cg_max_iter=3000)
+        gc.advance(5)
+        its0 = gc.cg_iterations(5)
+        t0 = time.perf_counter()
+        gc.advance(args.cg_steps)
+        wcg = time.perf_counter() - t0
+        its = gc.cg_iterations(args.cg_steps)
+        cg = {"steps_per_s": args.cg_steps / wcg, "ms_per_step": 1e3 * wcg / args.cg_steps,
+              "cg_iterations_first5": [int(v) for v in its0], "cg_iterations_mean": float(np.mean(its)),
+              "ms_per_iteration": 1e3 * wcg / max(1, int(np.sum(its))), "cgtol": 1e-7,
+              "note": "eager launches, one host read-back per CG iteration"}
+        log(f"stationary = F (CG force solve): {cg['steps_per_s']:.1f} steps/s, {cg['cg_iterations_mean']:.1f} "
+            f"iterations per step ({cg['ms_per_iteration']:.3f} ms per iteration); first steps {cg['cg_iterations_first5']}")
+        del gc
 
     dense = None
     if args.dense_sweep:
@@ -365,6 +387,7 @@ def main(argv=None):
         "tandem1024_adjoint_batch8": config5,
         "cpu_baseline": cpu,
         "dst_dense_mfma_vs_fft": dense,
+        "cg_form": cg,
     }
     print(json.dumps(out))
     return 0

@@ -39,6 +39,13 @@ struct ibfs_handle {
   int ncases = 0;
   int diag_cap = 0;
   bool use_graph = true;  // IBFS_NO_GRAPH=1: eager launches (for counter-collection profiling)
+  // force solve (src/parameters.f90:19-20,31): the factor (stationary) or
+  // cjgr warm-started from the previous fb, with the CG iterations of each
+  // step of the last advance call
+  int stationary = 1;
+  double cgtol = 1.0e-7;
+  int cg_max_iter = 3000;
+  std::vector<int> cg_iters;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 
@@ -323,7 +330,10 @@ namespace {
 // nsteps steps of every case; motion (concatenated per case, scaled by
 // delta) is set first.  The body of ibfs_advance.
 void advance_impl(ibfs_handle* h, int* itime, int nsteps, const double* motion, double* hist) {
-  if (!h->all_factors())
+  if (!h->stationary) {
+    if (h->ncases != 1 || !h->e64())
+      throw ArgError{IBFS_EINVAL, "the conjugate-gradient force solve needs a one-case plain FP64 handle"};
+  } else if (!h->all_factors())
     throw ArgError{IBFS_ESTATE, "no Cholesky factor: call ibfs_build_cholesky or ibfs_set_cholesky before the "
                                 "first step"};
   if (nsteps == 0) return;
@@ -339,9 +349,12 @@ void advance_impl(ibfs_handle* h, int* itime, int nsteps, const double* motion, 
     g->enqueue_motion();
   }
   HIPCHK(hipEventRecord(h->ev0, s));
+  if (!h->stationary) h->cg_iters.assign(nsteps, 0);
   for (int k = 0; k < nsteps; k++) {
     const int it = *itime + 1;
-    if (it == 1) {
+    if (!h->stationary) {
+      h->cg_iters[k] = h->groups[0]->step_cg(it == 1, h->cgtol, h->cg_max_iter);
+    } else if (it == 1) {
       h->enqueue_all(1);
     } else if (h->use_graph) {
       h->capture();
@@ -561,6 +574,30 @@ int ibfs_step(ibfs_handle* h, int* itime, const double* motion, double* fbo, dou
   if (circ) std::memcpy(circ, row.data() + 2 * k.nbody, sizeof(double) * k.L);
   if (slip) *slip = row[2 * k.nbody + k.L];
   if (fbo) return ibfs_get_state(h, nullptr, nullptr, nullptr, nullptr, fbo);
+  return IBFS_OK;
+}
+
+int ibfs_set_solver(ibfs_handle* h, int stationary, double cgtol, int cg_max_iter) {
+  if (!h || cg_max_iter < 0 || !(cgtol >= 0.0)) return IBFS_EINVAL;
+  h->stationary = stationary ? 1 : 0;
+  h->cgtol = cgtol;
+  h->cg_max_iter = cg_max_iter;
+  return IBFS_OK;
+}
+
+int ibfs_move_body(ibfs_handle* h, const double* xb, const double* yb) {
+  if (!xb || !yb) return IBFS_EINVAL;
+  return guarded(h, [&] {
+    if (h->ncases != 1) throw ArgError{IBFS_EINVAL, "moving bodies need a one-case handle"};
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->drop_graph();  // the captured step holds the old tables
+    h->groups[0]->move_body(0, xb, yb);
+  });
+}
+
+int ibfs_cg_iterations(const ibfs_handle* h, int nsteps, int* iters) {
+  if (!h || !iters || nsteps < 0) return IBFS_EINVAL;
+  for (int k = 0; k < nsteps; k++) iters[k] = k < (int)h->cg_iters.size() ? h->cg_iters[k] : 0;
   return IBFS_OK;
 }
 

@@ -255,6 +255,10 @@ struct GroupBase {
   std::vector<CaseHost> cs;
   std::vector<int> hoff;  // row offset of each case in the group's diagnostics row
   virtual void enqueue_step(int first) = 0;
+  // one step with the conjugate-gradient force solve; returns its iterations
+  virtual int step_cg(int first, double cgtol, int cg_max_iter) = 0;
+  // new point positions of case c and the rebuilt smearing tables
+  virtual void move_body(int c, const double* xb, const double* yb) = 0;
   virtual void enqueue_motion() = 0;
   virtual void set_motion(int c, const double* motion) = 0;
   virtual void set_state(int c, const double* om, const double* ro, const double* q, const double* q0,

@@ -115,6 +115,7 @@ struct Engine : GroupBase {
   long qbs() const { return (long)Lmax * nq; }
   bool is_double() const override { return sizeof(T) == 8; }
 
+  ~Engine() override { cg_release(); }
   Engine(std::vector<CaseHost> cases, hipStream_t stream) {
     st = stream;
     cs = std::move(cases);
@@ -169,76 +170,7 @@ struct Engine : GroupBase {
     if constexpr (kSolve64) upvec(laminv64, hlaminv64, st);
     upvec(lam1i, hl1i, st);
 
-    // smearing tables, concatenated
-    std::vector<int> fid, fptr, fpt, rptr, rface, ria, rib, vid, vface, bd;
-    std::vector<double> fw, rw, ws;
-    hcs.resize(B);
-    long loff = 0, goff = 0;
-    int foff = 0, hof = 0;
-    hoff.resize(B);
-    for (int b = 0; b < B; b++) {
-      const CaseHost& c = cs[b];
-      CaseSm& h = hcs[b];
-      h.nact = (int)c.face_id.size();
-      h.fbase = (int)fid.size();
-      h.pbase = (int)fptr.size();
-      const int ebase = (int)fpt.size();
-      fid.insert(fid.end(), c.face_id.begin(), c.face_id.end());
-      for (int v : c.face_ptr) fptr.push_back(v + ebase);
-      fpt.insert(fpt.end(), c.face_pt.begin(), c.face_pt.end());
-      fw.insert(fw.end(), c.face_w.begin(), c.face_w.end());
-      h.nf = c.nf;
-      h.rbase = (int)rptr.size();
-      const int rebase = (int)rface.size();
-      for (int v : c.frc_ptr) rptr.push_back(v + rebase);
-      rface.insert(rface.end(), c.frc_face.begin(), c.frc_face.end());
-      ria.insert(ria.end(), c.frc_ia.begin(), c.frc_ia.end());
-      rib.insert(rib.end(), c.frc_ib.begin(), c.frc_ib.end());
-      rw.insert(rw.end(), c.frc_w.begin(), c.frc_w.end());
-      h.nvtx = (int)c.vtx_id.size();
-      h.vbase = (int)vid.size();
-      vid.insert(vid.end(), c.vtx_id.begin(), c.vtx_id.end());
-      vface.insert(vface.end(), c.vtx_face.begin(), c.vtx_face.end());
-      h.foff = foff;
-      h.nb = c.d.nb;
-      h.loff = loff;
-      h.goff = goff;
-      h.nbody = c.nbody;
-      h.bmin = c.bmin;
-      h.ncol = c.ncol;
-      h.hoff = hof;
-      hoff[b] = hof;
-      h.dt = c.d.dt;
-      h.delta = c.delta;
-      ws.insert(ws.end(), c.wsum.begin(), c.wsum.begin() + c.nf);
-      bd.insert(bd.end(), c.d.bdyno.begin(), c.d.bdyno.end());
-      foff += c.nf;
-      loff += (long)c.nf * c.nf;
-      goff += (long)h.nvtx * c.nf;
-      hof += c.ncol;
-      nf_max = std::max(nf_max, c.nf);
-      nvtx_max = std::max(nvtx_max, h.nvtx);
-    }
-    nf_tot = foff;
-    mat_tot = loff;
-    gmat_tot = goff;
-    rowlen = hof;
-    upvec(face_id, fid, st);
-    upvec(face_ptr, fptr, st);
-    upvec(face_pt, fpt, st);
-    upvec(face_w, fw, st);
-    upvec(frc_ptr, rptr, st);
-    upvec(frc_face, rface, st);
-    upvec(frc_w, rw, st);
-    upvec(frc_ia, ria, st);
-    upvec(frc_ib, rib, st);
-    upvec(vtx_id, vid, st);
-    upvec(vtx_face, vface, st);
-    upvec(wsum, ws, st);
-    upvec(bdyno, bd, st);
-    upvec(csm, hcs, st);
-    sm = Smear{face_id.p, face_ptr.p, face_pt.p, face_w.p, frc_ptr.p, frc_face.p, frc_w.p, frc_ia.p, frc_ib.p,
-               vtx_id.p,   vtx_face.p, csm.p};
+    upload_smear();
 
     // state and initial condition (src/variables.f90:54-67)
     const size_t W = (size_t)B * Lmax * nw, Q = (size_t)B * Lmax * nq;
@@ -612,6 +544,13 @@ struct Engine : GroupBase {
 
   // --------------------------------------------------------------- one step
   void enqueue_step(int first) override {
+    enqueue_step1(first);
+    enqueue_step2_cholesky();
+    enqueue_step34();
+  }
+
+  // lastbc and STEP 1 of advance
+  void enqueue_step1(int first) {
     const long FB = fbs(), QB = qbs(), BB = (long)Lmax * nbc;
     if (base)  // tangent step: the edge values of the base state before its own step
       hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, base->omega.p,
@@ -658,8 +597,12 @@ struct Engine : GroupBase {
                   geo.mm1, 2, B, kStep1Col2);
       row_pass(SrcField<T>{T2.p, nw}, DstAdd<T>{omega.p + (long)lev * nw, FB, T(1)}, B, Lc.p, lev);
     }
-    // STEP 2 (src/operators.f90:142-171); motion is prepared per call.  The
-    // force kernel also writes the step-3 source RR = rot(reg(fb)).
+  }
+
+  // STEP 2 (src/operators.f90:142-171) with the factor; motion is prepared
+  // per call.  The force kernel also writes the step-3 source RR = rot(reg(fb)).
+  void enqueue_step2_cholesky() {
+    const long FB = fbs();
     const size_t lds = sizeof(double) * 2 * std::max(nf_max, 1);
     const dim3 gall(std::max(1, (nf_max + nvtx_max + kForceRows - 1) / kForceRows), B);
     const auto force = [&](auto s1) {
@@ -685,6 +628,11 @@ struct Engine : GroupBase {
       vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
       force((const T*)S.p);
     }
+  }
+
+  // STEPS 3 and 4, fluxes and diagnostics
+  void enqueue_step34() {
+    const long FB = fbs(), QB = qbs();
     // STEP 3 (src/operators.f90:174)
     ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, FB);
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
@@ -696,6 +644,188 @@ struct Engine : GroupBase {
                        partial.p, nchunk, Lc.p, kSolve64 ? (const double*)S64.p : nullptr);
     hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, sm, q.p, QB, motion.p, fb.p, wsum.p, bdyno.p,
                        partial.p, nchunk, Lc.p, diag.p, rowlen, counter.p, diag_cap);
+  }
+
+  // the compact smearing tables of every case, concatenated, on the device
+  // (and the per-case offsets of the force system that depend on them)
+  void upload_smear() {
+    nf_max = nvtx_max = 0;
+    std::vector<int> fid, fptr, fpt, rptr, rface, ria, rib, vid, vface, bd;
+    std::vector<double> fw, rw, ws;
+    hcs.resize(B);
+    long loff = 0, goff = 0;
+    int foff = 0, hof = 0;
+    hoff.resize(B);
+    for (int b = 0; b < B; b++) {
+      const CaseHost& c = cs[b];
+      CaseSm& h = hcs[b];
+      h.nact = (int)c.face_id.size();
+      h.fbase = (int)fid.size();
+      h.pbase = (int)fptr.size();
+      const int ebase = (int)fpt.size();
+      fid.insert(fid.end(), c.face_id.begin(), c.face_id.end());
+      for (int v : c.face_ptr) fptr.push_back(v + ebase);
+      fpt.insert(fpt.end(), c.face_pt.begin(), c.face_pt.end());
+      fw.insert(fw.end(), c.face_w.begin(), c.face_w.end());
+      h.nf = c.nf;
+      h.rbase = (int)rptr.size();
+      const int rebase = (int)rface.size();
+      for (int v : c.frc_ptr) rptr.push_back(v + rebase);
+      rface.insert(rface.end(), c.frc_face.begin(), c.frc_face.end());
+      ria.insert(ria.end(), c.frc_ia.begin(), c.frc_ia.end());
+      rib.insert(rib.end(), c.frc_ib.begin(), c.frc_ib.end());
+      rw.insert(rw.end(), c.frc_w.begin(), c.frc_w.end());
+      h.nvtx = (int)c.vtx_id.size();
+      h.vbase = (int)vid.size();
+      vid.insert(vid.end(), c.vtx_id.begin(), c.vtx_id.end());
+      vface.insert(vface.end(), c.vtx_face.begin(), c.vtx_face.end());
+      h.foff = foff;
+      h.nb = c.d.nb;
+      h.loff = loff;
+      h.goff = goff;
+      h.nbody = c.nbody;
+      h.bmin = c.bmin;
+      h.ncol = c.ncol;
+      h.hoff = hof;
+      hoff[b] = hof;
+      h.dt = c.d.dt;
+      h.delta = c.delta;
+      ws.insert(ws.end(), c.wsum.begin(), c.wsum.begin() + c.nf);
+      bd.insert(bd.end(), c.d.bdyno.begin(), c.d.bdyno.end());
+      foff += c.nf;
+      loff += (long)c.nf * c.nf;
+      goff += (long)h.nvtx * c.nf;
+      hof += c.ncol;
+      nf_max = std::max(nf_max, c.nf);
+      nvtx_max = std::max(nvtx_max, h.nvtx);
+    }
+    nf_tot = foff;
+    mat_tot = loff;
+    gmat_tot = goff;
+    rowlen = hof;
+    upvec(face_id, fid, st);
+    upvec(face_ptr, fptr, st);
+    upvec(face_pt, fpt, st);
+    upvec(face_w, fw, st);
+    upvec(frc_ptr, rptr, st);
+    upvec(frc_face, rface, st);
+    upvec(frc_w, rw, st);
+    upvec(frc_ia, ria, st);
+    upvec(frc_ib, rib, st);
+    upvec(vtx_id, vid, st);
+    upvec(vtx_face, vface, st);
+    upvec(wsum, ws, st);
+    upvec(bdyno, bd, st);
+    upvec(csm, hcs, st);
+    sm = Smear{face_id.p, face_ptr.p, face_pt.p, face_w.p, frc_ptr.p, frc_face.p, frc_w.p, frc_ia.p, frc_ib.p,
+               vtx_id.p,   vtx_face.p, csm.p};
+  }
+
+  // new positions of the points of case c (one-case handles): what advance
+  // does after move_body when a body moves (src/operators.f90:142-144):
+  // setup_reg's tables rebuilt (src/grid.f90:142-230) and uploaded.  The
+  // factor of the old geometry no longer applies; the step-3 source is
+  // cleared because the set of vertices next to the body changes.
+  void move_body(int c, const double* xb, const double* yb) override {
+    if (B != 1) throw ArgError{IBFS_EINVAL, "moving bodies need a one-case handle"};
+    CaseHost& h = cs[c];
+    for (int k = 0; k < h.d.nb; k++) {
+      h.d.xb[k] = xb[k];
+      h.d.yb[k] = yb[k];
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    build_smear(h);
+    upload_smear();
+    Gm.alloc(std::max<long>(gmat_tot, 1));  // rot(reg(A^-1)) rows follow the new vertex set
+    h.have_chol = false;
+    RR.zero(st);
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  // ------------------------------------------------ conjugate-gradient step
+  // advance with the force system solved by cjgr (stationary = F,
+  // src/operators.f90:169-171) instead of the factor: one case (B == 1),
+  // warm-started from the current fb.  The loop runs on the host, which
+  // reads the CG scalars back after every iteration and applies the
+  // reference's stopping test (iter < cg_max_iter and delta_new > cgtol^2).
+  DevBuf<T> cgRR, cgVB, cgSB, cgT1, cgT2;
+  DevBuf<double> cg_r, cg_d, cg_q;
+  DevBuf<CgState> cg_dev;
+  CgState* cg_host = nullptr;  // pinned
+
+  void cg_alloc() {
+    if (cg_host) return;
+    if (B != 1) throw ArgError{IBFS_EINVAL, "the conjugate-gradient force solve runs one case per handle"};
+    cgRR.alloc(nw); cgRR.zero(st);
+    cgVB.alloc(fbs()); cgVB.zero(st);
+    cgSB.alloc(fbs());
+    cgT1.alloc(nw);
+    cgT2.alloc(nw);
+    const size_t nfv = std::max(nf_max, 1);
+    cg_r.alloc(nfv); cg_d.alloc(nfv); cg_q.alloc(nfv);
+    cg_dev.alloc(1);
+    HIPCHK(hipHostMalloc((void**)&cg_host, sizeof(CgState), hipHostMallocDefault));
+  }
+  void cg_release() {
+    if (cg_host) (void)hipHostFree(cg_host);
+    cg_host = nullptr;
+  }
+
+  // out = a_times(x) (src/operators.f90:444-462) for case 0: ainv(rot(reg(x)))
+  // on level 1, vort2flux, regT of the level-1 fluxes
+  void a_times_one(const double* x, double* out) {
+    const int nf = cs[0].nf;
+    hipLaunchKernelGGL((k_rotreg<T, double>), dim3((hcs[0].nvtx + 255) / 256, 1), dim3(256), 0, st, sm, x, 0L,
+                       cgRR.p, (long)nw, 0);
+    ctci_like(SrcField<T>{cgRR.p, nw}, true, DstField<T>{cgVB.p, fbs()}, cgT1.p, cgT2.p, 1, nullptr, 0, lam1i.p, 0);
+    vort2flux(cgVB.p, cgSB.p, nullptr, 1, Lc.p, cgT1.p, cgT2.p);
+    hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf + 255) / 256, 1), dim3(256), 0, st, geo, sm, cgSB.p,
+                       Lmax > 1 ? cgSB.p + nw : nullptr, fbs(), (const double*)nullptr, out, (long)nf, Lc.p, 0);
+  }
+
+  CgState cg_read() {
+    HIPCHK(hipMemcpyAsync(cg_host, cg_dev.p, sizeof(CgState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return *cg_host;
+  }
+
+  // fb <- cjgr(fb, rhsf); returns the iteration count
+  int cg_solve(double cgtol, int cg_max_iter) {
+    const int nf = cs[0].nf;
+    if (nf == 0) return 0;
+    a_times_one(fb.p, cg_q.p);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p);
+    CgState c = cg_read();
+    const double eps = cgtol * cgtol;
+    for (int guard = 0; guard <= cg_max_iter && c.iter < cg_max_iter && c.delta_new > eps; guard++) {
+      a_times_one(cg_d.p, cg_q.p);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, nf, cg_d.p, cg_q.p, fb.p, cg_dev.p);
+      const int recompute = (c.iter + 1) % 50 == 0;
+      if (recompute) a_times_one(fb.p, cg_q.p);
+      hipLaunchKernelGGL(k_cg_resid, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p,
+                         recompute);
+      c = cg_read();
+    }
+    return c.iter;
+  }
+
+  // one step with the CG force solve (eager launches); returns the
+  // iteration count
+  int step_cg(int first, double cgtol, int cg_max_iter) override {
+    if constexpr (!std::is_same_v<T, double>)
+      throw ArgError{IBFS_EINVAL, "the conjugate-gradient force solve needs the plain FP64 form"};
+    cg_alloc();
+    enqueue_step1(first);
+    // STEP 2 (src/operators.f90:155-164): rhsf = regT(q_1) - motion
+    vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
+    hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf_max + 255) / 256, B), dim3(256), 0, st, geo, sm, S.p,
+                       Lmax > 1 ? S.p + nw : nullptr, fbs(), (const double*)motion.p, rhsf.p, 0L, Lc.p, -1);
+    const int it = cg_solve(cgtol, cg_max_iter);
+    // the step-3 source rot(reg(fb)) on the active vertices
+    hipLaunchKernelGGL((k_rotreg<T, double>), dim3((nvtx_max + 255) / 256, B), dim3(256), 0, st, sm, fb.p, 0L, RR.p,
+                       (long)nw, -1);
+    enqueue_step34();
+    return it;
   }
 
   void enqueue_motion() override {

@@ -932,6 +932,82 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
 }
 
 // ---------------------------------------------------------------------------
+// conjugate gradients on the force system: cjgr (src/operators.f90:393-441)
+// for one case, its vectors (nf doubles) in device memory and its scalars in
+// CgState.  Each kernel is one workgroup; the dot products are summed in
+// index order by one thread, as DOT_PRODUCT, and the vector updates are
+// not contracted into FMAs, so they round as the reference's statements.
+// ---------------------------------------------------------------------------
+struct CgState {
+  double delta_new, delta_old, alpha, beta;
+  int iter, pad;
+};
+
+__device__ __forceinline__ double dot_in_order(int nf, const double* __restrict__ a, const double* __restrict__ b) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  for (int f = 0; f < nf; f++) s = s + a[f] * b[f];
+  return s;
+}
+
+// r = b - Ax; d = r; delta_new = r . d; iter = 0 (src/operators.f90:407-413)
+__global__ __launch_bounds__(256) void k_cg_start(int nf, const double* __restrict__ b,
+                                                  const double* __restrict__ Ax, double* __restrict__ r,
+                                                  double* __restrict__ d, CgState* __restrict__ cs) {
+#pragma clang fp contract(off)
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+    const double v = b[f] - Ax[f];
+    r[f] = v;
+    d[f] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cs->delta_new = dot_in_order(nf, r, d);
+    cs->iter = 0;
+  }
+}
+
+// iter += 1; alpha = delta_new / (d . q); x = x + alpha d  (q = A d;
+// src/operators.f90:415-418)
+__global__ __launch_bounds__(256) void k_cg_alpha(int nf, const double* __restrict__ d,
+                                                  const double* __restrict__ q, double* __restrict__ x,
+                                                  CgState* __restrict__ cs) {
+#pragma clang fp contract(off)
+  __shared__ double alpha;
+  if (threadIdx.x == 0) {
+    alpha = cs->delta_new / dot_in_order(nf, d, q);
+    cs->alpha = alpha;
+    cs->iter = cs->iter + 1;
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) x[f] = x[f] + alpha * d[f];
+}
+
+// the residual, every 50th iteration from scratch (q = A x: r = b - q),
+// otherwise r = r - alpha q (q = A d); then delta_old = delta_new,
+// delta_new = r . r, beta = delta_new / delta_old, d = r + beta d
+// (src/operators.f90:420-431)
+__global__ __launch_bounds__(256) void k_cg_resid(int nf, const double* __restrict__ b,
+                                                  const double* __restrict__ q, double* __restrict__ r,
+                                                  double* __restrict__ d, CgState* __restrict__ cs, int recompute) {
+#pragma clang fp contract(off)
+  __shared__ double beta;
+  const double alpha = cs->alpha;
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) r[f] = recompute ? b[f] - q[f] : r[f] - alpha * q[f];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double dold = cs->delta_new;
+    const double dnew = dot_in_order(nf, r, r);
+    cs->delta_old = dold;
+    cs->delta_new = dnew;
+    beta = dnew / dold;
+    cs->beta = beta;
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) d[f] = r[f] + beta * d[f];
+}
+
+// ---------------------------------------------------------------------------
 // tail of the step: fluxes, circulation and diagnostics
 // ---------------------------------------------------------------------------
 // One launch after the last streamfunction of the step is final:

@@ -52,9 +52,10 @@ CONTAINS
     nw = ns
     nf = 2*nb
     nq = (m+1)*n + (n+1)*m
-    IF (ANY(bdyno>0)) THEN
-       WRITE(*,*) 'Moving bodies (bdyno > 0) are not supported by the GPU step'
-       STOP 'bdyno > 0'
+    IF (ANY(bdyno>0) .AND. stationary) THEN
+       ! a factor of the initial geometry would be wrong after the first move
+       WRITE(*,*) 'Moving bodies (bdyno > 0) need stationary = F (the CG force solve)'
+       STOP 'bdyno > 0 with stationary = T'
     END IF
     WRITE(111,*) "Set up grid and body..."
   END SUBROUTINE setup
@@ -110,5 +111,31 @@ CONTAINS
     END DO
     geom%ntot = geom%ntot+npt
   END SUBROUTINE circle
+
+  FUNCTION move_body(itime) RESULT(motion)
+    !-------------------------------------------------------------------------!
+    ! Prescribed motion of the moving bodies (bdyno > 0) at step itime, with
+    ! the interface of src/grid.f90:523-576: the point velocities in the
+    ! force layout (all x components, then all y components), scaled by
+    ! delta (src/grid.f90:573-574), and xb, yb updated to the positions of
+    ! this step.  No motion is prescribed here, as in the reference as
+    ! shipped: the points stay put and their velocities are zero.  A case
+    ! with a moving body sets, for every point i of it, motion(i) and
+    ! motion(nb+i) to the velocity at t = dt*itime and moves xb(i), yb(i) by
+    ! the displacement since the previous step (itime is the step number).
+    !-------------------------------------------------------------------------!
+    INTEGER :: itime
+    REAL(KIND(0.D0)), DIMENSION(MAX(nf,1)) :: motion
+    INTEGER :: i
+
+    motion = 0.d0
+    DO i = 1, nb
+       IF (bdyno(i) > 0) THEN
+          motion(i) = 0.d0
+          motion(nb+i) = 0.d0
+       END IF
+    END DO
+    motion = motion * delta
+  END FUNCTION move_body
 
 END MODULE grid

@@ -58,6 +58,26 @@ MODULE ibfs_c
        REAL(C_DOUBLE), DIMENSION(*) :: motion, hist
      END FUNCTION ibfs_advance
 
+     INTEGER(C_INT) FUNCTION ibfs_set_solver(h, stationary, cgtol, cg_max_iter) BIND(C, name='ibfs_set_solver')
+       IMPORT :: C_INT, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: h
+       INTEGER(C_INT), VALUE :: stationary, cg_max_iter
+       REAL(C_DOUBLE), VALUE :: cgtol
+     END FUNCTION ibfs_set_solver
+
+     INTEGER(C_INT) FUNCTION ibfs_move_body(h, xb, yb) BIND(C, name='ibfs_move_body')
+       IMPORT :: C_INT, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: h
+       REAL(C_DOUBLE), DIMENSION(*) :: xb, yb
+     END FUNCTION ibfs_move_body
+
+     INTEGER(C_INT) FUNCTION ibfs_cg_iterations(h, nsteps, iters) BIND(C, name='ibfs_cg_iterations')
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: h
+       INTEGER(C_INT), VALUE :: nsteps
+       INTEGER(C_INT), DIMENSION(*) :: iters
+     END FUNCTION ibfs_cg_iterations
+
      FUNCTION ibfs_last_error(h) BIND(C, name='ibfs_last_error') RESULT(msg)
        IMPORT :: C_PTR
        TYPE(C_PTR), VALUE :: h

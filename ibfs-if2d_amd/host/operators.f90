@@ -28,15 +28,10 @@ CONTAINS
     ! it.  As in the reference, a cached factor is trusted as is.
     !-------------------------------------------------------------------------!
     LOGICAL :: readchol
-    REAL(C_DOUBLE), DIMENSION(:,:), ALLOCATABLE :: cm
-    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE :: cv
 
     IF (.NOT. stationary) THEN
-       ! The CG path of moving bodies is not offloaded; a body with bdyno 0
-       ! is stationary, so the same matrix is factored once, in memory only.
-       ALLOCATE(cm(MAX(nf,1),MAX(nf,1)), cv(MAX(nf,1)))
-       CALL ibfs_check(ibfs_build_cholesky(handle, cm, cv), 'ibfs_build_cholesky')
-       DEALLOCATE(cm, cv)
+       ! the library solves each step's force system with cjgr
+       ! (src/operators.f90:169-171); there is no factor to build or cache
        have_factor = .TRUE.
        RETURN
     END IF
@@ -95,8 +90,12 @@ CONTAINS
     INTEGER :: itime
     INTEGER(C_INT) :: it_c, nsteps
     INTEGER :: k, s, ncol, t, b
+    INTEGER(C_INT), DIMENSION(:), ALLOCATABLE :: cgit
+    LOGICAL :: moving
 
+    moving = ANY(bdyno > 0)
     nsteps = MIN(nblock, istop - itime)
+    IF (moving) nsteps = 1  ! the body moves before every step
     DO t = itime + 1, itime + nsteps
        IF (MOD(t, isave) == 0) THEN
           nsteps = t - itime
@@ -118,8 +117,24 @@ CONTAINS
     END IF
     IF (.NOT. ALLOCATED(hist)) ALLOCATE(hist(ncol*nsteps))
 
+    IF (moving) THEN
+       ! BOUNDARY MOTION (src/operators.f90:142-144): the prescribed motion of
+       ! this step, then setup_reg on the device for the new positions
+       motion = move_body(itime + 1)
+       CALL ibfs_check(ibfs_move_body(handle, xb, yb), 'ibfs_move_body')
+    END IF
     it_c = itime
     CALL ibfs_check(ibfs_advance(handle, it_c, nsteps, motion, hist), 'ibfs_advance')
+    IF (.NOT. stationary) THEN  ! what cjgr reports (src/operators.f90:435-440)
+       ALLOCATE(cgit(nsteps))
+       CALL ibfs_check(ibfs_cg_iterations(handle, nsteps, cgit), 'ibfs_cg_iterations')
+       DO s = 1, nsteps
+          IF (cgit(s) >= cg_max_iter) WRITE(111,*) "WARNING!! CJGR used max iterations!"
+          WRITE(111,*) "CJGR used ", cgit(s), " iterations."
+       END DO
+       WRITE(*,*) "* CJGR iterations = ", cgit(nsteps)
+       DEALLOCATE(cgit)
+    END IF
 
     DO s = 1, nsteps
        k = (s-1)*ncol

@@ -28,6 +28,7 @@ CONTAINS
     INTEGER(C_INT) :: dims(12)
     INTEGER(C_INT), ALLOCATABLE :: bd(:)
     INTEGER :: uf
+    INTEGER(C_INT) :: st
 
     uf = 0
     IF (unif_flow) uf = 1
@@ -38,6 +39,11 @@ CONTAINS
                                 xb, yb, bd, uf, precision, handle), 'ibfs_create')
     CALL ibfs_check(ibfs_dims(handle, dims), 'ibfs_dims')
     IF (dims(3) /= mgridlev) STOP 'level count mismatch with the library'
+    ! the force solve: the factor for stationary bodies, else cjgr
+    ! (src/operators.f90:166-171)
+    st = 0
+    IF (stationary) st = 1
+    CALL ibfs_check(ibfs_set_solver(handle, st, cgtol, cg_max_iter), 'ibfs_set_solver')
     nbody = dims(9); bmin = dims(10); bmax = dims(11)
 
     ALLOCATE( omega(2:m,2:n,mgridlev), fb(MAX(nf,1)), rhs_old(2:m,2:n,mgridlev) )

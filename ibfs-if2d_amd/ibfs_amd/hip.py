@@ -57,6 +57,9 @@ def _load():
     L.ibfs_last_advance_ms.argtypes = [C.c_void_p, _D]
     L.ibfs_time_dst_pass.argtypes = [C.c_void_p, C.c_int, C.c_int, _D]
     L.ibfs_time_dst_gemm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _D]
+    L.ibfs_set_solver.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+    L.ibfs_cg_iterations.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.ibfs_move_body.argtypes = [C.c_void_p, _D, _D]
     L.ibfs_op_dst_gemm.argtypes = [C.c_void_p, C.c_int, _D, _D]
     L.ibfs_time_copy.argtypes = [C.c_void_p, C.c_long, C.c_int, _D]
     L.ibfs_tile_shapes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -188,6 +191,23 @@ class Solver:
         ms = C.c_double()
         self._chk(_lib.ibfs_time_dst_pass(self._h, axis, reps, C.byref(ms)), "ibfs_time_dst_pass")
         return ms.value
+
+    def set_solver(self, stationary, cgtol=1e-7, cg_max_iter=3000):
+        """stationary: the Cholesky factor (default); otherwise cjgr
+        (src/operators.f90:393-441) on the device, warm-started from fb."""
+        self._chk(_lib.ibfs_set_solver(self._h, 1 if stationary else 0, float(cgtol), int(cg_max_iter)),
+                  "ibfs_set_solver")
+
+    def move_body(self, xb, yb):
+        """New body point positions (the setup_reg after move_body,
+        src/operators.f90:142-144); the factor of the old geometry is dropped."""
+        self._chk(_lib.ibfs_move_body(self._h, _p(_f64(xb)), _p(_f64(yb))), "ibfs_move_body")
+
+    def cg_iterations(self, nsteps):
+        """CG iterations of each step of the last advance call."""
+        out = (C.c_int * max(nsteps, 1))()
+        self._chk(_lib.ibfs_cg_iterations(self._h, int(nsteps), out), "ibfs_cg_iterations")
+        return np.array(out[:nsteps])
 
     def time_dst_gemm(self, axis, ks=4, reps=200):
         """Mean device time (ms) of the dense matrix-core form of a DST-I pass

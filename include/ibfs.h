@@ -93,6 +93,31 @@ int ibfs_build_cholesky(ibfs_handle* h, double* cholmat, double* cholvec);
 int ibfs_step(ibfs_handle* h, int* itime, const double* motion, double* fb, double* force, double* circ,
               double* slip);
 
+/* The force solve of advance (src/operators.f90:166-171; namelist
+ * stationary, cgtol, cg_max_iter of src/parameters.f90:19-20,31).
+ * stationary != 0 (the default): fb = cholsl(rhsf) with the factor of
+ * ibfs_build_cholesky / ibfs_set_cholesky.  stationary == 0: cjgr
+ * (src/operators.f90:393-441), conjugate gradients on the device a_times,
+ * warm-started from the previous fb, stopping at iter == cg_max_iter or
+ * r.r <= cgtol^2; no factor is needed.  The CG form runs one case per
+ * handle in the plain FP64 form, with eager launches (the iteration count is
+ * data-dependent). */
+int ibfs_set_solver(ibfs_handle* h, int stationary, double cgtol, int cg_max_iter);
+
+/* New positions of the body points (one-case handles): replaces the
+ * setup_reg call that follows move_body when a body moves
+ * (src/operators.f90:142-144; tables of src/grid.f90:142-230).  The point
+ * velocities enter through the motion argument of the next step
+ * (move_body's result, scaled by delta).  The factor of the old geometry is
+ * dropped, so a moving body is stepped with ibfs_set_solver(h, 0, ...) as in
+ * the reference (stationary = F), or with a factor built anew. */
+int ibfs_move_body(ibfs_handle* h, const double* xb, const double* yb);
+
+/* The CG iteration counts of the steps of the last ibfs_advance / ibfs_step
+ * call (what cjgr writes to ibfs.inf, src/operators.f90:439); 0 for steps
+ * solved with the factor.  iters receives nsteps values. */
+int ibfs_cg_iterations(const ibfs_handle* h, int nsteps, int* iters);
+
 /* nsteps consecutive calls of advance with a constant motion vector.  hist
  * (may be NULL) receives nsteps rows of [fx.., fy.., circ(1..L), slip]
  * (2*nbody + L + 1 values per row), i.e. the lines of force.dat, circ.dat

@@ -24,6 +24,8 @@
  *   write_force    src/operators.f90:583-659   (the force values only)
  *   write_circ     src/operators.f90:662-699   (the circulation values only)
  *   choldc/cholsl  src/operators.f90:766-819
+ *   cjgr           src/operators.f90:393-441   (stationary = F force solve)
+ *   move + setup_reg src/operators.f90:142-144 (new point positions)
  *   preprocess     src/operators.f90:29-59     (no file cache here)
  *   setup_fft      src/myfft.f90:21-70
  *   dst/ctci/ainv  src/myfft.f90:83-124
@@ -184,6 +186,11 @@ typedef struct orc {
   fft_plan pm, pn;
   int dst_order; /* 0: along i then j; 1: along j then i (same transform, other rounding) */
   double *work, *tbuf; /* dst scratch, transpose buffer */
+  /* force solve (parameters.f90:19-20,31): stationary = Cholesky factor,
+   * else cjgr warm-started from the previous fb; cg_iters = count of the
+   * last cjgr call */
+  int stationary, cg_max_iter, cg_iters;
+  double cgtol;
 } orc_t;
 
 #define OM(a, i, j) (a)[((i)-2) + (size_t)((j)-2) * (o->m - 1)]
@@ -331,6 +338,9 @@ orc_t *orc_create(int m, int n, int L, double dt, double re, double len, double 
   }
   setup_reg(o);
   setup_fft(o);
+  o->stationary = 1;
+  o->cgtol = 1.0e-7;
+  o->cg_max_iter = 3000;
   size_t nw = (size_t)o->nw, nq = (size_t)o->nq, nf = (size_t)o->nf;
   o->omega = (double *)calloc(nw * L, sizeof(double));
   o->rhs_old = (double *)calloc(nw * L, sizeof(double));
@@ -634,6 +644,69 @@ void orc_cholsl(orc_t *o, const double *b, double *x) {
 #undef CM
 }
 
+/* DOT_PRODUCT of two Nf vectors, summed in index order */
+static double dotf(int nf, const double *a, const double *b) {
+  double s = 0.0;
+  for (int f = 0; f < nf; f++) s = s + a[f] * b[f];
+  return s;
+}
+
+/* cjgr: operators.f90:393-441.  Conjugate gradients on a_times, no
+ * preconditioner; x holds the initial guess on entry (the previous fb: the
+ * dummy is INTENT(OUT) but read before it is written, operators.f90:402,
+ * 409) and the solution on exit.  The residual is recomputed from b - A x
+ * every 50 iterations.  Returns the iteration count. */
+int orc_cjgr(orc_t *o, double *x, const double *b) {
+  const int nf = o->nf;
+  double *r = (double *)malloc(sizeof(double) * (nf > 0 ? nf : 1));
+  double *d = (double *)malloc(sizeof(double) * (nf > 0 ? nf : 1));
+  double *q = (double *)malloc(sizeof(double) * (nf > 0 ? nf : 1));
+  int iter = 0;
+  orc_a_times(o, x, q);
+  for (int f = 0; f < nf; f++) r[f] = b[f] - q[f];
+  for (int f = 0; f < nf; f++) d[f] = r[f];
+  double delta_new = dotf(nf, r, d);
+  const double eps = o->cgtol * o->cgtol;
+  while (iter < o->cg_max_iter && delta_new > eps) {
+    iter = iter + 1;
+    orc_a_times(o, d, q);
+    const double alpha = delta_new / dotf(nf, d, q);
+    for (int f = 0; f < nf; f++) x[f] = x[f] + alpha * d[f];
+    if (iter % 50 == 0) {
+      orc_a_times(o, x, q);
+      for (int f = 0; f < nf; f++) r[f] = b[f] - q[f];
+    } else {
+      for (int f = 0; f < nf; f++) r[f] = r[f] - alpha * q[f];
+    }
+    const double delta_old = delta_new;
+    delta_new = dotf(nf, r, r);
+    const double beta = delta_new / delta_old;
+    for (int f = 0; f < nf; f++) d[f] = r[f] + beta * d[f];
+  }
+  free(r); free(d); free(q);
+  o->cg_iters = iter;
+  return iter;
+}
+
+/* the force solve of advance: Cholesky (stationary) or cjgr
+ * (operators.f90:166-171), with cgtol and cg_max_iter of the namelist */
+void orc_set_solver(orc_t *o, int stationary, double cgtol, int cg_max_iter) {
+  o->stationary = stationary ? 1 : 0;
+  o->cgtol = cgtol;
+  o->cg_max_iter = cg_max_iter;
+}
+int orc_cg_iters(const orc_t *o) { return o->cg_iters; }
+
+/* New positions of the body points, then setup_reg: what advance does after
+ * move_body when a body moves (operators.f90:142-144, grid.f90:523-576) */
+void orc_move(orc_t *o, const double *xb, const double *yb) {
+  for (int k = 0; k < o->nb; k++) {
+    o->xb[k] = xb[k];
+    o->yb[k] = yb[k];
+  }
+  setup_reg(o);
+}
+
 /* preprocess: operators.f90:29-59, without the ibfs.chd cache (the caller
  * owns files).  Builds A column by column and factors it. */
 int orc_preprocess(orc_t *o) {
@@ -744,7 +817,7 @@ int orc_advance(orc_t *o, int *itime, const double *motion_in, int do_preprocess
   int rc = 0;
 
   *itime = *itime + 1;
-  if (*itime == 1 && do_preprocess) {
+  if (*itime == 1 && do_preprocess && o->stationary) {
     rc = orc_preprocess(o);
     if (rc) goto done;
   }
@@ -759,7 +832,8 @@ int orc_advance(orc_t *o, int *itime, const double *motion_in, int do_preprocess
   orc_vort2flux(o, o->omega, o->q, 1);
   orc_regT(o, o->q, rhsf);
   for (int f = 0; f < nf; f++) rhsf[f] = rhsf[f] - motion[f];
-  orc_cholsl(o, rhsf, o->fb);
+  if (o->stationary) orc_cholsl(o, rhsf, o->fb);
+  else orc_cjgr(o, o->fb, rhsf);
 
   /* STEP 3 */
   orc_reg(o, o->fb, fq);

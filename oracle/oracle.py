@@ -73,6 +73,12 @@ def lib():
         L.orc_advance.argtypes = [C.c_void_p, _I, _D, C.c_int, _D, _D, _D]
         L.orc_step1.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_dst_order.argtypes = [C.c_void_p, C.c_int]
+        L.orc_cjgr.restype = C.c_int
+        L.orc_cjgr.argtypes = [C.c_void_p, _D, _D]
+        L.orc_set_solver.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.orc_cg_iters.restype = C.c_int
+        L.orc_cg_iters.argtypes = [C.c_void_p]
+        L.orc_move.argtypes = [C.c_void_p, _D, _D]
         L.orc_dst1.argtypes = [_D, _D, C.c_int]
         L.orc_fft.argtypes = [_D, _D, C.c_int]
         _lib = L
@@ -291,6 +297,26 @@ class Oracle:
         y = np.zeros(self.nf)
         lib().orc_cholsl(self._h, _p(np.ascontiguousarray(b, np.float64)), _p(y))
         return y
+
+    def cjgr(self, b, x0=None):
+        """cjgr (src/operators.f90:393-441) from the guess x0 (zero if None).
+        Returns (x, iterations)."""
+        x = np.zeros(self.nf) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        it = lib().orc_cjgr(self._h, _p(x), _p(np.ascontiguousarray(b, np.float64)))
+        return x, it
+
+    def set_solver(self, stationary, cgtol=1e-7, cg_max_iter=3000):
+        """stationary: the factor's cholsl; otherwise cjgr warm-started from
+        the previous fb (src/operators.f90:166-171)."""
+        lib().orc_set_solver(self._h, 1 if stationary else 0, float(cgtol), int(cg_max_iter))
+
+    @property
+    def cg_iters(self):
+        return lib().orc_cg_iters(self._h)
+
+    def move(self, xb, yb):
+        """New body point positions and setup_reg (src/operators.f90:142-144)."""
+        lib().orc_move(self._h, _p(np.ascontiguousarray(xb, np.float64)), _p(np.ascontiguousarray(yb, np.float64)))
 
     def force(self, xg):
         y = np.zeros(2 * self.nbody)

@@ -23,15 +23,15 @@ EXE = os.path.join(ROOT, "ibfs-if2d_amd", "host", "ibfs_gpu")
 NML = """ &READ_PARAMETERS ISTART = {istart}, ISTOP = {istop}, ISAVE = 5, M = 32,
  N = 32, DT = 0.02, RE = 100., CGTOL = 1.E-7, CG_MAX_ITER = 3000,
  UNIF_FLOW = T, LEN = 4.0, OFFSETX = 2.0, OFFSETY = 2.0, MGRIDLEV = 3,
- MAKE_VORTEX = F, STATIONARY = T /
+ MAKE_VORTEX = F, STATIONARY = {stationary} /
 """
 
 
-def run_host(workdir, istart, istop, nblock=1):
+def run_host(workdir, istart, istop, nblock=1, stationary=True):
     os.makedirs(os.path.join(workdir, "input"), exist_ok=True)
     os.makedirs(os.path.join(workdir, "output"), exist_ok=True)
     with open(os.path.join(workdir, "input", "ibfs.inp"), "w") as f:
-        f.write(NML.format(istart=istart, istop=istop))
+        f.write(NML.format(istart=istart, istop=istop, stationary="T" if stationary else "F"))
     with open(os.path.join(workdir, "input", "ibfs_gpu.inp"), "w") as f:
         f.write(f" &IBFS_GPU PRECISION = 0, NBLOCK = {nblock} /\n")
     r = subprocess.run([EXE], cwd=workdir, capture_output=True, text=True, timeout=120)
@@ -125,3 +125,60 @@ def test_host_blocked_calls_write_the_same_files(host_run, tmp_path):
     run_host(d3, 0, 10, nblock=4)
     for fn in ("force.dat", "circ.dat", "slip.dat"):
         assert np.array_equal(read_table(os.path.join(d, "output", fn)), read_table(os.path.join(d3, "output", fn)))
+
+
+@pytest.fixture(scope="module")
+def host_cg_run(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("host_cg"))
+    run_host(d, 0, 10, stationary=False)
+    return d
+
+
+def test_host_cg_form_matches_oracle(host_cg_run):
+    """STATIONARY = F, the reference's default: each step's force system is
+    solved by cjgr (src/operators.f90:169-171) on the device, with the
+    namelist's cgtol and cg_max_iter; no factor and no ibfs.chd.  The CG
+    path depends on rounding (A is slightly nonsymmetric), so the force
+    history is held to 10x the spread of two oracle runs that differ only in
+    the rounding of their transforms."""
+    d = host_cg_run
+    assert not os.path.exists(os.path.join(d, "input", "ibfs.chd"))
+    with open(os.path.join(d, "output", "ibfs.inf")) as f:
+        inf = f.read()
+    assert inf.count("CJGR used") == 10
+    from ibfs_amd import geometry
+
+    xb, yb, bd = geometry.read_geom(os.path.join(d, "input", "geom.inp"))
+    oa = orc.Oracle(32, 32, 3, 0.02, 100.0, 4.0, 2.0, 2.0, xb, yb, bd)
+    ob = orc.Oracle(32, 32, 3, 0.02, 100.0, 4.0, 2.0, 2.0, xb, yb, bd)
+    ob.set_dst_order(1)
+    for o in (oa, ob):
+        o.set_solver(False, cgtol=1e-7, cg_max_iter=3000)
+    fa = np.array([oa.advance()[0] for _ in range(10)])
+    fb_ = np.array([ob.advance()[0] for _ in range(10)])
+    force = read_table(os.path.join(d, "output", "force.dat"))[:, 1:]
+    floor = np.max(np.abs(fb_ - fa))
+    assert np.max(np.abs(force - fa)) <= 10 * floor + 1e-12 * np.max(np.abs(fa))
+
+
+def test_host_moving_body_without_motion(host_cg_run, tmp_path):
+    """bdyno = 1 (a moving body): every step calls the host's move_body and
+    rebuilds the smearing tables on the device (src/operators.f90:142-144).
+    With no motion prescribed, as shipped, the run is the fixed-body CG run
+    exactly."""
+    d = host_cg_run
+    d2 = str(tmp_path)
+    shutil.copytree(os.path.join(d, "input"), os.path.join(d2, "input"))
+    gp = os.path.join(d2, "input", "geom.inp")
+    with open(gp) as f:
+        lines = f.read().split("\n")
+    out = [lines[0]]
+    for ln in lines[1:]:
+        p = ln.split()
+        if len(p) == 3:
+            out.append(f"{p[0]} {p[1]} 1")
+    with open(gp, "w") as f:
+        f.write("\n".join(out) + "\n")
+    run_host(d2, 0, 10, stationary=False)
+    for fn in ("force.dat", "circ.dat", "slip.dat"):
+        assert np.array_equal(read_table(os.path.join(d, "output", fn)), read_table(os.path.join(d2, "output", fn))), fn

@@ -424,3 +424,65 @@ def test_cylinder_drag_magnitude():
     for _ in range(200):
         f, c, s = o.advance()
     assert 1.0 < f[0] < 2.5
+
+
+# ---------------------------------------------------- cjgr and moving bodies
+@pytest.mark.parametrize("name", ["cyl32", "naca48x40"])
+def test_cjgr_solves_force_system(name, rng):
+    """cjgr (src/operators.f90:393-441) on a_times: the solution satisfies
+    A x = b to the CG tolerance (||b - A x|| <= cgtol up to the recursive
+    residual's drift) and matches a direct solve of the full A; from the
+    exact solution as the warm start it takes no iteration."""
+    o = make(name)
+    A = np.stack([o.a_column(i) for i in range(o.nf)], axis=1)
+    b = rng.standard_normal(o.nf) * o.delta
+    o.set_solver(False, cgtol=1e-10, cg_max_iter=3000)
+    x, it = o.cjgr(b)
+    assert 0 < it < 3000
+    assert np.linalg.norm(b - A @ x) < 1e-9
+    assert relerr(x, np.linalg.solve(A, b)) < 1e-6
+    x2, it2 = o.cjgr(b, x0=np.linalg.solve(A, b))
+    assert it2 <= 1
+    # the iteration cap is honoured
+    o.set_solver(False, cgtol=1e-30, cg_max_iter=7)
+    assert o.cjgr(b)[1] == 7
+
+
+def test_cjgr_step_matches_cholesky_step():
+    """stationary = F (cjgr warm-started from the previous fb) against the
+    factor (stationary = T, sym(A)): the two force solves differ by the
+    asymmetry of A (~1e-5 relative) and the CG tolerance, so the histories
+    agree to ~1e-4 of the force scale; after the first steps the warm start
+    cuts the iterations."""
+    a = case_args("cyl32")
+    oc = orc.Oracle(**a)
+    og = orc.Oracle(**a)
+    og.set_solver(False, cgtol=1e-9, cg_max_iter=3000)
+    hc, hg, its = [], [], []
+    for _ in range(12):
+        hc.append(oc.advance()[0])
+        hg.append(og.advance()[0])
+        its.append(og.cg_iters)
+    hc, hg = np.array(hc), np.array(hg)
+    assert np.max(np.abs(hc - hg)) < 1e-4 * np.max(np.abs(hc))
+    assert its[-1] < its[0]
+
+
+def test_move_translates_the_smearing():
+    """orc_move + setup_reg (src/operators.f90:142-144): moving every point
+    by two cells in x moves reg(f) by two u-face columns (the delta kernel is
+    translation invariant), and the partition of unity holds afterwards."""
+    a = case_args("cyl32")
+    o = orc.Oracle(**a)
+    f = np.random.default_rng(1).standard_normal(o.nf)
+    h0 = o.reg(f)
+    o.move(np.asarray(a["xb"]) + 2 * o.delta, np.asarray(a["yb"]))
+    h1 = o.reg(f)
+    m, n = o.m, o.n
+    u0 = h0[: (m + 1) * n].reshape(n, m + 1)
+    u1 = h1[: (m + 1) * n].reshape(n, m + 1)
+    assert np.max(np.abs(u1[:, 2:] - u0[:, :-2])) < 1e-12 * np.max(np.abs(u0))
+    ones = np.zeros(o.nf)
+    ones[: o.nb] = 1.0
+    hx = o.reg(ones)
+    assert abs(hx[: (m + 1) * n].sum() / o.nb - 1.0) < 1e-12  # sum of delta^2 phi phi over faces = 1 per point
