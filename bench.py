@@ -251,10 +251,22 @@ def main(argv=None):
             wb = time.perf_counter() - t0
             res[name] = args.batch * kb / wb
             res[name + "_tiles"] = ens.tile_shapes(0)
+            # the dominant kernel over the whole batch (the chip filled)
+            res[name + "_pair_us"] = ens.time_dst_pass(2, 100) * 1e3
             log(f"{name}, batch of {args.batch}: {res[name]:.1f} case-steps/s ({kb} steps, {wb * 1e3:.1f} ms); "
                 f"tile shapes {res[name + '_tiles']}")
             del ens
+        esz = {"mixed": 4, "fp64": 8}
+        batched_roofline = {name: {"launch_us": res[name + "_pair_us"],
+                                   "bytes_per_launch": 3 * esz[name] * g.nw * args.batch,
+                                   "achieved_GBs": 3 * esz[name] * g.nw * args.batch / (res[name + "_pair_us"] * 1e-6) / 1e9}
+                            for name in ("mixed", "fp64")}
+        for name, r in batched_roofline.items():
+            r["frac"] = r["achieved_GBs"] / HBM_PEAK_GBS
+            log(f"column pair over the batch of {args.batch} ({name}): {r['launch_us']:.1f} us -> "
+                f"{r['achieved_GBs']:.0f} GB/s ({100 * r['frac']:.1f}% of 8 TB/s)")
         chosen = {"mixed_one_case_steps_s": mixed_s, "batched_mixed_case_steps_s": res["mixed"],
+                  "column_pair_over_batch": batched_roofline,
                   "batched_fp64_case_steps_s": res["fp64"], "batch": args.batch,
                   "tile_shapes": {"one_case_fp64": g.tile_shapes(0), "batched_mixed": res["mixed_tiles"],
                                   "batched_fp64": res["fp64_tiles"]},

@@ -666,20 +666,23 @@ double time_launches(ibfs_handle* h, int reps, F&& f) {
   return ms / reps;
 }
 
-// one FFT pass over a zero W field of group 0, in the engine's element type
+// one FFT pass over zero W fields, one per case of group 0 (the group's
+// batch and tuned tile shape), in the engine's element type
 template <typename T>
 double time_fft_pass(ibfs_handle* h, Engine<T>* E, int axis, int reps) {
+  const int B = E->B;
+  const long nw = E->nw;
   DevBuf<T> a, b;
-  a.alloc(E->nw); b.alloc(E->nw);
+  a.alloc((size_t)B * nw); b.alloc((size_t)B * nw);
   a.zero(h->stream);
   return time_launches(h, reps, [&] {
-    if (axis == 0) E->row_pass(SrcField<T>{a.p, 0}, DstField<T>{b.p, 0}, 1);
-    else if (axis == 1) E->col_pass(SrcField<T>{a.p, 0}, DstField<T>{b.p, 0}, 1);
+    if (axis == 0) E->row_pass(SrcField<T>{a.p, nw}, DstField<T>{b.p, nw}, B);
+    else if (axis == 1) E->col_pass(SrcField<T>{a.p, nw}, DstField<T>{b.p, nw}, B);
     else
-      E->launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{E->geo, SrcField<T>{a.p, 0},
-                                                                        MidScale<T>{E->laminv.p}, DstField<T>{b.p, 0},
+      E->launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{E->geo, SrcField<T>{a.p, nw},
+                                                                        MidScale<T>{E->laminv.p}, DstField<T>{b.p, nw},
                                                                         nullptr, 0},
-                     E->geo.mm1, 2, 1);
+                     E->geo.mm1, 2, B);
   });
 }
 

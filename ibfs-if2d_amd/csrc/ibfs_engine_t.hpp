@@ -736,6 +736,7 @@ struct Engine : GroupBase {
     HIPCHK(hipStreamSynchronize(st));
     build_smear(h);
     upload_smear();
+    cg_drop_graphs();  // they hold the old tables
     Gm.alloc(std::max<long>(gmat_tot, 1));  // rot(reg(A^-1)) rows follow the new vertex set
     h.have_chol = false;
     RR.zero(st);
@@ -752,6 +753,10 @@ struct Engine : GroupBase {
   DevBuf<double> cg_r, cg_d, cg_q;
   DevBuf<CgState> cg_dev;
   CgState* cg_host = nullptr;  // pinned
+  // one CG iteration as a graph: [0] r -= alpha q, [1] the residual
+  // recomputed from b - A x (every 50th iteration)
+  hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
+  static constexpr int kCgBatch = 8;  // iterations replayed between two read-backs
 
   void cg_alloc() {
     if (cg_host) return;
@@ -766,9 +771,31 @@ struct Engine : GroupBase {
     cg_dev.alloc(1);
     HIPCHK(hipHostMalloc((void**)&cg_host, sizeof(CgState), hipHostMallocDefault));
   }
+  void cg_drop_graphs() {
+    for (auto& e : cg_exec) {
+      if (e) (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  }
   void cg_release() {
+    cg_drop_graphs();
     if (cg_host) (void)hipHostFree(cg_host);
     cg_host = nullptr;
+  }
+  void cg_capture() {
+    const int nf = cs[0].nf;
+    for (int v = 0; v < 2; v++) {
+      if (cg_exec[v]) continue;
+      hipGraph_t gr = nullptr;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      a_times_one(cg_d.p, cg_q.p);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, nf, cg_d.p, cg_q.p, fb.p, cg_dev.p);
+      if (v) a_times_one(fb.p, cg_q.p);
+      hipLaunchKernelGGL(k_cg_resid, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p, v);
+      HIPCHK(hipStreamEndCapture(st, &gr));
+      HIPCHK(hipGraphInstantiate(&cg_exec[v], gr, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(gr));
+    }
   }
 
   // out = a_times(x) (src/operators.f90:444-462) for case 0: ainv(rot(reg(x)))
@@ -789,21 +816,24 @@ struct Engine : GroupBase {
     return *cg_host;
   }
 
-  // fb <- cjgr(fb, rhsf); returns the iteration count
+  // fb <- cjgr(fb, rhsf); returns the iteration count.  Iterations are
+  // replayed from the captured graphs kCgBatch at a time, then the state is
+  // read back; replays after the loop test fails are no-ops (CgState::done),
+  // so the iterate is the one the loop would stop at.  A replay that is not
+  // a no-op performs iteration c.iter + k + 1, which fixes its variant.
   int cg_solve(double cgtol, int cg_max_iter) {
     const int nf = cs[0].nf;
     if (nf == 0) return 0;
+    cg_capture();
+    *cg_host = CgState{};
+    cg_host->eps = cgtol * cgtol;
+    cg_host->max_iter = cg_max_iter;
+    HIPCHK(hipMemcpyAsync(cg_dev.p, cg_host, sizeof(CgState), hipMemcpyHostToDevice, st));
     a_times_one(fb.p, cg_q.p);
     hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p);
     CgState c = cg_read();
-    const double eps = cgtol * cgtol;
-    for (int guard = 0; guard <= cg_max_iter && c.iter < cg_max_iter && c.delta_new > eps; guard++) {
-      a_times_one(cg_d.p, cg_q.p);
-      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, nf, cg_d.p, cg_q.p, fb.p, cg_dev.p);
-      const int recompute = (c.iter + 1) % 50 == 0;
-      if (recompute) a_times_one(fb.p, cg_q.p);
-      hipLaunchKernelGGL(k_cg_resid, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p,
-                         recompute);
+    for (int guard = 0; !c.done && guard <= cg_max_iter; guard += kCgBatch) {
+      for (int k = 0; k < kCgBatch; k++) HIPCHK(hipGraphLaunch(cg_exec[(c.iter + k + 1) % 50 == 0], st));
       c = cg_read();
     }
     return c.iter;

@@ -938,15 +938,34 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
 // index order by one thread, as DOT_PRODUCT, and the vector updates are
 // not contracted into FMAs, so they round as the reference's statements.
 // ---------------------------------------------------------------------------
+// done: the reference's loop test (iter < cg_max_iter and delta_new > eps)
+// has failed; every later CG kernel is then a no-op, so iterations can be
+// replayed in batches without changing the iterate.
 struct CgState {
-  double delta_new, delta_old, alpha, beta;
-  int iter, pad;
+  double delta_new, delta_old, alpha, beta, eps;
+  int iter, max_iter, done, pad;
 };
 
-__device__ __forceinline__ double dot_in_order(int nf, const double* __restrict__ a, const double* __restrict__ b) {
+// a . b summed in index order; called by every thread of the workgroup
+// (the vectors are staged through LDS in chunks by all threads, one memory
+// round trip per chunk, and thread 0 sums the chunk); the sum is returned
+// to thread 0
+constexpr int kDotChunk = 1024;
+__device__ double dot_in_order(int nf, const double* __restrict__ a, const double* __restrict__ b) {
 #pragma clang fp contract(off)
+  __shared__ double sa[kDotChunk], sb[kDotChunk];
   double s = 0.0;
-  for (int f = 0; f < nf; f++) s = s + a[f] * b[f];
+  for (int c0 = 0; c0 < nf; c0 += kDotChunk) {
+    const int cn = min(kDotChunk, nf - c0);
+    for (int f = threadIdx.x; f < cn; f += blockDim.x) {
+      sa[f] = a[c0 + f];
+      sb[f] = b[c0 + f];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int f = 0; f < cn; f++) s = s + sa[f] * sb[f];
+    __syncthreads();
+  }
   return s;
 }
 
@@ -961,9 +980,11 @@ __global__ __launch_bounds__(256) void k_cg_start(int nf, const double* __restri
     d[f] = v;
   }
   __syncthreads();
+  const double dn = dot_in_order(nf, r, d);
   if (threadIdx.x == 0) {
-    cs->delta_new = dot_in_order(nf, r, d);
+    cs->delta_new = dn;
     cs->iter = 0;
+    cs->done = !(0 < cs->max_iter && dn > cs->eps);
   }
 }
 
@@ -974,8 +995,10 @@ __global__ __launch_bounds__(256) void k_cg_alpha(int nf, const double* __restri
                                                   CgState* __restrict__ cs) {
 #pragma clang fp contract(off)
   __shared__ double alpha;
+  if (cs->done) return;
+  const double dq = dot_in_order(nf, d, q);
   if (threadIdx.x == 0) {
-    alpha = cs->delta_new / dot_in_order(nf, d, q);
+    alpha = cs->delta_new / dq;
     cs->alpha = alpha;
     cs->iter = cs->iter + 1;
   }
@@ -992,16 +1015,19 @@ __global__ __launch_bounds__(256) void k_cg_resid(int nf, const double* __restri
                                                   double* __restrict__ d, CgState* __restrict__ cs, int recompute) {
 #pragma clang fp contract(off)
   __shared__ double beta;
+  if (cs->done) return;
   const double alpha = cs->alpha;
   for (int f = threadIdx.x; f < nf; f += blockDim.x) r[f] = recompute ? b[f] - q[f] : r[f] - alpha * q[f];
   __syncthreads();
+  const double rr = dot_in_order(nf, r, r);
   if (threadIdx.x == 0) {
     const double dold = cs->delta_new;
-    const double dnew = dot_in_order(nf, r, r);
+    const double dnew = rr;
     cs->delta_old = dold;
     cs->delta_new = dnew;
     beta = dnew / dold;
     cs->beta = beta;
+    cs->done = !(cs->iter < cs->max_iter && dnew > cs->eps);
   }
   __syncthreads();
   for (int f = threadIdx.x; f < nf; f += blockDim.x) d[f] = r[f] + beta * d[f];

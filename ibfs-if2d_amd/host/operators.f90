@@ -23,9 +23,10 @@ CONTAINS
 
   SUBROUTINE preprocess
     !-------------------------------------------------------------------------!
-    ! src/operators.f90:29-59: reuse input/ibfs.chd if present, else build
-    ! the force-system matrix and its Cholesky factor (on the GPU) and cache
-    ! it.  As in the reference, a cached factor is trusted as is.
+    ! src/operators.f90:29-59: reuse input/ibfs.chd if it is this case's
+    ! factor (chd_is_current: record size and the key written beside it),
+    ! else build the force-system matrix and its Cholesky factor (on the
+    ! GPU) and cache it with its key.
     !-------------------------------------------------------------------------!
     LOGICAL :: readchol
 
@@ -36,6 +37,7 @@ CONTAINS
        RETURN
     END IF
     INQUIRE(file='input/ibfs.chd',exist=readchol)
+    IF (readchol) readchol = chd_is_current()
     IF (readchol) THEN
        CALL read_cholesky
        CALL ibfs_check(ibfs_set_cholesky(handle, cholmat, cholvec), 'ibfs_set_cholesky')

@@ -29,6 +29,7 @@ CONTAINS
     INTEGER(C_INT), ALLOCATABLE :: bd(:)
     INTEGER :: uf
     INTEGER(C_INT) :: st
+    LOGICAL :: readchol
 
     uf = 0
     IF (unif_flow) uf = 1
@@ -54,8 +55,13 @@ CONTAINS
        CALL read_variables(istart)
        CALL ibfs_check(ibfs_set_state(handle, omega, rhs_old, q, q0, fb), 'ibfs_set_state')
        IF (stationary) THEN
-          CALL read_cholesky
-          CALL ibfs_check(ibfs_set_cholesky(handle, cholmat, cholvec), 'ibfs_set_cholesky')
+          readchol = .FALSE.
+          INQUIRE(file='input/ibfs.chd', exist=readchol)
+          IF (readchol) readchol = chd_is_current()
+          IF (readchol) THEN  ! else preprocess builds it before the first step
+             CALL read_cholesky
+             CALL ibfs_check(ibfs_set_cholesky(handle, cholmat, cholvec), 'ibfs_set_cholesky')
+          END IF
        END IF
     END IF
   END SUBROUTINE setup_variables
@@ -72,8 +78,70 @@ CONTAINS
     OPEN(unit=100,file="input/ibfs.chd",form="unformatted",status="unknown")
     WRITE(100) cholmat, cholvec
     CLOSE(100)
+    ! what the factor was built for, checked before the file is reused
+    OPEN(unit=100,file="input/ibfs.chd.key",form="formatted",status="replace")
+    WRITE(100,"(*(ES25.17))") chd_key()
+    CLOSE(100)
     WRITE(111,*) "Saving cholesky decomposition..."
   END SUBROUTINE write_cholesky
+
+  FUNCTION chd_key() RESULT(key)
+    !-------------------------------------------------------------------------!
+    ! The inputs the force-system factor depends on: grid, levels, domain,
+    ! dt and Re (ainv of level 1 is in A), the point count and moments of
+    ! the point list that change with any point moved or reordered.
+    !-------------------------------------------------------------------------!
+    REAL(KIND(0.D0)), DIMENSION(14) :: key
+    INTEGER :: k
+    key = 0.d0
+    key(1) = m; key(2) = n; key(3) = mgridlev; key(4) = nb
+    key(5) = len; key(6) = offsetx; key(7) = offsety; key(8) = dt; key(9) = Re
+    DO k = 1, nb
+       key(10) = key(10) + xb(k)
+       key(11) = key(11) + yb(k)
+       key(12) = key(12) + REAL(k,KIND(0.D0))*xb(k)
+       key(13) = key(13) + REAL(k,KIND(0.D0))*yb(k)
+       key(14) = key(14) + xb(k)*xb(k) + yb(k)*yb(k)
+    END DO
+  END FUNCTION chd_key
+
+  LOGICAL FUNCTION chd_is_current()
+    !-------------------------------------------------------------------------!
+    ! The reference reuses input/ibfs.chd whatever run wrote it
+    ! (src/operators.f90:41-43).  Here a cached factor is used only if its
+    ! record has the size of this run's factor and, when input/ibfs.chd.key
+    ! is present (every factor this program writes has one), that key
+    ! matches this run.  A file without a key (written by the reference) is
+    ! trusted on its size alone.
+    !-------------------------------------------------------------------------!
+    INTEGER(KIND=8) :: sz, want
+    LOGICAL :: haskey
+    REAL(KIND(0.D0)), DIMENSION(14) :: key, cur
+    INTEGER :: k, ios
+
+    chd_is_current = .FALSE.
+    INQUIRE(file="input/ibfs.chd", size=sz)
+    want = 8_8 * (INT(nf,8)*INT(nf,8) + INT(nf,8)) + 8_8
+    IF (sz /= want) THEN
+       WRITE(111,*) "input/ibfs.chd has", sz, " bytes, this run's factor needs", want
+       RETURN
+    END IF
+    INQUIRE(file="input/ibfs.chd.key", exist=haskey)
+    IF (haskey) THEN
+       OPEN(unit=100,file="input/ibfs.chd.key",form="formatted",status="old")
+       READ(100,*,iostat=ios) key
+       CLOSE(100)
+       IF (ios /= 0) RETURN
+       cur = chd_key()
+       DO k = 1, 14
+          IF (ABS(key(k) - cur(k)) > 1.d-12*MAX(1.d0, ABS(cur(k)))) THEN
+             WRITE(111,*) "input/ibfs.chd was built for another case (key entry", k, ")"
+             RETURN
+          END IF
+       END DO
+    END IF
+    chd_is_current = .TRUE.
+  END FUNCTION chd_is_current
 
   SUBROUTINE read_cholesky
     OPEN(unit=100,file="input/ibfs.chd",form="unformatted",status="unknown")

@@ -185,9 +185,9 @@ class Solver:
         return ms.value
 
     def time_dst_pass(self, axis, reps=200):
-        """Mean device time (ms) of one pass over a W field: axis 0 = DST-I
-        along i, 1 = along j, 2 = the fused column pair of ctci (along j,
-        times laminv, along j)."""
+        """Mean device time (ms) of one pass over a W field per case of the
+        first group: axis 0 = DST-I along i, 1 = along j, 2 = the fused
+        column pair of ctci (along j, times laminv, along j)."""
         ms = C.c_double()
         self._chk(_lib.ibfs_time_dst_pass(self._h, axis, reps, C.byref(ms)), "ibfs_time_dst_pass")
         return ms.value
@@ -391,6 +391,7 @@ class Ensemble:
     _chk = Solver._chk
     tile_shapes = Solver.tile_shapes
     ngroups = Solver.ngroups
+    time_dst_pass = Solver.time_dst_pass
     _ncases = Solver._ncases
     _nf_total = Solver._nf_total
     objective = Solver.objective

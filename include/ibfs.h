@@ -209,10 +209,12 @@ int ibfs_last_advance_ms(const ibfs_handle* h, double* ms);
 int ibfs_tile_shapes(const ibfs_handle* h, int group, int* ngroups, int* shapes);
 
 /* Measurement only (no counterpart in the reference): launches one 1-D
- * DST-I pass over a W field reps times on the handle's stream and returns
- * the mean launch duration from HIP events.  axis 0 = along i (rows),
- * 1 = along j (columns), 2 = the fused column pair of ctci (DST-I along j,
- * times laminv, DST-I along j: one read and one write of the field). */
+ * DST-I pass over a W field per case of the handle's first group (one field
+ * for a single-case handle, B for a batch of B equal-size cases) reps times
+ * on the handle's stream and returns the mean launch duration from HIP
+ * events.  axis 0 = along i (rows), 1 = along j (columns), 2 = the fused
+ * column pair of ctci (DST-I along j, times laminv, DST-I along j: one read
+ * and one write of the field). */
 int ibfs_time_dst_pass(ibfs_handle* h, int axis, int reps, double* ms_per_launch);
 
 /* Measurement only (no counterpart in the reference): the dense form of a

@@ -182,3 +182,32 @@ def test_host_moving_body_without_motion(host_cg_run, tmp_path):
     run_host(d2, 0, 10, stationary=False)
     for fn in ("force.dat", "circ.dat", "slip.dat"):
         assert np.array_equal(read_table(os.path.join(d, "output", fn)), read_table(os.path.join(d2, "output", fn))), fn
+
+
+def test_host_rebuilds_a_stale_factor(host_run, tmp_path):
+    """A cached input/ibfs.chd built for another case is not reused (the
+    reference reuses whatever file is there, src/operators.f90:41-43): a run
+    with another dt and the first run's factor and key rebuilds the factor
+    and writes the same files as a run that starts without any."""
+    d, _ = host_run
+    assert os.path.exists(os.path.join(d, "input", "ibfs.chd.key"))
+    stale, fresh = str(tmp_path / "stale"), str(tmp_path / "fresh")
+    shutil.copytree(os.path.join(d, "input"), os.path.join(stale, "input"))
+    shutil.copytree(os.path.join(d, "input"), os.path.join(fresh, "input"))
+    for fn in ("ibfs.chd", "ibfs.chd.key"):
+        os.remove(os.path.join(fresh, "input", fn))
+    global NML
+    nml0 = NML
+    NML = NML.replace("DT = 0.02", "DT = 0.015")
+    try:
+        run_host(stale, 0, 4)
+        run_host(fresh, 0, 4)
+    finally:
+        NML = nml0
+    with open(os.path.join(stale, "output", "ibfs.inf")) as f:
+        assert "built for another case" in f.read()
+    assert np.array_equal(read_table(os.path.join(stale, "output", "force.dat")),
+                          read_table(os.path.join(fresh, "output", "force.dat")))
+    a = read_record(os.path.join(stale, "input", "ibfs.chd"))
+    b = read_record(os.path.join(fresh, "input", "ibfs.chd"))
+    assert np.array_equal(a, b)
