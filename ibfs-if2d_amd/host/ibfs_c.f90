@@ -78,6 +78,12 @@ MODULE ibfs_c
        INTEGER(C_INT), DIMENSION(*) :: iters
      END FUNCTION ibfs_cg_iterations
 
+     INTEGER(C_INT) FUNCTION ibfs_op_reg(h, h0, hq) BIND(C, name='ibfs_op_reg')
+       IMPORT :: C_INT, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: h
+       REAL(C_DOUBLE), DIMENSION(*) :: h0, hq
+     END FUNCTION ibfs_op_reg
+
      FUNCTION ibfs_last_error(h) BIND(C, name='ibfs_last_error') RESULT(msg)
        IMPORT :: C_PTR
        TYPE(C_PTR), VALUE :: h

@@ -2,6 +2,7 @@
 
     libibfs_hip.so   hipcc, from csrc/ibfs_capi.hip (kernels in the headers)
     host/ibfs_gpu    amdflang, the Fortran host linked against libibfs_hip.so
+    host/ibfs_post   amdflang, the post-processing program (Tecplot files)
 """
 from __future__ import annotations
 
@@ -13,6 +14,7 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))  # .../ibfs-if
 LIB = os.path.join(PKG, "libibfs_hip.so")
 HOST_DIR = os.path.join(PKG, "host")
 HOST_EXE = os.path.join(HOST_DIR, "ibfs_gpu")
+POST_EXE = os.path.join(HOST_DIR, "ibfs_post")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
@@ -42,7 +44,7 @@ def build_lib() -> str:
 
 
 def build_host() -> str | None:
-    if not all(os.path.exists(os.path.join(HOST_DIR, s)) for s in HOST_SOURCES):
+    if not all(os.path.exists(os.path.join(HOST_DIR, s)) for s in HOST_SOURCES + ["post.f90"]):
         return None
     if not os.path.exists(FC):
         return None
@@ -54,6 +56,11 @@ def build_host() -> str | None:
         _run([FC, "-O2", "-c", os.path.join(HOST_DIR, s), "-o", o, "-module-dir", bdir], cwd=bdir)
         objs.append(o)
     _run([FC, "-O2", *objs, "-o", HOST_EXE, "-L" + PKG, "-libfs_hip", "-Wl,-rpath,$ORIGIN/.."])
+    # the post-processing program: the same modules with post.f90 as the driver
+    po = os.path.join(bdir, "post.o")
+    _run([FC, "-O2", "-c", os.path.join(HOST_DIR, "post.f90"), "-o", po, "-module-dir", bdir], cwd=bdir)
+    mods = [o for o in objs if not o.endswith("main.o")]
+    _run([FC, "-O2", *mods, po, "-o", POST_EXE, "-L" + PKG, "-libfs_hip", "-Wl,-rpath,$ORIGIN/.."])
     shutil.rmtree(bdir, ignore_errors=True)
     return HOST_EXE
 

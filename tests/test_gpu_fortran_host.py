@@ -5,6 +5,7 @@ with the step done by the gfx950 library.  Its outputs are compared with the
 oracle run of the same case, and a restart from a saved state continues the
 run exactly."""
 import os
+import re
 import shutil
 import struct
 import subprocess
@@ -211,3 +212,63 @@ def test_host_rebuilds_a_stale_factor(host_run, tmp_path):
     a = read_record(os.path.join(stale, "input", "ibfs.chd"))
     b = read_record(os.path.join(fresh, "input", "ibfs.chd"))
     assert np.array_equal(a, b)
+
+
+def test_post_writes_the_tecplot_files(host_run, tmp_path):
+    """ibfs_post (the flow of src/post.f90) reads the saved restart files and
+    writes write_tecplot's zone per level (src/operators.f90:702-763):
+    coordinates, vertex velocity of q + q0 over the level spacing, the
+    finest level's reg(fb) force density and vorticity over the spacing
+    squared.  Checked against the same quantities from the restart record
+    and the oracle's reg, to the 5 significant digits of the E14.5 format."""
+    d, _ = host_run
+    d2 = str(tmp_path)
+    shutil.copytree(os.path.join(d, "input"), os.path.join(d2, "input"))
+    os.makedirs(os.path.join(d2, "output"))
+    os.makedirs(os.path.join(d2, "post"))
+    for it in (5, 10):
+        shutil.copy(os.path.join(d, "output", f"ibfs{it:07d}.var"), os.path.join(d2, "output"))
+    with open(os.path.join(d2, "input", "ibfs.inp"), "w") as f:
+        f.write(NML.format(istart=5, istop=10, stationary="T"))
+    r = subprocess.run([os.path.join(ROOT, "ibfs-if2d_amd", "host", "ibfs_post")], cwd=d2, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from ibfs_amd import geometry
+
+    xb, yb, bd = geometry.read_geom(os.path.join(d2, "input", "geom.inp"))
+    o = orc.Oracle(32, 32, 3, 0.02, 100.0, 4.0, 2.0, 2.0, xb, yb, bd)
+    m = n = 32
+    W, Q = o.nw * o.L, o.nq * o.L
+    var = read_record(os.path.join(d2, "output", "ibfs0000010.var"))
+    om, fb = var[:W], var[W:W + o.nf]
+    q, q0 = var[2 * W + o.nf:2 * W + o.nf + Q], var[2 * W + o.nf + Q:]
+    etf = o.reg(fb)
+    iu = lambda i, j: (i - 1) + (j - 1) * (m + 1)
+    iv = lambda i, j: (m + 1) * n + (i - 1) + (j - 1) * m
+    for lev in (1, 2, 3):
+        fn = os.path.join(d2, "post", f"ibfs_g{lev}t0000010_vel.dat")
+        with open(fn) as f:
+            lines = f.read().splitlines()
+        assert lines[0].strip().startswith("TITLE") and "DATAPACKING=POINT" in lines[10]  # 12 header lines
+        # Ew.d drops the E of three-digit exponents (0.12345-100)
+        num = lambda v: float(re.sub(r"(\d)([+-]\d{3})$", r"\1E\2", v))
+        data = np.array([[num(v) for v in ln.split()] for ln in lines[12:] if ln.strip()])
+        assert data.shape == ((m - 1) * (n - 1), 7)
+        fac = 2.0 ** (lev - 1)
+        dl = fac * o.delta
+        ql, q0l = q[(lev - 1) * o.nq:lev * o.nq], q0[(lev - 1) * o.nq:lev * o.nq]
+        exp = []
+        for j in range(2, n + 1):
+            for i in range(2, m + 1):
+                exp.append([fac * o.delta * (i - 1) - (fac - 1) / 2 * 4.0 - 2.0,
+                            fac * o.delta * (j - 1) - (fac - 1) / 2 * o.delta * n - 2.0,
+                            0.5 * (ql[iu(i, j)] + ql[iu(i, j - 1)] + q0l[iu(i, j)] + q0l[iu(i, j - 1)]) / dl,
+                            0.5 * (ql[iv(i, j)] + ql[iv(i - 1, j)] + q0l[iv(i, j)] + q0l[iv(i - 1, j)]) / dl,
+                            o.delta * (etf[iu(i, j)] + etf[iu(i, j - 1)]) / 0.02,
+                            o.delta * (etf[iv(i, j)] + etf[iv(i - 1, j)]) / 0.02,
+                            om[(lev - 1) * o.nw + (i - 2) + (j - 2) * (m - 1)] / dl / dl])
+        exp = np.array(exp)
+        for c in range(7):
+            # E14.5 keeps 5 significant digits: half a unit of the 5th is 5e-5 relative
+            scale = np.max(np.abs(exp[:, c]))
+            assert np.all(np.abs(data[:, c] - exp[:, c]) <= 5.01e-5 * np.abs(exp[:, c]) + 1e-9 * scale), (lev, c)
