@@ -67,6 +67,16 @@ struct op_load_post<Op, std::void_t<decltype(Op::kLoadPost)>> : std::bool_consta
 constexpr int kThreads = 256;
 constexpr int kMaxStages = 16;
 
+// Workgroup x of a row of n workgroups runs on XCD x % 8 (round-robin
+// dispatch over the 8 XCDs).  Returns x's place in an order that gives XCD k
+// the contiguous range [k q + min(k, r), ...) of q + (k < r) workgroups,
+// q = n / 8, r = n % 8: a bijection of [0, n) for every n, so that
+// neighbouring tiles, which share cache lines, share one XCD's L2.
+__device__ __forceinline__ int xcd_contiguous(int x, int n) {
+  const int q = n >> 3, r = n & 7, k = x & 7;
+  return k * q + min(k, r) + (x >> 3);
+}
+
 template <typename T>
 struct cplx {
   T re, im;
@@ -369,8 +379,12 @@ __device__ __forceinline__ void tw_load(cplx<T>* wr, const cplx<T>* __restrict__
   }
 }
 
-template <typename T, int N2, int P, int S>
-__device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* wr, int t) {
+// One Stockham stage from zi to zo.  In place (zi == zo, DB false) the
+// stage reads, waits at a barrier, then writes; with two buffers (DB) the
+// writes go to the other buffer at once, and the stage ends at its only
+// barrier.
+template <typename T, int N2, int P, int S, bool DB>
+__device__ __forceinline__ void stage(const cplx<T>* zi, cplx<T>* zo, const cplx<T>* wr, int t) {
   constexpr Sched sc = make_sched(N2);
   constexpr int R = sc.rad[S];
   constexpr int NS = sc.ns[S];
@@ -379,7 +393,8 @@ __device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* wr
   constexpr int NBT = (NR + TPF - 1) / TPF;
   constexpr int O = TwLayout<N2>::off(S);
   const int p = t / TPF, lt = t % TPF;
-  cplx<T>* zp = z + p * N2;
+  const cplx<T>* zp = zi + p * N2;
+  cplx<T>* zq = zo + p * N2;
   cplx<T> v[NBT][R];
 #pragma unroll
   for (int i = 0; i < NBT; i++) {
@@ -393,7 +408,7 @@ __device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* wr
       }
     }
   }
-  __syncthreads();
+  if constexpr (!DB) __syncthreads();
 #pragma unroll
   for (int i = 0; i < NBT; i++) {
     const int j = lt + i * TPF;
@@ -402,20 +417,34 @@ __device__ __forceinline__ void stage(cplx<T>* __restrict__ z, const cplx<T>* wr
       const int k = j % NS;
       const int d = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) zp[d + r * NS] = v[i][r];
+      for (int r = 0; r < R; r++) zq[d + r * NS] = v[i][r];
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N2, int P, int S>
-__device__ __forceinline__ void run(cplx<T>* z, const cplx<T>* wr, int t) {
+// All stages from za; returns the buffer holding the result (za, or zb
+// after an odd number of stages with two buffers).
+template <typename T, int N2, int P, int S, bool DB>
+__device__ __forceinline__ cplx<T>* run(cplx<T>* za, cplx<T>* zb, const cplx<T>* wr, int t) {
   constexpr Sched sc = make_sched(N2);
   if constexpr (S < sc.nst) {
-    stage<T, N2, P, S>(z, wr, t);
-    run<T, N2, P, S + 1>(z, wr, t);
+    if constexpr (DB) {
+      stage<T, N2, P, S, true>(za, zb, wr, t);
+      return run<T, N2, P, S + 1, true>(zb, za, wr, t);
+    } else {
+      stage<T, N2, P, S, false>(za, za, wr, t);
+      return run<T, N2, P, S + 1, false>(za, za, wr, t);
+    }
+  } else {
+    return za;
   }
 }
+
+// two LDS buffers when they fit in 32 KB (the single-case tiles), one
+// otherwise, so that the larger tiles keep their occupancy
+template <typename T, int N2, int P>
+constexpr bool kTwoBuf = 2L * P * N2 * (long)sizeof(cplx<T>) <= 32768;
 
 // Loads of one pass: every thread first issues all of its NL loads (values
 // kept in registers), then writes them into the odd extensions, so the loads
@@ -456,7 +485,8 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
   constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
   constexpr int NQ = (P * LEN + NT - 1) / NT;  // outputs (FFT, k) per thread
   constexpr bool kPre = op_store_pre<Op>::value;
-  __shared__ cplx<T> z[P * N2];
+  constexpr bool DB = kTwoBuf<T, N2, P>;
+  __shared__ cplx<T> z[(DB ? 2 : 1) * P * N2];
   const int t = threadIdx.x;
   const int b = blockIdx.y;
   int g = blockIdx.x;
@@ -487,14 +517,14 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
   }
   load_seqs<T, N2, P, AXIS, Op>(op, z, b, g, t);
   __syncthreads();
-  run<T, N2, P, 0>(z, wr, t);
+  const cplx<T>* zr = run<T, N2, P, 0, DB>(z, DB ? z + P * N2 : z, wr, t);
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     const int e = t + q * NT;
     if (e < P * LEN) {
       int p, k;
       out_at(e, p, k);
-      const cplx<T> Z = z[p * N2 + k];
+      const cplx<T> Z = zr[p * N2 + k];
       if constexpr (kPre) op.store_put(b, g, p, P, k, -Z.im, Z.re, oa[q], ob[q]);
       else op.store(b, g, p, P, k, -Z.im, Z.re);
     }
@@ -507,7 +537,9 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_col2_n(Op op, const cplx<T>* _
   constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
   constexpr int NQ = (P * LEN + NT - 1) / NT;  // (FFT, k) places per thread
   constexpr bool kCoef = op_has_mid_coef<Op>::value;
-  __shared__ cplx<T> z[P * N2];
+  constexpr bool DB = kTwoBuf<T, N2, P>;
+  __shared__ cplx<T> z[(DB ? 2 : 1) * P * N2];
+  cplx<T>* const z2 = DB ? z + P * N2 : z;
   const int t = threadIdx.x;
   const int b = blockIdx.y;
   int g = blockIdx.x;
@@ -533,35 +565,38 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_col2_n(Op op, const cplx<T>* _
   }
   load_seqs<T, N2, P, 1, Op>(op, z, b, g, t);
   __syncthreads();
-  run<T, N2, P, 0>(z, wr, t);
+  // the middle step works in place on the first transform's result (each
+  // place (p, k) is read and rewritten by one thread; the mirrored places
+  // N2-k and the zeros at 0 and N are read by no one)
+  cplx<T>* const zm = run<T, N2, P, 0, DB>(z, z2, wr, t);
   if constexpr (kCoef) {
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       const int e = t + q * NT;
       if (e < P * LEN) {
         const int k = e / P + 1, p = e - (k - 1) * P;
-        const cplx<T> Z = z[p * N2 + k];
+        const cplx<T> Z = zm[p * N2 + k];
         T va, vb;
         op.mid_apply(b, g, p, P, k, -Z.im, Z.re, ca[q], cb[q], va, vb);
-        z[p * N2 + k] = {va, vb};
-        z[p * N2 + N2 - k] = {-va, -vb};
+        zm[p * N2 + k] = {va, vb};
+        zm[p * N2 + N2 - k] = {-va, -vb};
       }
     }
     if (t < P) {
-      z[t * N2] = {T(0), T(0)};
-      z[t * N2 + N] = {T(0), T(0)};
+      zm[t * N2] = {T(0), T(0)};
+      zm[t * N2 + N] = {T(0), T(0)};
     }
     __syncthreads();
   } else {
-    col2_mid<T, Op>(op, z, b, g, P, N2, t, NT);
+    col2_mid<T, Op>(op, zm, b, g, P, N2, t, NT);
   }
-  run<T, N2, P, 0>(z, wr, t);
+  const cplx<T>* zr = run<T, N2, P, 0, DB>(zm, zm == z ? z2 : z, wr, t);
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     const int e = t + q * NT;
     if (e < P * LEN) {
       const int k = e / P + 1, p = e - (k - 1) * P;
-      const cplx<T> Z = z[p * N2 + k];
+      const cplx<T> Z = zr[p * N2 + k];
       op.store(b, g, p, P, k, -Z.im, Z.re);
     }
   }

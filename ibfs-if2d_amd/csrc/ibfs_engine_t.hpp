@@ -639,7 +639,7 @@ struct Engine : GroupBase {
     if constexpr (kSolve64) vort2flux_fine64(omega.p, B, Lc.p);
     else vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
     // fluxes and diagnostics (src/operators.f90:184-186)
-    const int ncurl = (nq + 255) / 256 * Lmax;
+    const int ncurl = (nq + 256 * kTailFaces - 1) / (256 * kTailFaces) * Lmax;
     hipLaunchKernelGGL(k_tail<T>, dim3(ncurl + nchunk * Lmax, B), dim3(256), 0, st, geo, S.p, q.p, omega.p, FB, QB,
                        partial.p, nchunk, Lc.p, kSolve64 ? (const double*)S64.p : nullptr);
     hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, sm, q.p, QB, motion.p, fb.p, wsum.p, bdyno.p,

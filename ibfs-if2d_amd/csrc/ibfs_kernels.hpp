@@ -451,8 +451,11 @@ __global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ 
 template <typename T>
 __global__ void k_coarsify2(Geo g, T* __restrict__ om, long bs, const int* __restrict__ Lc, int lev, int nblk) {
   const int wi = g.m / 2 - 1, wj = g.n / 2 - 1;
-  const int upper = blockIdx.x >= nblk;
-  const int t = (blockIdx.x - upper * nblk) * blockDim.x + threadIdx.x;
+  // XCD-contiguous, so the fine rows shared by the stencils of neighbouring
+  // groups stay in one L2
+  const int x = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int upper = x >= nblk;
+  const int t = (x - upper * nblk) * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   const int lc = lev + upper;  // level written
   if (t >= wi * wj || !active(Lc, b, lc)) return;
@@ -863,6 +866,7 @@ __global__ void k_make_g(Smear sm, int ci, const double* __restrict__ Ai, double
 // grid (ceil(max rows / kForceRows), B), 256 threads, LDS 2*nf_max doubles.
 constexpr int kForceThreads = 512;
 constexpr int kRowsPerWave = 2;
+constexpr int kColBlock = 10;
 constexpr int kForceRows = kForceThreads / 64 * kRowsPerWave;
 
 template <typename T, typename TS = T>
@@ -881,6 +885,31 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
   const int row0 = blockIdx.x * kForceRows;
   if (row0 >= nrows) return;
   const int NT = blockDim.x;
+  const int lane = tid & 63;
+  const int rbase = row0 + (tid >> 6) * kRowsPerWave;
+  const double* Mr[kRowsPerWave];
+  int xo[kRowsPerWave];
+  double acc[kRowsPerWave];
+#pragma unroll
+  for (int q = 0; q < kRowsPerWave; q++) {
+    const int r = min(rbase + q, nrows - 1);  // rows past the end repeat the last row and are not stored
+    const bool frow = r < nf;
+    Mr[q] = frow ? Ainv + c.loff + (long)r * nf : G + c.goff + (long)(r - nf) * nf;
+    xo[q] = (!frow && mode == 2) ? nf : 0;
+    acc[q] = 0.0;
+  }
+  // the first kColBlock columns per lane of this wave's rows, loaded before
+  // the right-hand side is evaluated (they do not depend on it)
+  double mv[kColBlock][kRowsPerWave];
+  auto load_block = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < kColBlock; u++) {
+      const int k = min(k0 + 64 * u, nf - 1);
+#pragma unroll
+      for (int q = 0; q < kRowsPerWave; q++) mv[u][q] = k >= 0 ? Mr[q][k] : 0.0;
+    }
+  };
+  load_block(lane);
   if (mode < 2) {
     const TS* s1b = s1 + b * bs;
     const TS* sc = (s2 != nullptr && Lc[b] > 1) ? s2 + b * bs : nullptr;
@@ -903,22 +932,19 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
     }
   }
   __syncthreads();
-  const int lane = tid & 63;
-  const int rbase = row0 + (tid >> 6) * kRowsPerWave;
-  const double* Mr[kRowsPerWave];
-  int xo[kRowsPerWave];
-  double acc[kRowsPerWave];
+  // the sums keep the column order k = lane, lane+64, ... of a plain loop
+  for (int k0 = lane;;) {
 #pragma unroll
-  for (int q = 0; q < kRowsPerWave; q++) {
-    const int r = min(rbase + q, nrows - 1);  // rows past the end repeat the last row and are not stored
-    const bool frow = r < nf;
-    Mr[q] = frow ? Ainv + c.loff + (long)r * nf : G + c.goff + (long)(r - nf) * nf;
-    xo[q] = (!frow && mode == 2) ? nf : 0;
-    acc[q] = 0.0;
-  }
-  for (int k = lane; k < nf; k += 64) {
+    for (int u = 0; u < kColBlock; u++) {
+      const int k = k0 + 64 * u;
+      if (k < nf) {
 #pragma unroll
-    for (int q = 0; q < kRowsPerWave; q++) acc[q] += Mr[q][k] * vec[xo[q] + k];
+        for (int q = 0; q < kRowsPerWave; q++) acc[q] += mv[u][q] * vec[xo[q] + k];
+      }
+    }
+    k0 += 64 * kColBlock;
+    if (k0 >= nf) break;
+    load_block(k0);
   }
 #pragma unroll
   for (int q = 0; q < kRowsPerWave; q++) {
@@ -1036,14 +1062,63 @@ __global__ __launch_bounds__(256) void k_cg_resid(int nf, const double* __restri
 // ---------------------------------------------------------------------------
 // tail of the step: fluxes, circulation and diagnostics
 // ---------------------------------------------------------------------------
+// Vertex indices of the two streamfunction values whose difference is the
+// curl at interior face f (u faces X(i,j+1) - X(i,j), v faces X(i,j) -
+// X(i+1,j), as curl_face); false for a face on the domain edge, whose value
+// takes the edge vector (curl_face).
+__device__ __forceinline__ bool curl_interior(const Geo& g, int f, int& ia, int& ib) {
+  const int m = g.m, n = g.n, nu = (m + 1) * n;
+  if (f < nu) {
+    const int i = f % (m + 1) + 1, j = f / (m + 1) + 1;
+    ia = wix(g, i, j + 1);
+    ib = wix(g, i, j);
+    return i > 1 && i < m + 1 && j > 1 && j < n;
+  }
+  const int fv = f - nu;
+  const int i = fv % m + 1, j = fv / m + 1;
+  ia = wix(g, i, j);
+  ib = wix(g, i + 1, j);
+  return j > 1 && j < n + 1 && i > 1 && i < m;
+}
+
 // One launch after the last streamfunction of the step is final:
-//  * workgroups x < ncurl = ceil(nq/256)*Lmax: q = curl(s) on level
-//    x / ceil(nq/256) (src/operators.f90:236-238, as k_curl_levels); with
-//    s1d set, the finest level's streamfunction is s1d (FP64);
+//  * workgroups x < ncurl = ceil(nq/(256*kTailFaces))*Lmax: q = curl(s) on
+//    level x / ceil(nq/(256*kTailFaces)) (src/operators.f90:236-238, as
+//    k_curl_levels), kTailFaces faces per thread whose interior loads are
+//    all issued before the first is used; with s1d set, the finest level's
+//    streamfunction is s1d (FP64).  These groups run in XCD-contiguous
+//    order, so the streamfunction rows shared by neighbouring face rows
+//    stay in one L2;
 //  * the next nchunk*Lmax: partial sums of omega for write_circ
 //    (src/operators.f90:675-684), chunk (x-ncurl) % nchunk of level
 //    (x-ncurl) / nchunk, at partial[((b*Lmax)+lev)*nchunk + chunk].
 // grid (ncurl + nchunk*Lmax, B), 256 threads.
+constexpr int kTailFaces = 4;
+template <typename T, typename TS>
+__device__ __forceinline__ void tail_curl(const Geo& g, int f0, const TS* __restrict__ sl, const T* __restrict__ sc,
+                                          T* __restrict__ ql) {
+  int ia[kTailFaces], ib[kTailFaces];
+  bool in[kTailFaces];
+  TS va[kTailFaces], vb[kTailFaces];
+#pragma unroll
+  for (int u = 0; u < kTailFaces; u++) {
+    const int f = f0 + 256 * u;
+    in[u] = f < g.nq && curl_interior(g, f, ia[u], ib[u]);
+    if (!in[u]) ia[u] = ib[u] = 0;
+  }
+#pragma unroll
+  for (int u = 0; u < kTailFaces; u++) {
+    va[u] = sl[ia[u]];
+    vb[u] = sl[ib[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < kTailFaces; u++) {
+    const int f = f0 + 256 * u;
+    if (f >= g.nq) break;
+    ql[f] = in[u] ? T(va[u] - vb[u]) : T(curl_face<TS, T>(g, f, sl, sc));
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T* __restrict__ q,
                                               const T* __restrict__ om, long bs, long qbs,
@@ -1051,20 +1126,20 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
                                               const double* __restrict__ s1d) {
   __shared__ double red[4];
   const int tid = threadIdx.x, b = blockIdx.y;
-  const int ncb = (g.nq + 255) / 256, ncurl = ncb * g.Lmax;
-  const int x = blockIdx.x;
-  if (x < ncurl) {
-    const int lev = x / ncb, f = (x - lev * ncb) * 256 + tid;
-    if (f >= g.nq || !active(Lc, b, lev)) return;
+  const int ncb = (g.nq + 256 * kTailFaces - 1) / (256 * kTailFaces), ncurl = ncb * g.Lmax;
+  if ((int)blockIdx.x < ncurl) {
+    const int x = xcd_contiguous(blockIdx.x, ncurl);
+    const int lev = x / ncb, f0 = (x - lev * ncb) * 256 * kTailFaces + tid;
+    if (!active(Lc, b, lev)) return;
     const T* sl = s + b * bs + (long)lev * g.nw;
     const T* sc = has_coarser(Lc, b, lev) ? sl + g.nw : nullptr;
     if (lev == 0 && s1d != nullptr)  // finest level solved in FP64
-      q[b * qbs + f] = T(curl_face<double, T>(g, f, s1d + b * bs, sc));
+      tail_curl<T, double>(g, f0, s1d + b * bs, sc, q + b * qbs);
     else
-      q[b * qbs + (long)lev * g.nq + f] = curl_face<T>(g, f, sl, sc);
+      tail_curl<T, T>(g, f0, sl, sc, q + b * qbs + (long)lev * g.nq);
     return;
   }
-  const int xc = x - ncurl, lev = xc / nchunk, ch = xc - lev * nchunk;
+  const int xc = blockIdx.x - ncurl, lev = xc / nchunk, ch = xc - lev * nchunk;
   if (!active(Lc, b, lev)) return;
   const long per = (g.nw + nchunk - 1) / nchunk;
   const long lo = ch * per, hi = min((long)g.nw, lo + per);
@@ -1098,6 +1173,19 @@ __global__ __launch_bounds__(1024) void k_diag(Geo g, Smear sm, const T* __restr
   const int slot0 = counter[b];
   const int slot = slot0 < capacity ? slot0 : capacity - 1;
   double* out = out_base + (long)slot * rowlen + c.hoff;
+  // the force terms of this thread's first point, loaded here with the
+  // partials and the slip tables rather than after the slip's reduction
+  const int pbase = c.foff / 2;
+  int bd0 = -1;
+  double fv0[2] = {0.0, 0.0};
+  if (tid < c.nb) {
+    bd0 = bdyno[pbase + tid];
+#pragma unroll
+    for (int comp = 0; comp < 2; comp++) {
+      const int f = comp * c.nb + tid;
+      fv0[comp] = wsum[c.foff + f] * (fb[c.foff + f] / c.dt);
+    }
+  }
   // circulation: wave w sums the partials of level w
   for (int lv = tid >> 6; lv < L; lv += NT >> 6) {
     const double* pp = partial + ((long)b * g.Lmax + lv) * nchunk;
@@ -1123,11 +1211,11 @@ __global__ __launch_bounds__(1024) void k_diag(Geo g, Smear sm, const T* __restr
   mx = block_max(mx, red);
   if (tid == 0) out[2 * c.nbody + L] = mx;
   // forces, one body and component at a time
-  const int pbase = c.foff / 2;
   for (int bd = 0; bd < c.nbody; bd++)
     for (int comp = 0; comp < 2; comp++) {
       double acc = 0.0;
-      for (int p = tid; p < c.nb; p += NT)
+      if (bd0 == c.bmin + bd) acc += fv0[comp];
+      for (int p = tid + NT; p < c.nb; p += NT)
         if (bdyno[pbase + p] == c.bmin + bd) {
           const int f = comp * c.nb + p;
           acc += wsum[c.foff + f] * (fb[c.foff + f] / c.dt);
