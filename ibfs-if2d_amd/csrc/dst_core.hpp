@@ -335,7 +335,7 @@ constexpr Sched make_sched(int n2) {
   for (int i = 0; i < k; i++) {
     s.ns[i] = ns;
     s.toff[i] = off;
-    off += ns * (s.rad[i] - 1);
+    off += ns;  // one factor exp(-2 pi i k / (ns R)) per butterfly index k
     ns *= s.rad[i];
   }
   s.ntw = off;
@@ -343,8 +343,11 @@ constexpr Sched make_sched(int n2) {
 }
 
 // Twiddles of one thread, all stages, held in registers: stage s with
-// ns > 1 owns NBT(s) * (R(s)-1) factors starting at off(s), loaded once at
-// kernel start so that no stage waits on a memory load.
+// ns > 1 owns NBT(s) * (R(s)-1) factors starting at off(s).  The table holds
+// only w = exp(-2 pi i k / (ns R)) for each k; a thread loads one factor per
+// butterfly and stage at kernel start and forms w^2 .. w^(R-1) by complex
+// products, so that no stage waits on a memory load and the loads are a
+// third of those of a table of every power.
 template <int N2>
 struct TwLayout {
   static constexpr Sched sc = make_sched(N2);
@@ -371,8 +374,14 @@ __device__ __forceinline__ void tw_load(cplx<T>* wr, const cplx<T>* __restrict__
       for (int i = 0; i < NBT; i++) {
         const int j = lt + i * TPF;
         const int k = (j < NR ? j : 0) % NS;
+        const cplx<T> w = tw[sc.toff[S] + k];
+        cplx<T> pw = w;
+        wr[O + i * (R - 1)] = w;
 #pragma unroll
-        for (int r = 1; r < R; r++) wr[O + i * (R - 1) + r - 1] = tw[sc.toff[S] + k * (R - 1) + r - 1];
+        for (int r = 2; r < R; r++) {
+          pw = cmul(pw, w);
+          wr[O + i * (R - 1) + r - 1] = pw;
+        }
       }
     }
     tw_load<T, N2, S + 1>(wr, tw, lt);
@@ -461,8 +470,18 @@ __device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g
     int s = 0, idx = 0;
     if (AXIS == 0) { s = e / LEN; idx = e - s * LEN; }
     else { idx = e / (2 * P); s = e - idx * (2 * P); }
-    if constexpr (kPost) x[q] = (NL * NT == TOT || e < TOT) ? op.load_aux(b, g, s, P, idx, aux[q]) : T(0);
-    else x[q] = (NL * NT == TOT || e < TOT) ? op.load(b, g, s, P, idx) : T(0);
+    if constexpr (kPost) {
+      // evaluated at an existing place whatever the test, then selected: the
+      // loader's loads of every element are issued without a branch
+      const bool in = NL * NT == TOT || e < TOT;
+      const int ec = in ? e : TOT - 1;
+      if (AXIS == 0) { s = ec / LEN; idx = ec - s * LEN; }
+      else { idx = ec / (2 * P); s = ec - idx * (2 * P); }
+      const T v = op.load_aux(b, g, s, P, idx, aux[q]);
+      x[q] = in ? v : T(0);
+    } else {
+      x[q] = (NL * NT == TOT || e < TOT) ? op.load(b, g, s, P, idx) : T(0);
+    }
   }
 #pragma unroll
   for (int q = 0; q < NL; q++) {

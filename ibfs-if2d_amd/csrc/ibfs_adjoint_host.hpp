@@ -100,7 +100,7 @@ struct AdjointRunner {
     // STEP 4 reverse: fluxes and vorticity of the final vort2flux
     mp_reverse(AQ.p, nullptr, T(1));
     // STEP 3 reverse: afb_total = afb_J - regT(curl(ainv(aw_1), 0)); z = A^-1 (regT(..) - afb_J) = -adrhsf
-    E.ctci_like(SrcField<T>{AW.p, FB}, true, DstField<T>{U.p, nw}, T1.p, T2.p, B, nullptr, 0, E.lam1i.p, FB);
+    E.ctci_like(SrcField<T>{AW.p, FB}, true, DstField<T>{U.p, nw}, T1.p, T2.p, B, nullptr, 0, E.lam1i.p, E.tbs());
     if (in_window)
       hipLaunchKernelGGL(k_force_adj, grid(E.nf_max), dim3(256), 0, st, E.sm, E.wsum.p, cd, cl, jscale, AFB.p);
     else
@@ -122,7 +122,7 @@ struct AdjointRunner {
       // aX = S lam1i S aw*, then aw_old += aw* + 2 vfac D0 aX (k_step1_adj_incr)
       E.row_pass(SrcField<T>{AW.p + (long)k * nw, FB}, DstField<T>{T1.p, nw}, B, Lc, k);
       E.launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{g, SrcField<T>{T1.p, nw},
-                                                                   MidTab<T>{E.lam1i.p + (long)k * nw, (long)Lmax * nw},
+                                                                   MidTab<T>{E.lam1i.p + (long)k * E.nwp, E.tbs()},
                                                                    DstField<T>{T2.p, nw}, Lc, k},
                     g.mm1, 2, B, Engine<T>::kStep1Col2);
       E.row_pass(SrcField<T>{T2.p, nw}, DstField<T>{AX.p, nw}, B, Lc, k);

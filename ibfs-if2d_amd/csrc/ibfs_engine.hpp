@@ -252,6 +252,7 @@ bool factor_235(int n, std::vector<int>& rad);
 struct GroupBase {
   virtual ~GroupBase() = default;
   int B = 0, m = 0, n = 0, Lmax = 0, nw = 0, nq = 0, nbc = 0, rowlen = 0;
+  long nwp = 0;  // entries of one pair-major spectral table (pm_size)
   std::vector<CaseHost> cs;
   std::vector<int> hoff;  // row offset of each case in the group's diagnostics row
   virtual void enqueue_step(int first) = 0;

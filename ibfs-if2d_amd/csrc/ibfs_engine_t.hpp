@@ -5,7 +5,7 @@
 namespace ibfs {
 
 // Per-stage twiddles of a size-specialised pass: stage s, butterfly index
-// k < ns, factor r = 1..R-1 holds exp(-2 pi i k r / (ns R)).
+// k < ns holds exp(-2 pi i k / (ns R)) (the kernels form the other powers).
 template <typename T>
 bool make_spec_twiddles(int n2, DevBuf<cplx<T>>& tw, hipStream_t st) {
   static const int kSizes[] = {400, 512, 640, 768, 1024, 2048};
@@ -14,11 +14,10 @@ bool make_spec_twiddles(int n2, DevBuf<cplx<T>>& tw, hipStream_t st) {
   std::vector<cplx<T>> t(std::max(sc.ntw, 1));
   for (int s = 0; s < sc.nst; s++) {
     const int R = sc.rad[s], ns = sc.ns[s];
-    for (int k = 0; k < ns; k++)
-      for (int r = 1; r < R; r++) {
-        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) / (long double)(ns * R);
-        t[sc.toff[s] + k * (R - 1) + (r - 1)] = {(T)cosl(a), (T)sinl(a)};
-      }
+    for (int k = 0; k < ns; k++) {
+      long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)(ns * R);
+      t[sc.toff[s] + k] = {(T)cosl(a), (T)sinl(a)};
+    }
   }
   tw.alloc(t.size());
   tw.upload(t.data(), t.size(), st);
@@ -80,6 +79,7 @@ struct Engine : GroupBase {
   DevBuf<double> laminv64, S64, U1, U2;
 
   DevBuf<int> Lc;
+  const int* gate = nullptr;  // Lc.p, or nullptr when every case has Lmax levels
   DevBuf<T> con1, con2, vfac;
   DevBuf<T> laminv, lam1i;
   DevBuf<T> omega, rhs_old, q, q0, S, T1, T2, T3, RR, lastbc;
@@ -112,6 +112,7 @@ struct Engine : GroupBase {
   Engine* base = nullptr;  // set on a tangent-linear engine: the state it is linearised about
 
   long fbs() const { return (long)Lmax * nw; }
+  long tbs() const { return (long)Lmax * nwp; }  // case stride of lam1i (pair-major tables)
   long qbs() const { return (long)Lmax * nq; }
   bool is_double() const override { return sizeof(T) == 8; }
 
@@ -128,6 +129,7 @@ struct Engine : GroupBase {
     nq = (m + 1) * n + (n + 1) * m;
     nbc = 2 * (m + 1) + 2 * (n + 1);
     geo = Geo{m, n, Lmax, m - 1, n - 1, nw, nq, nbc, 0, n + 1, 2 * (n + 1), 2 * (n + 1) + m + 1};
+    nwp = pm_size(geo);
     norm = T(cs[0].normalize);
     refine = sizeof(T) == 4;
     fs.init(m, n, st);
@@ -136,16 +138,18 @@ struct Engine : GroupBase {
     // per-case scalars and eigenvalue tables (src/myfft.f90:42-69)
     std::vector<int> hL(B);
     std::vector<T> hc1((size_t)B * Lmax, T(0)), hc2(hc1), hvf(hc1);
-    std::vector<T> hlaminv(nw), hl1i((size_t)B * Lmax * nw, T(0));
-    std::vector<double> hlaminv64(nw);
+    // the column pair's tables are pair-major (pm_ix), zero in the padding
+    std::vector<T> hlaminv(nwp, T(0)), hl1i((size_t)B * Lmax * nwp, T(0));
+    std::vector<double> hlaminv64(nwp, 0.0);
     std::vector<double> lam(nw);
     for (int j = 1; j <= n - 1; j++)
       for (int i = 1; i <= m - 1; i++) {
         const size_t e = (size_t)(i - 1) + (size_t)(j - 1) * (m - 1);
+        const long ep = pm_ix(geo, i - 1, j - 1);
         lam[e] = -2.0 * (std::cos(kPiRef * (double)(float)i / (double)(float)m) +
                          std::cos(kPiRef * (double)(float)j / (double)(float)n) - 2.0);
-        hlaminv[e] = T(1.0 / lam[e] / cs[0].normalize);
-        hlaminv64[e] = 1.0 / lam[e] / cs[0].normalize;
+        hlaminv[ep] = T(1.0 / lam[e] / cs[0].normalize);
+        hlaminv64[ep] = 1.0 / lam[e] / cs[0].normalize;
       }
     for (int b = 0; b < B; b++) {
       const CaseHost& c = cs[b];
@@ -156,13 +160,18 @@ struct Engine : GroupBase {
         hc2[b * Lmax + k] = T(c.con2[k]);
         hvf[b * Lmax + k] = T(c.vfac[k]);
         const double del22 = del2 * std::pow(4.0, k);
-        const size_t base = ((size_t)b * Lmax + k) * nw;
-        for (int e = 0; e < nw; e++) {
-          hl1i[base + e] = T(1.0 / (1.0 + 0.5 * c.d.dt * lam[e] / del22 / c.d.re));
-        }
+        const size_t base = ((size_t)b * Lmax + k) * nwp;
+        for (int jj = 0; jj < n - 1; jj++)
+          for (int ii = 0; ii < m - 1; ii++) {
+            const size_t e = (size_t)ii + (size_t)jj * (m - 1);
+            hl1i[base + pm_ix(geo, ii, jj)] = T(1.0 / (1.0 + 0.5 * c.d.dt * lam[e] / del22 / c.d.re));
+          }
       }
     }
     upvec(Lc, hL, st);
+    // the step's kernels take no level gate when every case has all levels
+    gate = Lc.p;
+    if (std::all_of(hL.begin(), hL.end(), [&](int l) { return l == Lmax; })) gate = nullptr;
     upvec(con1, hc1, st);
     upvec(con2, hc2, st);
     upvec(vfac, hvf, st);
@@ -404,17 +413,17 @@ struct Engine : GroupBase {
               break;
             case kStep1Row:
               {  // the freestream form, as the step launches it
-                SrcStep1<T, true> s1f{T2.p, nullptr, qz.p, nullptr, bz.p, T3.p, con1.p, con2.p, vfac.p, Lc.p,
+                SrcStep1<T, true> s1f{T2.p, nullptr, qz.p, nullptr, bz.p, T3.p, con1.p, con2.p, vfac.p, gate,
                                       (long)nw, (long)nq, (long)nbc, 0, 0};
                 s1f.q0c = q0c.p;
-                launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geo, s1f, DstField<T>{T1.p, nw}, Lc.p, 0},
+                launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geo, s1f, DstField<T>{T1.p, nw}, gate, 0},
                           geo.nm1, 2, B, kStep1Row);
               }
               break;
             default:
               launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{geo, SrcField<T>{T1.p, nw},
-                                                                         MidTab<T>{lam1i.p, (long)Lmax * nw},
-                                                                         DstField<T>{T2.p, nw}, Lc.p, 0},
+                                                                         MidTab<T>{lam1i.p, tbs()},
+                                                                         DstField<T>{T2.p, nw}, gate, 0},
                           geo.mm1, 2, B, kStep1Col2);
               break;
           }
@@ -554,9 +563,9 @@ struct Engine : GroupBase {
     const long FB = fbs(), QB = qbs(), BB = (long)Lmax * nbc;
     if (base)  // tangent step: the edge values of the base state before its own step
       hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, base->omega.p,
-                         base->lastbc.p, Lc.p);
+                         base->lastbc.p, gate);
     hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, omega.p, lastbc.p,
-                       Lc.p);
+                       gate);
     // STEP 1 (src/operators.f90:103-134), level slots Lmax..1, in increment
     // form: omega += S[lam1i * S(X + 2 vfac D0 omega)] (see SrcStep1)
     for (int lev = Lmax - 1; lev >= 0; lev--) {
@@ -569,7 +578,7 @@ struct Engine : GroupBase {
                      con1.p,
                      con2.p,
                      vfac.p,
-                     Lc.p,
+                     gate,
                      FB,
                      QB,
                      BB,
@@ -585,17 +594,17 @@ struct Engine : GroupBase {
         SrcStep1<T, true> s1f{s1.om, s1.omc, s1.q, nullptr, s1.lastbc, s1.rhs_old, s1.con1, s1.con2, s1.vfac,
                               s1.Lc, s1.fbs, s1.qbs, s1.bbs, s1.lev, s1.first};
         s1f.q0c = q0c.p;
-        launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geo, s1f, DstField<T>{T1.p, nw}, Lc.p, lev}, geo.nm1, 2,
+        launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geo, s1f, DstField<T>{T1.p, nw}, gate, lev}, geo.nm1, 2,
                   B, kStep1Row);
       } else {
-        launch<0>(RowOp<T, SrcStep1<T>, DstField<T>>{geo, s1, DstField<T>{T1.p, nw}, Lc.p, lev}, geo.nm1, 2, B,
+        launch<0>(RowOp<T, SrcStep1<T>, DstField<T>>{geo, s1, DstField<T>{T1.p, nw}, gate, lev}, geo.nm1, 2, B,
                   kStep1Row);
       }
       launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{geo, SrcField<T>{T1.p, nw},
-                                                                 MidTab<T>{lam1i.p + (long)lev * nw, (long)Lmax * nw},
-                                                                 DstField<T>{T2.p, nw}, Lc.p, lev},
+                                                                 MidTab<T>{lam1i.p + (long)lev * nwp, tbs()},
+                                                                 DstField<T>{T2.p, nw}, gate, lev},
                   geo.mm1, 2, B, kStep1Col2);
-      row_pass(SrcField<T>{T2.p, nw}, DstAdd<T>{omega.p + (long)lev * nw, FB, T(1)}, B, Lc.p, lev);
+      row_pass(SrcField<T>{T2.p, nw}, DstAdd<T>{omega.p + (long)lev * nw, FB, T(1)}, B, gate, lev);
     }
   }
 
@@ -610,22 +619,22 @@ struct Engine : GroupBase {
       const TS* s2 = Lmax > 1 ? s1 + nw : nullptr;
       if (!refine) {
         hipLaunchKernelGGL((k_force_solve<T, TS>), gall, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
-                           Lc.p, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 0);
+                           gate, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 0);
       } else {  // one FP64 refinement sweep: fb += A^-1 (rhsf - A fb)
         const dim3 gf(std::max(1, (nf_max + kForceRows - 1) / kForceRows), B);
         hipLaunchKernelGGL((k_force_solve<T, TS>), gf, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
-                           Lc.p, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 1);
+                           gate, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 1);
         hipLaunchKernelGGL(k_symv_residual, dim3((nf_max + 3) / 4, B), dim3(256), 0, st, sm, Asym.p, rhsf.p, fb.p,
                            rtmp.p);
         hipLaunchKernelGGL((k_force_solve<T, TS>), gall, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
-                           Lc.p, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)rtmp.p, 2);
+                           gate, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)rtmp.p, 2);
       }
     };
     if constexpr (kSolve64) {
-      vort2flux64(omega.p, B, Lc.p);
+      vort2flux64(omega.p, B, gate);
       force((const double*)S64.p);
     } else {
-      vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
+      vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
       force((const T*)S.p);
     }
   }
@@ -634,16 +643,16 @@ struct Engine : GroupBase {
   void enqueue_step34() {
     const long FB = fbs(), QB = qbs();
     // STEP 3 (src/operators.f90:174)
-    ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, FB);
+    ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, tbs());
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
-    if constexpr (kSolve64) vort2flux_fine64(omega.p, B, Lc.p);
-    else vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
+    if constexpr (kSolve64) vort2flux_fine64(omega.p, B, gate);
+    else vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
     // fluxes and diagnostics (src/operators.f90:184-186)
     const int ncurl = (nq + 256 * kTailFaces - 1) / (256 * kTailFaces) * Lmax;
     hipLaunchKernelGGL(k_tail<T>, dim3(ncurl + nchunk * Lmax, B), dim3(256), 0, st, geo, S.p, q.p, omega.p, FB, QB,
-                       partial.p, nchunk, Lc.p, kSolve64 ? (const double*)S64.p : nullptr);
+                       partial.p, nchunk, gate, kSolve64 ? (const double*)S64.p : nullptr);
     hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, sm, q.p, QB, motion.p, fb.p, wsum.p, bdyno.p,
-                       partial.p, nchunk, Lc.p, diag.p, rowlen, counter.p, diag_cap);
+                       partial.p, nchunk, gate, diag.p, rowlen, counter.p, diag_cap);
   }
 
   // the compact smearing tables of every case, concatenated, on the device
@@ -805,9 +814,9 @@ struct Engine : GroupBase {
     hipLaunchKernelGGL((k_rotreg<T, double>), dim3((hcs[0].nvtx + 255) / 256, 1), dim3(256), 0, st, sm, x, 0L,
                        cgRR.p, (long)nw, 0);
     ctci_like(SrcField<T>{cgRR.p, nw}, true, DstField<T>{cgVB.p, fbs()}, cgT1.p, cgT2.p, 1, nullptr, 0, lam1i.p, 0);
-    vort2flux(cgVB.p, cgSB.p, nullptr, 1, Lc.p, cgT1.p, cgT2.p);
+    vort2flux(cgVB.p, cgSB.p, nullptr, 1, gate, cgT1.p, cgT2.p);
     hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf + 255) / 256, 1), dim3(256), 0, st, geo, sm, cgSB.p,
-                       Lmax > 1 ? cgSB.p + nw : nullptr, fbs(), (const double*)nullptr, out, (long)nf, Lc.p, 0);
+                       Lmax > 1 ? cgSB.p + nw : nullptr, fbs(), (const double*)nullptr, out, (long)nf, gate, 0);
   }
 
   CgState cg_read() {
@@ -847,9 +856,9 @@ struct Engine : GroupBase {
     cg_alloc();
     enqueue_step1(first);
     // STEP 2 (src/operators.f90:155-164): rhsf = regT(q_1) - motion
-    vort2flux(omega.p, S.p, nullptr, B, Lc.p, T2.p, T3.p);
+    vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
     hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf_max + 255) / 256, B), dim3(256), 0, st, geo, sm, S.p,
-                       Lmax > 1 ? S.p + nw : nullptr, fbs(), (const double*)motion.p, rhsf.p, 0L, Lc.p, -1);
+                       Lmax > 1 ? S.p + nw : nullptr, fbs(), (const double*)motion.p, rhsf.p, 0L, gate, -1);
     const int it = cg_solve(cgtol, cg_max_iter);
     // the step-3 source rot(reg(fb)) on the active vertices
     hipLaunchKernelGGL((k_rotreg<T, double>), dim3((nvtx_max + 255) / 256, B), dim3(256), 0, st, sm, fb.p, 0L, RR.p,
@@ -1005,7 +1014,7 @@ struct Engine : GroupBase {
     std::vector<int> hg(Bc, L);
     upvec(gate, hg, st);
     std::vector<double> hF((size_t)Bc * nf);
-    const T* l1i = lam1i.p + (long)c * bsL;  // level 1 of case c
+    const T* l1i = lam1i.p + (long)c * tbs();  // level 1 of case c
     for (int c0 = 0; c0 < nf; c0 += Bc) {
       const int bb = std::min(Bc, nf - c0);
       std::fill(hF.begin(), hF.end(), 0.0);

@@ -35,8 +35,13 @@ __device__ __forceinline__ int uix(const Geo& g, int i, int j) { return (i - 1) 
 __device__ __forceinline__ int vix(const Geo& g, int i, int j) { return (g.m + 1) * g.n + (i - 1) + (j - 1) * g.m; }
 __device__ __forceinline__ int wix(const Geo& g, int i, int j) { return (i - 2) + (j - 2) * g.mm1; }
 
+// Lc == nullptr: every case has all g.Lmax levels (no per-case load, so no
+// kernel waits on one before its first field load)
 __device__ __forceinline__ bool active(const int* Lc, int b, int lev) { return Lc == nullptr || lev < Lc[b]; }
-__device__ __forceinline__ bool has_coarser(const int* Lc, int b, int lev) { return lev + 1 < Lc[b]; }
+__device__ __forceinline__ int levels(const Geo& g, const int* Lc, int b) { return Lc == nullptr ? g.Lmax : Lc[b]; }
+__device__ __forceinline__ bool has_coarser(const Geo& g, const int* Lc, int b, int lev) {
+  return lev + 1 < levels(g, Lc, b);
+}
 
 // get_bc (src/operators.f90:336-367) evaluated at one 1-based edge position.
 // r is the coarser level's vertex field; the reference's assumed-shape dummy
@@ -84,6 +89,45 @@ __device__ __forceinline__ T applybc_vec(const Geo& g, T acc, int i, int j, cons
   return acc;
 }
 
+// Branch-free forms of getbc_at, applybc_add and applybc_vec for loaders
+// that issue all their loads before using any: every candidate value is
+// loaded from an in-range place and the terms are then selected, with the
+// same operands, order and arithmetic, so the results are identical.  (The
+// bottom/top and left/right conditions exclude each other for m, n > 2.)
+template <typename T>
+__device__ __forceinline__ T getbc_sel(const Geo& g, const T* __restrict__ r, int pos, T fac) {
+  const bool horiz = pos > g.bottom;
+  const bool hi = horiz ? pos > g.top : pos > g.right;
+  const int k0 = pos - (horiz ? (hi ? g.top : g.bottom) : (hi ? g.right : g.left)) - 1;
+  const bool odd = (k0 & 1) != 0;
+  const int c1 = (horiz ? g.m : g.n) / 4 + (odd ? (k0 + 1) / 2 : k0 / 2);
+  const int fx = horiz ? (hi ? 3 * g.n / 4 : g.n / 4) : (hi ? 3 * g.m / 4 : g.m / 4);
+  auto ix = [&](int a, int c) { return horiz ? (a - 1) + (c - 1) * g.mm1 : (c - 1) + (a - 1) * g.mm1; };
+  const int a1 = ix(c1, fx), a2 = odd ? ix(c1 - 1, fx) : a1;
+  const T v1 = r[a1], v2 = r[a2];
+  const T v = odd ? T(0.5) * (v1 + v2) : v1;
+  return v * fac;
+}
+template <typename T>
+__device__ __forceinline__ T applybc_add_sel(const Geo& g, T acc, int i, int j, const T* __restrict__ rc, T gfac,
+                                             T fac) {
+  const bool ej = j == 2 || j == g.n, ei = i == 2 || i == g.m;
+  const T gj = getbc_sel<T>(g, rc, (j == g.n ? g.top : g.bottom) + i, gfac);
+  const T gi = getbc_sel<T>(g, rc, (i == g.m ? g.right : g.left) + j, gfac);
+  acc = ej ? acc + fac * gj : acc;
+  acc = ei ? acc + fac * gi : acc;
+  return acc;
+}
+template <typename T>
+__device__ __forceinline__ T applybc_vec_sel(const Geo& g, T acc, int i, int j, const T* __restrict__ bc, T fac) {
+  const bool ej = j == 2 || j == g.n, ei = i == 2 || i == g.m;
+  const T vj = bc[(j == g.n ? g.top : g.bottom) + i - 1];
+  const T vi = bc[(i == g.m ? g.right : g.left) + j - 1];
+  acc = ej ? acc + fac * vj : acc;
+  acc = ei ? acc + fac * vi : acc;
+  return acc;
+}
+
 // rot(nonlinear(omega, q, q0, bc)) at vertex (i,j):
 // nonlinear src/operators.f90:465-510, rot src/operators.f90:291-304.
 // kFree: the freestream form of q0 (src/variables.f90:59-63), u0 on every u
@@ -94,15 +138,14 @@ __device__ __forceinline__ T rot_nonlinear(const Geo& g, int i, int j, const T* 
                                            const T* __restrict__ q, const T* __restrict__ q0,
                                            const T* __restrict__ bc, T u0 = T(0)) {
   const int m = g.m, n = g.n;
+  // the edge value or the field value, one load from the selected place
   auto OmU = [&](int ii, int jj) -> T {  // along j at column ii (2..m)
-    if (jj == 1) return bc[g.bottom + ii - 1];
-    if (jj == n + 1) return bc[g.top + ii - 1];
-    return om[wix(g, ii, jj)];
+    const T* p = jj == 1 ? bc + (g.bottom + ii - 1) : jj == n + 1 ? bc + (g.top + ii - 1) : om + wix(g, ii, jj);
+    return *p;
   };
   auto OmV = [&](int ii, int jj) -> T {  // along i at row jj (2..n)
-    if (ii == 1) return bc[g.left + jj - 1];
-    if (ii == m + 1) return bc[g.right + jj - 1];
-    return om[wix(g, ii, jj)];
+    const T* p = ii == 1 ? bc + (g.left + jj - 1) : ii == m + 1 ? bc + (g.right + jj - 1) : om + wix(g, ii, jj);
+    return *p;
   };
   auto vavg = [&](int ii, int jj) -> T {
     const int a = vix(g, ii, jj), b = vix(g, ii - 1, jj);
@@ -142,7 +185,7 @@ struct SrcVortBC {
   const T* om; const TC* sc; long bs; const int* Lc; int lev;
   __device__ TS operator()(const Geo& g, int b, int ii, int jj) const {
     TS v = TS(om[b * bs + ii + jj * g.mm1]);
-    if (sc == nullptr || !has_coarser(Lc, b, lev)) return v;
+    if (sc == nullptr || !has_coarser(g, Lc, b, lev)) return v;
     const int i = ii + 2, j = jj + 2;
     if (i == 2 || i == g.m || j == 2 || j == g.n) {
       if constexpr (std::is_same_v<TS, TC>) v = applybc_add<TS>(g, v, i, j, sc + b * bs, TS(1), TS(1));
@@ -198,8 +241,11 @@ struct RowOp {
   __device__ T load_aux(int b, int gr, int s, int P, int idx, T& aux) const {
     aux = T(0);
     if constexpr (kLoadPost) {
+      // evaluated on a row that exists whatever the test, so no branch holds
+      // the loads back
       const int row = gr * 2 * P + s;
-      return (row < g.nm1 && active(Lc, b, lev)) ? src.value(g, b, idx, row, aux) : T(0);
+      const T v = src.value(g, b, idx, row < g.nm1 ? row : g.nm1 - 1, aux);
+      return (row < g.nm1 && active(Lc, b, lev)) ? v : T(0);
     } else {
       return load(b, gr, s, P, idx);
     }
@@ -253,6 +299,14 @@ struct ColOp {
 };
 
 // spectral factors of the fused column pair (value at column ii, row jj)
+// Their tables are stored pair-major: the entries of columns 2p and 2p+1 at
+// row jj are adjacent, at (p (n-1) + jj) * 2, so the workgroup of a column
+// pair reads its entries contiguously (pm_size(g) entries per table, the
+// last column padded when m-1 is odd).
+__host__ __device__ __forceinline__ long pm_ix(const Geo& g, int ii, int jj) {
+  return ((long)(ii >> 1) * g.nm1 + jj) * 2 + (ii & 1);
+}
+__host__ __device__ __forceinline__ long pm_size(const Geo& g) { return (long)((g.mm1 + 1) / 2) * g.nm1 * 2; }
 // tab * v: ctci's laminv (src/myfft.f90:106), shared by the group
 // coef gives the table entry and apply the factor given that entry (the
 // size-specialised column pair reads the entries together with its inputs);
@@ -260,8 +314,8 @@ struct ColOp {
 template <typename T>
 struct MidScale {
   const T* tab;
-  __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const { return tab[ii + jj * g.mm1] * v; }
-  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[ii + jj * g.mm1]; }
+  __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const { return tab[pm_ix(g, ii, jj)] * v; }
+  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[pm_ix(g, ii, jj)]; }
   __device__ T apply(T c, T v) const { return c * v; }
 };
 // tab * v / norm: ainv's lam1i(:,:,1) / normalize (src/myfft.f90:121), per case
@@ -269,9 +323,9 @@ template <typename T>
 struct MidScaleDiv {
   const T* tab; T norm; long tbs;
   __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const {
-    return tab[b * tbs + ii + jj * g.mm1] * v / norm;
+    return tab[b * tbs + pm_ix(g, ii, jj)] * v / norm;
   }
-  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[b * tbs + ii + jj * g.mm1]; }
+  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[b * tbs + pm_ix(g, ii, jj)]; }
   __device__ T apply(T c, T v) const { return c * v / norm; }
 };
 
@@ -363,25 +417,30 @@ struct SrcStep1 {
                              lastbc + b * bbs);
       if (bom) rhs = rhs + rot_nonlinear<T>(g, i, j, bom + b * fbs, q + b * qbs, q0 + b * qbs, blb + b * bbs);
     }
-    const T rold = first ? rhs : rhs_old[b * fbs + e];
+    // every load below is unconditional (in-range places, terms selected),
+    // so a thread's elements issue their loads together
+    const T ro = rhs_old[b * fbs + e];
+    const T rold = first ? rhs : ro;
     const int c = b * g.Lmax + lev;
-    T bcs = T(0);
-    if (i == 2 || i == g.m || j == 2 || j == g.n) {
-      bcs = applybc_vec<T>(g, bcs, i, j, lastbc + b * bbs, vfac[c]);
-      if (omc != nullptr && has_coarser(Lc, b, lev)) bcs = applybc_add<T>(g, bcs, i, j, omc + b * fbs, T(0.25), vfac[c]);
-    }
-    const T lap = (ii > 0 ? w[e - 1] : T(0)) + (ii + 1 < g.mm1 ? w[e + 1] : T(0)) + (jj > 0 ? w[e - g.mm1] : T(0)) +
-                  (jj + 1 < g.nm1 ? w[e + g.mm1] : T(0)) - T(4) * w[e];
+    // edge terms (zero away from the edge rows and columns)
+    T bcs = applybc_vec_sel<T>(g, T(0), i, j, lastbc + b * bbs, vfac[c]);
+    const bool hc = omc != nullptr && has_coarser(g, Lc, b, lev);
+    const T bcc = applybc_add_sel<T>(g, bcs, i, j, omc != nullptr ? omc + b * fbs : w, T(0.25), vfac[c]);
+    bcs = hc ? bcc : bcs;
+    const T wl = w[ii > 0 ? e - 1 : e], wr = w[ii + 1 < g.mm1 ? e + 1 : e];
+    const T wd = w[jj > 0 ? e - g.mm1 : e], wu = w[jj + 1 < g.nm1 ? e + g.mm1 : e];
+    const T lap = (ii > 0 ? wl : T(0)) + (ii + 1 < g.mm1 ? wr : T(0)) + (jj > 0 ? wd : T(0)) +
+                  (jj + 1 < g.nm1 ? wu : T(0)) - T(4) * w[e];
     return con1[c] * rhs + con2[c] * rold + bcs + T(2) * vfac[c] * lap;
   }
   __device__ void post(const Geo& g, int b, int ii, int jj, T rhs) const { rhs_old[b * fbs + ii + jj * g.mm1] = rhs; }
 };
-// tab * v with a per-case table (step 1's lam1i of level lev)
+// tab * v with a per-case table (step 1's lam1i of level lev), pair-major
 template <typename T>
 struct MidTab {
   const T* tab; long tbs;
-  __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const { return tab[b * tbs + ii + jj * g.mm1] * v; }
-  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[b * tbs + ii + jj * g.mm1]; }
+  __device__ T operator()(const Geo& g, int b, int ii, int jj, T v) const { return tab[b * tbs + pm_ix(g, ii, jj)] * v; }
+  __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[b * tbs + pm_ix(g, ii, jj)]; }
   __device__ T apply(T c, T v) const { return c * v; }
 };
 // a = a + scale * v (the step-1 increment; adjoint accumulations)
@@ -411,7 +470,7 @@ __global__ void k_lastbc(Geo g, const T* __restrict__ om, T* __restrict__ lastbc
   const int lev = t / g.nbc, p = t - lev * g.nbc;
   const long fb = (long)b * g.Lmax;
   T v = T(0);
-  if (has_coarser(Lc, b, lev)) v = getbc_at<T>(g, om + (fb + lev + 1) * g.nw, p + 1, T(0.25));
+  if (has_coarser(g, Lc, b, lev)) v = getbc_at<T>(g, om + (fb + lev + 1) * g.nw, p + 1, T(0.25));
   lastbc[(fb + lev) * g.nbc + p] = v;
 }
 
@@ -513,7 +572,7 @@ __global__ void k_curl(Geo g, const T* __restrict__ s, const T* __restrict__ sc,
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (f >= g.nq || !active(Lc, b, lev)) return;
-  const T* c = (sc != nullptr && (Lc == nullptr || has_coarser(Lc, b, lev))) ? sc + b * bs : nullptr;
+  const T* c = (sc != nullptr && has_coarser(g, Lc, b, lev)) ? sc + b * bs : nullptr;
   vel[b * vbs + f] = curl_face<T>(g, f, s + b * bs, c);
 }
 
@@ -527,7 +586,7 @@ __global__ void k_curl_levels(Geo g, const T* __restrict__ s, T* __restrict__ ve
   const int lev = blockIdx.y, b = blockIdx.z;
   if (f >= g.nq || !active(Lc, b, lev)) return;
   const T* sl = s + b * bs + (long)lev * g.nw;
-  const bool coarser = lev + 1 < g.Lmax && (Lc == nullptr || has_coarser(Lc, b, lev));
+  const bool coarser = lev + 1 < g.Lmax && has_coarser(g, Lc, b, lev);
   vel[b * vbs + (long)lev * g.nq + f] = curl_face<T>(g, f, sl, coarser ? sl + g.nw : nullptr);
 }
 
@@ -745,7 +804,7 @@ __global__ void k_regT_curl(Geo g, Smear sm, const T* __restrict__ s1, const T* 
   const int ci = one_case >= 0 ? one_case : b;
   const CaseSm c = sm.cs[ci];
   if (f >= c.nf) return;
-  const T* sc = (s2 != nullptr && Lc[b] > 1) ? s2 + b * bs : nullptr;  // Lc indexed by batch entry
+  const T* sc = (s2 != nullptr && levels(g, Lc, b) > 1) ? s2 + b * bs : nullptr;  // Lc indexed by batch entry
   const long o = obs > 0 ? b * obs + f : c.foff + f;
   out[o] = regT_curl_at<T>(g, sm, c, f, s1 + b * bs, sc, sub);
 }
@@ -912,7 +971,7 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
   load_block(lane);
   if (mode < 2) {
     const TS* s1b = s1 + b * bs;
-    const TS* sc = (s2 != nullptr && Lc[b] > 1) ? s2 + b * bs : nullptr;
+    const TS* sc = (s2 != nullptr && levels(g, Lc, b) > 1) ? s2 + b * bs : nullptr;
     for (int f0 = tid; f0 < nf; f0 += 2 * NT) {
       const int f1 = f0 + NT;
       const double v0 = regT_curl_at<TS>(g, sm, c, f0, s1b, sc, motion);
@@ -1132,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
     const int lev = x / ncb, f0 = (x - lev * ncb) * 256 * kTailFaces + tid;
     if (!active(Lc, b, lev)) return;
     const T* sl = s + b * bs + (long)lev * g.nw;
-    const T* sc = has_coarser(Lc, b, lev) ? sl + g.nw : nullptr;
+    const T* sc = has_coarser(g, Lc, b, lev) ? sl + g.nw : nullptr;
     if (lev == 0 && s1d != nullptr)  // finest level solved in FP64
       tail_curl<T, double>(g, f0, s1d + b * bs, sc, q + b * qbs);
     else
@@ -1169,7 +1228,7 @@ __global__ __launch_bounds__(1024) void k_diag(Geo g, Smear sm, const T* __restr
   __shared__ double red[16];
   const int tid = threadIdx.x, b = blockIdx.x, lane = tid & 63, NT = blockDim.x;
   const CaseSm c = sm.cs[b];
-  const int L = Lc[b];
+  const int L = levels(g, Lc, b);
   const int slot0 = counter[b];
   const int slot = slot0 < capacity ? slot0 : capacity - 1;
   double* out = out_base + (long)slot * rowlen + c.hoff;
