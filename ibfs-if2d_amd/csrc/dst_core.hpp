@@ -63,6 +63,13 @@ template <class Op, class = void>
 struct op_load_post : std::false_type {};
 template <class Op>
 struct op_load_post<Op, std::void_t<decltype(Op::kLoadPost)>> : std::bool_constant<Op::kLoadPost> {};
+//   kGather, Gathered, gather(b, g, s, P, idx, G) / combine(..., G, aux):
+//       a side-store loader in two phases, the loads of every element of a
+//       thread issued before any value is formed.
+template <class Op, class = void>
+struct op_gather : std::false_type {};
+template <class Op>
+struct op_gather<Op, std::void_t<decltype(Op::kGather)>> : std::bool_constant<Op::kGather> {};
 
 constexpr int kThreads = 256;
 constexpr int kMaxStages = 16;
@@ -464,8 +471,32 @@ __device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g
   constexpr int TOT = 2 * P * LEN, NL = (TOT + NT - 1) / NT;
   constexpr bool kPost = op_load_post<Op>::value;
   T x[NL], aux[kPost ? NL : 1];
+  if constexpr (op_gather<Op>::value) {
+    // every element's loads first, then every element's value
+    typename Op::Gathered gv[NL];
 #pragma unroll
-  for (int q = 0; q < NL; q++) {
+    for (int q = 0; q < NL; q++) {
+      const int e = t + q * NT;
+      const int ec = (NL * NT == TOT || e < TOT) ? e : TOT - 1;
+      int s, idx;
+      if (AXIS == 0) { s = ec / LEN; idx = ec - s * LEN; }
+      else { idx = ec / (2 * P); s = ec - idx * (2 * P); }
+      op.gather(b, g, s, P, idx, gv[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+      const int e = t + q * NT;
+      const bool in = NL * NT == TOT || e < TOT;
+      const int ec = in ? e : TOT - 1;
+      int s, idx;
+      if (AXIS == 0) { s = ec / LEN; idx = ec - s * LEN; }
+      else { idx = ec / (2 * P); s = ec - idx * (2 * P); }
+      const T v = op.combine(b, g, s, P, idx, gv[q], aux[q]);
+      x[q] = in ? v : T(0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NL && !op_gather<Op>::value; q++) {
     const int e = t + q * NT;
     int s = 0, idx = 0;
     if (AXIS == 0) { s = e / LEN; idx = e - s * LEN; }

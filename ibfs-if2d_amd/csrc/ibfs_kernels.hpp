@@ -104,8 +104,32 @@ __device__ __forceinline__ T getbc_sel(const Geo& g, const T* __restrict__ r, in
   const int fx = horiz ? (hi ? 3 * g.n / 4 : g.n / 4) : (hi ? 3 * g.m / 4 : g.m / 4);
   auto ix = [&](int a, int c) { return horiz ? (a - 1) + (c - 1) * g.mm1 : (c - 1) + (a - 1) * g.mm1; };
   const int a1 = ix(c1, fx), a2 = odd ? ix(c1 - 1, fx) : a1;
+  // both values are used whatever odd is (0.5 (v1 + v1) == v1 exactly), so
+  // neither load is moved into a branch
   const T v1 = r[a1], v2 = r[a2];
-  const T v = odd ? T(0.5) * (v1 + v2) : v1;
+  const T v = T(0.5) * (v1 + (odd ? v2 : v1));
+  return v * fac;
+}
+// getbc_sel in two parts: the two places it reads (the same place twice
+// when it needs one), and the value from what was read there
+__device__ __forceinline__ void getbc_places(const Geo& g, int pos, int& a1, int& a2) {
+  const bool horiz = pos > g.bottom;
+  const bool hi = horiz ? pos > g.top : pos > g.right;
+  const int k0 = pos - (horiz ? (hi ? g.top : g.bottom) : (hi ? g.right : g.left)) - 1;
+  const bool odd = (k0 & 1) != 0;
+  const int c1 = (horiz ? g.m : g.n) / 4 + (odd ? (k0 + 1) / 2 : k0 / 2);
+  const int fx = horiz ? (hi ? 3 * g.n / 4 : g.n / 4) : (hi ? 3 * g.m / 4 : g.m / 4);
+  auto ix = [&](int a, int c) { return horiz ? (a - 1) + (c - 1) * g.mm1 : (c - 1) + (a - 1) * g.mm1; };
+  a1 = ix(c1, fx);
+  a2 = odd ? ix(c1 - 1, fx) : a1;
+}
+template <typename T>
+__device__ __forceinline__ T getbc_value(const Geo& g, int pos, T v1, T v2, T fac) {
+  const bool horiz = pos > g.bottom;
+  const bool hi = horiz ? pos > g.top : pos > g.right;
+  const int k0 = pos - (horiz ? (hi ? g.top : g.bottom) : (hi ? g.right : g.left)) - 1;
+  const bool odd = (k0 & 1) != 0;
+  const T v = T(0.5) * (v1 + (odd ? v2 : v1));
   return v * fac;
 }
 template <typename T>
@@ -225,6 +249,17 @@ template <class S, class = void>
 struct src_has_post : std::false_type {};
 template <class S>
 struct src_has_post<S, std::void_t<decltype(&S::post)>> : std::true_type {};
+// Src::gather/combine present and enabled (a loader in two phases)?
+template <class S, class = void>
+struct src_gathered {
+  using type = char;
+  static constexpr bool value = false;
+};
+template <class S>
+struct src_gathered<S, std::void_t<typename S::Gathered>> {
+  using type = typename S::Gathered;
+  static constexpr bool value = S::kGather;
+};
 
 // single-field row pass: rows jj; FFT p pairs rows 2Pg+2p and 2Pg+2p+1.
 // Cases whose level count is <= lev are skipped (Lc == nullptr: none).
@@ -255,6 +290,19 @@ struct RowOp {
       const int row = gr * 2 * P + s;
       if (row < g.nm1 && active(Lc, b, lev)) src.post(g, b, idx, row, aux);
     }
+  }
+  // loaders in two phases (Src::gather/combine): the loads of an element on
+  // a row that exists whatever the test, then the value
+  static constexpr bool kGather = src_gathered<Src>::value;
+  using Gathered = typename src_gathered<Src>::type;
+  __device__ void gather(int b, int gr, int s, int P, int idx, Gathered& G) const {
+    const int row = gr * 2 * P + s;
+    src.gather(g, b, idx, row < g.nm1 ? row : g.nm1 - 1, G);
+  }
+  __device__ T combine(int b, int gr, int s, int P, int idx, const Gathered& G, T& aux) const {
+    const int row = gr * 2 * P + s;
+    const T v = src.combine(g, b, idx, row < g.nm1 ? row : g.nm1 - 1, G, aux);
+    return (row < g.nm1 && active(Lc, b, lev)) ? v : T(0);
   }
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     if (!active(Lc, b, lev)) return;
@@ -397,6 +445,7 @@ struct SrcStep1 {
   const T* bq0 = nullptr;
   const T* blb = nullptr;
   const T* q0c = nullptr;  // [b*Lmax + lev]: the u-face value of the freestream q0 (kFree)
+  static constexpr bool kGather = kFree;  // the freestream form loads in two phases
   __device__ T operator()(const Geo& g, int b, int ii, int jj) const {
     T rhs;
     const T v = value(g, b, ii, jj, rhs);
@@ -410,9 +459,8 @@ struct SrcStep1 {
     const T* w = om + b * fbs;
     const int e = ii + jj * g.mm1;
     const int i = ii + 2, j = jj + 2;
-    if constexpr (kFree) {
-      rhs = rot_nonlinear<T, true>(g, i, j, w, q + b * qbs, nullptr, lastbc + b * bbs, q0c[b * g.Lmax + lev]);
-    } else {
+    if constexpr (kFree) return value_free(g, b, ii, jj, rhs);
+    if constexpr (!kFree) {
       rhs = rot_nonlinear<T>(g, i, j, w, bom ? bq + b * qbs : q + b * qbs, bom ? bq0 + b * qbs : q0 + b * qbs,
                              lastbc + b * bbs);
       if (bom) rhs = rhs + rot_nonlinear<T>(g, i, j, bom + b * fbs, q + b * qbs, q0 + b * qbs, blb + b * bbs);
@@ -432,6 +480,99 @@ struct SrcStep1 {
     const T lap = (ii > 0 ? wl : T(0)) + (ii + 1 < g.mm1 ? wr : T(0)) + (jj > 0 ? wd : T(0)) +
                   (jj + 1 < g.nm1 ? wu : T(0)) - T(4) * w[e];
     return con1[c] * rhs + con2[c] * rold + bcs + T(2) * vfac[c] * lap;
+  }
+  // The freestream form in two phases.  gather loads every value of the
+  // element from an in-range place (the vorticity and its four neighbours,
+  // their edge replacements, 12 fluxes, rhs_old, the edge terms) and
+  // combine selects the edge values and does the arithmetic of
+  // rot_nonlinear<T, true> and of the general path above, term for term.
+  // The size-specialised passes gather all elements of a thread before
+  // combining any, so their loads are in flight together.
+  struct Gathered {
+    T C, wl, wr, wd, wu, bU, bD, bR, bL;
+    T v_a, v_b, v_c, v_d, v_e, v_f, u_a, u_b, u_c, u_d, u_e, u_f;
+    T ro, lj, li, gj1, gj2, gi1, gi2;
+  };
+  __device__ void gather(const Geo& g, int b, int ii, int jj, Gathered& G) const {
+    const T* w = om + b * fbs;
+    const T* qq = q + b * qbs;
+    const T* bc = lastbc + b * bbs;
+    const T* rc = omc != nullptr ? omc + b * fbs : w;
+    const int e = ii + jj * g.mm1;
+    const int i = ii + 2, j = jj + 2;
+    G.C = w[e];
+    G.wl = w[ii > 0 ? e - 1 : e];
+    G.wr = w[ii + 1 < g.mm1 ? e + 1 : e];
+    G.wd = w[jj > 0 ? e - g.mm1 : e];
+    G.wu = w[jj + 1 < g.nm1 ? e + g.mm1 : e];
+    G.bU = bc[g.top + i - 1];
+    G.bD = bc[g.bottom + i - 1];
+    G.bR = bc[g.right + j - 1];
+    G.bL = bc[g.left + j - 1];
+    G.v_a = qq[vix(g, i, j + 1)];
+    G.v_b = qq[vix(g, i - 1, j + 1)];
+    G.v_c = qq[vix(g, i, j)];
+    G.v_d = qq[vix(g, i - 1, j)];
+    G.v_e = qq[vix(g, i, j - 1)];
+    G.v_f = qq[vix(g, i - 1, j - 1)];
+    G.u_a = qq[uix(g, i + 1, j)];
+    G.u_b = qq[uix(g, i + 1, j - 1)];
+    G.u_c = qq[uix(g, i, j)];
+    G.u_d = qq[uix(g, i, j - 1)];
+    G.u_e = qq[uix(g, i - 1, j)];
+    G.u_f = qq[uix(g, i - 1, j - 1)];
+    G.ro = rhs_old[b * fbs + e];
+    // applybc_vec_sel / applybc_add_sel operands (get_bc places of getbc_sel)
+    G.lj = bc[(j == g.n ? g.top : g.bottom) + i - 1];
+    G.li = bc[(i == g.m ? g.right : g.left) + j - 1];
+    int a1, a2;
+    getbc_places(g, (j == g.n ? g.top : g.bottom) + i, a1, a2);
+    G.gj1 = rc[a1];
+    G.gj2 = rc[a2];
+    getbc_places(g, (i == g.m ? g.right : g.left) + j, a1, a2);
+    G.gi1 = rc[a1];
+    G.gi2 = rc[a2];
+  }
+  __device__ T combine(const Geo& g, int b, int ii, int jj, const Gathered& G, T& rhs) const {
+    const int i = ii + 2, j = jj + 2, m = g.m, n = g.n;
+    const int c = b * g.Lmax + lev;
+    const T u0 = q0c[c];
+    const T fac = vfac[c];
+    // OmU(i, j+1), OmU(i, j-1), OmV(i+1, j), OmV(i-1, j) of rot_nonlinear
+    const T nU = j == n ? G.bU : G.wu, nD = j == 2 ? G.bD : G.wd;
+    const T nR = i == m ? G.bR : G.wr, nL = i == 2 ? G.bL : G.wl;
+    const T va_up = T(0.5) * (G.v_a + G.v_b + T(0) + T(0));  // vavg(i, j+1)
+    const T va_c = T(0.5) * (G.v_c + G.v_d + T(0) + T(0));   // vavg(i, j)
+    const T va_dn = T(0.5) * (G.v_e + G.v_f + T(0) + T(0));  // vavg(i, j-1)
+    const T ua_r = T(0.5) * (G.u_a + G.u_b + u0 + u0);       // uavg(i+1, j)
+    const T ua_c = T(0.5) * (G.u_c + G.u_d + u0 + u0);       // uavg(i, j)
+    const T ua_l = T(0.5) * (G.u_e + G.u_f + u0 + u0);       // uavg(i-1, j)
+    const T fqu_c = T(0.5) * (va_up * nU + va_c * G.C);      // fqu(i, j)
+    const T fqu_d = T(0.5) * (va_c * G.C + va_dn * nD);      // fqu(i, j-1)
+    const T fqv_c = T(-0.5) * (ua_r * nR + ua_c * G.C);      // fqv(i, j)
+    const T fqv_l = T(-0.5) * (ua_c * G.C + ua_l * nL);      // fqv(i-1, j)
+    rhs = fqv_c - fqv_l - fqu_c + fqu_d;
+    const T rold = first ? rhs : G.ro;
+    // applybc_vec_sel, then applybc_add_sel when a coarser level exists
+    const bool ej = j == 2 || j == n, ei = i == 2 || i == m;
+    T bcs = T(0);
+    bcs = ej ? bcs + fac * G.lj : bcs;
+    bcs = ei ? bcs + fac * G.li : bcs;
+    const T gj = getbc_value(g, (j == n ? g.top : g.bottom) + i, G.gj1, G.gj2, T(0.25));
+    const T gi = getbc_value(g, (i == m ? g.right : g.left) + j, G.gi1, G.gi2, T(0.25));
+    T bcc = bcs;
+    bcc = ej ? bcc + fac * gj : bcc;
+    bcc = ei ? bcc + fac * gi : bcc;
+    const bool hc = omc != nullptr && has_coarser(g, Lc, b, lev);
+    bcs = hc ? bcc : bcs;
+    const T lap = (ii > 0 ? G.wl : T(0)) + (ii + 1 < g.mm1 ? G.wr : T(0)) + (jj > 0 ? G.wd : T(0)) +
+                  (jj + 1 < g.nm1 ? G.wu : T(0)) - T(4) * G.C;
+    return con1[c] * rhs + con2[c] * rold + bcs + T(2) * fac * lap;
+  }
+  __device__ T value_free(const Geo& g, int b, int ii, int jj, T& rhs) const {
+    Gathered G;
+    gather(g, b, ii, jj, G);
+    return combine(g, b, ii, jj, G, rhs);
   }
   __device__ void post(const Geo& g, int b, int ii, int jj, T rhs) const { rhs_old[b * fbs + ii + jj * g.mm1] = rhs; }
 };
