@@ -66,6 +66,15 @@ struct op_load_post<Op, std::void_t<decltype(Op::kLoadPost)>> : std::bool_consta
 //   kGather, Gathered, gather(b, g, s, P, idx, G) / combine(..., G, aux):
 //       a side-store loader in two phases, the loads of every element of a
 //       thread issued before any value is formed.
+//   kSide, side_count(), side(b, x, t): independent work that the pass's
+//       first side_groups(blockDim) workgroups do instead of a transform
+//       (a multiple of 8 of them, so that the transform groups keep their
+//       XCD order); side_count() == 0: none.
+template <class Op, class = void>
+struct op_side : std::false_type {};
+template <class Op>
+struct op_side<Op, std::void_t<decltype(Op::kSide)>> : std::bool_constant<Op::kSide> {};
+__host__ __device__ __forceinline__ int side_groups(int count, int nt) { return ((count + nt - 1) / nt + 7) & ~7; }
 template <class Op, class = void>
 struct op_gather : std::false_type {};
 template <class Op>
@@ -215,7 +224,15 @@ __global__ __launch_bounds__(kThreads) void dst_pass(Op op, FftPlan<T> plan) {
   const int len = N - 1;
   const int P = EMAX / n2;          // FFTs per workgroup
   const int b = blockIdx.y;
-  const int g = blockIdx.x;
+  int g = blockIdx.x;
+  if constexpr (op_side<Op>::value) {
+    const int sg = op.side_count() > 0 ? side_groups(op.side_count(), blockDim.x) : 0;
+    if (g < sg) {
+      op.side(b, g, threadIdx.x);
+      return;
+    }
+    g -= sg;
+  }
 
   // zero the two fixed points of every odd extension
   for (int p = threadIdx.x; p < P; p += kThreads) {
@@ -491,7 +508,9 @@ __device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g
       int s, idx;
       if (AXIS == 0) { s = ec / LEN; idx = ec - s * LEN; }
       else { idx = ec / (2 * P); s = ec - idx * (2 * P); }
-      const T v = op.combine(b, g, s, P, idx, gv[q], aux[q]);
+      T a = T(0);
+      const T v = op.combine(b, g, s, P, idx, gv[q], a);
+      if constexpr (kPost) aux[q] = a;
       x[q] = in ? v : T(0);
     }
   }
@@ -539,8 +558,17 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
   __shared__ cplx<T> z[(DB ? 2 : 1) * P * N2];
   const int t = threadIdx.x;
   const int b = blockIdx.y;
-  int g = blockIdx.x;
-  if ((gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  int g = blockIdx.x, ng = gridDim.x;
+  if constexpr (op_side<Op>::value) {
+    const int sg = op.side_count() > 0 ? side_groups(op.side_count(), NT) : 0;
+    if (g < sg) {
+      op.side(b, g, t);
+      return;
+    }
+    g -= sg;
+    ng -= sg;
+  }
+  if ((ng & 7) == 0) g = (g & 7) * (ng >> 3) + (g >> 3);
   auto out_at = [&](int e, int& p, int& k) {
     if (AXIS == 0) { p = e / LEN; k = e - p * LEN + 1; }
     else { k = e / P + 1; p = e - (k - 1) * P; }

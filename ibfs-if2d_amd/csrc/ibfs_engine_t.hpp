@@ -217,6 +217,7 @@ struct Engine : GroupBase {
     T3.alloc((size_t)B * nw);
     RR.alloc((size_t)B * nw); RR.zero(st);
     lastbc.alloc((size_t)B * Lmax * nbc);
+    lastbc.zero(st);  // the levels without a coarser level keep zero edge values
     const size_t nff = std::max(nf_tot, 1);
     fb.alloc(nff); fb.zero(st);
     motion_in.alloc(nff); motion_in.zero(st);
@@ -265,7 +266,9 @@ struct Engine : GroupBase {
   void launch_spec(const Op& op, const cplx<op_value_t<Op>>* stw, int nseq, int mult, int batch) {
     using V = op_value_t<Op>;
     if constexpr (p_fits<V>(NN, P)) {
-      const int groups = (nseq + mult * P - 1) / (mult * P);
+      int groups = (nseq + mult * P - 1) / (mult * P);
+      if constexpr (op_side<Op>::value)
+        if (op.side_count() > 0) groups += side_groups(op.side_count(), P * NN / 4);
       hipLaunchKernelGGL((spec::dst_pass_n<V, NN, P, AXIS, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
                          stw);
     }
@@ -311,7 +314,9 @@ struct Engine : GroupBase {
     }
     const FftPlan<V>& plan = AXIS == 0 ? F.pm : F.pn;
     const int Pg = kEmax / n2;
-    const int groups = (nseq + mult * Pg - 1) / (mult * Pg);
+    int groups = (nseq + mult * Pg - 1) / (mult * Pg);
+    if constexpr (op_side<Op>::value)
+      if (op.side_count() > 0) groups += side_groups(op.side_count(), kThreads);
     hipLaunchKernelGGL((dst_pass<V, kEmax, AXIS, Op>), dim3(groups, batch), dim3(kThreads),
                        kEmax * sizeof(cplx<V>), st, op, plan);
   }
@@ -561,11 +566,15 @@ struct Engine : GroupBase {
   // lastbc and STEP 1 of advance
   void enqueue_step1(int first) {
     const long FB = fbs(), QB = qbs(), BB = (long)Lmax * nbc;
+    // freestream form: lastbc is side work of the first row pass (the
+    // coarsest slot, which reads no level's lastbc that it writes)
+    const bool fold_lastbc = !base && q0_uniform() && Lmax > 1;
     if (base)  // tangent step: the edge values of the base state before its own step
       hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, base->omega.p,
                          base->lastbc.p, gate);
-    hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, omega.p, lastbc.p,
-                       gate);
+    if (!fold_lastbc)
+      hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, geo, omega.p, lastbc.p,
+                         gate);
     // STEP 1 (src/operators.f90:103-134), level slots Lmax..1, in increment
     // form: omega += S[lam1i * S(X + 2 vfac D0 omega)] (see SrcStep1)
     for (int lev = Lmax - 1; lev >= 0; lev--) {
@@ -594,6 +603,11 @@ struct Engine : GroupBase {
         SrcStep1<T, true> s1f{s1.om, s1.omc, s1.q, nullptr, s1.lastbc, s1.rhs_old, s1.con1, s1.con2, s1.vfac,
                               s1.Lc, s1.fbs, s1.qbs, s1.bbs, s1.lev, s1.first};
         s1f.q0c = q0c.p;
+        if (fold_lastbc && lev == Lmax - 1) {
+          s1f.lastbc_all = lastbc.p;
+          s1f.om_all = omega.p;
+          s1f.side_count = Lmax * nbc;
+        }
         launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geo, s1f, DstField<T>{T1.p, nw}, gate, lev}, geo.nm1, 2,
                   B, kStep1Row);
       } else {

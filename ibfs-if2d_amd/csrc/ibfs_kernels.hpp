@@ -207,6 +207,49 @@ struct SrcField {
 template <typename T, typename TS = T, typename TC = TS>
 struct SrcVortBC {
   const T* om; const TC* sc; long bs; const int* Lc; int lev;
+  // two phases for the size-specialised passes (see SrcStep1::gather): the
+  // vertex value and the four get_bc values an edge vertex may add
+  static constexpr bool kGather = true;
+  struct Gathered {
+    T v;
+    TC gj1, gj2, gi1, gi2;
+  };
+  __device__ void gather(const Geo& g, int b, int ii, int jj, Gathered& G) const {
+    const int i = ii + 2, j = jj + 2;
+    G.v = om[b * bs + ii + jj * g.mm1];
+    G.gj1 = G.gj2 = G.gi1 = G.gi2 = TC(0);
+    // only edge vertices add get_bc values (a branch around loads alone
+    // waits for nothing)
+    if (sc != nullptr && (i == 2 || i == g.m || j == 2 || j == g.n)) {
+      const TC* rc = sc + b * bs;
+      int a1, a2;
+      getbc_places(g, (j == g.n ? g.top : g.bottom) + i, a1, a2);
+      G.gj1 = rc[a1];
+      G.gj2 = rc[a2];
+      getbc_places(g, (i == g.m ? g.right : g.left) + j, a1, a2);
+      G.gi1 = rc[a1];
+      G.gi2 = rc[a2];
+    }
+  }
+  // the value of operator() from the gathered loads (applybc_add_sel order)
+  __device__ TS combine(const Geo& g, int b, int ii, int jj, const Gathered& G, TS& aux) const {
+    TS v = TS(G.v);
+    if (sc == nullptr || !has_coarser(g, Lc, b, lev)) return v;
+    const int i = ii + 2, j = jj + 2;
+    const bool ej = j == 2 || j == g.n, ei = i == 2 || i == g.m;
+    const TC gj = getbc_value<TC>(g, (j == g.n ? g.top : g.bottom) + i, G.gj1, G.gj2, TC(1));
+    const TC gi = getbc_value<TC>(g, (i == g.m ? g.right : g.left) + j, G.gi1, G.gi2, TC(1));
+    if constexpr (std::is_same_v<TS, TC>) {
+      v = ej ? v + TS(1) * gj : v;
+      v = ei ? v + TS(1) * gi : v;
+    } else {
+      TC acc = TC(0);
+      acc = ej ? acc + TC(1) * gj : acc;
+      acc = ei ? acc + TC(1) * gi : acc;
+      v = (ej || ei) ? v + TS(acc) : v;
+    }
+    return v;
+  }
   __device__ TS operator()(const Geo& g, int b, int ii, int jj) const {
     TS v = TS(om[b * bs + ii + jj * g.mm1]);
     if (sc == nullptr || !has_coarser(g, Lc, b, lev)) return v;
@@ -249,6 +292,11 @@ template <class S, class = void>
 struct src_has_post : std::false_type {};
 template <class S>
 struct src_has_post<S, std::void_t<decltype(&S::post)>> : std::true_type {};
+// Src::side present (independent work done by extra workgroups of the pass)?
+template <class S, class = void>
+struct src_has_side : std::false_type {};
+template <class S>
+struct src_has_side<S, std::void_t<decltype(&S::side)>> : std::true_type {};
 // Src::gather/combine present and enabled (a loader in two phases)?
 template <class S, class = void>
 struct src_gathered {
@@ -291,6 +339,15 @@ struct RowOp {
       if (row < g.nm1 && active(Lc, b, lev)) src.post(g, b, idx, row, aux);
     }
   }
+  // side work of the loader (Src::side), done by extra workgroups
+  static constexpr bool kSide = src_has_side<Src>::value;
+  __host__ __device__ int side_count() const {
+    if constexpr (kSide) return src.side_count;
+    else return 0;
+  }
+  __device__ void side(int b, int x, int t) const {
+    if constexpr (kSide) src.side(g, b, x, t);
+  }
   // loaders in two phases (Src::gather/combine): the loads of an element on
   // a row that exists whatever the test, then the value
   static constexpr bool kGather = src_gathered<Src>::value;
@@ -304,6 +361,7 @@ struct RowOp {
     const T v = src.combine(g, b, idx, row < g.nm1 ? row : g.nm1 - 1, G, aux);
     return (row < g.nm1 && active(Lc, b, lev)) ? v : T(0);
   }
+
   __device__ void store(int b, int gr, int p, int P, int k, T ya, T yb) const {
     if (!active(Lc, b, lev)) return;
     const int r0 = gr * 2 * P + 2 * p, r1 = r0 + 1;
@@ -446,6 +504,22 @@ struct SrcStep1 {
   const T* blb = nullptr;
   const T* q0c = nullptr;  // [b*Lmax + lev]: the u-face value of the freestream q0 (kFree)
   static constexpr bool kGather = kFree;  // the freestream form loads in two phases
+  // lastbc of every level as side work (the job of k_lastbc), for the launch
+  // of the coarsest level slot, which neither reads those levels' lastbc nor
+  // writes any omega: lastbc_all[(b Lmax + k) nbc + p] = get_bc(omega_all of
+  // level k+1, 0.25) for the levels k with a coarser level; the others are
+  // not written (they hold zeros).  side_count = Lmax * nbc, 0 for none.
+  T* lastbc_all = nullptr;
+  const T* om_all = nullptr;
+  int side_count = 0;
+  __device__ void side(const Geo& g, int b, int x, int t) const {
+    const int u = x * (int)blockDim.x + t;
+    if (u >= side_count) return;
+    const int k = u / g.nbc, p = u - k * g.nbc;
+    if (!has_coarser(g, Lc, b, k)) return;
+    const long fb = (long)b * g.Lmax;
+    lastbc_all[(fb + k) * g.nbc + p] = getbc_at<T>(g, om_all + (fb + k + 1) * g.nw, p + 1, T(0.25));
+  }
   __device__ T operator()(const Geo& g, int b, int ii, int jj) const {
     T rhs;
     const T v = value(g, b, ii, jj, rhs);
@@ -505,10 +579,6 @@ struct SrcStep1 {
     G.wr = w[ii + 1 < g.mm1 ? e + 1 : e];
     G.wd = w[jj > 0 ? e - g.mm1 : e];
     G.wu = w[jj + 1 < g.nm1 ? e + g.mm1 : e];
-    G.bU = bc[g.top + i - 1];
-    G.bD = bc[g.bottom + i - 1];
-    G.bR = bc[g.right + j - 1];
-    G.bL = bc[g.left + j - 1];
     G.v_a = qq[vix(g, i, j + 1)];
     G.v_b = qq[vix(g, i - 1, j + 1)];
     G.v_c = qq[vix(g, i, j)];
@@ -522,16 +592,25 @@ struct SrcStep1 {
     G.u_e = qq[uix(g, i - 1, j)];
     G.u_f = qq[uix(g, i - 1, j - 1)];
     G.ro = rhs_old[b * fbs + e];
-    // applybc_vec_sel / applybc_add_sel operands (get_bc places of getbc_sel)
-    G.lj = bc[(j == g.n ? g.top : g.bottom) + i - 1];
-    G.li = bc[(i == g.m ? g.right : g.left) + j - 1];
-    int a1, a2;
-    getbc_places(g, (j == g.n ? g.top : g.bottom) + i, a1, a2);
-    G.gj1 = rc[a1];
-    G.gj2 = rc[a2];
-    getbc_places(g, (i == g.m ? g.right : g.left) + j, a1, a2);
-    G.gi1 = rc[a1];
-    G.gi2 = rc[a2];
+    // the edge values and the applybc_vec_sel / applybc_add_sel operands
+    // (get_bc places of getbc_sel), loaded for edge vertices only: a branch
+    // around loads alone waits for nothing; the others are never selected
+    G.bU = G.bD = G.bR = G.bL = G.lj = G.li = G.gj1 = G.gj2 = G.gi1 = G.gi2 = T(0);
+    if (i == 2 || i == g.m || j == 2 || j == g.n) {
+      G.bU = bc[g.top + i - 1];
+      G.bD = bc[g.bottom + i - 1];
+      G.bR = bc[g.right + j - 1];
+      G.bL = bc[g.left + j - 1];
+      G.lj = bc[(j == g.n ? g.top : g.bottom) + i - 1];
+      G.li = bc[(i == g.m ? g.right : g.left) + j - 1];
+      int a1, a2;
+      getbc_places(g, (j == g.n ? g.top : g.bottom) + i, a1, a2);
+      G.gj1 = rc[a1];
+      G.gj2 = rc[a2];
+      getbc_places(g, (i == g.m ? g.right : g.left) + j, a1, a2);
+      G.gi1 = rc[a1];
+      G.gi2 = rc[a2];
+    }
   }
   __device__ T combine(const Geo& g, int b, int ii, int jj, const Gathered& G, T& rhs) const {
     const int i = ii + 2, j = jj + 2, m = g.m, n = g.n;
