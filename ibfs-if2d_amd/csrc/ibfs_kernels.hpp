@@ -746,15 +746,44 @@ __global__ void k_coarsify2(Geo g, T* __restrict__ om, long bs, const int* __res
     om[b * bs + (long)lc * g.nw + wix(g, i, j)] = restrict_at<T>(F1, ii, jj);
     return;
   }
-  // level lev values: inside its centre, coarsified from level lev-1 here
+  // level lev values: inside its centre, coarsified from level lev-1 here.
+  // The nine restrictions read one 7 x 7 block of level lev-1 around
+  // (X, Y), the level lev-1 place of (ii, jj); the block (places clamped
+  // into the field, used only where they are in it) and the nine stored
+  // level lev values (read at (2, 2), outside the centre being written, for
+  // places inside it, whose value is recomputed) are all loaded before any
+  // is used.  The arithmetic is restrict_at's, so the result is unchanged.
   const T* r0 = om + b * bs + (long)(lev - 1) * g.nw;
-  auto F0 = [&](int a, int c) -> T { return r0[wix(g, a, c)]; };
-  auto Fm = [&](int a, int c) -> T {
-    if (a >= g.m / 4 + 2 && a <= 3 * g.m / 4 && c >= g.n / 4 + 2 && c <= 3 * g.n / 4)
-      return restrict_at<T>(F0, 2 * a - g.m / 2 - 1, 2 * c - g.n / 2 - 1);
-    return F1(a, c);
-  };
-  om[b * bs + (long)lc * g.nw + wix(g, i, j)] = restrict_at<T>(Fm, ii, jj);
+  const int X = 2 * ii - g.m / 2 - 1, Y = 2 * jj - g.n / 2 - 1;
+  T blk[7][7];
+#pragma unroll
+  for (int dy = 0; dy < 7; dy++)
+#pragma unroll
+    for (int dx = 0; dx < 7; dx++) {
+      const int xa = min(max(X - 3 + dx, 2), g.m), ya = min(max(Y - 3 + dy, 2), g.n);
+      blk[dy][dx] = r0[wix(g, xa, ya)];
+    }
+  bool in[3][3];
+  T f1v[3][3];
+#pragma unroll
+  for (int dc = 0; dc < 3; dc++)
+#pragma unroll
+    for (int da = 0; da < 3; da++) {
+      const int a = ii - 1 + da, c = jj - 1 + dc;
+      in[dc][da] = a >= g.m / 4 + 2 && a <= 3 * g.m / 4 && c >= g.n / 4 + 2 && c <= 3 * g.n / 4;
+      f1v[dc][da] = F1(in[dc][da] ? 2 : a, in[dc][da] ? 2 : c);
+    }
+  // (ii-1+da, jj-1+dc) is level lev-1 place (X+2da-2, Y+2dc-2), block entry
+  // (1+2da, 1+2dc); the indices below are all constants
+  auto B0 = [&](int a, int c) -> T { return blk[c][a]; };
+  T fm[3][3];
+#pragma unroll
+  for (int dc = 0; dc < 3; dc++)
+#pragma unroll
+    for (int da = 0; da < 3; da++)
+      fm[dc][da] = in[dc][da] ? restrict_at<T>(B0, 1 + 2 * da, 1 + 2 * dc) : f1v[dc][da];
+  auto Fm = [&](int a, int c) -> T { return fm[c][a]; };
+  om[b * bs + (long)lc * g.nw + wix(g, i, j)] = restrict_at<T>(Fm, 1, 1);
 }
 
 // curl face value (src/operators.f90:243-288) for 0-based face f; the edge
