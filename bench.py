@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per launch of the fused column pair at 512^2 (FP64): FETCH_SIZE
 # and WRITE_SIZE per dispatch from two rocprofv3 --pmc runs, each scaled by
 # what the counter reads for a copy kernel of the same size
-# (profiles/dst_col2_512x5_fp64_hbm_counters.md: 2632.8 KiB x 1.983 fetched +
+# (profiles/dst_col2_512x5_fp64_hbm_counters.md: 2353.6 KiB x 1.980 fetched +
 # 2135.8 KiB x 1.000 written)
-PAIR_TRAFFIC_BYTES = int((2632.8 * 1.983 + 2135.8 * 1.000) * 1024)
+PAIR_TRAFFIC_BYTES = int((2353.6 * 1.980 + 2135.8 * 1.000) * 1024)
 
 
 def step_bytes(m, n, L, nf, s=8):
