@@ -66,15 +66,14 @@ struct op_load_post<Op, std::void_t<decltype(Op::kLoadPost)>> : std::bool_consta
 //   kGather, Gathered, gather(b, g, s, P, idx, G) / combine(..., G, aux):
 //       a side-store loader in two phases, the loads of every element of a
 //       thread issued before any value is formed.
-//   kSide, side_count(), side(b, x, t): independent work that the pass's
-//       first side_groups(blockDim) workgroups do instead of a transform
-//       (a multiple of 8 of them, so that the transform groups keep their
-//       XCD order); side_count() == 0: none.
+//   kSide, side_groups_for(nt), side(b, x, t): independent work that the
+//       pass's first side_groups_for(blockDim) workgroups do instead of a
+//       transform (a multiple of 8 of them, so that the transform groups
+//       keep their XCD order); 0: none.
 template <class Op, class = void>
 struct op_side : std::false_type {};
 template <class Op>
 struct op_side<Op, std::void_t<decltype(Op::kSide)>> : std::bool_constant<Op::kSide> {};
-__host__ __device__ __forceinline__ int side_groups(int count, int nt) { return ((count + nt - 1) / nt + 7) & ~7; }
 template <class Op, class = void>
 struct op_gather : std::false_type {};
 template <class Op>
@@ -226,7 +225,7 @@ __global__ __launch_bounds__(kThreads) void dst_pass(Op op, FftPlan<T> plan) {
   const int b = blockIdx.y;
   int g = blockIdx.x;
   if constexpr (op_side<Op>::value) {
-    const int sg = op.side_count() > 0 ? side_groups(op.side_count(), blockDim.x) : 0;
+    const int sg = op.side_groups_for(blockDim.x);
     if (g < sg) {
       op.side(b, g, threadIdx.x);
       return;
@@ -560,7 +559,7 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
   const int b = blockIdx.y;
   int g = blockIdx.x, ng = gridDim.x;
   if constexpr (op_side<Op>::value) {
-    const int sg = op.side_count() > 0 ? side_groups(op.side_count(), NT) : 0;
+    const int sg = op.side_groups_for(NT);
     if (g < sg) {
       op.side(b, g, t);
       return;

@@ -350,6 +350,15 @@ void advance_impl(ibfs_handle* h, int* itime, int nsteps, const double* motion, 
   }
   HIPCHK(hipEventRecord(h->ev0, s));
   if (!h->stationary) h->cg_iters.assign(nsteps, 0);
+  // each step's diagnostics row is written by the next step's first pass;
+  // the last one is flushed after the loop
+  for (auto& g : h->groups) g->set_side_diag(true);
+  struct SideDiagOff {  // other paths (the adjoint's forward steps) write their rows in the step
+    ibfs_handle* h;
+    ~SideDiagOff() {
+      for (auto& g : h->groups) g->set_side_diag(false);
+    }
+  } side_diag_off{h};
   for (int k = 0; k < nsteps; k++) {
     const int it = *itime + 1;
     if (!h->stationary) {
@@ -364,6 +373,7 @@ void advance_impl(ibfs_handle* h, int* itime, int nsteps, const double* motion, 
     }
     *itime = it;
   }
+  for (auto& g : h->groups) g->flush_diag();
   HIPCHK(hipEventRecord(h->ev1, s));
   HIPCHK(hipStreamSynchronize(s));
   float ms = 0.f;

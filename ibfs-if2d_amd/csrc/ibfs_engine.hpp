@@ -267,6 +267,10 @@ struct GroupBase {
   virtual void get_state(int c, double* om, double* ro, double* q, double* q0, double* fb) = 0;
   virtual void upload_factor(int c) = 0;
   virtual void reset_counters() = 0;
+  // defer each step's diagnostics row to the next step's first pass, and
+  // write the rows still pending
+  virtual void set_side_diag(bool on) = 0;
+  virtual void flush_diag() = 0;
   virtual void ensure_diag(int cap) = 0;
   virtual void download_diag(double* dst, int nsteps, int ldrow, int col0) = 0;
   virtual bool is_double() const = 0;

@@ -105,6 +105,8 @@ struct Engine : GroupBase {
   DevBuf<double> partial, diag;
   DevBuf<double> Ainv, Gm;  // A^-1 (Nf x Nf per case) and rot(reg(A^-1)) (nvtx x Nf per case)
   DevBuf<int> counter;
+  DevBuf<int> dpend;        // per case: a diagnostics row is pending (deferred to the next step's first pass)
+  bool side_diag = false;   // defer the rows (set by the advance loop, which flushes the last one)
   int diag_cap = 0, nchunk = 64;
   int nf_tot = 0, nf_max = 0, nvtx_max = 0;
   long mat_tot = 0, gmat_tot = 0;
@@ -232,6 +234,8 @@ struct Engine : GroupBase {
     partial.alloc((size_t)B * Lmax * nchunk);
     counter.alloc(B);
     counter.zero(st);
+    dpend.alloc(B);
+    dpend.zero(st);
     ensure_diag(1);
     HIPCHK(hipStreamSynchronize(st));
     tune();
@@ -268,7 +272,7 @@ struct Engine : GroupBase {
     if constexpr (p_fits<V>(NN, P)) {
       int groups = (nseq + mult * P - 1) / (mult * P);
       if constexpr (op_side<Op>::value)
-        if (op.side_count() > 0) groups += side_groups(op.side_count(), P * NN / 4);
+        groups += op.side_groups_for(P * NN / 4);
       hipLaunchKernelGGL((spec::dst_pass_n<V, NN, P, AXIS, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
                          stw);
     }
@@ -316,7 +320,7 @@ struct Engine : GroupBase {
     const int Pg = kEmax / n2;
     int groups = (nseq + mult * Pg - 1) / (mult * Pg);
     if constexpr (op_side<Op>::value)
-      if (op.side_count() > 0) groups += side_groups(op.side_count(), kThreads);
+      groups += op.side_groups_for(kThreads);
     hipLaunchKernelGGL((dst_pass<V, kEmax, AXIS, Op>), dim3(groups, batch), dim3(kThreads),
                        kEmax * sizeof(cplx<V>), st, op, plan);
   }
@@ -607,6 +611,10 @@ struct Engine : GroupBase {
           s1f.lastbc_all = lastbc.p;
           s1f.om_all = omega.p;
           s1f.side_count = Lmax * nbc;
+          if (diag_deferred()) {  // the previous step's pending row
+            s1f.has_diag = 1;
+            s1f.dg = diag_args(dpend.p);
+          }
         }
         launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geo, s1f, DstField<T>{T1.p, nw}, gate, lev}, geo.nm1, 2,
                   B, kStep1Row);
@@ -663,10 +671,37 @@ struct Engine : GroupBase {
     else vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
     // fluxes and diagnostics (src/operators.f90:184-186)
     const int ncurl = (nq + 256 * kTailFaces - 1) / (256 * kTailFaces) * Lmax;
+    const bool defer = diag_deferred();
     hipLaunchKernelGGL(k_tail<T>, dim3(ncurl + nchunk * Lmax, B), dim3(256), 0, st, geo, S.p, q.p, omega.p, FB, QB,
-                       partial.p, nchunk, gate, kSolve64 ? (const double*)S64.p : nullptr);
-    hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, sm, q.p, QB, motion.p, fb.p, wsum.p, bdyno.p,
-                       partial.p, nchunk, gate, diag.p, rowlen, counter.p, diag_cap);
+                       partial.p, nchunk, gate, kSolve64 ? (const double*)S64.p : nullptr,
+                       defer ? dpend.p : (int*)nullptr);
+    if (!defer)
+      hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, diag_args(nullptr));
+  }
+
+  // the arguments of the diagnostics row (pend: the pending flags, or
+  // nullptr to write the row unconditionally)
+  DiagArgs<T> diag_args(int* pend) const {
+    return DiagArgs<T>{sm, q.p, qbs(), motion.p, fb.p, wsum.p, bdyno.p, partial.p, nchunk, gate,
+                       diag.p, rowlen, counter.p, diag_cap, pend};
+  }
+  // The row of a step is written by the next step's first row pass (side
+  // work, one more workgroup) when the advance loop asks for it and that
+  // pass exists: the freestream form with lastbc as side work, and a
+  // workgroup size of a multiple of 64 for the row's reductions.
+  bool diag_deferred() const {
+    if (!side_diag || base || !q0_uniform() || Lmax < 2) return false;
+    if (!fs.spec_m) return true;  // the generic pass: 256 threads
+    const int nt = ptile[kStep1Row] * fs.pm.n2 / 4;
+    return nt % 64 == 0;
+  }
+  void set_side_diag(bool on) override {
+    if (on) dpend.zero(st);  // no row left pending by an interrupted call
+    side_diag = on;
+  }
+  // writes the pending rows (after the last step of an advance call)
+  void flush_diag() override {
+    hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, diag_args(dpend.p));
   }
 
   // the compact smearing tables of every case, concatenated, on the device

@@ -191,6 +191,63 @@ __device__ __forceinline__ T rot_nonlinear(const Geo& g, int i, int j, const T* 
 }
 
 // ---------------------------------------------------------------------------
+// smearing tables (E / E^T) on compact per-case tables, see the smearing section
+// ---------------------------------------------------------------------------
+// Active faces of case b: face_id[fbase+a] (0-based face of that case's q),
+// entries face_ptr[pbase+a]..face_ptr[pbase+a+1] of (face_pt, face_w): the
+// case-local force index (x or y component) and the weight, in increasing
+// IB-point order (the ELL order of src/grid.f90:197-228).
+// Force gather: force f of case b has entries frc_ptr[rbase+f]..+1 of
+// (frc_face, frc_w) in increasing face order (the accumulation order of
+// regT, src/operators.f90:541-545).
+// Active vertices: vtx_id[vbase+v] (0-based W index) and
+// vtx_face[4(vbase+v)..+3] = active-face slots (case-local) of v(i,j),
+// v(i-1,j), u(i,j), u(i,j-1) (-1: face has no weight).
+struct CaseSm {
+  int nact, fbase, pbase;  // faces
+  int nf, rbase;           // forces
+  int nvtx, vbase;         // vertices
+  int foff;                // offset of the case's force vectors
+  int nb;                  // IB points
+  long loff;               // offset of the case's Nf x Nf matrices
+  long goff;               // offset of the case's nvtx x Nf matrix rot(reg(A^-1))
+  int nbody, bmin, ncol, hoff;  // diagnostics row layout
+  double dt, delta;
+};
+
+struct Smear {
+  const int* face_id;
+  const int* face_ptr;
+  const int* face_pt;
+  const double* face_w;
+  const int* frc_ptr;
+  const int* frc_face;
+  const double* frc_w;
+  const int* frc_ia;  // per force entry: curl(s) = s[ia] - s[ib] at an interior face, ia = -1 on the edge
+  const int* frc_ib;
+  const int* vtx_id;
+  const int* vtx_face;
+  const CaseSm* cs;
+};
+
+// Arguments of the diagnostics row (k_diag), also run as side work of the
+// next step's first pass.  pend[b] != 0: case b has a row to write (set by
+// k_tail when the row is deferred; pend == nullptr: always write).
+template <typename T>
+struct DiagArgs {
+  Smear sm;
+  const T* q; long qbs;
+  const double* motion; const double* fb; const double* wsum; const int* bdyno;
+  const double* partial; int nchunk;
+  const int* Lc;
+  double* out_base; int rowlen;
+  int* counter; int capacity;
+  int* pend;
+};
+template <typename T>
+__device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red);
+
+// ---------------------------------------------------------------------------
 // pass operators (see dst_core.hpp for the load/store protocol)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -341,8 +398,8 @@ struct RowOp {
   }
   // side work of the loader (Src::side), done by extra workgroups
   static constexpr bool kSide = src_has_side<Src>::value;
-  __host__ __device__ int side_count() const {
-    if constexpr (kSide) return src.side_count;
+  __host__ __device__ int side_groups_for(int nt) const {
+    if constexpr (kSide) return src.side_groups_for(nt);
     else return 0;
   }
   __device__ void side(int b, int x, int t) const {
@@ -508,11 +565,30 @@ struct SrcStep1 {
   // of the coarsest level slot, which neither reads those levels' lastbc nor
   // writes any omega: lastbc_all[(b Lmax + k) nbc + p] = get_bc(omega_all of
   // level k+1, 0.25) for the levels k with a coarser level; the others are
-  // not written (they hold zeros).  side_count = Lmax * nbc, 0 for none.
+  // not written (they hold zeros).  side_count = Lmax * nbc, 0 for none;
+  // side_groups_for(nt) workgroups (a multiple of 8) do the side work.
   T* lastbc_all = nullptr;
   const T* om_all = nullptr;
   int side_count = 0;
+  // With has_diag, one more side workgroup writes the pending diagnostics
+  // row of the previous step (diag_row; nt must be a multiple of 64), which
+  // reads q, fb, the circulation partials and the motion: no launch of the
+  // step before k_tail writes them.
+  DiagArgs<T> dg{};
+  int has_diag = 0;
+  __host__ __device__ int side_groups_for(int nt) const {
+    const int n = (side_count + nt - 1) / nt + (has_diag ? 1 : 0);
+    return (n + 7) & ~7;
+  }
   __device__ void side(const Geo& g, int b, int x, int t) const {
+    const int nl = (side_count + (int)blockDim.x - 1) / (int)blockDim.x;
+    if (x >= nl) {
+      if (has_diag && x == nl) {
+        __shared__ double red[16];
+        diag_row<T>(g, dg, b, red);
+      }
+      return;
+    }
     const int u = x * (int)blockDim.x + t;
     if (u >= side_count) return;
     const int k = u / g.nbc, p = u - k * g.nbc;
@@ -861,43 +937,6 @@ __global__ void k_rot_nonlinear(Geo g, const T* __restrict__ om, const T* __rest
 // ---------------------------------------------------------------------------
 // smearing (E / E^T) on compact per-case tables
 // ---------------------------------------------------------------------------
-// Active faces of case b: face_id[fbase+a] (0-based face of that case's q),
-// entries face_ptr[pbase+a]..face_ptr[pbase+a+1] of (face_pt, face_w): the
-// case-local force index (x or y component) and the weight, in increasing
-// IB-point order (the ELL order of src/grid.f90:197-228).
-// Force gather: force f of case b has entries frc_ptr[rbase+f]..+1 of
-// (frc_face, frc_w) in increasing face order (the accumulation order of
-// regT, src/operators.f90:541-545).
-// Active vertices: vtx_id[vbase+v] (0-based W index) and
-// vtx_face[4(vbase+v)..+3] = active-face slots (case-local) of v(i,j),
-// v(i-1,j), u(i,j), u(i,j-1) (-1: face has no weight).
-struct CaseSm {
-  int nact, fbase, pbase;  // faces
-  int nf, rbase;           // forces
-  int nvtx, vbase;         // vertices
-  int foff;                // offset of the case's force vectors
-  int nb;                  // IB points
-  long loff;               // offset of the case's Nf x Nf matrices
-  long goff;               // offset of the case's nvtx x Nf matrix rot(reg(A^-1))
-  int nbody, bmin, ncol, hoff;  // diagnostics row layout
-  double dt, delta;
-};
-
-struct Smear {
-  const int* face_id;
-  const int* face_ptr;
-  const int* face_pt;
-  const double* face_w;
-  const int* frc_ptr;
-  const int* frc_face;
-  const double* frc_w;
-  const int* frc_ia;  // per force entry: curl(s) = s[ia] - s[ib] at an interior face, ia = -1 on the edge
-  const int* frc_ib;
-  const int* vtx_id;
-  const int* vtx_face;
-  const CaseSm* cs;
-};
-
 // s = sum over entries e = lo..hi-1, in order, of w[e] * value(idx[e]).
 // The entries are taken K at a time: all K index/weight loads are issued,
 // then all K value loads, then the K products are added in entry order, so
@@ -1400,6 +1439,8 @@ __device__ __forceinline__ bool curl_interior(const Geo& g, int f, int& ia, int&
 //  * the next nchunk*Lmax: partial sums of omega for write_circ
 //    (src/operators.f90:675-684), chunk (x-ncurl) % nchunk of level
 //    (x-ncurl) / nchunk, at partial[((b*Lmax)+lev)*nchunk + chunk].
+// With pend set, pend[b] = 1: the step's diagnostics row is left to the
+// next launch that writes pending rows (diag_row).
 // grid (ncurl + nchunk*Lmax, B), 256 threads.
 constexpr int kTailFaces = 4;
 template <typename T, typename TS>
@@ -1431,9 +1472,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T* __restrict__ q,
                                               const T* __restrict__ om, long bs, long qbs,
                                               double* __restrict__ partial, int nchunk, const int* __restrict__ Lc,
-                                              const double* __restrict__ s1d) {
+                                              const double* __restrict__ s1d, int* __restrict__ pend) {
   __shared__ double red[4];
   const int tid = threadIdx.x, b = blockIdx.y;
+  if (pend != nullptr && blockIdx.x == 0 && tid == 0) pend[b] = 1;  // the row is deferred to the next pass
   const int ncb = (g.nq + 256 * kTailFaces - 1) / (256 * kTailFaces), ncurl = ncb * g.Lmax;
   if ((int)blockIdx.x < ncurl) {
     const int x = xcd_contiguous(blockIdx.x, ncurl);
@@ -1458,62 +1500,52 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
   if (tid == 0) partial[((long)b * g.Lmax + lev) * nchunk + ch] = acc;
 }
 
-// One workgroup per case: the diagnostics row of the step,
-// [fx(bmin..bmax), fy(bmin..bmax), circ(1..L), slip] at
-// out_base + slot*rowlen + hoff, slot = counter[b] (then incremented):
-//   force: write_force (src/operators.f90:583-636) = 2*delta * sum over
-//          faces of reg(fb/dt restricted to the body), evaluated as
-//          2*delta * sum_p wsum(p) fb(p)/dt with wsum(p) the sum of the
-//          point's weights on u (x) or v (y) faces;
-//   circ:  the sum of the level's partial sums from k_tail;
-//   slip:  write_slip (src/operators.f90:564) = max |regT(q_1) - motion|.
+// The diagnostics row of case b (see k_diag), by one workgroup whose size is a multiple of 64 (red:
+// LDS for one double per wave); a pending row (pend != nullptr) is written
+// only if pend[b] is set, and pend[b] is then cleared.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_diag(Geo g, Smear sm, const T* __restrict__ q, long qbs,
-                                              const double* __restrict__ motion, const double* __restrict__ fb,
-                                              const double* __restrict__ wsum, const int* __restrict__ bdyno,
-                                              const double* __restrict__ partial, int nchunk,
-                                              const int* __restrict__ Lc, double* __restrict__ out_base, int rowlen,
-                                              int* __restrict__ counter, int capacity) {
-  __shared__ double red[16];
-  const int tid = threadIdx.x, b = blockIdx.x, lane = tid & 63, NT = blockDim.x;
+__device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red) {
+  const Smear& sm = a.sm;
+  const int tid = threadIdx.x, lane = tid & 63, NT = blockDim.x;
+  if (a.pend != nullptr && a.pend[b] == 0) return;
   const CaseSm c = sm.cs[b];
-  const int L = levels(g, Lc, b);
-  const int slot0 = counter[b];
-  const int slot = slot0 < capacity ? slot0 : capacity - 1;
-  double* out = out_base + (long)slot * rowlen + c.hoff;
+  const int L = levels(g, a.Lc, b);
+  const int slot0 = a.counter[b];
+  const int slot = slot0 < a.capacity ? slot0 : a.capacity - 1;
+  double* out = a.out_base + (long)slot * a.rowlen + c.hoff;
   // the force terms of this thread's first point, loaded here with the
   // partials and the slip tables rather than after the slip's reduction
   const int pbase = c.foff / 2;
   int bd0 = -1;
   double fv0[2] = {0.0, 0.0};
   if (tid < c.nb) {
-    bd0 = bdyno[pbase + tid];
+    bd0 = a.bdyno[pbase + tid];
 #pragma unroll
     for (int comp = 0; comp < 2; comp++) {
       const int f = comp * c.nb + tid;
-      fv0[comp] = wsum[c.foff + f] * (fb[c.foff + f] / c.dt);
+      fv0[comp] = a.wsum[c.foff + f] * (a.fb[c.foff + f] / c.dt);
     }
   }
   // circulation: wave w sums the partials of level w
   for (int lv = tid >> 6; lv < L; lv += NT >> 6) {
-    const double* pp = partial + ((long)b * g.Lmax + lv) * nchunk;
+    const double* pp = a.partial + ((long)b * g.Lmax + lv) * a.nchunk;
     double acc = 0.0;
-    for (int k = lane; k < nchunk; k += 64) acc += pp[k];
+    for (int k = lane; k < a.nchunk; k += 64) acc += pp[k];
     acc = wave_sum(acc);
     if (lane == 0) out[2 * c.nbody + lv] = acc;
   }
   // slip, two forces per thread and pass
-  const T* q1 = q + b * qbs;
+  const T* q1 = a.q + b * a.qbs;
   auto qv = [&](int face) { return double(q1[face]); };
   double mx = 0.0;
   for (int f0 = tid; f0 < c.nf; f0 += 2 * NT) {
     const int f1 = f0 + NT;
     const double e0 = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f0], sm.frc_ptr[c.rbase + f0 + 1], sm.frc_face,
-                                        sm.frc_w, qv) - motion[c.foff + f0];
+                                        sm.frc_w, qv) - a.motion[c.foff + f0];
     double e1 = 0.0;
     if (f1 < c.nf)
       e1 = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f1], sm.frc_ptr[c.rbase + f1 + 1], sm.frc_face, sm.frc_w, qv) -
-           motion[c.foff + f1];
+           a.motion[c.foff + f1];
     mx = fmax(mx, fmax(fabs(e0), fabs(e1)));
   }
   mx = block_max(mx, red);
@@ -1524,14 +1556,33 @@ __global__ __launch_bounds__(1024) void k_diag(Geo g, Smear sm, const T* __restr
       double acc = 0.0;
       if (bd0 == c.bmin + bd) acc += fv0[comp];
       for (int p = tid + NT; p < c.nb; p += NT)
-        if (bdyno[pbase + p] == c.bmin + bd) {
+        if (a.bdyno[pbase + p] == c.bmin + bd) {
           const int f = comp * c.nb + p;
-          acc += wsum[c.foff + f] * (fb[c.foff + f] / c.dt);
+          acc += a.wsum[c.foff + f] * (a.fb[c.foff + f] / c.dt);
         }
       acc = block_sum(acc, red);
       if (tid == 0) out[comp * c.nbody + bd] = acc * 2.0 * c.delta;
     }
-  if (tid == 0) counter[b] = slot0 + 1;
+  // every thread read pend[b] before the reductions' barriers
+  if (tid == 0) {
+    a.counter[b] = slot0 + 1;
+    if (a.pend != nullptr) a.pend[b] = 0;
+  }
+}
+
+// One workgroup per case: the diagnostics row of the step,
+// [fx(bmin..bmax), fy(bmin..bmax), circ(1..L), slip] at
+// out_base + slot*rowlen + hoff, slot = counter[b] (then incremented):
+//   force: write_force (src/operators.f90:583-636) = 2*delta * sum over
+//          faces of reg(fb/dt restricted to the body), evaluated as
+//          2*delta * sum_p wsum(p) fb(p)/dt with wsum(p) the sum of the
+//          point's weights on u (x) or v (y) faces;
+//   circ:  the sum of the level's partial sums from k_tail;
+//   slip:  write_slip (src/operators.f90:564) = max |regT(q_1) - motion|.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_diag(Geo g, DiagArgs<T> a) {
+  __shared__ double red[16];
+  diag_row<T>(g, a, blockIdx.x, red);
 }
 
 }  // namespace ibfs
