@@ -674,19 +674,19 @@ struct Engine : GroupBase {
     const bool defer = diag_deferred();
     hipLaunchKernelGGL(k_tail<T>, dim3(ncurl + nchunk * Lmax, B), dim3(256), 0, st, geo, S.p, q.p, omega.p, FB, QB,
                        partial.p, nchunk, gate, kSolve64 ? (const double*)S64.p : nullptr,
-                       defer ? dpend.p : (int*)nullptr);
+                       defer ? dpend.p : (int*)nullptr, counter.p);
     if (!defer)
       hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, diag_args(nullptr));
   }
 
   // the arguments of the diagnostics row (pend: the pending flags, or
   // nullptr to write the row unconditionally)
-  DiagArgs<T> diag_args(int* pend) const {
+  DiagArgs<T> diag_args(int* pend, int clear = 0) const {
     return DiagArgs<T>{sm, q.p, qbs(), motion.p, fb.p, wsum.p, bdyno.p, partial.p, nchunk, gate,
-                       diag.p, rowlen, counter.p, diag_cap, pend};
+                       diag.p, rowlen, counter.p, diag_cap, pend, clear};
   }
   // The row of a step is written by the next step's first row pass (side
-  // work, one more workgroup) when the advance loop asks for it and that
+  // work, three more workgroups) when the advance loop asks for it and that
   // pass exists: the freestream form with lastbc as side work, and a
   // workgroup size of a multiple of 64 for the row's reductions.
   bool diag_deferred() const {
@@ -701,7 +701,7 @@ struct Engine : GroupBase {
   }
   // writes the pending rows (after the last step of an advance call)
   void flush_diag() override {
-    hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, diag_args(dpend.p));
+    hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, diag_args(dpend.p, 1));
   }
 
   // the compact smearing tables of every case, concatenated, on the device

@@ -231,8 +231,11 @@ struct Smear {
 };
 
 // Arguments of the diagnostics row (k_diag), also run as side work of the
-// next step's first pass.  pend[b] != 0: case b has a row to write (set by
-// k_tail when the row is deferred; pend == nullptr: always write).
+// next step's first pass.  pend == nullptr: the row goes to slot counter[b],
+// which is then incremented.  Otherwise the row is deferred: k_tail sets
+// pend[b] and increments counter[b], and the row (slot counter[b] - 1) is
+// written only while pend[b] is set; clear: pend[b] is then cleared (the
+// flush after the last step; the side work leaves it to the flush).
 template <typename T>
 struct DiagArgs {
   Smear sm;
@@ -242,10 +245,10 @@ struct DiagArgs {
   const int* Lc;
   double* out_base; int rowlen;
   int* counter; int capacity;
-  int* pend;
+  int* pend; int clear;
 };
 template <typename T>
-__device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red);
+__device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red, int part);
 
 // ---------------------------------------------------------------------------
 // pass operators (see dst_core.hpp for the load/store protocol)
@@ -570,22 +573,22 @@ struct SrcStep1 {
   T* lastbc_all = nullptr;
   const T* om_all = nullptr;
   int side_count = 0;
-  // With has_diag, one more side workgroup writes the pending diagnostics
-  // row of the previous step (diag_row; nt must be a multiple of 64), which
-  // reads q, fb, the circulation partials and the motion: no launch of the
-  // step before k_tail writes them.
+  // With has_diag, three more side workgroups write the pending diagnostics
+  // row of the previous step (diag_row parts 0-2; nt must be a multiple of
+  // 64), which reads q, fb, the circulation partials and the motion: no
+  // launch of the step before k_tail writes them.
   DiagArgs<T> dg{};
   int has_diag = 0;
   __host__ __device__ int side_groups_for(int nt) const {
-    const int n = (side_count + nt - 1) / nt + (has_diag ? 1 : 0);
+    const int n = (side_count + nt - 1) / nt + (has_diag ? 3 : 0);
     return (n + 7) & ~7;
   }
   __device__ void side(const Geo& g, int b, int x, int t) const {
     const int nl = (side_count + (int)blockDim.x - 1) / (int)blockDim.x;
-    if (x >= nl) {
-      if (has_diag && x == nl) {
+    if (x >= nl) {  // the row's circulation, slip and forces, one workgroup each
+      if (has_diag && x - nl < 3) {
         __shared__ double red[16];
-        diag_row<T>(g, dg, b, red);
+        diag_row<T>(g, dg, b, red, x - nl);
       }
       return;
     }
@@ -1439,8 +1442,9 @@ __device__ __forceinline__ bool curl_interior(const Geo& g, int f, int& ia, int&
 //  * the next nchunk*Lmax: partial sums of omega for write_circ
 //    (src/operators.f90:675-684), chunk (x-ncurl) % nchunk of level
 //    (x-ncurl) / nchunk, at partial[((b*Lmax)+lev)*nchunk + chunk].
-// With pend set, pend[b] = 1: the step's diagnostics row is left to the
-// next launch that writes pending rows (diag_row).
+// With pend set, pend[b] = 1 and counter[b] += 1: the step's diagnostics
+// row (slot counter[b] - 1) is left to the next launch that writes pending
+// rows (diag_row).
 // grid (ncurl + nchunk*Lmax, B), 256 threads.
 constexpr int kTailFaces = 4;
 template <typename T, typename TS>
@@ -1472,10 +1476,14 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T* __restrict__ q,
                                               const T* __restrict__ om, long bs, long qbs,
                                               double* __restrict__ partial, int nchunk, const int* __restrict__ Lc,
-                                              const double* __restrict__ s1d, int* __restrict__ pend) {
+                                              const double* __restrict__ s1d, int* __restrict__ pend,
+                                              int* __restrict__ counter) {
   __shared__ double red[4];
   const int tid = threadIdx.x, b = blockIdx.y;
-  if (pend != nullptr && blockIdx.x == 0 && tid == 0) pend[b] = 1;  // the row is deferred to the next pass
+  if (pend != nullptr && blockIdx.x == 0 && tid == 0) {  // the row is deferred to the next pass
+    pend[b] = 1;
+    counter[b] += 1;
+  }
   const int ncb = (g.nq + 256 * kTailFaces - 1) / (256 * kTailFaces), ncurl = ncb * g.Lmax;
   if ((int)blockIdx.x < ncurl) {
     const int x = xcd_contiguous(blockIdx.x, ncurl);
@@ -1500,17 +1508,19 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
   if (tid == 0) partial[((long)b * g.Lmax + lev) * nchunk + ch] = acc;
 }
 
-// The diagnostics row of case b (see k_diag), by one workgroup whose size is a multiple of 64 (red:
-// LDS for one double per wave); a pending row (pend != nullptr) is written
-// only if pend[b] is set, and pend[b] is then cleared.
+// The diagnostics row of case b (see k_diag and DiagArgs), by one workgroup
+// whose size is a multiple of 64 (red: LDS for one double per wave).  part
+// -1 writes the whole row; 0, 1, 2 its circulation, slip and forces (three
+// workgroups that write disjoint entries).
 template <typename T>
-__device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red) {
+__device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red, int part) {
   const Smear& sm = a.sm;
   const int tid = threadIdx.x, lane = tid & 63, NT = blockDim.x;
-  if (a.pend != nullptr && a.pend[b] == 0) return;
+  const bool deferred = a.pend != nullptr;
+  if (deferred && a.pend[b] == 0) return;
   const CaseSm c = sm.cs[b];
   const int L = levels(g, a.Lc, b);
-  const int slot0 = a.counter[b];
+  const int slot0 = deferred ? a.counter[b] - 1 : a.counter[b];
   const int slot = slot0 < a.capacity ? slot0 : a.capacity - 1;
   double* out = a.out_base + (long)slot * a.rowlen + c.hoff;
   // the force terms of this thread's first point, loaded here with the
@@ -1518,7 +1528,7 @@ __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red)
   const int pbase = c.foff / 2;
   int bd0 = -1;
   double fv0[2] = {0.0, 0.0};
-  if (tid < c.nb) {
+  if ((part < 0 || part == 2) && tid < c.nb) {
     bd0 = a.bdyno[pbase + tid];
 #pragma unroll
     for (int comp = 0; comp < 2; comp++) {
@@ -1527,31 +1537,32 @@ __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red)
     }
   }
   // circulation: wave w sums the partials of level w
-  for (int lv = tid >> 6; lv < L; lv += NT >> 6) {
+  for (int lv = tid >> 6; (part < 0 || part == 0) && lv < L; lv += NT >> 6) {
     const double* pp = a.partial + ((long)b * g.Lmax + lv) * a.nchunk;
     double acc = 0.0;
     for (int k = lane; k < a.nchunk; k += 64) acc += pp[k];
     acc = wave_sum(acc);
     if (lane == 0) out[2 * c.nbody + lv] = acc;
   }
-  // slip, two forces per thread and pass
-  const T* q1 = a.q + b * a.qbs;
-  auto qv = [&](int face) { return double(q1[face]); };
-  double mx = 0.0;
-  for (int f0 = tid; f0 < c.nf; f0 += 2 * NT) {
-    const int f1 = f0 + NT;
-    const double e0 = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f0], sm.frc_ptr[c.rbase + f0 + 1], sm.frc_face,
-                                        sm.frc_w, qv) - a.motion[c.foff + f0];
-    double e1 = 0.0;
-    if (f1 < c.nf)
-      e1 = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f1], sm.frc_ptr[c.rbase + f1 + 1], sm.frc_face, sm.frc_w, qv) -
-           a.motion[c.foff + f1];
-    mx = fmax(mx, fmax(fabs(e0), fabs(e1)));
+  // slip, three forces per thread and pass
+  if (part < 0 || part == 1) {
+    const T* q1 = a.q + b * a.qbs;
+    auto qv = [&](int face) { return double(q1[face]); };
+    auto slip_at = [&](int f) {
+      return f < c.nf ? fabs(gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f], sm.frc_ptr[c.rbase + f + 1], sm.frc_face,
+                                               sm.frc_w, qv) - a.motion[c.foff + f])
+                      : 0.0;
+    };
+    double mx = 0.0;
+    for (int f0 = tid; f0 < c.nf; f0 += 3 * NT) {
+      const double e0 = slip_at(f0), e1 = slip_at(f0 + NT), e2 = slip_at(f0 + 2 * NT);
+      mx = fmax(mx, fmax(e0, fmax(e1, e2)));
+    }
+    mx = block_max(mx, red);
+    if (tid == 0) out[2 * c.nbody + L] = mx;
   }
-  mx = block_max(mx, red);
-  if (tid == 0) out[2 * c.nbody + L] = mx;
   // forces, one body and component at a time
-  for (int bd = 0; bd < c.nbody; bd++)
+  for (int bd = 0; (part < 0 || part == 2) && bd < c.nbody; bd++)
     for (int comp = 0; comp < 2; comp++) {
       double acc = 0.0;
       if (bd0 == c.bmin + bd) acc += fv0[comp];
@@ -1563,10 +1574,9 @@ __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red)
       acc = block_sum(acc, red);
       if (tid == 0) out[comp * c.nbody + bd] = acc * 2.0 * c.delta;
     }
-  // every thread read pend[b] before the reductions' barriers
   if (tid == 0) {
-    a.counter[b] = slot0 + 1;
-    if (a.pend != nullptr) a.pend[b] = 0;
+    if (!deferred) a.counter[b] = slot0 + 1;
+    else if (a.clear) a.pend[b] = 0;  // every thread read pend[b] before the reductions' barriers
   }
 }
 
@@ -1582,7 +1592,7 @@ __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red)
 template <typename T>
 __global__ __launch_bounds__(1024) void k_diag(Geo g, DiagArgs<T> a) {
   __shared__ double red[16];
-  diag_row<T>(g, a, blockIdx.x, red);
+  diag_row<T>(g, a, blockIdx.x, red, -1);
 }
 
 }  // namespace ibfs
