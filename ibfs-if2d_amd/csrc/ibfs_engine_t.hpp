@@ -97,6 +97,8 @@ struct Engine : GroupBase {
 
   DevBuf<int> face_id, face_ptr, face_pt, frc_ptr, frc_face, frc_ia, frc_ib, vtx_id, vtx_face, bdyno;
   DevBuf<double> face_w, frc_w, wsum;
+  DevBuf<int> f9_face, f9_ia, f9_ib;  // the force gather padded to kFrcK entries per force
+  DevBuf<double> f9_w;
   DevBuf<CaseSm> csm;
   std::vector<CaseSm> hcs;
   Smear sm{};
@@ -776,7 +778,39 @@ struct Engine : GroupBase {
     upvec(bdyno, bd, st);
     upvec(csm, hcs, st);
     sm = Smear{face_id.p, face_ptr.p, face_pt.p, face_w.p, frc_ptr.p, frc_face.p, frc_w.p, frc_ia.p, frc_ib.p,
-               vtx_id.p,   vtx_face.p, csm.p};
+               vtx_id.p,   vtx_face.p, csm.p, hcs[0], B == 1};
+    // the padded force gather, when no force has more than kFrcK entries
+    const size_t nslots = (size_t)std::max(foff, 1) * kFrcK;
+    std::vector<int> p9f(nslots, 0), p9a(nslots, 0), p9b(nslots, 0);
+    std::vector<double> p9w(nslots, 0.0);
+    bool pad_ok = true;
+    for (int b = 0; b < B && pad_ok; b++) {
+      const CaseHost& c = cs[b];
+      for (int f = 0; f < c.nf && pad_ok; f++) {
+        const int lo = c.frc_ptr[f], hi = c.frc_ptr[f + 1];
+        if (hi - lo > kFrcK) {
+          pad_ok = false;
+          break;
+        }
+        for (int e = lo; e < hi; e++) {
+          const size_t slot = (size_t)(hcs[b].foff + f) * kFrcK + (e - lo);
+          p9f[slot] = c.frc_face[e];
+          p9w[slot] = c.frc_w[e];
+          p9a[slot] = c.frc_ia[e];
+          p9b[slot] = c.frc_ib[e];
+        }
+      }
+    }
+    if (pad_ok) {
+      upvec(f9_face, p9f, st);
+      upvec(f9_w, p9w, st);
+      upvec(f9_ia, p9a, st);
+      upvec(f9_ib, p9b, st);
+      sm.f9_face = f9_face.p;
+      sm.f9_w = f9_w.p;
+      sm.f9_ia = f9_ia.p;
+      sm.f9_ib = f9_ib.p;
+    }
   }
 
   // new positions of the points of case c (one-case handles): what advance

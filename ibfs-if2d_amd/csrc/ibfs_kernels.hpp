@@ -228,7 +228,23 @@ struct Smear {
   const int* vtx_id;
   const int* vtx_face;
   const CaseSm* cs;
+  // a group of one case also carries its record by value, so the kernels
+  // of the step start their table loads without first loading it
+  CaseSm c0;
+  int single;
+  // the force gather padded to kFrcK entries per force, at (foff + f) *
+  // kFrcK + k (zero weight and index 0 in the unused slots), so a force's
+  // entries are loaded without first loading its range; nullptr when some
+  // force has more entries
+  const int* f9_face;
+  const double* f9_w;
+  const int* f9_ia;
+  const int* f9_ib;
 };
+__device__ __forceinline__ CaseSm case_sm(const Smear& sm, int b) {
+  if (sm.single) return sm.c0;
+  return sm.cs[b];
+}
 
 // Arguments of the diagnostics row (k_diag), also run as side work of the
 // next step's first pass.  pend == nullptr: the row goes to slot counter[b],
@@ -1014,8 +1030,33 @@ __device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, co
   // entries' index, weight and field loads are each issued together; faces
   // on the domain edge (ia < 0, only for bodies touching the edge) fall back
   // to curl_face with its get_bc terms.  Same values as curl_face.
-  const int lo = sm.frc_ptr[c.rbase + f], hi = sm.frc_ptr[c.rbase + f + 1];
   double s = 0.0;
+  if (sm.f9_w != nullptr) {  // padded: the slots past a force's entries add +0
+    const int base = (c.foff + f) * kFrcK;
+    int ia[kFrcK], ib[kFrcK];
+    double ww[kFrcK];
+    T va[kFrcK], vb[kFrcK];
+#pragma unroll
+    for (int k = 0; k < kFrcK; k++) {
+      ia[k] = sm.f9_ia[base + k];
+      ib[k] = sm.f9_ib[base + k];
+      ww[k] = sm.f9_w[base + k];
+    }
+#pragma unroll
+    for (int k = 0; k < kFrcK; k++) {
+      va[k] = s1[ia[k] < 0 ? 0 : ia[k]];
+      vb[k] = s1[ib[k] < 0 ? 0 : ib[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kFrcK; k++) {
+      const double v =
+          ia[k] >= 0 ? double(va[k] - vb[k]) : double(curl_face<T, TC>(g, sm.f9_face[base + k], s1, sc));
+      s = s + ww[k] * v;
+    }
+    if (sub) s = s - sub[c.foff + f];
+    return s;
+  }
+  const int lo = sm.frc_ptr[c.rbase + f], hi = sm.frc_ptr[c.rbase + f + 1];
   for (int base = lo; base < hi; base += kFrcK) {
     int ia[kFrcK], ib[kFrcK];
     double ww[kFrcK];
@@ -1229,7 +1270,7 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
                                                      int mode) {
   extern __shared__ double vec[];
   const int b = blockIdx.y, tid = threadIdx.x;
-  const CaseSm c = sm.cs[b];
+  const CaseSm c = case_sm(sm, b);
   const int nf = c.nf;
   const int nrows = mode == 1 ? nf : nf + c.nvtx;
   const int row0 = blockIdx.x * kForceRows;
@@ -1517,25 +1558,34 @@ __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red,
   const Smear& sm = a.sm;
   const int tid = threadIdx.x, lane = tid & 63, NT = blockDim.x;
   const bool deferred = a.pend != nullptr;
-  if (deferred && a.pend[b] == 0) return;
-  const CaseSm c = sm.cs[b];
+  const int pending = deferred ? a.pend[b] : 1;  // tested after the first loads are issued
+  const CaseSm c = case_sm(sm, b);
   const int L = levels(g, a.Lc, b);
   const int slot0 = deferred ? a.counter[b] - 1 : a.counter[b];
   const int slot = slot0 < a.capacity ? slot0 : a.capacity - 1;
   double* out = a.out_base + (long)slot * a.rowlen + c.hoff;
-  // the force terms of this thread's first point, loaded here with the
+  // the force terms of this thread's first points, loaded here with the
   // partials and the slip tables rather than after the slip's reduction
+  // (the first kDiagPts points of the thread, p = tid + i NT)
+  constexpr int kDiagPts = 3;
   const int pbase = c.foff / 2;
-  int bd0 = -1;
-  double fv0[2] = {0.0, 0.0};
-  if ((part < 0 || part == 2) && tid < c.nb) {
-    bd0 = a.bdyno[pbase + tid];
+  int bdp[kDiagPts];
+  double fvp[kDiagPts][2];
 #pragma unroll
-    for (int comp = 0; comp < 2; comp++) {
-      const int f = comp * c.nb + tid;
-      fv0[comp] = a.wsum[c.foff + f] * (a.fb[c.foff + f] / c.dt);
+  for (int i = 0; i < kDiagPts; i++) {
+    const int p = tid + i * NT;
+    bdp[i] = -1;
+    fvp[i][0] = fvp[i][1] = 0.0;
+    if ((part < 0 || part == 2) && p < c.nb) {
+      bdp[i] = a.bdyno[pbase + p];
+#pragma unroll
+      for (int comp = 0; comp < 2; comp++) {
+        const int f = comp * c.nb + p;
+        fvp[i][comp] = a.wsum[c.foff + f] * (a.fb[c.foff + f] / c.dt);
+      }
     }
   }
+  if (!pending) return;
   // circulation: wave w sums the partials of level w
   for (int lv = tid >> 6; (part < 0 || part == 0) && lv < L; lv += NT >> 6) {
     const double* pp = a.partial + ((long)b * g.Lmax + lv) * a.nchunk;
@@ -1548,10 +1598,27 @@ __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red,
   if (part < 0 || part == 1) {
     const T* q1 = a.q + b * a.qbs;
     auto qv = [&](int face) { return double(q1[face]); };
-    auto slip_at = [&](int f) {
-      return f < c.nf ? fabs(gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f], sm.frc_ptr[c.rbase + f + 1], sm.frc_face,
-                                               sm.frc_w, qv) - a.motion[c.foff + f])
-                      : 0.0;
+    auto slip_at = [&](int f) -> double {
+      if (f >= c.nf) return 0.0;
+      double r;
+      if (sm.f9_w != nullptr) {  // padded: the slots past a force's entries add +0
+        const int base = (c.foff + f) * kFrcK;
+        int id[kFrcK];
+        double ww[kFrcK], v[kFrcK];
+#pragma unroll
+        for (int k = 0; k < kFrcK; k++) {
+          id[k] = sm.f9_face[base + k];
+          ww[k] = sm.f9_w[base + k];
+        }
+#pragma unroll
+        for (int k = 0; k < kFrcK; k++) v[k] = qv(id[k]);
+        r = 0.0;
+#pragma unroll
+        for (int k = 0; k < kFrcK; k++) r = r + ww[k] * v[k];
+      } else {
+        r = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f], sm.frc_ptr[c.rbase + f + 1], sm.frc_face, sm.frc_w, qv);
+      }
+      return fabs(r - a.motion[c.foff + f]);
     };
     double mx = 0.0;
     for (int f0 = tid; f0 < c.nf; f0 += 3 * NT) {
@@ -1565,8 +1632,10 @@ __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red,
   for (int bd = 0; (part < 0 || part == 2) && bd < c.nbody; bd++)
     for (int comp = 0; comp < 2; comp++) {
       double acc = 0.0;
-      if (bd0 == c.bmin + bd) acc += fv0[comp];
-      for (int p = tid + NT; p < c.nb; p += NT)
+#pragma unroll
+      for (int i = 0; i < kDiagPts; i++)
+        if (bdp[i] == c.bmin + bd) acc += fvp[i][comp];
+      for (int p = tid + kDiagPts * NT; p < c.nb; p += NT)
         if (a.bdyno[pbase + p] == c.bmin + bd) {
           const int f = comp * c.nb + p;
           acc += a.wsum[c.foff + f] * (a.fb[c.foff + f] / c.dt);
