@@ -779,7 +779,7 @@ struct Engine : GroupBase {
     upvec(csm, hcs, st);
     sm = Smear{face_id.p, face_ptr.p, face_pt.p, face_w.p, frc_ptr.p, frc_face.p, frc_w.p, frc_ia.p, frc_ib.p,
                vtx_id.p,   vtx_face.p, csm.p, hcs[0], B == 1};
-    // the padded force gather, when no force has more than kFrcK entries
+    // the padded force gather (slot-major), when no force has more than kFrcK entries
     const size_t nslots = (size_t)std::max(foff, 1) * kFrcK;
     std::vector<int> p9f(nslots, 0), p9a(nslots, 0), p9b(nslots, 0);
     std::vector<double> p9w(nslots, 0.0);
@@ -793,7 +793,7 @@ struct Engine : GroupBase {
           break;
         }
         for (int e = lo; e < hi; e++) {
-          const size_t slot = (size_t)(hcs[b].foff + f) * kFrcK + (e - lo);
+          const size_t slot = (size_t)(e - lo) * std::max(foff, 1) + hcs[b].foff + f;
           p9f[slot] = c.frc_face[e];
           p9w[slot] = c.frc_w[e];
           p9a[slot] = c.frc_ia[e];
@@ -810,6 +810,7 @@ struct Engine : GroupBase {
       sm.f9_w = f9_w.p;
       sm.f9_ia = f9_ia.p;
       sm.f9_ib = f9_ib.p;
+      sm.f9_n = std::max(foff, 1);
     }
   }
 

@@ -232,14 +232,16 @@ struct Smear {
   // of the step start their table loads without first loading it
   CaseSm c0;
   int single;
-  // the force gather padded to kFrcK entries per force, at (foff + f) *
-  // kFrcK + k (zero weight and index 0 in the unused slots), so a force's
-  // entries are loaded without first loading its range; nullptr when some
-  // force has more entries
+  // the force gather padded to kFrcK entries per force, slot-major: entry k
+  // of force foff + f at k * f9_n + foff + f (zero weight and index 0 in the
+  // unused slots), so a force's entries are loaded without first loading
+  // its range and a wave's loads of one slot are contiguous; nullptr when
+  // some force has more entries
   const int* f9_face;
   const double* f9_w;
   const int* f9_ia;
   const int* f9_ib;
+  int f9_n;
 };
 __device__ __forceinline__ CaseSm case_sm(const Smear& sm, int b) {
   if (sm.single) return sm.c0;
@@ -1032,15 +1034,15 @@ __device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, co
   // to curl_face with its get_bc terms.  Same values as curl_face.
   double s = 0.0;
   if (sm.f9_w != nullptr) {  // padded: the slots past a force's entries add +0
-    const int base = (c.foff + f) * kFrcK;
+    const int base = c.foff + f;
     int ia[kFrcK], ib[kFrcK];
     double ww[kFrcK];
     T va[kFrcK], vb[kFrcK];
 #pragma unroll
     for (int k = 0; k < kFrcK; k++) {
-      ia[k] = sm.f9_ia[base + k];
-      ib[k] = sm.f9_ib[base + k];
-      ww[k] = sm.f9_w[base + k];
+      ia[k] = sm.f9_ia[k * sm.f9_n + base];
+      ib[k] = sm.f9_ib[k * sm.f9_n + base];
+      ww[k] = sm.f9_w[k * sm.f9_n + base];
     }
 #pragma unroll
     for (int k = 0; k < kFrcK; k++) {
@@ -1050,7 +1052,7 @@ __device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, co
 #pragma unroll
     for (int k = 0; k < kFrcK; k++) {
       const double v =
-          ia[k] >= 0 ? double(va[k] - vb[k]) : double(curl_face<T, TC>(g, sm.f9_face[base + k], s1, sc));
+          ia[k] >= 0 ? double(va[k] - vb[k]) : double(curl_face<T, TC>(g, sm.f9_face[k * sm.f9_n + base], s1, sc));
       s = s + ww[k] * v;
     }
     if (sub) s = s - sub[c.foff + f];
@@ -1602,13 +1604,13 @@ __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red,
       if (f >= c.nf) return 0.0;
       double r;
       if (sm.f9_w != nullptr) {  // padded: the slots past a force's entries add +0
-        const int base = (c.foff + f) * kFrcK;
+        const int base = c.foff + f;
         int id[kFrcK];
         double ww[kFrcK], v[kFrcK];
 #pragma unroll
         for (int k = 0; k < kFrcK; k++) {
-          id[k] = sm.f9_face[base + k];
-          ww[k] = sm.f9_w[base + k];
+          id[k] = sm.f9_face[k * sm.f9_n + base];
+          ww[k] = sm.f9_w[k * sm.f9_n + base];
         }
 #pragma unroll
         for (int k = 0; k < kFrcK; k++) v[k] = qv(id[k]);
