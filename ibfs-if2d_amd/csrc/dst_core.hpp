@@ -328,9 +328,11 @@ __global__ __launch_bounds__(kThreads) void dst_col2(Op op, FftPlan<T> plan) {
 // compile-time constant, so the radix schedule, every index division and the
 // twiddle-table offsets fold into shifts and constants.  One FFT is served by
 // TPF = N2/4 threads (one radix-4 butterfly per thread per stage); a
-// workgroup holds P FFTs.  Twiddles come from a per-stage table laid out
-// [stage][k][r-1] so that a butterfly reads its R-1 factors contiguously.
-// The workgroup order is made XCD-aware: consecutive workgroups are
+// workgroup holds P FFTs.  Twiddles come from a per-stage table of
+// w = exp(-2 pi i k / (ns R)), one load per butterfly and stage, the other
+// powers formed in registers (TwLayout, tw_load).  Tiles of at most 32 KB
+// of LDS ping-pong between two buffers (one barrier per stage).  The
+// workgroup order is made XCD-aware: consecutive workgroups are
 // dispatched round-robin over the 8 XCDs, so group g is remapped to give
 // every XCD one contiguous range of sequences.  Column segments shared by
 // neighbouring column groups, and the neighbouring rows that the stencil
