@@ -103,6 +103,7 @@ struct FftPlan {
   int nst;                 // number of stages
   int rad[kMaxStages];     // radix of each stage
   const cplx<T>* tw;       // n2 twiddles exp(-2 pi i k / n2)
+  int nfft;                // FFTs per workgroup, set per launch (nfft * n2 <= EMAX)
 };
 
 template <typename T>
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(kThreads) void dst_pass(Op op, FftPlan<T> plan) {
   const int n2 = plan.n2;
   const int N = n2 / 2;
   const int len = N - 1;
-  const int P = EMAX / n2;          // FFTs per workgroup
+  const int P = plan.nfft;          // FFTs per workgroup
   const int b = blockIdx.y;
   int g = blockIdx.x;
   if constexpr (op_side<Op>::value) {
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(kThreads) void dst_col2(Op op, FftPlan<T> plan) {
   const int n2 = plan.n2;
   const int N = n2 / 2;
   const int len = N - 1;
-  const int P = EMAX / n2;
+  const int P = plan.nfft;
   const int b = blockIdx.y;
   const int g = blockIdx.x;
   for (int p = threadIdx.x; p < P; p += kThreads) {

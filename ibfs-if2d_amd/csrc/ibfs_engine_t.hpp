@@ -31,6 +31,7 @@ void make_plan(int n2, FftPlan<T>& plan, DevBuf<cplx<T>>& tw, hipStream_t s) {
     throw ArgError{1, "grid size not supported: 2*m and 2*n must factor into 2, 3, 5 and be <= " +
                           std::to_string(kEmax)};
   plan.n2 = n2;
+  plan.nfft = kEmax / n2;
   plan.nst = (int)rad.size();
   for (int i = 0; i < kMaxStages; i++) plan.rad[i] = i < plan.nst ? rad[i] : 1;
   std::vector<cplx<T>> t(n2);
@@ -318,13 +319,22 @@ struct Engine : GroupBase {
         default: break;
       }
     }
-    const FftPlan<V>& plan = AXIS == 0 ? F.pm : F.pn;
-    const int Pg = kEmax / n2;
-    int groups = (nseq + mult * Pg - 1) / (mult * Pg);
+    FftPlan<V> plan = AXIS == 0 ? F.pm : F.pn;
+    plan.nfft = generic_fanout(n2, nseq, mult, batch);
+    int groups = (nseq + mult * plan.nfft - 1) / (mult * plan.nfft);
     if constexpr (op_side<Op>::value)
       groups += op.side_groups_for(kThreads);
     hipLaunchKernelGGL((dst_pass<V, kEmax, AXIS, Op>), dim3(groups, batch), dim3(kThreads),
-                       kEmax * sizeof(cplx<V>), st, op, plan);
+                       plan.nfft * n2 * sizeof(cplx<V>), st, op, plan);
+  }
+
+  // FFTs per workgroup of a generic pass over nseq sequences: as many as
+  // fit in kEmax values while the launch still has a workgroup for every
+  // CU, else one.  (Filling kEmax regardless left a 127-row field to four
+  // workgroups: 16 us per pass instead of 6.)
+  static int generic_fanout(int n2, int nseq, int mult, int batch) {
+    const long per = (long)nseq * batch / (mult * 256L);
+    return (int)std::max(1L, std::min((long)(kEmax / n2), per));
   }
 
   // fused column pair (dst_col2) over nseq columns
@@ -352,10 +362,11 @@ struct Engine : GroupBase {
         default: break;
       }
     }
-    const int Pg = kEmax / n2;
-    const int groups = (nseq + mult * Pg - 1) / (mult * Pg);
-    hipLaunchKernelGGL((dst_col2<V, kEmax, Op>), dim3(groups, batch), dim3(kThreads), kEmax * sizeof(cplx<V>), st,
-                       op, F.pn);
+    FftPlan<V> plan = F.pn;
+    plan.nfft = generic_fanout(n2, nseq, mult, batch);
+    const int groups = (nseq + mult * plan.nfft - 1) / (mult * plan.nfft);
+    hipLaunchKernelGGL((dst_col2<V, kEmax, Op>), dim3(groups, batch), dim3(kThreads),
+                       plan.nfft * n2 * sizeof(cplx<V>), st, op, plan);
   }
 
   // ------------------------------------------------------------ autotuner

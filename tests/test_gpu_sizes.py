@@ -1,5 +1,6 @@
 """Parity at the production grid sizes (the size-specialised DST passes):
-transforms at every configured FFT length, and a few full steps of the
+transforms at every configured FFT length and at sizes outside that set
+(the generic passes), and a few full steps of the
 200^2 x 3-level cylinder (the CPU reference case) and of the 512^2 x 5-level
 airfoil (the headline case) against the oracle.  The oracle cannot afford
 its own preprocess at these sizes, so both sides use the device factor,
@@ -36,6 +37,26 @@ def test_transforms_at_production_sizes(m, rng):
     assert relerr(g.ainv(x), o.ainv(x)) < 1e-12
     if m <= 512:
         assert relerr(g.op_dst_gemm(x, 4), o.dst(x)) < 1e-13
+
+
+@pytest.mark.parametrize("m,n", [(1200, 16), (16, 1200), (600, 600)])
+def test_transforms_on_the_generic_passes(m, n, rng):
+    """Sizes with no size-specialised pass run the generic one, with one
+    FFT per workgroup where a pass has few sequences and several where it
+    has many: a transform longer than 2048 (2m = 2400), the 1199 short
+    columns of the 1200 x 16 grid (two FFTs per workgroup), and 599^2."""
+    from ibfs_amd import geometry, hip
+
+    len_ = 4.0
+    delta = len_ / m
+    xb, yb, bd = geometry.circle(delta, 4 * delta, 0.0, 0.0)
+    a = dict(m=m, n=n, mgridlev=2, dt=0.25 * delta, re=100.0, len_=len_, offx=len_ / 2, offy=len_ * n / m / 2,
+             xb=xb, yb=yb, bdyno=bd)
+    o, g = orc.Oracle(**a), hip.Solver(**a)
+    x = rng.standard_normal(o.nw)
+    assert relerr(g.dst(x), o.dst(x)) < 1e-13
+    assert relerr(g.ctci(x), o.ctci(x)) < 1e-12
+    assert relerr(g.ainv(x), o.ainv(x)) < 1e-12
 
 
 def run_pair(o, g, nsteps, spot_cols):
