@@ -8,7 +8,7 @@ namespace ibfs {
 // k < ns holds exp(-2 pi i k / (ns R)) (the kernels form the other powers).
 template <typename T>
 bool make_spec_twiddles(int n2, DevBuf<cplx<T>>& tw, hipStream_t st) {
-  static const int kSizes[] = {400, 512, 640, 768, 1024, 2048};
+  static const int kSizes[] = {256, 400, 512, 600, 640, 768, 1024, 1200, 2048};
   if (std::find(std::begin(kSizes), std::end(kSizes), n2) == std::end(kSizes)) return false;
   const spec::Sched sc = spec::make_sched(n2);
   std::vector<cplx<T>> t(std::max(sc.ntw, 1));
@@ -308,11 +308,14 @@ struct Engine : GroupBase {
     launch_spec<NN, PP, AXIS, Op>(op, stw, nseq, mult, batch); \
     return;
 #define IBFS_SPEC_SIZE(NN) IBFS_SPEC_CASE(NN, 1) IBFS_SPEC_CASE(NN, 2) IBFS_SPEC_CASE(NN, 4)
+        IBFS_SPEC_SIZE(256)
         IBFS_SPEC_SIZE(400)
         IBFS_SPEC_SIZE(512)
+        IBFS_SPEC_SIZE(600)
         IBFS_SPEC_SIZE(640)
         IBFS_SPEC_SIZE(768)
         IBFS_SPEC_SIZE(1024)
+        IBFS_SPEC_SIZE(1200)
         IBFS_SPEC_SIZE(2048)
 #undef IBFS_SPEC_SIZE
 #undef IBFS_SPEC_CASE
@@ -351,11 +354,14 @@ struct Engine : GroupBase {
     launch_spec_col2<NN, PP, Op>(op, nseq, mult, batch); \
     return;
 #define IBFS_SPEC_SIZE(NN) IBFS_SPEC_CASE(NN, 1) IBFS_SPEC_CASE(NN, 2) IBFS_SPEC_CASE(NN, 4)
+        IBFS_SPEC_SIZE(256)
         IBFS_SPEC_SIZE(400)
         IBFS_SPEC_SIZE(512)
+        IBFS_SPEC_SIZE(600)
         IBFS_SPEC_SIZE(640)
         IBFS_SPEC_SIZE(768)
         IBFS_SPEC_SIZE(1024)
+        IBFS_SPEC_SIZE(1200)
         IBFS_SPEC_SIZE(2048)
 #undef IBFS_SPEC_SIZE
 #undef IBFS_SPEC_CASE
@@ -388,7 +394,7 @@ struct Engine : GroupBase {
     for (int k = 0; k < kKinds; k++) {
       const bool rowtype = k == kRow || k == kStep1Row;
       const int n2 = rowtype ? fs.pm.n2 : fs.pn.n2;
-      if (!(rowtype ? fs.spec_m : fs.spec_n)) continue;  // generic pass: P fixed by its LDS budget
+      if (!(rowtype ? fs.spec_m : fs.spec_n)) continue;  // generic pass: P chosen per launch (generic_fanout)
       std::vector<int> cand;
       for (int P : {1, 2, 4})
         if (p_fits<T>(n2, P)) cand.push_back(P);

@@ -28,7 +28,7 @@ def solver_pair(m, L, len_, offx, offy, body, dt, re):
     return orc.Oracle(**a), hip.Solver(**a)
 
 
-@pytest.mark.parametrize("m", [200, 256, 320, 384, 512, 1024])
+@pytest.mark.parametrize("m", [128, 200, 256, 300, 320, 384, 512, 600, 1024])
 def test_transforms_at_production_sizes(m, rng):
     o, g = solver_pair(m, 3, 4.0, 2.0, 2.0, "circle", 0.004, 100.0)
     x = rng.standard_normal(o.nw)
@@ -39,12 +39,12 @@ def test_transforms_at_production_sizes(m, rng):
         assert relerr(g.op_dst_gemm(x, 4), o.dst(x)) < 1e-13
 
 
-@pytest.mark.parametrize("m,n", [(1200, 16), (16, 1200), (600, 600)])
+@pytest.mark.parametrize("m,n", [(1200, 16), (16, 1200), (480, 480)])
 def test_transforms_on_the_generic_passes(m, n, rng):
     """Sizes with no size-specialised pass run the generic one, with one
     FFT per workgroup where a pass has few sequences and several where it
     has many: a transform longer than 2048 (2m = 2400), the 1199 short
-    columns of the 1200 x 16 grid (two FFTs per workgroup), and 599^2."""
+    columns of the 1200 x 16 grid (two FFTs per workgroup), and 479^2."""
     from ibfs_amd import geometry, hip
 
     len_ = 4.0
@@ -85,6 +85,14 @@ def test_cylinder_200_three_levels():
     o, g = solver_pair(200, 3, 4.0, 2.0, 2.0, "circle", 0.004, 100.0)
     assert o.nb == 157 and g.nb == 157
     run_pair(o, g, 4, [0, 78, 313])
+
+
+@pytest.mark.parametrize("m", [128, 300])
+def test_cylinder_three_levels_other_sizes(m):
+    """Cylinder steps at the other specialised lengths (2m = 256: one wave
+    per FFT; 2m = 600: radices 4, 2, 3, 5, 5): 3 steps."""
+    o, g = solver_pair(m, 3, 4.0, 2.0, 2.0, "circle", 1.0 / m, 100.0)
+    run_pair(o, g, 3, [0, o.nf // 2, o.nf - 1])
 
 
 def test_airfoil_512_five_levels():
