@@ -244,7 +244,10 @@ struct Engine : GroupBase {
     tune();
   }
   void tile_shapes(int* out) const override {
-    for (int k = 0; k < kKinds; k++) out[k] = ptile[k];
+    for (int k = 0; k < kKinds; k++) {  // 0: the kind runs the generic pass
+      const bool rowtype = k == kRow || k == kStep1Row;
+      out[k] = (rowtype ? fs.spec_m : fs.spec_n) ? ptile[k] : 0;
+    }
   }
 
   // ------------------------------------------------------------------ passes

@@ -31,6 +31,7 @@ def solver_pair(m, L, len_, offx, offy, body, dt, re):
 @pytest.mark.parametrize("m", [128, 200, 256, 300, 320, 384, 512, 600, 1024])
 def test_transforms_at_production_sizes(m, rng):
     o, g = solver_pair(m, 3, 4.0, 2.0, 2.0, "circle", 0.004, 100.0)
+    assert min(g.tile_shapes().values()) >= 1  # every kind on a specialised pass
     x = rng.standard_normal(o.nw)
     assert relerr(g.dst(x), o.dst(x)) < 1e-13
     assert relerr(g.ctci(x), o.ctci(x)) < 1e-12
@@ -53,6 +54,7 @@ def test_transforms_on_the_generic_passes(m, n, rng):
     a = dict(m=m, n=n, mgridlev=2, dt=0.25 * delta, re=100.0, len_=len_, offx=len_ / 2, offy=len_ * n / m / 2,
              xb=xb, yb=yb, bdyno=bd)
     o, g = orc.Oracle(**a), hip.Solver(**a)
+    assert set(g.tile_shapes().values()) == {0}  # every kind on the generic pass
     x = rng.standard_normal(o.nw)
     assert relerr(g.dst(x), o.dst(x)) < 1e-13
     assert relerr(g.ctci(x), o.ctci(x)) < 1e-12
