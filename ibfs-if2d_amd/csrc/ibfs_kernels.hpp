@@ -1256,7 +1256,7 @@ __global__ void k_make_g(Smear sm, int ci, const double* __restrict__ Ai, double
 // gathers are independent; each wave holds kRowsPerWave rows and walks the
 // columns once, so the row loads of one column block are all in flight
 // together.
-// grid (ceil(max rows / kForceRows), B), 256 threads, LDS 2*nf_max doubles.
+// grid (ceil(max rows / kForceRows), B), kForceThreads (512) threads, LDS 2*nf_max doubles.
 constexpr int kForceThreads = 512;
 constexpr int kRowsPerWave = 2;
 constexpr int kColBlock = 10;

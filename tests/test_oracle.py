@@ -194,6 +194,30 @@ def test_coarsify_full_weighting(name, rng):
 
 
 @pytest.mark.parametrize("name", ["cyl32", "naca48x40"])
+def test_coarsify_weights_on_a_quadratic(name):
+    """Full weighting (src/operators.f90:326-330) of f = x^2 + 3y^2 - 2xy:
+    the weights (centre 1, edge neighbours 1/2, corners 1/4) give
+    sum w f(x_c + dx, y_c + dy) = 4 f(x_c, y_c) + (2 + 3*2) h^2 with h the fine
+    spacing (the edge pair contributes 2*(1/2) h^2 per axis, the four corners
+    4*(1/4) h^2, the corners' xy terms cancel).  Swapping 1/2 and 1/4 keeps the
+    sum of weights 4 and the zero first moments, so linear fields cannot tell,
+    but gives 4f + 10h^2 here.  Mutation tried once on the CPU: the two
+    weights swapped in orc_coarsify -> this test fails (relative error 4.2e-4 and
+    1.7e-4 on the two cases)."""
+    a = case_args(name)
+    o = orc.Oracle(**a)
+    o._off = (a["offx"], a["offy"])
+    X1, Y1 = level_xy(o, 1)
+    X2, Y2 = level_xy(o, 2)
+    f = lambda x, y: x**2 + 3 * y**2 - 2 * x * y
+    out = o.coarsify(np.zeros(o.nw), f(X1, Y1).ravel()).reshape(o.n - 1, o.m - 1)
+    ci = slice(o.m // 4, 3 * o.m // 4 - 1)
+    cj = slice(o.n // 4, 3 * o.n // 4 - 1)
+    h = o.delta
+    assert relerr(out[cj, ci], 4.0 * f(X2, Y2)[cj, ci] + 8.0 * h * h) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["cyl32", "naca48x40"])
 def test_get_bc_reproduces_linear_fields(name):
     a = case_args(name)
     o = orc.Oracle(**a)
@@ -255,6 +279,95 @@ def test_nonlinear_uniform_advection(name, rng):
     # q and q0 enter only as their sum
     r2 = o.rot(o.nonlinear(w, q - 0.125, q0 + 0.125, bcv))
     assert relerr(r2, r) < 1e-14
+
+
+def vertex_grid(o, offx, offy):
+    """Physical coordinates of every vertex of the finest level, edges
+    included, as [j, i] arrays (vertex (i, j) at ((i-1) delta - offx,
+    (j-1) delta - offy); u faces sit half a cell above a vertex, v faces half
+    a cell right of it: src/grid.f90:150-220)."""
+    xv = o.delta * np.arange(0, o.m + 1) - offx
+    yv = o.delta * np.arange(0, o.n + 1) - offy
+    return np.meshgrid(xv, yv)
+
+
+def fluxes_from_streamfunction(P):
+    """Face fluxes of a streamfunction given at every vertex ([j, i], edges
+    included): the flux through a u face is psi(top) - psi(bottom), through a
+    v face psi(left) - psi(right); exact for any psi."""
+    qu = P[1:, :] - P[:-1, :]
+    qv = P[:, :-1] - P[:, 1:]
+    return np.concatenate([qu.ravel(), qv.ravel()])
+
+
+def edge_vector(V, m, n):
+    """Reference edge vector (left, right, bottom, top; src/grid.f90:73-76)
+    of a vertex field given with its edges ([j, i])."""
+    return np.concatenate([V[:, 0], V[:, m], V[0, :], V[n, :]])
+
+
+@pytest.mark.parametrize("name", ["cyl32", "naca48x40"])
+def test_nonlinear_linear_shear(name, rng):
+    """A shear flux u(y) = a y + c on the u faces and v(x) = b x + d on the v
+    faces (divergence free).  The flux at a vertex is the linear function at
+    the vertex's own coordinates (the vertex lies midway between the faces
+    below and above it, left and right of it), so the rotational form is
+    central advection by it:
+        rot(nonlinear) = -u(y_j)/2 (w[i+1]-w[i-1]) - v(x_i)/2 (w[j+1]-w[j-1]).
+    Unlike a uniform flux this tells the averaging neighbours apart: an
+    average over the face above instead of below (src/operators.f90:477-486)
+    evaluates u half a cell off.  Mutation tried once on the CPU: uavg from
+    the faces j and j+1 in orc_nonlinear -> this test fails (relative error
+    0.49); vavg from the faces i and i+1 -> 0.014."""
+    a = case_args(name)
+    o = orc.Oracle(**a)
+    m, n, d = o.m, o.n, o.delta
+    XV, YV = vertex_grid(o, a["offx"], a["offy"])
+    u_lin = lambda y: 0.4 * y - 0.3
+    v_lin = lambda x: -0.7 * x + 0.2
+    q = np.zeros(o.nq)
+    yu = d * np.arange(0, n) - a["offy"] + 0.5 * d  # u-face rows
+    xv = d * np.arange(0, m) - a["offx"] + 0.5 * d  # v-face columns
+    q[: (m + 1) * n] = (np.ones((n, m + 1)) * u_lin(yu)[:, None]).ravel() * d
+    q[(m + 1) * n:] = (np.ones((n + 1, m)) * v_lin(xv)[None, :]).ravel() * d
+    W = rng.standard_normal((n + 1, m + 1))
+    r = o.rot(o.nonlinear(W[1:n, 1:m].ravel(), q, np.zeros(o.nq), edge_vector(W, m, n)))
+    X, Y = XV[1:n, 1:m], YV[1:n, 1:m]
+    ref = (-0.5 * u_lin(Y) * d * (W[1:n, 2:] - W[1:n, :m - 1])
+           - 0.5 * v_lin(X) * d * (W[2:, 1:m] - W[:n - 1, 1:m]))
+    assert relerr(r, ref.ravel()) < 1e-13
+
+
+def test_nonlinear_second_order_advection():
+    """Smooth flow: q from the streamfunction psi (face fluxes exact), vorticity
+    stored as circulation w(x, y) delta^2 with exact edge values.  Then
+    rot(nonlinear(omega, q)) / delta^4 -> -(u dw/dx + v dw/dy) with
+    u = dpsi/dy, v = -dpsi/dx, and the error falls by 4 per halving of delta
+    (second order: src/operators.f90:465-510 is centred).  An averaging index
+    shifted by one face breaks the second order.  Mutation tried once on the
+    CPU: vavg from the faces i and i+1 in orc_nonlinear -> error ratios 0.28
+    and 12.9, this test fails."""
+    psi = lambda x, y: np.sin(np.pi * x) * np.cos(0.5 * np.pi * y) + 0.3 * x * y
+    u_ = lambda x, y: -0.5 * np.pi * np.sin(np.pi * x) * np.sin(0.5 * np.pi * y) + 0.3 * x
+    v_ = lambda x, y: -(np.pi * np.cos(np.pi * x) * np.cos(0.5 * np.pi * y) + 0.3 * y)
+    g = lambda x, y: np.exp(-(x - 0.2) ** 2 - (y + 0.1) ** 2)
+    w = lambda x, y: g(x, y) * np.cos(x + 2 * y)
+    wx = lambda x, y: g(x, y) * (-2 * (x - 0.2) * np.cos(x + 2 * y) - np.sin(x + 2 * y))
+    wy = lambda x, y: g(x, y) * (-2 * (y + 0.1) * np.cos(x + 2 * y) - 2 * np.sin(x + 2 * y))
+    errs = []
+    for m in (32, 64, 128):
+        o = orc.Oracle(m, m, 1, 0.01, 100.0, 2.0, 1.0, 1.0, np.zeros(0), np.zeros(0))
+        d = o.delta
+        XV, YV = vertex_grid(o, 1.0, 1.0)
+        Wv = w(XV, YV) * d * d
+        q = fluxes_from_streamfunction(psi(XV, YV))
+        r = o.rot(o.nonlinear(Wv[1:m, 1:m].ravel(), q, np.zeros(o.nq), edge_vector(Wv, m, m)))
+        X, Y = XV[1:m, 1:m], YV[1:m, 1:m]
+        exact = -(u_(X, Y) * wx(X, Y) + v_(X, Y) * wy(X, Y))
+        errs.append(np.max(np.abs(r.reshape(m - 1, m - 1) / d**4 - exact)))
+    ratios = [errs[i] / errs[i + 1] for i in range(len(errs) - 1)]
+    assert all(3.7 < r < 4.3 for r in ratios), ratios
+    assert errs[-1] < 1e-2
 
 
 def roma(r):
@@ -344,33 +457,89 @@ def test_no_body_uniform_flow_stays_uniform():
     assert np.max(np.abs(o.omega)) == 0.0 and np.max(np.abs(o.q)) == 0.0
 
 
+def check_step1_stencil(o, w0, ro0, q, q0, w1, ro1, first):
+    """STEP 1 solves, on every level k (1-based, spacing D_k = delta 2^(k-1)),
+        (I + b D) w* = (I - b D) w + dt/D_k^2 (3/2 N - 1/2 N_old) + b (lastbc + newbc)
+    with b = dt/(2 Re D_k^2) and D = C^T C (the Taira-Colonius CN/AB2
+    scheme); at the first step N_old = N (explicit Euler).  The edge values of
+    level k come from level k+1: stored vorticity is circulation per cell,
+    w_k = D_k^2 w_phys, so an edge value of level k is
+    (D_k / D_{k+1})^2 = 1/4 times level k+1's interpolated value.  That factor
+    is formed here from the spacings (get_bc itself, with fac = 1, is pinned
+    on linear fields); the oracle's own constant is never read.  The coarsest
+    level has zero edge values.  Returns the worst relative residual."""
+    nw, nq, L = o.nw, o.nq, o.L
+    worst = 0.0
+    for k in range(L):
+        dk = o.delta * 2**k
+        b = 0.5 * o.dt / (o.re * dk**2)
+        if k + 1 < L:
+            scale = (dk / (2 * dk)) ** 2
+            last = scale * o.get_bc(w0[(k + 1) * nw:(k + 2) * nw], 1.0)
+            new = scale * o.get_bc(w1[(k + 1) * nw:(k + 2) * nw], 1.0)
+        else:
+            last = new = np.zeros(o.nbc)
+        sl = slice(k * nw, (k + 1) * nw)
+        N = o.rot(o.nonlinear(w0[sl], q[k * nq:(k + 1) * nq], q0[k * nq:(k + 1) * nq], last))
+        Nold = N if first else ro0[sl]
+        lhs = w1[sl] + b * lap5(w1[sl], o.m, o.n, split_bc(new, o.m, o.n))
+        rhs = (w0[sl] - b * lap5(w0[sl], o.m, o.n, split_bc(last, o.m, o.n))
+               + o.dt / dk**2 * (1.5 * N - 0.5 * Nold))
+        worst = max(worst, relerr(lhs, rhs))
+        assert relerr(ro1[sl], N) < 1e-15  # saved for the next step
+    return worst
+
+
 @pytest.mark.parametrize("name", list(CASES))
-def test_step1_is_cn_ab2(name, rng):
-    """STEP 1 solves (I + b D) w* = (I - b D) w + dt/D^2 (3/2 N - 1/2 N_old)
-    + b (lastbc + newbc) with b = dt/(2 Re D_k^2) and D = C^T C (the
-    Taira-Colonius CN/AB2 scheme); checked on the finest level."""
+@pytest.mark.parametrize("first", [False, True])
+def test_step1_is_cn_ab2(name, first, rng):
+    """The intermediate vorticity of STEP 1 (src/operators.f90:94-134)
+    satisfies the CN/AB2 stencil equation on every level (AB1 at the first
+    step).  Mutations tried once on the CPU, each fails this test: the 0.25
+    of get_bc in orc_step1 set to 0.5 (residual 1.0e-3, and 0.09 with the
+    lastbc 0.25 set to 0.5); the first-step copy rhs_old = rhs removed
+    (first=True: residual 0.10)."""
     o = make(name)
     o.omega[:] = 1e-2 * rng.standard_normal(o.omega.size)
     o.q[:] = 1e-2 * rng.standard_normal(o.q.size)
-    o.rhs_old[:] = 1e-3 * rng.standard_normal(o.rhs_old.size)
-    w0 = o.omega.copy()
-    ro0 = o.rhs_old.copy()
-    q = o.q.copy()
-    q0 = o.q0.copy()
-    o.step1(first=False)
-    w1 = o.omega.copy()
-    nw, nq = o.nw, o.nq
-    k = 0  # finest level
-    dk = o.delta * 2**k
-    b = 0.5 * o.dt / (o.re * dk**2)
-    last = o.get_bc(w0[nw:2 * nw], 0.25)
-    new = o.get_bc(w1[nw:2 * nw], 0.25)
-    N = o.rot(o.nonlinear(w0[:nw], q[:nq], q0[:nq], last))
-    lhs = w1[:nw] + b * lap5(w1[:nw], o.m, o.n, split_bc(new, o.m, o.n))
-    rhs = (w0[:nw] - b * lap5(w0[:nw], o.m, o.n, split_bc(last, o.m, o.n))
-           + o.dt / dk**2 * (1.5 * N - 0.5 * ro0[:nw]))
-    assert relerr(lhs, rhs) < 1e-11
-    assert relerr(o.rhs_old[:nw], N) < 1e-15
+    o.rhs_old[:] = 1e-3 * rng.standard_normal(o.rhs_old.size)  # overwritten at the first step
+    w0, ro0, q, q0 = o.omega.copy(), o.rhs_old.copy(), o.q.copy(), o.q0.copy()
+    o.step1(first=first)
+    assert check_step1_stencil(o, w0, ro0, q, q0, o.omega.copy(), o.rhs_old.copy(), first) < 1e-11
+
+
+def test_advance_takes_the_euler_start_at_itime_1(rng):
+    """advance honours itime == 1 (src/operators.f90:115-117): from a
+    non-zero state, the call that makes itime 1 ignores rhs_old (AB1) and is
+    bit-identical to a call at itime 2 whose rhs_old holds this step's own
+    nonlinear term N (AB2 with N_old = N is AB1); a call at itime 2 with
+    another rhs_old differs."""
+    a = case_args("cyl32")
+    ref = orc.Oracle(**a)
+    ref.advance()  # builds the factor
+    cm, cv = ref.cholmat.copy(), ref.cholvec.copy()
+    w = 1e-2 * rng.standard_normal(ref.omega.size)
+    garbage = 1e-3 * rng.standard_normal(ref.rhs_old.size)
+    probe = orc.Oracle(**a)
+    probe.omega[:] = w
+    probe.step1(first=True)
+    N = probe.rhs_old.copy()  # the nonlinear term of the initial state
+
+    def run(it0, ro):
+        o = orc.Oracle(**a)
+        o.set_cholesky(cm, cv)
+        o.omega[:] = w
+        o.rhs_old[:] = ro
+        o.itime = it0
+        f, c, s = o.advance()
+        assert o.itime == it0 + 1
+        return o, f
+
+    o1, f1 = run(0, garbage)
+    o2, f2 = run(1, N)
+    o3, f3 = run(1, garbage)
+    assert np.array_equal(o1.omega, o2.omega) and np.array_equal(o1.fb, o2.fb) and np.array_equal(f1, f2)
+    assert relerr(o3.omega, o1.omega) > 1e-8
 
 
 @pytest.fixture(scope="module")
