@@ -358,7 +358,7 @@ __device__ __forceinline__ T rot_nonlinear_tlm(const Geo& g, int i, int j, const
 
 // afb = wfx * cd + wfy * cl per point (J = mean over the window of
 // cd * sum of body x forces + cl * sum of y forces): write afb into out.
-__global__ void k_force_adj(Smear sm, const double* __restrict__ wsum, double cd, double cl, double scale,
+static __global__ void k_force_adj(Smear sm, const double* __restrict__ wsum, double cd, double cl, double scale,
                             double* __restrict__ afb) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const CaseSm c = sm.cs[blockIdx.y];
@@ -368,7 +368,7 @@ __global__ void k_force_adj(Smear sm, const double* __restrict__ wsum, double cd
 }
 
 // au[f] += delta * z[f]
-__global__ void k_axpy_case(Smear sm, const double* __restrict__ z, double a, double* __restrict__ y, int use_delta) {
+static __global__ void k_axpy_case(Smear sm, const double* __restrict__ z, double a, double* __restrict__ y, int use_delta) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const CaseSm c = sm.cs[blockIdx.y];
   if (f >= c.nf) return;

@@ -318,6 +318,18 @@ int ibfs_case_build_cholesky(ibfs_handle* h, int c, double* cholmat, double* cho
 int ibfs_build_cholesky(ibfs_handle* h, double* cholmat, double* cholvec) {
   return ibfs_case_build_cholesky(h, 0, cholmat, cholvec);
 }
+int ibfs_case_set_actuators(ibfs_handle* h, int c, int n_act, const int* npt, const double* par) {
+  return guarded(h, [&] {
+    if (c < 0 || c >= h->ncases) throw ArgError{IBFS_EINVAL, "case index out of range"};
+    if (n_act < 0 || (n_act > 0 && (!npt || !par))) throw ArgError{IBFS_EINVAL, "bad actuator arguments"};
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->drop_graph();  // the captured step has no actuation launch, or old tables
+    h->gr(c).set_actuators(h->where[c].second, n_act, npt, par);
+  });
+}
+int ibfs_set_actuators(ibfs_handle* h, int n_act, const int* npt, const double* par) {
+  return ibfs_case_set_actuators(h, 0, n_act, npt, par);
+}
 int ibfs_ensemble_build_cholesky(ibfs_handle* h) {
   return guarded(h, [&] {
     for (int c = 0; c < h->ncases; c++) h->build_factor(c);
@@ -338,8 +350,14 @@ void advance_impl(ibfs_handle* h, int* itime, int nsteps, const double* motion, 
                                 "first step"};
   if (nsteps == 0) return;
   hipStream_t s = h->stream;
-  h->ensure_diag(nsteps);
   long p = 0;
+  if (motion) {
+    long nft = 0;
+    for (int c = 0; c < h->ncases; c++) nft += h->ch(c).nf;
+    for (long f = 0; f < nft; f++)
+      if (!std::isfinite(motion[f])) throw ArgError{IBFS_EINVAL, "motion has a non-finite entry"};
+  }
+  h->ensure_diag(nsteps);
   for (int c = 0; c < h->ncases; c++) {
     h->gr(c).set_motion(h->where[c].second, motion ? motion + p : nullptr);
     p += h->ch(c).nf;
@@ -347,6 +365,7 @@ void advance_impl(ibfs_handle* h, int* itime, int nsteps, const double* motion, 
   for (auto& g : h->groups) {
     g->reset_counters();
     g->enqueue_motion();
+    g->set_itime(*itime);
   }
   HIPCHK(hipEventRecord(h->ev0, s));
   if (!h->stationary) h->cg_iters.assign(nsteps, 0);
@@ -416,6 +435,8 @@ void accumulate_objective(ibfs_handle* h, const double* hist, int nrows, int it_
 // the one group of a handle whose cases share a grid size
 GroupBase* single_group(ibfs_handle* h) {
   if (h->groups.size() != 1) throw ArgError{IBFS_EINVAL, "tangent/adjoint runs need all cases of one grid size"};
+  // the control u_B drives every point; actuator jets would override it
+  if (h->groups[0]->has_actuators()) throw ArgError{IBFS_EINVAL, "tangent/adjoint runs take no actuators"};
   return h->groups[0].get();
 }
 

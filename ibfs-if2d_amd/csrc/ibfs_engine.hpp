@@ -261,6 +261,10 @@ struct GroupBase {
   // new point positions of case c and the rebuilt smearing tables
   virtual void move_body(int c, const double* xb, const double* yb) = 0;
   virtual void enqueue_motion() = 0;
+  // actuator slots of case c and the step counter their jets are timed by
+  virtual void set_actuators(int c, int n_act, const int* npt, const double* par) = 0;
+  virtual bool has_actuators() const = 0;
+  virtual void set_itime(int itime) = 0;
   virtual void set_motion(int c, const double* motion) = 0;
   virtual void set_state(int c, const double* om, const double* ro, const double* q, const double* q0,
                          const double* fb) = 0;

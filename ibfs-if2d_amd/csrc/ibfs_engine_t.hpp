@@ -73,8 +73,10 @@ struct Engine : GroupBase {
   // FP32 transforms there, their rounding, of order eps*|psi|, reaches the
   // slip at the body, and the force that removes it carries per-step noise
   // of ~1e-4 of the force scale (mean forces are unaffected).  Step 4 solves
-  // the finest level in FP64 too, so that the stored fluxes there (the
-  // difference of nearby streamfunction values) keep FP32 relative accuracy.
+  // every level in FP64 too: the stored fluxes are differences of nearby
+  // streamfunction values, so an FP32 streamfunction would give them a
+  // relative error of about eps*|psi|/|dpsi| ~ eps*m/pi (1.6e-5 measured on
+  // level 2 at 512^2), against FP32 rounding of the flux itself in FP64.
   static constexpr bool kSolve64 = sizeof(T) == 4;
   FftSet<double> fs64;
   DevBuf<double> laminv64, S64, U1, U2;
@@ -104,7 +106,14 @@ struct Engine : GroupBase {
   std::vector<CaseSm> hcs;
   Smear sm{};
 
-  DevBuf<double> fb, motion_in, motion, rhsf, ytmp, rtmp, Linv, Asym;
+  DevBuf<double> fb, motion_in, motion, rq0, rhsf, ytmp, rtmp, Linv, Asym;
+  // actuator slots (src/grid.f90:579-718): the points of every case's
+  // actuators (act_ptr: the per-case ranges), and each case's step counter,
+  // which the step's k_actuation advances
+  std::vector<std::vector<ActPt>> hact;
+  DevBuf<ActPt> act_pts;
+  DevBuf<int> act_ptr, dit;
+  bool has_act = false;
   DevBuf<double> partial, diag;
   DevBuf<double> Ainv, Gm;  // A^-1 (Nf x Nf per case) and rot(reg(A^-1)) (nvtx x Nf per case)
   DevBuf<int> counter;
@@ -227,6 +236,8 @@ struct Engine : GroupBase {
     fb.alloc(nff); fb.zero(st);
     motion_in.alloc(nff); motion_in.zero(st);
     motion.alloc(nff); motion.zero(st);
+    rq0.alloc(nff); rq0.zero(st);
+    hact.assign(B, {});
     rhsf.alloc(nff);
     ytmp.alloc(nff);
     rtmp.alloc(nff);
@@ -562,26 +573,12 @@ struct Engine : GroupBase {
               geo.nm1, 2, batch);
   }
   // vort2flux streamfunctions of an FP32 engine with every level solved in
-  // FP64 into S64 (step 2: they feed the force system)
+  // FP64 into S64 (steps 2 and 4: the force system and the stored fluxes)
   void vort2flux64(T* om, int batch, const int* gate) {
     coarsify(om, batch, gate);
     for (int lev = Lmax - 1; lev >= 0; lev--)
       ctci64<double>(om, (lev + 1 < Lmax) ? S64.p + (long)(lev + 1) * nw : nullptr, batch, gate, lev);
   }
-  // vort2flux streamfunctions of an FP32 engine with the coarse levels in
-  // FP32 into S and the finest level in FP64 into S64 (step 4: the stored
-  // fluxes of the finest level come from the FP64 solve)
-  void vort2flux_fine64(T* om, int batch, const int* gate) {
-    const long bs = fbs();
-    coarsify(om, batch, gate);
-    for (int lev = Lmax - 1; lev >= 1; lev--) {
-      T* sc = (lev + 1 < Lmax) ? S.p + (long)(lev + 1) * nw : nullptr;
-      ctci_like(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, false, DstField<T>{S.p + (long)lev * nw, bs},
-                T2.p, T3.p, batch, gate, lev);
-    }
-    ctci64<T>(om, Lmax > 1 ? S.p + nw : nullptr, batch, gate, 0);
-  }
-
   // --------------------------------------------------------------- one step
   void enqueue_step(int first) override {
     enqueue_step1(first);
@@ -653,8 +650,10 @@ struct Engine : GroupBase {
   }
 
   // STEP 2 (src/operators.f90:142-171) with the factor; motion is prepared
-  // per call.  The force kernel also writes the step-3 source RR = rot(reg(fb)).
+  // per call (and per step at the actuator points).  The force kernel also
+  // writes the step-3 source RR = rot(reg(fb)).
   void enqueue_step2_cholesky() {
+    enqueue_actuation();
     const long FB = fbs();
     const size_t lds = sizeof(double) * 2 * std::max(nf_max, 1);
     const dim3 gall(std::max(1, (nf_max + nvtx_max + kForceRows - 1) / kForceRows), B);
@@ -689,7 +688,7 @@ struct Engine : GroupBase {
     // STEP 3 (src/operators.f90:174)
     ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, tbs());
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
-    if constexpr (kSolve64) vort2flux_fine64(omega.p, B, gate);
+    if constexpr (kSolve64) vort2flux64(omega.p, B, gate);
     else vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
     // fluxes and diagnostics (src/operators.f90:184-186)
     const int ncurl = (nq + 256 * kTailFaces - 1) / (256 * kTailFaces) * Lmax;
@@ -959,6 +958,7 @@ struct Engine : GroupBase {
       throw ArgError{IBFS_EINVAL, "the conjugate-gradient force solve needs the plain FP64 form"};
     cg_alloc();
     enqueue_step1(first);
+    enqueue_actuation();
     // STEP 2 (src/operators.f90:155-164): rhsf = regT(q_1) - motion
     vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
     hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf_max + 255) / 256, B), dim3(256), 0, st, geo, sm, S.p,
@@ -973,7 +973,54 @@ struct Engine : GroupBase {
 
   void enqueue_motion() override {
     hipLaunchKernelGGL(k_motion<T>, dim3((nf_max + 255) / 256, B), dim3(256), 0, st, sm, q0.p, qbs(), motion_in.p,
-                       motion.p);
+                       motion.p, rq0.p);
+  }
+
+  // The actuators of case c (src/grid.f90:579-633): n_act slots whose points
+  // are the last sum(npt) points of the case, in slot order (as actuation
+  // addresses them, nb - n_act_pt + count); par holds each slot's namelist
+  // read_act_parameters (x_j, y_j, h_j, u_j, u_j_osc, theta_j, omg_j).
+  void set_actuators(int c, int n_act, const int* npt, const double* par) override {
+    const CaseHost& h = cs[c];
+    int tot = 0;
+    for (int a = 0; a < n_act; a++) {
+      if (npt[a] < 1) throw ArgError{IBFS_EINVAL, "an actuator slot needs at least one point"};
+      tot += npt[a];
+    }
+    if (tot > h.d.nb) throw ArgError{IBFS_EINVAL, "actuator slots have more points than the case"};
+    std::vector<ActPt> pts;
+    int count = 0;
+    for (int a = 0; a < n_act; a++) {
+      const double* p = par + 7 * a;
+      for (int k = 0; k < 7; k++)
+        if (!std::isfinite(p[k])) throw ArgError{IBFS_EINVAL, "an actuator parameter is not finite"};
+      const double th = kPiRef / 180.0 * p[5];  // u_jet: pi/180.d0*theta_j
+      for (int k = 0; k < npt[a]; k++, count++)
+        pts.push_back(ActPt{h.d.nb - tot + count, p[3], p[4], p[6] * h.d.dt, std::cos(th), std::sin(th)});
+    }
+    hact[c] = std::move(pts);
+    std::vector<ActPt> all;
+    std::vector<int> ptr{0};
+    for (int b = 0; b < B; b++) {
+      all.insert(all.end(), hact[b].begin(), hact[b].end());
+      ptr.push_back((int)all.size());
+    }
+    has_act = !all.empty();
+    upvec(act_pts, all, st);
+    upvec(act_ptr, ptr, st);
+    if (!dit.p) dit.alloc(B);
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  bool has_actuators() const override { return has_act; }
+  // the step counter k_actuation reads (the itime of the step before the next)
+  void set_itime(int itime) override {
+    if (!has_act) return;
+    std::vector<int> v(B, itime);
+    dit.upload(v.data(), B, st);
+  }
+  void enqueue_actuation() {
+    if (has_act)
+      hipLaunchKernelGGL(k_actuation, dim3(B), dim3(256), 0, st, sm, act_ptr.p, act_pts.p, rq0.p, dit.p, motion.p);
   }
   void set_motion(int c, const double* mo) override {
     if (mo) motion_in.upload(mo, cs[c].nf, st, hcs[c].foff);

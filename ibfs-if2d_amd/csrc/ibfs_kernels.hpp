@@ -1143,10 +1143,12 @@ __global__ void k_regT_curl(Geo g, Smear sm, const T* __restrict__ s1, const T* 
   out[o] = regT_curl_at<T>(g, sm, c, f, s1 + b * bs, sc, sub);
 }
 
-// motion_eff = motion_in - regT(q0_1) (src/operators.f90:155), per case
+// motion_eff = motion_in - regT(q0_1) (src/operators.f90:155), per case;
+// regT(q0_1) is kept in rq0 for the actuator points, whose motion is
+// replaced every step (k_actuation)
 template <typename T>
 __global__ void k_motion(Smear sm, const T* __restrict__ q0, long qbs, const double* __restrict__ motion_in,
-                         double* __restrict__ motion) {
+                         double* __restrict__ motion, double* __restrict__ rq0) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   const CaseSm c = sm.cs[b];
@@ -1155,6 +1157,41 @@ __global__ void k_motion(Smear sm, const T* __restrict__ q0, long qbs, const dou
   const double s = gather_sum<kFrcK>(sm.frc_ptr[c.rbase + f], sm.frc_ptr[c.rbase + f + 1], sm.frc_face, sm.frc_w,
                                      [&](int face) { return double(q0b[face]); });
   motion[c.foff + f] = motion_in[c.foff + f] - s;
+  rq0[c.foff + f] = s;
+}
+
+// One point of an actuator slot (src/grid.f90:579-633, 636-718): at step it
+// the jet velocity is (u_j + u_j_osc sin(omg_j dt it)) (cos th, sin th) with
+// th = pi/180 theta_j.
+struct ActPt {
+  int f;             // the point's x force in the case (its y force is f + nb)
+  double uj, ujo;    // steady and oscillating amplitude
+  double omgdt;      // omg_j * dt
+  double cth, sth;   // cos th, sin th
+};
+
+// actuation (src/operators.f90:150-152 with src/grid.f90:579-633): the
+// motion of every actuator point for this step, u_jet * delta - regT(q0_1),
+// replacing what the caller's motion vector held there (as actuation
+// overwrites move_body's result).  One workgroup per case; itime[b] holds the
+// case's last step and is advanced here, once per step, so a captured step
+// replays with the right time.  The products are rounded one by one, in the
+// reference's order.
+__global__ void k_actuation(Smear sm, const int* __restrict__ aptr, const ActPt* __restrict__ pts,
+                            const double* __restrict__ rq0, int* __restrict__ itime,
+                            double* __restrict__ motion) {
+  const int b = blockIdx.x;
+  const CaseSm c = sm.cs[b];
+  const int it = itime[b] + 1;
+  __syncthreads();  // every thread has read itime[b] before it is advanced
+  for (int p = aptr[b] + threadIdx.x; p < aptr[b + 1]; p += blockDim.x) {
+    const ActPt a = pts[p];
+    const double u = __dadd_rn(a.uj, __dmul_rn(a.ujo, sin(__dmul_rn(a.omgdt, (double)it))));
+    const int fx = c.foff + a.f, fy = fx + c.nb;
+    motion[fx] = __dmul_rn(__dmul_rn(u, a.cth), c.delta) - rq0[fx];
+    motion[fy] = __dmul_rn(__dmul_rn(u, a.sth), c.delta) - rq0[fy];
+  }
+  if (threadIdx.x == 0) itime[b] = it;
 }
 
 // ---------------------------------------------------------------------------
@@ -1162,7 +1199,7 @@ __global__ void k_motion(Smear sm, const T* __restrict__ q0, long qbs, const dou
 // explicit inverse Li = L^{-1} of the Cholesky factor (column-major, lower):
 // y = Li r, then x = Li^T y.  One wave per output row, 4 rows per block.
 // ---------------------------------------------------------------------------
-__global__ void k_trmv_lower(Smear sm, const double* __restrict__ Li, const double* __restrict__ r,
+static __global__ void k_trmv_lower(Smear sm, const double* __restrict__ Li, const double* __restrict__ r,
                              double* __restrict__ y) {
   const CaseSm c = sm.cs[blockIdx.y];
   const int lane = threadIdx.x & 63;
@@ -1176,7 +1213,7 @@ __global__ void k_trmv_lower(Smear sm, const double* __restrict__ Li, const doub
   if (lane == 0) y[c.foff + i] = s;
 }
 // x[i] (+)= sum_{k>=i} Li(k,i) y[k]   (column i of Li, contiguous)
-__global__ void k_trmv_upper_t(Smear sm, const double* __restrict__ Li, const double* __restrict__ y,
+static __global__ void k_trmv_upper_t(Smear sm, const double* __restrict__ Li, const double* __restrict__ y,
                                double* __restrict__ x, int accumulate) {
   const CaseSm c = sm.cs[blockIdx.y];
   const int lane = threadIdx.x & 63;
@@ -1192,7 +1229,7 @@ __global__ void k_trmv_upper_t(Smear sm, const double* __restrict__ Li, const do
 // refinement residual r = b - A x with the symmetric A the factor was built
 // from (upper triangle of the ibfs.chd matrix, stored here as a full
 // matrix, so row i is read as the contiguous column i)
-__global__ void k_symv_residual(Smear sm, const double* __restrict__ A, const double* __restrict__ bvec,
+static __global__ void k_symv_residual(Smear sm, const double* __restrict__ A, const double* __restrict__ bvec,
                                 const double* __restrict__ x, double* __restrict__ r) {
   const CaseSm c = sm.cs[blockIdx.y];
   const int lane = threadIdx.x & 63;
@@ -1207,7 +1244,7 @@ __global__ void k_symv_residual(Smear sm, const double* __restrict__ A, const do
 
 // A^-1 = Li^T Li of case ci (column-major, both triangles written from one
 // product, so the stored matrix is exactly symmetric).  grid over nf^2
-__global__ void k_make_ainv(Smear sm, int ci, const double* __restrict__ Li, double* __restrict__ Ai) {
+static __global__ void k_make_ainv(Smear sm, int ci, const double* __restrict__ Li, double* __restrict__ Ai) {
   const CaseSm c = sm.cs[ci];
   const int nf = c.nf;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1225,7 +1262,7 @@ __global__ void k_make_ainv(Smear sm, int ci, const double* __restrict__ Li, dou
 // G = rot(reg(A^-1)) of case ci on its active vertices: G[v*nf + j] is the
 // step-3 source at vertex v for the unit force j (row-major, nvtx x nf).
 // grid over nvtx*nf
-__global__ void k_make_g(Smear sm, int ci, const double* __restrict__ Ai, double* __restrict__ G) {
+static __global__ void k_make_g(Smear sm, int ci, const double* __restrict__ Ai, double* __restrict__ G) {
   const CaseSm c = sm.cs[ci];
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)c.nvtx * c.nf) return;
@@ -1389,7 +1426,7 @@ __device__ double dot_in_order(int nf, const double* __restrict__ a, const doubl
 }
 
 // r = b - Ax; d = r; delta_new = r . d; iter = 0 (src/operators.f90:407-413)
-__global__ __launch_bounds__(256) void k_cg_start(int nf, const double* __restrict__ b,
+static __global__ __launch_bounds__(256) void k_cg_start(int nf, const double* __restrict__ b,
                                                   const double* __restrict__ Ax, double* __restrict__ r,
                                                   double* __restrict__ d, CgState* __restrict__ cs) {
 #pragma clang fp contract(off)
@@ -1409,7 +1446,7 @@ __global__ __launch_bounds__(256) void k_cg_start(int nf, const double* __restri
 
 // iter += 1; alpha = delta_new / (d . q); x = x + alpha d  (q = A d;
 // src/operators.f90:415-418)
-__global__ __launch_bounds__(256) void k_cg_alpha(int nf, const double* __restrict__ d,
+static __global__ __launch_bounds__(256) void k_cg_alpha(int nf, const double* __restrict__ d,
                                                   const double* __restrict__ q, double* __restrict__ x,
                                                   CgState* __restrict__ cs) {
 #pragma clang fp contract(off)
@@ -1429,7 +1466,7 @@ __global__ __launch_bounds__(256) void k_cg_alpha(int nf, const double* __restri
 // otherwise r = r - alpha q (q = A d); then delta_old = delta_new,
 // delta_new = r . r, beta = delta_new / delta_old, d = r + beta d
 // (src/operators.f90:420-431)
-__global__ __launch_bounds__(256) void k_cg_resid(int nf, const double* __restrict__ b,
+static __global__ __launch_bounds__(256) void k_cg_resid(int nf, const double* __restrict__ b,
                                                   const double* __restrict__ q, double* __restrict__ r,
                                                   double* __restrict__ d, CgState* __restrict__ cs, int recompute) {
 #pragma clang fp contract(off)
@@ -1478,8 +1515,8 @@ __device__ __forceinline__ bool curl_interior(const Geo& g, int f, int& ia, int&
 //  * workgroups x < ncurl = ceil(nq/(256*kTailFaces))*Lmax: q = curl(s) on
 //    level x / ceil(nq/(256*kTailFaces)) (src/operators.f90:236-238, as
 //    k_curl_levels), kTailFaces faces per thread whose interior loads are
-//    all issued before the first is used; with s1d set, the finest level's
-//    streamfunction is s1d (FP64).  These groups run in XCD-contiguous
+//    all issued before the first is used; with s1d set, the streamfunctions
+//    of every level are s1d (FP64).  These groups run in XCD-contiguous
 //    order, so the streamfunction rows shared by neighbouring face rows
 //    stay in one L2;
 //  * the next nchunk*Lmax: partial sums of omega for write_circ
@@ -1490,8 +1527,8 @@ __device__ __forceinline__ bool curl_interior(const Geo& g, int f, int& ia, int&
 // rows (diag_row).
 // grid (ncurl + nchunk*Lmax, B), 256 threads.
 constexpr int kTailFaces = 4;
-template <typename T, typename TS>
-__device__ __forceinline__ void tail_curl(const Geo& g, int f0, const TS* __restrict__ sl, const T* __restrict__ sc,
+template <typename T, typename TS, typename TC = T>
+__device__ __forceinline__ void tail_curl(const Geo& g, int f0, const TS* __restrict__ sl, const TC* __restrict__ sc,
                                           T* __restrict__ ql) {
   int ia[kTailFaces], ib[kTailFaces];
   bool in[kTailFaces];
@@ -1511,7 +1548,7 @@ __device__ __forceinline__ void tail_curl(const Geo& g, int f0, const TS* __rest
   for (int u = 0; u < kTailFaces; u++) {
     const int f = f0 + 256 * u;
     if (f >= g.nq) break;
-    ql[f] = in[u] ? T(va[u] - vb[u]) : T(curl_face<TS, T>(g, f, sl, sc));
+    ql[f] = in[u] ? T(va[u] - vb[u]) : T(curl_face<TS, TC>(g, f, sl, sc));
   }
 }
 
@@ -1532,12 +1569,14 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
     const int x = xcd_contiguous(blockIdx.x, ncurl);
     const int lev = x / ncb, f0 = (x - lev * ncb) * 256 * kTailFaces + tid;
     if (!active(Lc, b, lev)) return;
-    const T* sl = s + b * bs + (long)lev * g.nw;
-    const T* sc = has_coarser(g, Lc, b, lev) ? sl + g.nw : nullptr;
-    if (lev == 0 && s1d != nullptr)  // finest level solved in FP64
-      tail_curl<T, double>(g, f0, s1d + b * bs, sc, q + b * qbs);
-    else
-      tail_curl<T, T>(g, f0, sl, sc, q + b * qbs + (long)lev * g.nq);
+    const bool hc = has_coarser(g, Lc, b, lev);
+    if (s1d != nullptr) {  // every level solved in FP64: the differences in FP64, rounded once
+      const double* sl = s1d + b * bs + (long)lev * g.nw;
+      tail_curl<T, double, double>(g, f0, sl, hc ? sl + g.nw : nullptr, q + b * qbs + (long)lev * g.nq);
+    } else {
+      const T* sl = s + b * bs + (long)lev * g.nw;
+      tail_curl<T, T>(g, f0, sl, hc ? sl + g.nw : nullptr, q + b * qbs + (long)lev * g.nq);
+    }
     return;
   }
   const int xc = blockIdx.x - ncurl, lev = xc / nchunk, ch = xc - lev * nchunk;

@@ -14,6 +14,13 @@ MODULE grid
   INTEGER, DIMENSION(:), ALLOCATABLE :: bdyno
   INTEGER :: nq, nb, nf, ns, nw
 
+  ! actuator slots: points per slot and each slot's read_act_parameters
+  ! (x_j, y_j, h_j, u_j, u_j_osc, theta_j, omg_j), handed to the library,
+  ! which evaluates the jets on the device every step
+  INTEGER, PARAMETER :: max_act = 100
+  INTEGER, DIMENSION(max_act) :: act_npt = 0
+  REAL(KIND(0.D0)), DIMENSION(7, max_act) :: act_par = 0.d0
+
   INTEGER, PARAMETER :: maxpts = 100000
   TYPE bodylist
      INTEGER :: ntot = 0
@@ -64,7 +71,12 @@ CONTAINS
     !-------------------------------------------------------------------------!
     ! input/geom.inp (Nb, then "x y bodyno" lines, src/grid.f90:270-276), or
     ! the reference's default body: a circle of diameter 1 at the origin
-    ! (src/grid.f90:353), written back to input/geom.inp.
+    ! (src/grid.f90:353), written back to input/geom.inp.  Actuators
+    ! (input/actuator.NNN.inp, NNN = 001..100) are handled as in
+    ! src/grid.f90:286-316,383-392: a geom.inp with negative body numbers
+    ! already holds their slots (their point counts are read back from
+    ! input/slot.NNN.inp); otherwise each actuator's slot is appended with
+    ! body number -NNN and geom.inp is rewritten.
     !-------------------------------------------------------------------------!
     TYPE(bodylist) :: geom
     REAL(KIND(0.D0)) :: ds
@@ -82,18 +94,153 @@ CONTAINS
        CLOSE(8)
        WRITE(111,*) "Reading geometry from geom.inp"
        WRITE(*,*) "Reading geometry from geom.inp"
+       IF (ANY(geom%bodyno(1:geom%ntot) < 0)) THEN
+          CALL count_actuators
+       ELSE
+          CALL append_actuators( ds, geom )
+          CALL write_geometry( geom, 'replace' )
+          WRITE(111,*) "...appended actuator info to geom.inp"
+       END IF
     ELSE
        CALL circle( ds, 1.d0, 0.0d0, 0.d0, geom, 0)
-       OPEN(unit=8,file='input/geom.inp',form='formatted',status='new')
-       WRITE(8,*) geom%ntot
-       DO i=1,geom%ntot
-          WRITE(8,*) geom%xbody(i), geom%ybody(i), geom%bodyno(i)
-       END DO
-       CLOSE(8)
+       CALL append_actuators( ds, geom )
+       CALL write_geometry( geom, 'new' )
        WRITE(111,*) "Generated geometry from primatives."
        WRITE(*,*) "Generated geometry from primatives."
     END IF
   END SUBROUTINE setup_geometry
+
+  SUBROUTINE write_geometry( geom, stat )
+    TYPE(bodylist) :: geom
+    CHARACTER(*) :: stat
+    INTEGER :: i
+    OPEN(unit=8,file='input/geom.inp',form='formatted',status=stat)
+    WRITE(8,*) geom%ntot
+    DO i=1,geom%ntot
+       WRITE(8,*) geom%xbody(i), geom%ybody(i), geom%bodyno(i)
+    END DO
+    CLOSE(8)
+  END SUBROUTINE write_geometry
+
+  SUBROUTINE read_actuator( i_act, par )
+    ! the namelist of input/actuator.NNN.inp (src/grid.f90:262-263,309-311)
+    INTEGER :: i_act
+    REAL(KIND(0.D0)), DIMENSION(7) :: par
+    CHARACTER(3) :: act_num
+    REAL(KIND(0.D0)) :: x_j, y_j, h_j, u_j, u_j_osc, theta_j, omg_j
+    NAMELIST /read_act_parameters/ x_j,y_j,h_j,u_j,u_j_osc,theta_j,omg_j
+
+    x_j = 0.d0; y_j = 0.d0; h_j = 0.d0; u_j = 0.d0; u_j_osc = 0.d0; theta_j = 0.d0; omg_j = 0.d0
+    WRITE(act_num,"(I3.3)") i_act
+    OPEN(unit=7,file="input/actuator."//act_num//".inp",form='formatted',status='old')
+    READ(unit=7,nml=read_act_parameters)
+    CLOSE(7)
+    par = (/ x_j, y_j, h_j, u_j, u_j_osc, theta_j, omg_j /)
+  END SUBROUTINE read_actuator
+
+  SUBROUTINE append_actuators( ds, geom )
+    ! every input/actuator.NNN.inp adds its slot with body number -NNN
+    ! (src/grid.f90:303-316 and 383-392)
+    TYPE(bodylist) :: geom
+    REAL(KIND(0.D0)) :: ds
+    INTEGER :: i_act, ntot0
+    CHARACTER(3) :: act_num
+    LOGICAL :: readact
+
+    DO i_act = 1, max_act
+       WRITE(act_num,"(I3.3)") i_act
+       INQUIRE(file="input/actuator."//act_num//".inp",exist=readact)
+       IF (readact) THEN
+          n_act = n_act + 1
+          IF (n_act /= i_act) STOP 'actuator files must be numbered 001, 002, ... without gaps'
+          CALL read_actuator( i_act, act_par(:,n_act) )
+          ntot0 = geom%ntot
+          CALL actuator_slot( act_par(1,n_act), act_par(2,n_act), act_par(3,n_act), ds, &
+                              act_par(6,n_act), geom, -i_act )
+          act_npt(n_act) = geom%ntot - ntot0
+       END IF
+    END DO
+  END SUBROUTINE append_actuators
+
+  SUBROUTINE count_actuators
+    ! slots already in geom.inp: their point counts from input/slot.NNN.inp
+    ! (src/grid.f90:286-300)
+    INTEGER :: i_act, n_pt
+    CHARACTER(3) :: act_num
+    LOGICAL :: readact
+
+    DO i_act = 1, max_act
+       WRITE(act_num,"(I3.3)") i_act
+       INQUIRE(file="input/actuator."//act_num//".inp",exist=readact)
+       IF (readact) THEN
+          OPEN(unit=9,file="input/slot."//act_num//".inp",form='formatted',status='old')
+          READ(9,*) n_pt
+          CLOSE(9)
+          n_act = n_act + 1
+          IF (n_act /= i_act) STOP 'actuator files must be numbered 001, 002, ... without gaps'
+          n_act_pt = n_act_pt + n_pt
+          act_npt(n_act) = n_pt
+          CALL read_actuator( i_act, act_par(:,n_act) )
+       END IF
+    END DO
+  END SUBROUTINE count_actuators
+
+  SUBROUTINE actuator_slot( x, y, hs, ds, angle, geom, num )
+    !-------------------------------------------------------------------------!
+    ! src/grid.f90:636-718: INT(hs/ds)+1 points on the segment of width hs
+    ! through (x, y), normal to the jet direction angle (degrees), or the
+    ! single point (x, y) when hs < ds; body number num.  Logs the slot in
+    ! input/slot.dat and its point count in input/slot.NNN.inp.
+    !-------------------------------------------------------------------------!
+    TYPE(bodylist) :: geom
+    REAL(KIND(0.D0)) :: x, y, hs, ds, angle, seg
+    INTEGER :: num, npt, i
+    CHARACTER(3) :: act_num
+    LOGICAL :: ex
+
+    npt = INT(hs/ds)
+    IF (geom%ntot + npt + 1 > maxpts) STOP 'too many body points'
+    IF (npt == 0) THEN
+       geom%xbody(geom%ntot+1) = x
+       geom%ybody(geom%ntot+1) = y
+       geom%bodyno(geom%ntot+1) = num
+       geom%ntot = geom%ntot + 1
+    ELSE
+       DO i = 1, npt+1
+          seg = REAL(i-1)*hs/REAL(npt)
+          geom%xbody(geom%ntot+i) = x - hs/2.0d0*COS(pi*(angle+90.)/180.) + seg*COS(pi*(angle+90.)/180.)
+          geom%ybody(geom%ntot+i) = y - hs/2.0d0*SIN(pi*(angle+90.)/180.) + seg*SIN(pi*(angle+90.)/180.)
+          geom%bodyno(geom%ntot+i) = num
+       END DO
+       geom%ntot = geom%ntot + npt + 1
+    END IF
+
+    WRITE(act_num,"(I3.3)") -num
+    IF (num == -1) THEN
+       OPEN(unit=116,file="input/slot.dat",form="formatted",status="replace")
+       OPEN(unit=118,file="input/slot."//act_num//".inp",form="formatted",status="replace")
+    ELSE
+       INQUIRE(file="input/slot.dat",exist=ex)
+       IF (ex) THEN
+          OPEN(unit=116,file="input/slot.dat",form="formatted",status="old",position="append")
+       ELSE
+          OPEN(unit=116,file="input/slot.dat",form="formatted",status="new")
+       END IF
+       OPEN(unit=118,file="input/slot."//act_num//".inp",form="formatted",status="replace")
+    END IF
+    WRITE(116,*) " ----------- ACTUATOR No. ",-num
+    WRITE(116,*) "   o Desired slot width               = ",hs
+    IF (npt == 0) THEN
+       WRITE(116,*) "   o Actual slot width in simulation  = ",ds
+    ELSE
+       WRITE(116,*) "   o Actual slot width in simulation  = ",hs
+    END IF
+    WRITE(116,*) "   o Number of pts used to model slot = ",(npt+1)
+    CLOSE(116)
+    WRITE(118,*) (npt+1)
+    CLOSE(118)
+    n_act_pt = n_act_pt + (npt + 1)
+  END SUBROUTINE actuator_slot
 
   SUBROUTINE circle( ds, dia, x, y, geom, num )
     ! npt = INT(pi*dia/ds) points on the circle, first at angle 0

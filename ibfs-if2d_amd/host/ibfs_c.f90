@@ -71,6 +71,14 @@ MODULE ibfs_c
        REAL(C_DOUBLE), DIMENSION(*) :: xb, yb
      END FUNCTION ibfs_move_body
 
+     INTEGER(C_INT) FUNCTION ibfs_set_actuators(h, n_act, npt, par) BIND(C, name='ibfs_set_actuators')
+       IMPORT :: C_INT, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: h
+       INTEGER(C_INT), VALUE :: n_act
+       INTEGER(C_INT), DIMENSION(*) :: npt
+       REAL(C_DOUBLE), DIMENSION(*) :: par
+     END FUNCTION ibfs_set_actuators
+
      INTEGER(C_INT) FUNCTION ibfs_cg_iterations(h, nsteps, iters) BIND(C, name='ibfs_cg_iterations')
        IMPORT :: C_INT, C_PTR
        TYPE(C_PTR), VALUE :: h

@@ -45,6 +45,8 @@ CONTAINS
     st = 0
     IF (stationary) st = 1
     CALL ibfs_check(ibfs_set_solver(handle, st, cgtol, cg_max_iter), 'ibfs_set_solver')
+    ! the jets of the actuator slots (actuation, src/operators.f90:150-152)
+    IF (n_act > 0) CALL ibfs_check(ibfs_set_actuators(handle, n_act, act_npt, act_par), 'ibfs_set_actuators')
     nbody = dims(9); bmin = dims(10); bmax = dims(11)
 
     ALLOCATE( omega(2:m,2:n,mgridlev), fb(MAX(nf,1)), rhs_old(2:m,2:n,mgridlev) )
@@ -165,7 +167,14 @@ CONTAINS
   SUBROUTINE read_variables(it)
     CHARACTER(7) :: charit
     INTEGER :: it
+    LOGICAL :: ex
     WRITE(charit,"(I7.7)") it
+    INQUIRE(file="output/ibfs"//charit//".var", exist=ex)
+    IF (.NOT. ex) THEN
+       WRITE(*,*) "Restart file output/ibfs"//charit//".var not found (istart =", it, ")"
+       WRITE(111,*) "Restart file output/ibfs"//charit//".var not found (istart =", it, ")"
+       STOP 'restart file not found'
+    END IF
     OPEN(unit=100,file="output/ibfs"//charit//".var",form="unformatted",status="unknown")
     READ(100) omega, fb, rhs_old, q, q0
     CLOSE(100)

@@ -64,6 +64,24 @@ def naca4(ds: float, t: float = 0.12, chord: float = 1.0, aoa_deg: float = 0.0, 
     return rx + x_le, ry + y_le, np.full(npt, num, dtype=np.int32)
 
 
+def actuator_slot(x: float, y: float, hs: float, ds: float, angle: float, num: int):
+    """The points of one actuator slot (src/grid.f90:636-718): a segment of
+    width hs centred on (x, y), normal to the jet direction angle (degrees),
+    as npt + 1 points with npt = INT(hs/ds) (one point at (x, y) when hs <
+    ds), all with body number num (negative: -1 for actuator 001, ...).
+    Same expressions and order as the reference: the segment direction is
+    pi*(angle + 90)/180 and point i sits at x - hs/2 cos + (i-1) hs/npt cos."""
+    npt = int(hs / ds)
+    if npt == 0:
+        return np.array([x]), np.array([y]), np.array([num], dtype=np.int32)
+    a = PI_REF * (angle + 90.0) / 180.0
+    ca, sa = math.cos(a), math.sin(a)
+    seg = [float(i - 1) * hs / float(npt) for i in range(1, npt + 2)]
+    xb = np.array([x - hs / 2.0 * ca + sg * ca for sg in seg])
+    yb = np.array([y - hs / 2.0 * sa + sg * sa for sg in seg])
+    return xb, yb, np.full(npt + 1, num, dtype=np.int32)
+
+
 def write_geom(path: str, xb, yb, bdyno) -> None:
     """geom.inp: Nb, then 'x y bodyno' per line (src/grid.f90:272-275)."""
     with open(path, "w") as f:

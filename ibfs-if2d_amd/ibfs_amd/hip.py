@@ -60,6 +60,8 @@ def _load():
     L.ibfs_set_solver.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
     L.ibfs_cg_iterations.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
     L.ibfs_move_body.argtypes = [C.c_void_p, _D, _D]
+    L.ibfs_set_actuators.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), _D]
+    L.ibfs_case_set_actuators.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), _D]
     L.ibfs_op_dst_gemm.argtypes = [C.c_void_p, C.c_int, _D, _D]
     L.ibfs_time_copy.argtypes = [C.c_void_p, C.c_long, C.c_int, _D]
     L.ibfs_tile_shapes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -202,6 +204,19 @@ class Solver:
         """New body point positions (the setup_reg after move_body,
         src/operators.f90:142-144); the factor of the old geometry is dropped."""
         self._chk(_lib.ibfs_move_body(self._h, _p(_f64(xb)), _p(_f64(yb))), "ibfs_move_body")
+
+    def set_actuators(self, npt, params):
+        """Actuator slots (actuation/u_jet, src/grid.f90:579-633): npt[a]
+        points for slot a, the last sum(npt) points of the case in slot
+        order; params[a] = (x_j, y_j, h_j, u_j, u_j_osc, theta_j, omg_j), the
+        slot's read_act_parameters namelist.  The jets are evaluated on the
+        device every step and replace the motion argument at those points."""
+        npt = np.ascontiguousarray(npt, dtype=np.int32).ravel()
+        par = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        if par.size != 7 * npt.size:
+            raise ValueError("params needs 7 values per actuator")
+        self._chk(_lib.ibfs_set_actuators(self._h, int(npt.size), npt.ctypes.data_as(C.POINTER(C.c_int)),
+                                          _p(par) if par.size else None), "ibfs_set_actuators")
 
     def cg_iterations(self, nsteps):
         """CG iterations of each step of the last advance call."""

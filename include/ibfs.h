@@ -85,7 +85,8 @@ int ibfs_build_cholesky(ibfs_handle* h, double* cholmat, double* cholvec);
  * *itime == 1 the explicit Euler start is taken (lines 115-117).  motion
  * (Nf, may be NULL = 0) is what move_body/actuation would return, already
  * scaled by delta (src/grid.f90:573-574); the library subtracts regT(q0)
- * (line 155).  Outputs (each may be NULL):
+ * (line 155).  A non-finite motion entry is rejected (IBFS_EINVAL) before
+ * any step runs.  Outputs (each may be NULL):
  *   fb    : Nf forces, the reference's fb
  *   force : [fx(bmin..bmax), fy(bmin..bmax)] as written by write_force (line 184)
  *   circ  : L circulations as written by write_circ (line 185)
@@ -112,6 +113,24 @@ int ibfs_set_solver(ibfs_handle* h, int stationary, double cgtol, int cg_max_ite
  * dropped, so a moving body is stepped with ibfs_set_solver(h, 0, ...) as in
  * the reference (stationary = F), or with a factor built anew. */
 int ibfs_move_body(ibfs_handle* h, const double* xb, const double* yb);
+
+/* Actuator slots: replaces actuation and u_jet (src/grid.f90:579-633),
+ * which advance calls every step when actuators exist
+ * (src/operators.f90:150-152).  The n_act slots own the last
+ * npt[0] + ... + npt[n_act-1] points of the case's point list, in slot order
+ * (actuator_slot appends them with bdyno = -1, -2, ..., src/grid.f90:636-718;
+ * actuation addresses them as nb - n_act_pt + count).  par holds 7 values per
+ * slot, its namelist read_act_parameters in order: x_j, y_j, h_j, u_j,
+ * u_j_osc, theta_j (degrees), omg_j.  Every step, on the device, each point
+ * of slot a gets the motion
+ *   (u_j + u_j_osc sin(omg_j dt itime)) (cos, sin)(pi/180 theta_j) * delta
+ * in place of what the motion argument of ibfs_advance holds there.  The
+ * reference re-reads the namelist files every step; the host reads them once
+ * and passes them here.  n_act = 0 removes the actuators.  Handles with
+ * actuators take no tangent or adjoint runs (their control drives every
+ * point). */
+int ibfs_set_actuators(ibfs_handle* h, int n_act, const int* npt, const double* par);
+int ibfs_case_set_actuators(ibfs_handle* h, int c, int n_act, const int* npt, const double* par);
 
 /* The CG iteration counts of the steps of the last ibfs_advance / ibfs_step
  * call (what cjgr writes to ibfs.inf, src/operators.f90:439); 0 for steps

@@ -109,6 +109,30 @@ def fft(x: np.ndarray) -> np.ndarray:
     return y.view(np.complex128)
 
 
+def actuation(itime, nb, delta, dt, npt, params, motion=None):
+    """actuation and u_jet (src/grid.f90:579-633), the motion the reference
+    hands to the step when actuators exist (src/operators.f90:150-152): a
+    copy of motion (zeros if None, length 2 nb) with, for every point of slot
+    a (the last sum(npt) points, in slot order), x and y entries
+      (u_j + u_j_osc sin(omg_j dt itime)) (cos, sin)(pi/180 theta_j) * delta,
+    params[a] = (x_j, y_j, h_j, u_j, u_j_osc, theta_j, omg_j)."""
+    import math
+
+    pi = 3.1415926535897932384  # src/parameters.f90:35
+    out = np.zeros(2 * nb) if motion is None else np.array(motion, dtype=np.float64)
+    n_act_pt = int(np.sum(npt))
+    count = 0
+    for a, n_pt in enumerate(npt):
+        _, _, _, u_j, u_j_osc, theta_j, omg_j = (float(v) for v in params[a])
+        for _ in range(int(n_pt)):
+            count += 1
+            i = nb - n_act_pt + count  # 1-based point index
+            base = u_j + u_j_osc * math.sin(omg_j * dt * float(itime))
+            out[i - 1] = base * math.cos(pi / 180.0 * theta_j) * delta
+            out[nb + i - 1] = base * math.sin(pi / 180.0 * theta_j) * delta
+    return out
+
+
 class Oracle:
     """One reference-layout simulation state plus its operators."""
 

@@ -1,0 +1,121 @@
+"""The 64-case uneven ensemble (cases.ensemble_64, seed 0), in the mixed
+form, on an eight-case slice that covers every grid size (256, 320, 384,
+512), every level count (3, 4, 5), both bodies (cylinders and NACA0012
+airfoils at their drawn angles) and eight different Reynolds numbers:
+
+* each case of the mixed ensemble equals its own single-case mixed run bit
+  for bit (same kernels, same arithmetic, batched launches);
+* the mixed ensemble stays within 1e-5 of the FP64 ensemble on every case's
+  force history (relative to the case's force scale) over 100 steps, and on
+  the vorticity and fluxes of every level;
+* for each grid size, the FP64 single-case run matches the oracle for 2
+  steps with the device factor (the oracle cannot afford its own preprocess
+  at these sizes), so the FP64 yardstick is itself pinned."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+SLICE = [0, 1, 2, 3, 4, 5, 8, 31]
+
+
+def slice_cases():
+    from ibfs_amd import cases
+
+    allc = cases.ensemble_64(seed=0)
+    sl = [allc[i] for i in SLICE]
+    assert {c["m"] for c in sl} == {256, 320, 384, 512}
+    assert {c["mgridlev"] for c in sl} == {3, 4, 5}
+    assert len({round(c["re"], 6) for c in sl}) == len(sl)
+    return sl
+
+
+@pytest.fixture(scope="module")
+def factors():
+    from ibfs_amd import hip
+
+    sl = slice_cases()
+    ens = hip.Ensemble(sl)
+    out = [ens.build_case_cholesky(i) for i in range(len(sl))]
+    ens.close()
+    return out
+
+
+def test_mixed_ensemble_equals_single_case_runs(factors):
+    from ibfs_amd import hip
+
+    sl = slice_cases()
+    ens = hip.Ensemble(sl, precision=hip.MIXED)
+    for i, (cm, cv) in enumerate(factors):
+        ens.set_case_cholesky(i, cm, cv)
+    hist = ens.advance(12)
+    for i, c in enumerate(sl):
+        s = hip.Solver(**c, precision=hip.MIXED)
+        s.set_cholesky(*factors[i])
+        sh = s.advance(12)
+        assert np.array_equal(ens.case_rows(hist, i), sh), f"case {SLICE[i]} history"
+        es, ss = ens.get_state(i), s.get_state()
+        for key in ("omega", "q", "rhs_old", "fb"):
+            assert np.array_equal(es[key], ss[key]), f"case {SLICE[i]} {key}"
+        s.close()
+
+
+def test_mixed_ensemble_within_1e5_of_fp64(factors):
+    from ibfs_amd import hip
+
+    sl = slice_cases()
+    e64 = hip.Ensemble(sl)
+    e32 = hip.Ensemble(sl, precision=hip.MIXED)
+    for i, (cm, cv) in enumerate(factors):
+        e64.set_case_cholesky(i, cm, cv)
+        e32.set_case_cholesky(i, cm, cv)
+    h64 = e64.advance(100)
+    h32 = e32.advance(100)
+    worst = {}
+    for i, c in enumerate(sl):
+        r64, r32 = e64.case_rows(h64, i), e32.case_rows(h32, i)
+        s64, s32 = e64.get_state(i), e32.get_state(i)
+        nbody = e64.dims[i]["nbody"]
+        fscale = np.max(np.abs(r64[:, : 2 * nbody]))
+        errs = {"force": np.max(np.abs(r32[:, : 2 * nbody] - r64[:, : 2 * nbody])) / fscale,
+                "fb": relerr(s32["fb"], s64["fb"])}
+        L, nw, nq = (e64.dims[i][k] for k in ("L", "nw", "nq"))
+        for k in range(L):
+            errs[f"omega{k + 1}"] = relerr(s32["omega"][k * nw:(k + 1) * nw], s64["omega"][k * nw:(k + 1) * nw])
+            errs[f"q{k + 1}"] = relerr(s32["q"][k * nq:(k + 1) * nq], s64["q"][k * nq:(k + 1) * nq])
+        print(f"case {SLICE[i]} m={c['m']} L={L}", {k: f"{v:.1e}" for k, v in errs.items()})
+        for k, v in errs.items():
+            worst[k[:1]] = max(worst.get(k[:1], 0.0), v)
+            assert v < 1e-5, f"case {SLICE[i]} {k} {v:.2e}"
+    print("worst per field kind", worst)
+
+
+@pytest.mark.parametrize("m", [256, 320, 384, 512])
+def test_fp64_case_of_each_size_matches_oracle(m, factors):
+    from ibfs_amd import hip
+
+    sl = slice_cases()
+    i = next(k for k, c in enumerate(sl) if c["m"] == m)
+    c = sl[i]
+    o = orc.Oracle(**c)
+    g = hip.Solver(**c)
+    cm, cv = factors[i]
+    g.set_cholesky(cm, cv)
+    o.set_cholesky(cm, cv)
+    nf = o.nf
+    A = cm.reshape(nf, nf).T
+    for j in (0, nf - 1):  # the upper triangle keeps A: spot columns against a_times
+        assert relerr(A[: j + 1, j], o.a_column(j)[: j + 1]) < 1e-11
+    oh = np.array([np.concatenate([f, cc, [s]]) for f, cc, s in (o.advance() for _ in range(2))])
+    gh = g.advance(2)
+    st = g.get_state()
+    for k in range(o.L):
+        w = slice(k * o.nw, (k + 1) * o.nw)
+        qq = slice(k * o.nq, (k + 1) * o.nq)
+        assert relerr(st["omega"][w], o.omega[w]) < 1e-11, f"omega level {k + 1}"
+        assert relerr(st["q"][qq], o.q[qq]) < 1e-11, f"q level {k + 1}"
+    assert relerr(st["fb"], o.fb) < 1e-11
+    assert np.max(np.abs(gh[:, :2] - oh[:, :2])) < 1e-11 * np.max(np.abs(oh[:, :2]))

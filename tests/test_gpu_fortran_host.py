@@ -40,12 +40,16 @@ def run_host(workdir, istart, istop, nblock=1, stationary=True):
     return r
 
 
-def read_table(path):
+def read_table(path, ncol=None):
+    """A list-directed output file; with ncol, the values are taken in order
+    and reshaped (list-directed records may wrap onto several lines)."""
     rows = []
     with open(path) as f:
         for line in f:
             if line.strip():
                 rows.append([float(v.replace("D", "E")) for v in line.split()])
+    if ncol is not None:
+        return np.array([v for r in rows for v in r]).reshape(-1, ncol)
     return np.array(rows)
 
 
@@ -272,3 +276,88 @@ def test_post_writes_the_tecplot_files(host_run, tmp_path):
             # E14.5 keeps 5 significant digits: half a unit of the 5th is 5e-5 relative
             scale = np.max(np.abs(exp[:, c]))
             assert np.all(np.abs(data[:, c] - exp[:, c]) <= 5.01e-5 * np.abs(exp[:, c]) + 1e-9 * scale), (lev, c)
+
+
+ACT = [" &READ_ACT_PARAMETERS X_J = 0.9, Y_J = 0.0, H_J = 0.3, U_J = 0.5, U_J_OSC = 0.25, THETA_J = 0.0,"
+       " OMG_J = 3.0 /\n",
+       " &READ_ACT_PARAMETERS X_J = 0.0, Y_J = 0.9, H_J = 0.1, U_J = 0.0, U_J_OSC = 0.8, THETA_J = 90.0,"
+       " OMG_J = 5.0 /\n"]
+
+
+def test_host_actuators_match_oracle(tmp_path):
+    """Two input/actuator.NNN.inp files and no geom.inp: the host appends
+    the slots to the default circle with body numbers -1, -2
+    (src/grid.f90:383-392, 636-718), writes geom.inp and slot.NNN.inp, and
+    the library drives the jets every step.  force.dat gets the columns of
+    bodies -2..0; every output matches the oracle with actuation's motion.
+    A second run reads the slots back from geom.inp and slot.NNN.inp
+    (src/grid.f90:286-300) and gives the same files."""
+    d = str(tmp_path / "a")
+    os.makedirs(os.path.join(d, "input"))
+    for k, txt in enumerate(ACT):
+        with open(os.path.join(d, "input", f"actuator.{k + 1:03d}.inp"), "w") as f:
+            f.write(txt)
+    run_host(d, 0, 10)
+    from ibfs_amd import geometry
+
+    xb, yb, bd = geometry.read_geom(os.path.join(d, "input", "geom.inp"))
+    assert list(np.unique(bd)) == [-2, -1, 0]
+    npt = [int(open(os.path.join(d, "input", f"slot.{k:03d}.inp")).read().split()[0]) for k in (1, 2)]
+    assert npt == [3, 1] and np.count_nonzero(bd == -1) == 3 and np.count_nonzero(bd == -2) == 1
+    cx, cy, _ = geometry.circle(4.0 / 32, 1.0, 0.0, 0.0)
+    sx, sy, _ = geometry.actuator_slot(0.9, 0.0, 0.3, 4.0 / 32, 0.0, -1)
+    assert np.allclose(xb, np.concatenate([cx, sx, [0.0]]), rtol=0, atol=1e-15)
+    assert np.allclose(yb, np.concatenate([cy, sy, [0.9]]), rtol=0, atol=1e-15)
+    force = read_table(os.path.join(d, "output", "force.dat"), 7)
+    assert force.shape == (10, 7) and np.array_equal(force[:, 0], np.arange(1, 11))
+    params = [(0.9, 0.0, 0.3, 0.5, 0.25, 0.0, 3.0), (0.0, 0.9, 0.1, 0.0, 0.8, 90.0, 5.0)]
+    o = orc.Oracle(32, 32, 3, 0.02, 100.0, 4.0, 2.0, 2.0, xb, yb, bd)
+    chd = read_record(os.path.join(d, "input", "ibfs.chd"))
+    o.set_cholesky(chd[: o.nf * o.nf], chd[o.nf * o.nf:])
+    of = []
+    for it in range(1, 11):
+        f, c, s = o.advance(motion=orc.actuation(it, o.nb, o.delta, 0.02, npt, params))
+        of.append(f)
+    of = np.array(of)
+    for b in range(3):
+        cols = [b, 3 + b]
+        assert np.max(np.abs(force[:, 1:][:, cols] - of[:, cols])) < 1e-11 * np.max(np.abs(of[:, cols])), b
+    # a second run from the written geometry: slots counted from slot.NNN.inp
+    d2 = str(tmp_path / "b")
+    shutil.copytree(os.path.join(d, "input"), os.path.join(d2, "input"))
+    run_host(d2, 0, 10)
+    assert np.array_equal(read_table(os.path.join(d2, "output", "force.dat"), 7), force)
+
+
+def test_host_stops_on_a_missing_restart_file(tmp_path):
+    """istart = 5 with no output/ibfs0000005.var: the host stops with a
+    message naming the file (the reference would hit end-of-file)."""
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "input"))
+    os.makedirs(os.path.join(d, "output"))
+    with open(os.path.join(d, "input", "ibfs.inp"), "w") as f:
+        f.write(NML.format(istart=5, istop=10, stationary="T"))
+    r = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 or "restart file not found" in (r.stdout + r.stderr)
+    assert "ibfs0000005.var not found" in r.stdout
+
+
+def test_host_stops_on_a_bad_cached_factor(host_run, tmp_path):
+    """A cached input/ibfs.chd whose factor has a non-positive pivot (a
+    non-SPD matrix) is refused by the library (IBFS_ECHOL) and the host stops
+    with the library's message, as the reference stops in choldc."""
+    d, _ = host_run
+    d2 = str(tmp_path)
+    shutil.copytree(os.path.join(d, "input"), os.path.join(d2, "input"))
+    os.makedirs(os.path.join(d2, "output"))
+    path = os.path.join(d2, "input", "ibfs.chd")
+    raw = bytearray(open(path, "rb").read())
+    n = struct.unpack("<i", raw[:4])[0] // 8
+    nf = int((-1 + (1 + 4 * n) ** 0.5) / 2)
+    struct.pack_into("<d", raw, 4 + 8 * (nf * nf + 3), -1.0)  # cholvec(4) < 0
+    open(path, "wb").write(bytes(raw))
+    with open(os.path.join(d2, "input", "ibfs.inp"), "w") as f:
+        f.write(NML.format(istart=0, istop=2, stationary="T"))
+    r = subprocess.run([EXE], cwd=d2, capture_output=True, text=True, timeout=120)
+    out = r.stdout + r.stderr
+    assert "ibfs_set_cholesky" in out and "code" in out and "non-positive" in out

@@ -220,3 +220,19 @@ def test_errors_are_reported():
     g = hip.Solver(**a)
     with pytest.raises(hip.IBFSError, match="Cholesky"):
         g.advance(1)  # the factor is required before the first step
+    # a factor with a non-positive pivot (what choldc reports for a matrix
+    # that is not SPD, src/operators.f90:782-784) is refused: IBFS_ECHOL
+    cm, cv = g.build_cholesky()
+    bad = cv.copy()
+    bad[3] = 0.0
+    with pytest.raises(hip.IBFSError, match=r"\(3\).*non-positive"):
+        g.set_cholesky(cm, bad)
+    # a non-finite motion entry is rejected before any step runs: IBFS_EINVAL
+    g.set_cholesky(cm, cv)
+    mo = np.zeros(g.nf)
+    mo[5] = np.nan
+    with pytest.raises(hip.IBFSError, match=r"\(1\).*non-finite"):
+        g.advance(2, motion=mo)
+    assert g.itime == 0
+    h = g.advance(1)  # the handle still steps
+    assert np.all(np.isfinite(h))

@@ -129,3 +129,17 @@ def test_tile_shapes_do_not_change_results(monkeypatch):
         print("tile", tile, shapes)
         for k in base:
             assert np.array_equal(st[k], base[k]), k
+
+
+def test_tandem_1024_six_levels():
+    """The large case (1024^2 x 6 levels, dx 0.01, two d = 1 cylinders 3 d
+    apart, 628 points, Re 200, dt 0.0015): 2 FP64 steps with the device
+    factor on both sides.  At Nf = 1256 this pins the step's explicit-inverse
+    force solve (rows of A^-1 = L^-T L^-1) against the oracle's two triangular
+    solves of cholsl (src/operators.f90:794-819)."""
+    from ibfs_amd import cases, hip
+
+    a = cases.tandem_1024()
+    o, g = orc.Oracle(**a), hip.Solver(**a)
+    assert g.nb == 628 and g.L == 6
+    run_pair(o, g, 2, [0, 627, 1255])

@@ -655,3 +655,73 @@ def test_move_translates_the_smearing():
     ones[: o.nb] = 1.0
     hx = o.reg(ones)
     assert abs(hx[: (m + 1) * n].sum() / o.nb - 1.0) < 1e-12  # sum of delta^2 phi phi over faces = 1 per point
+
+
+# ------------------------------------------- the 200^2 x 3 full-length fixture
+def test_cyl200_fixture_first_steps():
+    """tests/golden/cyl200_history.npz (made by make_cyl200_history.py beside
+    it) holds the oracle's 2000-step history of the 200^2 x 3 cylinder; the
+    device is compared with it.  Here the oracle re-derives its factor and
+    the first 20 rows, which must agree with the stored ones exactly, and the
+    stored run must show the expected flow: drag positive and O(1), lift at
+    rounding level (mirror symmetry), and the CPU-vs-CPU floor small."""
+    from ibfs_amd import cases
+
+    fx = np.load(os.path.join(GOLDEN, "cyl200_history.npz"))
+    args = cases.cylinder_200()
+    o = orc.Oracle(**args)
+    o.preprocess()
+    cm, cv = o.cholmat.copy(), o.cholvec.copy()
+    assert np.array_equal(fx["chol_sum"], np.array([cm.sum(), (cm * cm).sum(), cv.sum()]))
+    o.set_cholesky(cm, cv)
+    rows = np.array([np.concatenate([f, c, [s]]) for f, c, s in (o.advance() for _ in range(20))])
+    assert np.array_equal(rows, fx["hist_a"][:20])
+    ha = fx["hist_a"]
+    assert ha.shape == (2000, 2 + 3 + 1)
+    assert np.all(ha[10:, 0] > 0) and 0.5 < ha[-1, 0] < 3.0
+    assert np.max(np.abs(ha[:, 1])) < 1e-3 * np.max(np.abs(ha[:, 0]))
+    assert np.max(fx["floor_omega"]) < 1e-9 and np.max(fx["floor_force"]) < 1e-9
+
+
+# ------------------------------------------------------------------ actuators
+@pytest.mark.parametrize("hs,angle", [(0.3, 0.0), (0.55, 30.0), (0.1, 90.0), (0.25, -45.0)])
+def test_actuator_slot_points(hs, angle):
+    """actuator_slot (src/grid.f90:636-718): INT(hs/ds) + 1 points spaced
+    hs/INT(hs/ds) apart on a segment of width hs centred on the slot, normal
+    to the jet direction; a single point at the centre when hs < ds."""
+    from ibfs_amd import geometry
+
+    ds, x, y = 0.125, 0.7, -0.2
+    px, py, pb = geometry.actuator_slot(x, y, hs, ds, angle, -3)
+    npt = int(hs / ds)
+    assert len(px) == npt + 1 and np.all(pb == -3)
+    if npt == 0:
+        assert px[0] == x and py[0] == y
+        return
+    th = np.deg2rad(angle)
+    assert abs(px.mean() - x) < 1e-14 and abs(py.mean() - y) < 1e-14
+    d = np.hypot(np.diff(px), np.diff(py))
+    assert np.allclose(d, hs / npt, rtol=0, atol=1e-14)
+    assert abs(np.hypot(px[-1] - px[0], py[-1] - py[0]) - hs) < 1e-14
+    # the segment is normal to the jet (cos th, sin th)
+    assert np.max(np.abs(np.diff(px) * np.cos(th) + np.diff(py) * np.sin(th))) < 1e-14
+
+
+def test_oracle_actuation_closed_form():
+    """actuation/u_jet (src/grid.f90:579-633): only the last sum(npt) points
+    move, slot by slot; the jet speed is |u_j + u_j_osc sin(omg_j dt it)|
+    times delta along theta_j; other entries of the motion are kept."""
+    nb, delta, dt = 40, 0.05, 0.01
+    npt = [3, 1]
+    params = [(0.9, 0.0, 0.3, 0.5, 0.25, 0.0, 3.0), (0.0, 0.9, 0.1, 0.0, 0.8, 90.0, 5.0)]
+    base = np.arange(2 * nb, dtype=float)
+    for it in (1, 7, 40):
+        mo = orc.actuation(it, nb, delta, dt, npt, params, base)
+        assert np.array_equal(mo[: nb - 4], base[: nb - 4]) and np.array_equal(mo[nb:2 * nb - 4], base[nb:2 * nb - 4])
+        for a, pts in ((0, range(nb - 4, nb - 1)), (1, [nb - 1])):
+            _, _, _, uj, ujo, th, om = params[a]
+            speed = uj + ujo * np.sin(om * dt * it)
+            for i in pts:
+                assert abs(np.hypot(mo[i], mo[nb + i]) - abs(speed) * delta) < 1e-15
+                assert abs(mo[i] - speed * np.cos(np.deg2rad(th)) * delta) < 1e-15
+                assert abs(mo[nb + i] - speed * np.sin(np.deg2rad(th)) * delta) < 1e-15
