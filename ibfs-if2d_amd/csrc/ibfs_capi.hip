@@ -213,6 +213,9 @@ CaseDesc make_desc(int m, int n, int mgridlev, int nb, double dt, double re, dou
 
 }  // namespace
 
+// the message of the calling thread's last failed handle-free call
+const char* noalloc_last_error();
+
 // ============================================================================
 // C-ABI
 // ============================================================================
@@ -990,11 +993,165 @@ int ibfs_op_cholsl(ibfs_handle* h, const double* b, double* x) {
   });
 }
 
-const char* ibfs_last_error(const ibfs_handle* h) { return h ? h->err.c_str() : "null handle"; }
+const char* ibfs_last_error(const ibfs_handle* h) { return h ? h->err.c_str() : noalloc_last_error(); }
 
 int ibfs_destroy(ibfs_handle* h) {
   if (!h) return IBFS_EINVAL;
   delete h;
+  return IBFS_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// ibfs_ctci_noalloc: ctci (src/myfft.f90:97-109) on caller-owned device
+// buffers, with no allocation at all.  Its intermediates, the pair-major
+// eigenvalue table and the twiddles of both axes live in one file-scope
+// device array, `scratch`, whose address on each device is looked up once
+// (hipGetSymbolAddress) and kept; the array's host-side name is never passed
+// to a kernel.  Every call on a device shares that device's array (calls that
+// overlap on one device race); every entry a call reads is written by the
+// same call first, so nothing is left over from an earlier call.
+// ============================================================================
+namespace {
+
+constexpr int kNaMaxMN = 1024;                                   // m, n <= 1024
+constexpr long kNaMaxW = (long)(kNaMaxMN - 1) * (kNaMaxMN - 1);  // (m-1)(n-1)
+constexpr long kNaMaxPM = (long)kNaMaxMN * (kNaMaxMN - 1);        // pm_size <= m (n-1)
+constexpr int kNaMaxTw = 2048;                                    // spec twiddles per axis (1365 at 2m = 2048)
+// doubles: T1 (W) | T2 (W) | laminv, pair-major | twiddles along i | along j (complex)
+constexpr long kNaScratch = 2 * kNaMaxW + kNaMaxPM + 2L * 2 * kNaMaxTw;
+constexpr int kNaMaxDev = 64;
+
+__device__ __align__(256) double scratch[kNaScratch];
+
+std::mutex na_mu;
+double* na_ptr[kNaMaxDev] = {};       // the array's address on each device
+thread_local char na_err[256] = "";   // message of the calling thread's last failure
+
+// the size-specialised FFT lengths; the no-alloc entry takes only these
+constexpr int kNaSizes[] = {256, 400, 512, 600, 640, 768, 1024, 1200, 2048};
+bool na_spec(int n2) {
+  for (int s : kNaSizes)
+    if (s == n2) return true;
+  return false;
+}
+
+// ctci's table laminv = 1/(lam 4mn) (src/myfft.f90:57-59,106) in the
+// pair-major layout of the column pair (padding entries 0), and the per-stage
+// twiddles exp(-2 pi i k/(ns R)) of the two axes' schedules
+__global__ void k_na_tables(Geo g, double norm, double* __restrict__ lpm, spec::Sched sm,
+                            cplx<double>* __restrict__ twm, spec::Sched sn, cplx<double>* __restrict__ twn) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long npm = pm_size(g);
+  if (t < npm) {
+    const long pr = t >> 1, ii = 2 * (pr / g.nm1) + (t & 1), jj = pr % g.nm1;
+    double v = 0.0;
+    if (ii < g.mm1) {
+      const double lam = -2.0 * (cos(kPiRef * (double)(float)(ii + 1) / (double)(float)g.m) +
+                                 cos(kPiRef * (double)(float)(jj + 1) / (double)(float)g.n) - 2.0);
+      v = 1.0 / lam / norm;
+    }
+    lpm[t] = v;
+    return;
+  }
+  long u = t - npm;
+  for (int ax = 0; ax < 2; ax++) {
+    const spec::Sched& sc = ax == 0 ? sm : sn;
+    if (u < sc.ntw) {
+      int s = 0;
+      while (s + 1 < sc.nst && u >= sc.toff[s + 1]) s++;
+      const long k = u - sc.toff[s];
+      double sn_, cs_;
+      sincospi(-2.0 * (double)k / (double)(sc.ns[s] * sc.rad[s]), &sn_, &cs_);
+      (ax == 0 ? twm : twn)[u] = cplx<double>{cs_, sn_};
+      return;
+    }
+    u -= sc.ntw;
+  }
+}
+
+template <int NN>
+void na_row(const RowOp<double, SrcField<double>, DstField<double>>& op, const cplx<double>* tw, hipStream_t st) {
+  hipLaunchKernelGGL((spec::dst_pass_n<double, NN, 1, 0, RowOp<double, SrcField<double>, DstField<double>>>),
+                     dim3((op.g.nm1 + 1) / 2, 1), dim3(NN / 4), 0, st, op, tw);
+}
+template <int NN>
+void na_col2(const Col2Op<double, SrcField<double>, MidScale<double>, DstField<double>>& op, const cplx<double>* tw,
+             hipStream_t st) {
+  hipLaunchKernelGGL((spec::dst_col2_n<double, NN, 1, Col2Op<double, SrcField<double>, MidScale<double>,
+                                                            DstField<double>>>),
+                     dim3((op.g.mm1 + 1) / 2, 1), dim3(NN / 4), 0, st, op, tw);
+}
+#define IBFS_NA_DISPATCH(FN, N2, ...) \
+  switch (N2) {                       \
+    case 256: FN<256>(__VA_ARGS__); break;   \
+    case 400: FN<400>(__VA_ARGS__); break;   \
+    case 512: FN<512>(__VA_ARGS__); break;   \
+    case 600: FN<600>(__VA_ARGS__); break;   \
+    case 640: FN<640>(__VA_ARGS__); break;   \
+    case 768: FN<768>(__VA_ARGS__); break;   \
+    case 1024: FN<1024>(__VA_ARGS__); break; \
+    case 1200: FN<1200>(__VA_ARGS__); break; \
+    default: FN<2048>(__VA_ARGS__); break;   \
+  }
+
+int na_fail(int code, const char* msg) {
+  std::snprintf(na_err, sizeof(na_err), "%s", msg);
+  return code;
+}
+
+}  // namespace
+
+const char* noalloc_last_error() { return na_err[0] ? na_err : "null handle"; }
+
+extern "C" {
+
+int ibfs_ctci_noalloc(int m, int n, const double* omega, double* s, void* stream) {
+  if (m < 4 || n < 4 || m > kNaMaxMN || n > kNaMaxMN || !na_spec(2 * m) || !na_spec(2 * n))
+    return na_fail(IBFS_EINVAL, "ibfs_ctci_noalloc: m and n must be one of 128, 200, 256, 300, 320, 384, 512, "
+                                "600, 1024 (the size-specialised transforms; scratch holds up to 1024 x 1024)");
+  if (!omega || !s) return na_fail(IBFS_EINVAL, "ibfs_ctci_noalloc: null field");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNaMaxDev)
+    return na_fail(IBFS_EHIP, "ibfs_ctci_noalloc: no current device");
+  double* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_ptr[dev]) {  // once per device, with that device current
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) return na_fail(IBFS_EHIP, hipGetErrorString(e));
+      na_ptr[dev] = static_cast<double*>(p);
+    }
+    base = na_ptr[dev];
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const Geo g{m, n, 1, m - 1, n - 1, (m - 1) * (n - 1), (m + 1) * n + (n + 1) * m, 2 * (m + 1) + 2 * (n + 1),
+              0, n + 1, 2 * (n + 1), 2 * (n + 1) + m + 1};
+  const long W = g.nw, npm = pm_size(g);
+  double* t1 = base;
+  double* t2 = base + W;
+  double* lpm = base + 2 * W;
+  cplx<double>* twm = reinterpret_cast<cplx<double>*>(base + 2 * kNaMaxW + kNaMaxPM);
+  cplx<double>* twn = twm + kNaMaxTw;
+  const spec::Sched sm = spec::make_sched(2 * m), sn = spec::make_sched(2 * n);
+  const double norm = 4.0 * (double)(float)((long)m * (long)n);  // src/myfft.f90:42
+  const long ntab = npm + sm.ntw + sn.ntw;
+  hipLaunchKernelGGL(k_na_tables, dim3((unsigned)((ntab + 255) / 256)), dim3(256), 0, st, g, norm, lpm, sm, twm, sn,
+                     twn);
+  // S . diag(laminv) . S as a row pass, the fused column pair and a row pass
+  const RowOp<double, SrcField<double>, DstField<double>> r1{g, SrcField<double>{omega, 0}, DstField<double>{t1, 0},
+                                                             nullptr, 0};
+  const Col2Op<double, SrcField<double>, MidScale<double>, DstField<double>> c2{
+      g, SrcField<double>{t1, 0}, MidScale<double>{lpm}, DstField<double>{t2, 0}, nullptr, 0};
+  const RowOp<double, SrcField<double>, DstField<double>> r2{g, SrcField<double>{t2, 0}, DstField<double>{s, 0},
+                                                             nullptr, 0};
+  IBFS_NA_DISPATCH(na_row, 2 * m, r1, twm, st)
+  IBFS_NA_DISPATCH(na_col2, 2 * n, c2, twn, st)
+  IBFS_NA_DISPATCH(na_row, 2 * m, r2, twm, st)
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return na_fail(IBFS_EHIP, hipGetErrorString(e));
   return IBFS_OK;
 }
 

@@ -254,6 +254,25 @@ int ibfs_op_dst_gemm(ibfs_handle* h, int ks, const double* in, double* out);
  * HBM byte counters (FETCH_SIZE / WRITE_SIZE) against known traffic. */
 int ibfs_time_copy(ibfs_handle* h, long n, int reps, double* ms_per_launch);
 
+/* No counterpart in the reference: an entry point that allocates nothing,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It is
+ * ibfs_op_ctci (the Poisson solve C^T C s = omega with zero edge values,
+ * src/myfft.f90:97-109) on device pointers the caller owns: omega and s
+ * hold (m-1)*(n-1) doubles each, i fastest (they may be the same buffer);
+ * stream is a hipStream_t (NULL: the default stream), and the call only
+ * enqueues work on it.  No handle is needed.  Intermediates, the eigenvalue
+ * table and the twiddles live in one library-owned device array of about
+ * 25 MB per GPU (held until the process ends), so m and n are limited to the
+ * size-specialised transforms with m, n <= 1024: 128, 200, 256, 300, 320,
+ * 384, 512, 600, 1024; other sizes return IBFS_EINVAL.  Calls that overlap
+ * on one GPU share that array and race: serialise them (one stream, or
+ * events).  Errors: ibfs_last_error(NULL) gives the calling thread's last
+ * message. */
+int ibfs_ctci_noalloc(int m, int n, const double* omega, double* s, void* stream);
+
+/* The message of the last failed call on h; with h == NULL, of the calling
+ * thread's last failed handle-free call. */
 const char* ibfs_last_error(const ibfs_handle* h);
 int ibfs_destroy(ibfs_handle* h);
 
