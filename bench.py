@@ -11,8 +11,8 @@ Prints one JSON line.  ``value`` is steps/s of the plain FP64 form (the
 drop-in).  ``chosen`` reports the mixed FP32/FP64 form for one case and for
 a batch of independent copies of the case in one graph per step (aggregate
 case-steps/s), with its per-field error against the FP64 run.  ``roofline`` is
-the step's dominant kernel (the fused column pair of ctci, timed live with
-HIP events on the library's stream); ``cpu_baseline`` is the oracle's FP64
+the step's dominant kernel (the fused column pair of ctci, timed live inside
+steps with events on its own dispatches); ``cpu_baseline`` is the oracle's FP64
 step, single thread, on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -31,11 +31,24 @@ sys.path.insert(0, os.path.join(ROOT, "ibfs-if2d_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per launch of the fused column pair at 512^2 (FP64): FETCH_SIZE
-# and WRITE_SIZE per dispatch from two rocprofv3 --pmc runs, each scaled by
-# what the counter reads for a copy kernel of the same size
-# (profiles/dst_col2_512x5_fp64_hbm_counters.md: 2353.6 KiB x 1.980 fetched +
-# 2135.8 KiB x 1.000 written)
-PAIR_TRAFFIC_BYTES = int((2353.6 * 1.980 + 2135.8 * 1.000) * 1024)
+# and WRITE_SIZE per dispatch from two rocprofv3 --pmc runs of this build,
+# each scaled by what the counter reads for a copy kernel of the same size;
+# the summary of those runs records the figure on a line
+# "traffic_bytes_per_launch: N"
+PAIR_COUNTERS = os.path.join(ROOT, "profiles", "dst_col2_512x5_fp64_hbm_counters.md")
+
+
+def pair_traffic():
+    """(bytes per launch, source) from the committed counter summary, or
+    (None, reason) when it is absent."""
+    try:
+        with open(PAIR_COUNTERS) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1].split()[0]), os.path.relpath(PAIR_COUNTERS, ROOT)
+    except OSError:
+        pass
+    return None, "no counter summary found"
 
 
 def step_bytes(m, n, L, nf, s=8):
@@ -156,14 +169,20 @@ def main(argv=None):
     # ---- dominant kernel of the step: the fused column pair of ctci (DST-I
     # along j, times laminv, DST-I along j), 10 of the step's launches; per
     # launch it reads the W field and the W table laminv and writes W doubles
+    # in-step duration: events on the kernel's own dispatches during 20 eager
+    # steps (the rocprofv3 trace of the graph-replayed step is the check);
+    # isolated back-to-back launches are reported beside it
     pair_bytes = 3 * 8 * g.nw
+    pair_us, pair_n = g.time_step_kernel(20)
     t_pair = g.time_dst_pass(2, 200)
     t_row = g.time_dst_pass(0, 200)
     t_col = g.time_dst_pass(1, 200)
-    achieved = pair_bytes / (t_pair * 1e-3) / 1e9
+    achieved = pair_bytes / (pair_us * 1e-6) / 1e9
+    traffic, traffic_src = pair_traffic()
     B = step_bytes(g.m, g.n, g.L, g.nf)
-    log(f"column pair {t_pair * 1e3:.2f} us ({achieved:.0f} GB/s); DST pass: row {t_row * 1e3:.2f} us, column "
-        f"{t_col * 1e3:.2f} us; step bytes {B / 1e6:.1f} MB -> whole-step {B * steps_s / 1e9:.1f} GB/s")
+    log(f"column pair in the step {pair_us:.2f} us over {pair_n} launches ({achieved:.0f} GB/s); isolated "
+        f"{t_pair * 1e3:.2f} us; DST pass: row {t_row * 1e3:.2f} us, column {t_col * 1e3:.2f} us; step bytes "
+        f"{B / 1e6:.1f} MB -> whole-step {B * steps_s / 1e9:.1f} GB/s")
     # streaming copies: one of the pass's size (the counter calibration point)
     # and a large one (the measured copy bandwidth of this GPU)
     t_copy_w = g.time_copy(g.nw, 200)
@@ -385,10 +404,11 @@ def main(argv=None):
                                "impulsive start", "m": g.m, "n": g.n, "levels": g.L, "nb": g.nb,
                    "precision": "plain FP64", "parallelism": "single GPU, one case"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": PAIR_TRAFFIC_BYTES,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "dst_col2_n (fused column pair of ctci over a 511x511 FP64 field)",
-                     "launch_us": t_pair * 1e3, "bytes_per_launch": pair_bytes,
-                     "row_pass_us": t_row * 1e3, "col_pass_us": t_col * 1e3,
+                     "launch_us": pair_us, "launch_us_note": "mean in-step duration over the kernel's "
+                     f"{pair_n} dispatches in 20 steps (events on the dispatches)", "bytes_per_launch": pair_bytes,
+                     "isolated_launch_us": t_pair * 1e3, "row_pass_us": t_row * 1e3, "col_pass_us": t_col * 1e3,
                      "copy_same_size_us": t_copy_w * 1e3, "measured_copy_GBs": copy_gbs},
         "step_roofline": {"bytes_per_step": B, "achieved_GBs": B * steps_s / 1e9,
                           "frac": B * steps_s / 1e9 / HBM_PEAK_GBS},

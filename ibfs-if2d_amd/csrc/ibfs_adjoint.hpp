@@ -367,6 +367,33 @@ static __global__ void k_force_adj(Smear sm, const double* __restrict__ wsum, do
   afb[c.foff + f] = scale * coef * 2.0 * c.delta / c.dt * wsum[c.foff + f];
 }
 
+// The adjoint of one step's diagnostics row (write_force and write_circ,
+// src/operators.f90:583-684) with the row's cotangent grow, laid out like a
+// history row: afb of point force f = (2 delta / dt) wsum_f times the
+// cotangent of its body's x or y force column (body bdyno_p - bmin).
+static __global__ void k_force_adj_row(Smear sm, const double* __restrict__ wsum, const int* __restrict__ bdyno,
+                                       const double* __restrict__ grow, double* __restrict__ afb) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const CaseSm c = sm.cs[blockIdx.y];
+  if (f >= c.nf) return;
+  const int p = f < c.nb ? f : f - c.nb, comp = f < c.nb ? 0 : 1;
+  const double g = grow[c.hoff + comp * c.nbody + (bdyno[c.foff / 2 + p] - c.bmin)];
+  afb[c.foff + f] = g * 2.0 * c.delta / c.dt * wsum[c.foff + f];
+}
+
+// circ(k) = sum of omega_k (write_circ): aw_k += the cotangent of circ(k),
+// read from the row grow at the case's circulation columns
+template <typename T>
+__global__ void k_circ_adj(Geo g, Smear sm, const double* __restrict__ grow, T* __restrict__ aw, long bs,
+                           const int* __restrict__ Lc) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int b = blockIdx.y / g.Lmax, lev = blockIdx.y % g.Lmax;
+  if (e >= g.nw || !active(Lc, b, lev)) return;
+  const CaseSm c = sm.cs[b];
+  const double gc = grow[c.hoff + 2 * c.nbody + lev];
+  if (gc != 0.0) aw[b * bs + (long)lev * g.nw + e] += T(gc);
+}
+
 // au[f] += delta * z[f]
 static __global__ void k_axpy_case(Smear sm, const double* __restrict__ z, double a, double* __restrict__ y, int use_delta) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;

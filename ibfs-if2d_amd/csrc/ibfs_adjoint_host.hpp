@@ -91,18 +91,26 @@ struct AdjointRunner {
   }
 
   // One adjoint step.  In: AW, ARO, AQ = adjoints of the step's outputs
-  // (omega, rhs_old, q); wold/qold = the forward state at the start of the
-  // step.  Out: AW, ARO, AQ = adjoints of the step's inputs; AU += dJ/du_B.
-  void step(const T* wold, const T* qold, int first, bool in_window, double cd, double cl, double jscale) {
+  // (omega, rhs_old, q); grow = the cotangent of the step's diagnostics row
+  // on the device (history-row layout, the cases at their row offsets: body
+  // x and y forces and circulations; the slip entry is not differentiated),
+  // or nullptr; wold/qold = the forward state at the start of the step.
+  // Out: AW, ARO, AQ = adjoints of the step's inputs; Z = the adjoint of the
+  // step's motion argument; AU += delta * Z (dJ/du_B).
+  void step(const T* wold, const T* qold, int first, const double* grow) {
     const int* Lc = E.Lc.p;
     const Geo& g = E.geo;
     hipLaunchKernelGGL(k_lastbc<T>, dim3((nbc * Lmax + 255) / 256, B), dim3(256), 0, st, g, wold, LB.p, Lc);
+    if (grow)  // circulations are sums of the step's final vorticity
+      hipLaunchKernelGGL(k_circ_adj<T>, dim3((nw + 255) / 256, B * Lmax), dim3(256), 0, st, g, E.sm, grow, AW.p,
+                         FB, Lc);
     // STEP 4 reverse: fluxes and vorticity of the final vort2flux
     mp_reverse(AQ.p, nullptr, T(1));
     // STEP 3 reverse: afb_total = afb_J - regT(curl(ainv(aw_1), 0)); z = A^-1 (regT(..) - afb_J) = -adrhsf
     E.ctci_like(SrcField<T>{AW.p, FB}, true, DstField<T>{U.p, nw}, T1.p, T2.p, B, nullptr, 0, E.lam1i.p, E.tbs());
-    if (in_window)
-      hipLaunchKernelGGL(k_force_adj, grid(E.nf_max), dim3(256), 0, st, E.sm, E.wsum.p, cd, cl, jscale, AFB.p);
+    if (grow)
+      hipLaunchKernelGGL(k_force_adj_row, grid(E.nf_max), dim3(256), 0, st, E.sm, E.wsum.p, E.bdyno.p, grow,
+                         AFB.p);
     else
       AFB.zero(st);
     hipLaunchKernelGGL(k_regT_curl<T>, grid(E.nf_max), dim3(256), 0, st, g, E.sm, U.p, (const T*)nullptr,

@@ -26,6 +26,20 @@ bool factor_235(int n, std::vector<int>& rad) {
 }
 }  // namespace ibfs
 
+// The forward sweep of an adjoint run (ibfs_adjoint_forward), kept in the
+// handle for the reverse sweep: the state (omega, rhs_old, q) every ckpt
+// steps, the final state and the motion every step used.
+struct TapeBase {
+  virtual ~TapeBase() = default;
+  int N = 0, ckpt = 1, nck = 0, it0 = 0;
+  std::vector<int> ck_it;
+  std::vector<double> mo;  // delta * u_B, concatenated per case
+};
+template <typename T>
+struct Tape : TapeBase {
+  DevBuf<T> CW, CRO, CQ, FW, FRO, FQ;
+};
+
 struct ibfs_handle {
   int precision = 0;
   std::string err;
@@ -48,6 +62,8 @@ struct ibfs_handle {
   std::vector<int> cg_iters;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // the forward sweep of ibfs_adjoint_forward, kept until ibfs_adjoint_reverse
+  std::unique_ptr<TapeBase> tape;
 
   ~ibfs_handle() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -479,12 +495,10 @@ void tangent_impl(ibfs_handle* h, Engine<T>* E, int* itime, int nspin, int nwin,
   accumulate_objective(h, ht.data(), N, it0 + 1, it0 + nspin, nwin, cd, cl, dJ);
 }
 
-// checkpointed adjoint run of ibfs_adjoint_gradient on the single group E
+// the transposes assume no smearing weight on the two outermost rings of
+// faces of the finest level (bodies well inside the domain)
 template <typename T>
-void adjoint_impl(ibfs_handle* h, Engine<T>* E, int* itime, int nspin, int nwin, int ckpt, const double* ub,
-                  double cd, double cl, double* J, double* grad) {
-  // the transposes assume no smearing weight on the two outermost rings of
-  // faces of the finest level (bodies well inside the domain)
+void check_adjoint_geometry(Engine<T>* E) {
   for (int c = 0; c < E->B; c++) {
     const CaseHost& k = E->cs[c];
     const int m = k.d.m, n = k.d.n, nu = (m + 1) * n;
@@ -496,61 +510,165 @@ void adjoint_impl(ibfs_handle* h, Engine<T>* E, int* itime, int nspin, int nwin,
         throw ArgError{IBFS_EINVAL, "adjoint: a body smears onto the outer rings of the finest grid"};
     }
   }
+}
+
+// forward sweep of an adjoint run: nsteps steps with motion = delta * u_B,
+// the state checkpointed every ckpt steps into the handle's tape, the
+// history rows into hist (nsteps x rowlen, may be NULL)
+template <typename T>
+void adjoint_forward_impl(ibfs_handle* h, Engine<T>* E, int* itime, int nsteps, int ckpt, const double* ub,
+                          double* hist) {
+  check_adjoint_geometry(E);
   hipStream_t st = h->stream;
-  const int N = nspin + nwin, it0 = *itime;
-  const int nck = (N + ckpt - 1) / ckpt;
+  auto tp = std::make_unique<Tape<T>>();
+  Tape<T>& t = *tp;
+  t.N = nsteps;
+  t.ckpt = ckpt;
+  t.nck = (nsteps + ckpt - 1) / ckpt;
+  t.it0 = *itime;
+  t.mo = scaled_motion(h, ub);
   const size_t W = (size_t)E->B * E->fbs(), Q = (size_t)E->B * E->qbs();
-  DevBuf<T> CW, CRO, CQ, SW, SQ, FW, FRO, FQ;
-  CW.alloc(W * nck); CRO.alloc(W * nck); CQ.alloc(Q * nck);
-  SW.alloc(W * ckpt); SQ.alloc(Q * ckpt);
-  FW.alloc(W); FRO.alloc(W); FQ.alloc(Q);
+  t.CW.alloc(W * t.nck); t.CRO.alloc(W * t.nck); t.CQ.alloc(Q * t.nck);
+  t.FW.alloc(W); t.FRO.alloc(W); t.FQ.alloc(Q);
   auto d2d = [&](T* dst, const T* src, size_t n) {
     HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
   };
-  // forward sweep with checkpoints
-  std::vector<double> mo = scaled_motion(h, ub);
-  for (int c = 0; c < h->ncases; c++) J[c] = 0.0;
-  std::vector<double> hist((size_t)ckpt * h->rowlen);
-  std::vector<int> ck_it(nck);
-  int it = it0;
-  for (int s = 0; s < nck; s++) {
-    d2d(CW.p + W * s, E->omega.p, W);
-    d2d(CRO.p + W * s, E->rhs_old.p, W);
-    d2d(CQ.p + Q * s, E->q.p, Q);
-    ck_it[s] = it;
-    const int ns = std::min(ckpt, N - s * ckpt);
-    const int itb = it;
-    advance_impl(h, &it, ns, mo.data(), hist.data());
-    accumulate_objective(h, hist.data(), ns, itb + 1, it0 + nspin, nwin, cd, cl, J);
+  std::vector<double> rows((size_t)ckpt * h->rowlen);
+  t.ck_it.resize(t.nck);
+  int it = *itime;
+  for (int s = 0; s < t.nck; s++) {
+    d2d(t.CW.p + W * s, E->omega.p, W);
+    d2d(t.CRO.p + W * s, E->rhs_old.p, W);
+    d2d(t.CQ.p + Q * s, E->q.p, Q);
+    t.ck_it[s] = it;
+    const int ns = std::min(ckpt, nsteps - s * ckpt);
+    advance_impl(h, &it, ns, t.mo.data(), rows.data());
+    if (hist) std::memcpy(hist + (size_t)s * ckpt * h->rowlen, rows.data(), sizeof(double) * ns * h->rowlen);
   }
-  d2d(FW.p, E->omega.p, W);
-  d2d(FRO.p, E->rhs_old.p, W);
-  d2d(FQ.p, E->q.p, Q);
-  // reverse sweep: recompute each segment's states, then step back through it
+  d2d(t.FW.p, E->omega.p, W);
+  d2d(t.FRO.p, E->rhs_old.p, W);
+  d2d(t.FQ.p, E->q.p, Q);
+  HIPCHK(hipStreamSynchronize(st));
+  *itime = it;
+  h->tape = std::move(tp);
+}
+
+// reverse sweep over the handle's tape: ghist holds the cotangents of the
+// tape's nsteps history rows; each segment's states are recomputed from its
+// checkpoint, then the segment is stepped back.  grad = dJ/du_B.  The handle
+// ends at the forward sweep's final state.
+template <typename T>
+void adjoint_reverse_impl(ibfs_handle* h, Engine<T>* E, const double* ghist, double* grad) {
+  auto* tp = dynamic_cast<Tape<T>*>(h->tape.get());
+  if (!tp) throw ArgError{IBFS_ESTATE, "no forward sweep to reverse: call ibfs_adjoint_forward first"};
+  Tape<T>& t = *tp;
+  hipStream_t st = h->stream;
+  const int rl = h->rowlen;
+  const size_t W = (size_t)E->B * E->fbs(), Q = (size_t)E->B * E->qbs();
+  DevBuf<T> SW, SQ;
+  SW.alloc(W * t.ckpt);
+  SQ.alloc(Q * t.ckpt);
+  DevBuf<double> GH;
+  GH.alloc((size_t)t.N * rl);
+  GH.upload(ghist, (size_t)t.N * rl, st);
+  std::vector<char> nonzero(t.N, 0);
+  for (int r = 0; r < t.N; r++)
+    for (int c = 0; c < rl && !nonzero[r]; c++) nonzero[r] = ghist[(size_t)r * rl + c] != 0.0;
+  auto d2d = [&](T* dst, const T* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+  };
+  // the motion of the forward sweep, for the recomputed steps
+  for (int c = 0; c < E->B; c++) E->set_motion(c, t.mo.data() + E->hcs[c].foff);
+  E->enqueue_motion();
   AdjointRunner<T> R(*E);
-  for (int s = nck - 1; s >= 0; s--) {
-    d2d(E->omega.p, CW.p + W * s, W);
-    d2d(E->rhs_old.p, CRO.p + W * s, W);
-    d2d(E->q.p, CQ.p + Q * s, Q);
-    const int ns = std::min(ckpt, N - s * ckpt);
+  for (int s = t.nck - 1; s >= 0; s--) {
+    d2d(E->omega.p, t.CW.p + W * s, W);
+    d2d(E->rhs_old.p, t.CRO.p + W * s, W);
+    d2d(E->q.p, t.CQ.p + Q * s, Q);
+    const int ns = std::min(t.ckpt, t.N - s * t.ckpt);
     for (int j = 0; j < ns; j++) {
       d2d(SW.p + W * j, E->omega.p, W);
       d2d(SQ.p + Q * j, E->q.p, Q);
-      E->enqueue_step(ck_it[s] + j + 1 == 1);
+      E->enqueue_step(t.ck_it[s] + j + 1 == 1);
     }
     for (int j = ns - 1; j >= 0; j--) {
-      const int n = ck_it[s] + j + 1;  // itime of this step
-      R.step(SW.p + W * j, SQ.p + Q * j, n == 1, n > it0 + nspin, cd, cl, 1.0 / nwin);
+      const int n = t.ck_it[s] + j + 1;  // itime of this step
+      const int r = n - t.it0 - 1;       // its row in the tape
+      R.step(SW.p + W * j, SQ.p + Q * j, n == 1, nonzero[r] ? GH.p + (size_t)r * rl : nullptr);
     }
     HIPCHK(hipStreamSynchronize(st));
   }
   // back to the final forward state
-  d2d(E->omega.p, FW.p, W);
-  d2d(E->rhs_old.p, FRO.p, W);
-  d2d(E->q.p, FQ.p, Q);
+  d2d(E->omega.p, t.FW.p, W);
+  d2d(E->rhs_old.p, t.FRO.p, W);
+  d2d(E->q.p, t.FQ.p, Q);
   R.AU.download(grad, E->nf_tot, st);
   HIPCHK(hipStreamSynchronize(st));
-  *itime = it;
+}
+
+// the history-row cotangent of the mean-force objective J (cd * x forces +
+// cl * y forces of every body, averaged over the window rows)
+std::vector<double> mean_force_rows(ibfs_handle* h, int N, int it0, int nspin, int nwin, double cd, double cl) {
+  std::vector<double> g((size_t)N * h->rowlen, 0.0);
+  std::vector<int> col0(h->ncases);
+  ibfs_ensemble_info(h, nullptr, nullptr, col0.data());
+  for (int r = 0; r < N; r++) {
+    if (it0 + r + 1 <= it0 + nspin) continue;
+    for (int c = 0; c < h->ncases; c++) {
+      const CaseHost& k = h->ch(c);
+      for (int b = 0; b < k.nbody; b++) {
+        g[(size_t)r * h->rowlen + col0[c] + b] = cd / nwin;
+        g[(size_t)r * h->rowlen + col0[c] + k.nbody + b] = cl / nwin;
+      }
+    }
+  }
+  return g;
+}
+
+// checkpointed adjoint run of ibfs_adjoint_gradient on the single group E:
+// the forward sweep, the mean-force cotangent, the reverse sweep
+template <typename T>
+void adjoint_impl(ibfs_handle* h, Engine<T>* E, int* itime, int nspin, int nwin, int ckpt, const double* ub,
+                  double cd, double cl, double* J, double* grad) {
+  const int N = nspin + nwin, it0 = *itime;
+  std::vector<double> hist((size_t)N * h->rowlen);
+  adjoint_forward_impl(h, E, itime, N, ckpt, ub, hist.data());
+  for (int c = 0; c < h->ncases; c++) J[c] = 0.0;
+  accumulate_objective(h, hist.data(), N, it0 + 1, it0 + nspin, nwin, cd, cl, J);
+  const std::vector<double> g = mean_force_rows(h, N, it0, nspin, nwin, cd, cl);
+  adjoint_reverse_impl(h, E, g.data(), grad);
+  h->tape.reset();
+}
+
+// vector-Jacobian product of one step of a one-case handle (see ibfs_step_vjp)
+template <typename T>
+void step_vjp_impl(ibfs_handle* h, Engine<T>* E, int itime, const double* omega, const double* q,
+                   const double* a_omega, const double* a_rhs_old, const double* a_q, const double* a_row,
+                   double* g_omega, double* g_rhs_old, double* g_q, double* g_motion) {
+  if (h->ncases != 1) throw ArgError{IBFS_EINVAL, "ibfs_step_vjp takes a one-case handle"};
+  check_adjoint_geometry(E);
+  hipStream_t st = h->stream;
+  const size_t W = (size_t)E->fbs(), Q = (size_t)E->qbs();
+  DevBuf<T> SW, SQ;
+  SW.alloc(W);
+  SQ.alloc(Q);
+  E->put(SW, 0, omega, W);
+  E->put(SQ, 0, q, Q);
+  AdjointRunner<T> R(*E);
+  if (a_omega) E->put(R.AW, 0, a_omega, W);
+  if (a_rhs_old) E->put(R.ARO, 0, a_rhs_old, W);
+  if (a_q) E->put(R.AQ, 0, a_q, Q);
+  DevBuf<double> GR;
+  if (a_row) {
+    GR.alloc(std::max(h->rowlen, 1));
+    GR.upload(a_row, h->rowlen, st);
+  }
+  R.step(SW.p, SQ.p, itime == 1, a_row ? GR.p : nullptr);
+  if (g_omega) E->get(R.AW, 0, g_omega, W);
+  if (g_rhs_old) E->get(R.ARO, 0, g_rhs_old, W);
+  if (g_q) E->get(R.AQ, 0, g_q, Q);
+  if (g_motion) R.Z.download(g_motion, E->nf_tot, st);
+  HIPCHK(hipStreamSynchronize(st));
 }
 
 }  // namespace
@@ -578,6 +696,41 @@ int ibfs_tangent_linear(ibfs_handle* h, int* itime, int nspin, int nwin, const d
     GroupBase* G = single_group(h);
     if (auto* E = dynamic_cast<Engine<double>*>(G)) tangent_impl(h, E, itime, nspin, nwin, ub, dub, cd, cl, J, dJ);
     else tangent_impl(h, dynamic_cast<Engine<float>*>(G), itime, nspin, nwin, ub, dub, cd, cl, J, dJ);
+  });
+}
+
+int ibfs_adjoint_forward(ibfs_handle* h, int* itime, int nsteps, int ckpt, const double* ub, double* hist) {
+  if (!h || !itime || nsteps < 1 || ckpt < 1) return IBFS_EINVAL;
+  return guarded(h, [&] {
+    if (!h->all_factors()) throw ArgError{IBFS_ESTATE, "no Cholesky factor"};
+    h->tape.reset();
+    GroupBase* G = single_group(h);
+    if (auto* E = dynamic_cast<Engine<double>*>(G)) adjoint_forward_impl(h, E, itime, nsteps, ckpt, ub, hist);
+    else adjoint_forward_impl(h, dynamic_cast<Engine<float>*>(G), itime, nsteps, ckpt, ub, hist);
+  });
+}
+
+int ibfs_adjoint_reverse(ibfs_handle* h, const double* ghist, double* grad) {
+  if (!h || !ghist || !grad) return IBFS_EINVAL;
+  return guarded(h, [&] {
+    GroupBase* G = single_group(h);
+    if (auto* E = dynamic_cast<Engine<double>*>(G)) adjoint_reverse_impl(h, E, ghist, grad);
+    else adjoint_reverse_impl(h, dynamic_cast<Engine<float>*>(G), ghist, grad);
+  });
+}
+
+int ibfs_step_vjp(ibfs_handle* h, int itime, const double* omega, const double* q, const double* a_omega,
+                  const double* a_rhs_old, const double* a_q, const double* a_row, double* g_omega,
+                  double* g_rhs_old, double* g_q, double* g_motion) {
+  if (!h || itime < 1 || !omega || !q) return IBFS_EINVAL;
+  return guarded(h, [&] {
+    if (!h->all_factors()) throw ArgError{IBFS_ESTATE, "no Cholesky factor"};
+    GroupBase* G = single_group(h);
+    if (auto* E = dynamic_cast<Engine<double>*>(G))
+      step_vjp_impl(h, E, itime, omega, q, a_omega, a_rhs_old, a_q, a_row, g_omega, g_rhs_old, g_q, g_motion);
+    else
+      step_vjp_impl(h, dynamic_cast<Engine<float>*>(G), itime, omega, q, a_omega, a_rhs_old, a_q, a_row, g_omega,
+                    g_rhs_old, g_q, g_motion);
   });
 }
 
@@ -657,6 +810,43 @@ __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, lon
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
 }  // namespace
+
+int ibfs_time_step_kernel(ibfs_handle* h, int nsteps, double* us_mean, int* launches) {
+  if (!h || nsteps < 1 || !us_mean) return IBFS_EINVAL;
+  return guarded(h, [&] {
+    Engine<double>* E = h->e64();
+    if (!E || h->groups.size() != 1) throw ArgError{IBFS_EINVAL, "the step-kernel probe needs an FP64 one-group handle"};
+    if (!h->stationary || !h->all_factors()) throw ArgError{IBFS_ESTATE, "the probe steps need the factor form"};
+    typename Engine<double>::Probe pr;
+    pr.ev.resize((size_t)nsteps * 4 * E->Lmax + 8);
+    for (auto& e : pr.ev) {
+      HIPCHK(hipEventCreate(&e.first));
+      HIPCHK(hipEventCreate(&e.second));
+    }
+    struct Cleanup {
+      ibfs_handle* h; Engine<double>* E; typename Engine<double>::Probe* pr; bool g;
+      ~Cleanup() {
+        E->probe = nullptr;
+        h->use_graph = g;
+        for (auto& e : pr->ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+      }
+    } cleanup{h, E, &pr, h->use_graph};
+    std::vector<double> hist((size_t)nsteps * h->rowlen);
+    h->use_graph = false;  // eager launches: events cannot ride in a captured graph
+    E->probe = &pr;
+    int itime = 1;  // the steps are never the Euler start
+    advance_impl(h, &itime, nsteps, nullptr, hist.data());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double sum = 0.0;
+    for (size_t k = 0; k < pr.next; k++) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, pr.ev[k].first, pr.ev[k].second));
+      sum += ms;
+    }
+    *us_mean = pr.next ? 1e3 * sum / (double)pr.next : 0.0;
+    if (launches) *launches = (int)pr.next;
+  });
+}
 
 int ibfs_time_copy(ibfs_handle* h, long n, int reps, double* ms_per_launch) {
   if (!h || !ms_per_launch || reps < 1 || n < 1) return IBFS_EINVAL;

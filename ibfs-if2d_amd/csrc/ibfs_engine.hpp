@@ -7,6 +7,7 @@
 // force system (right-hand side accumulation, Cholesky solve, one refinement
 // sweep) and every reduction in FP64.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -124,6 +124,19 @@ struct Engine : GroupBase {
   long mat_tot = 0, gmat_tot = 0;
   bool refine = false;
   Engine* base = nullptr;  // set on a tangent-linear engine: the state it is linearised about
+  // Step 1 with the levels batched (enqueue_step1_levels): every level's
+  // bulk solve in one set of launches over the flattened (case, level)
+  // items (geoL: one level per item), then every level's correction for the
+  // new coarser edge values in another.  T1L, T2L: one field per item.
+  bool lev_batch = false;
+  Geo geoL{};
+  DevBuf<T> T1L, T2L;
+  // the band correction (k_band_coef, k_band_add): beta per item, the band
+  // width per level, the coefficients c_k rho_k^(d+1) per item, edge, d, k
+  DevBuf<double> band_beta, band_C;
+  DevBuf<int> band_w;
+  std::vector<int> hband_w;
+  int band_wmax = 0, band_nmax = 0;
 
   long fbs() const { return (long)Lmax * nw; }
   long tbs() const { return (long)Lmax * nwp; }  // case stride of lam1i (pair-major tables)
@@ -253,6 +266,65 @@ struct Engine : GroupBase {
     ensure_diag(1);
     HIPCHK(hipStreamSynchronize(st));
     tune();
+    lev_batch = levels_can_batch();
+    if (lev_batch) {
+      geoL = geo;
+      geoL.Lmax = 1;
+      T1L.alloc((size_t)B * Lmax * nw);
+      T2L.alloc((size_t)B * Lmax * nw);
+      std::vector<double> hb((size_t)B * Lmax, 1.0);
+      for (int b = 0; b < B; b++)
+        for (int k = 0; k < Lmax; k++) {
+          const double dk = cs[b].delta * std::pow(2.0, k);
+          hb[(size_t)b * Lmax + k] = 0.5 * cs[b].d.dt / (cs[b].d.re * dk * dk);
+        }
+      upvec(band_beta, hb, st);
+      upvec(band_w, hband_w, st);
+      band_wmax = *std::max_element(hband_w.begin(), hband_w.end());
+      band_nmax = std::max(m, n);
+      band_C.alloc((size_t)B * Lmax * 4 * band_wmax * band_nmax);
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  }
+
+  // Whether step 1 may run with the levels batched.  The reference solves
+  // the levels coarse to fine because level k's edge values come from the
+  // new vorticity of level k+1 (src/operators.f90:103-107).  Those edge
+  // values enter linearly, so level k's new vorticity is its bulk solve
+  // (old edge values, nonlinear term) plus the response
+  // (I + beta_k C^T C)^-1 applied to the edge source, and level k samples
+  // level k+1 (get_bc) only on the lines i = m/4, 3m/4, j = n/4, 3n/4 of
+  // level k+1, at least min(m,n)/4 cells from its edges.  Level k+1's own
+  // edge response decays from its edges by at most r = exp(-acosh(1 +
+  // 1/(2 beta))) per cell, beta = dt/(2 Re Delta_{k+1}^2) (the slowest mode
+  // of the 1-D recurrence normal to an edge), so it is below r^(min(m,n)/4)
+  // of the edge source there: the batched form takes the bulk solve of level
+  // k+1 for level k's edge values when that factor is below 1e-17 on every
+  // level with a finer level, for every case.  It is used for one or two
+  // cases with all levels (batches are throughput-bound; one level's extra
+  // transform pair would cost more than the launches it saves) and can be
+  // switched off with IBFS_STEP1_LEVELS=0.
+  // The correction itself is written on the w_k rows and columns next to
+  // each edge of level k, w_k = ceil(ln(1e17) / kappa_k) for the level's own
+  // beta (the dropped terms are below 1e-17 of the edge source), which must
+  // stay under a quarter of the grid (hband_w; filled here).
+  bool levels_can_batch() {
+    const char* env = std::getenv("IBFS_STEP1_LEVELS");
+    if (env && env[0] == '0') return false;
+    if (gate != nullptr || Lmax < 2 || B > 2) return false;
+    const double need = std::log(1e17);
+    hband_w.assign(Lmax, 1);
+    for (const CaseHost& c : cs)
+      for (int k = 0; k < c.L; k++) {
+        const double dk = c.delta * std::pow(2.0, k);
+        const double beta = 0.5 * c.d.dt / (c.d.re * dk * dk);
+        const double kappa = std::acosh(1.0 + 1.0 / (2.0 * beta));
+        if (k >= 1 && kappa * (std::min(m, n) / 4) < need) return false;
+        if (k + 1 < c.L) hband_w[k] = std::max(hband_w[k], (int)std::ceil(need / kappa));
+      }
+    for (int w : hband_w)
+      if (4 * w >= std::min(m, n)) return false;
+    return true;
   }
   void tile_shapes(int* out) const override {
     for (int k = 0; k < kKinds; k++) {  // 0: the kind runs the generic pass
@@ -299,10 +371,27 @@ struct Engine : GroupBase {
     using V = op_value_t<Op>;
     if constexpr (p_fits<V>(NN, P)) {
       const int groups = (nseq + mult * P - 1) / (mult * P);
+      if constexpr (std::is_same_v<Op, Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>>) {
+        if (probe && probe->next < probe->ev.size()) {  // measurement: events on the dispatch itself
+          auto& e = probe->ev[probe->next++];
+          hipExtLaunchKernelGGL((spec::dst_col2_n<V, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st,
+                                e.first, e.second, 0, op, fset<V>().stw_n.p);
+          return;
+        }
+      }
       hipLaunchKernelGGL((spec::dst_col2_n<V, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
                          fset<V>().stw_n.p);
     }
   }
+  // Measurement only: with probe set (eager steps, never a captured graph),
+  // every launch of the ctci column pair (the step's dominant kernel) carries
+  // a start and a stop event on its own dispatch, so their difference is the
+  // kernel's duration inside the step.
+  struct Probe {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t next = 0;
+  };
+  Probe* probe = nullptr;
 
   // Launch one pass over nseq sequences along the other axis; mult = 2 when
   // a workgroup's P FFTs take 2P sequences of one field, 1 when they pair
@@ -586,8 +675,63 @@ struct Engine : GroupBase {
     enqueue_step34();
   }
 
+  // STEP 1 with the levels batched (see levels_can_batch): the head kernel
+  // (lastbc of every level, and the previous step's pending diagnostics
+  // row), the bulk solve of every level with its old edge values (a row
+  // pass, the lam1i column pair and an adding row pass over all (case,
+  // level) items), then every level's correction for the new edge values of
+  // its coarser level, from that level's bulk solve (k_band_coef,
+  // k_band_add)
+  void enqueue_step1_levels(int first) {
+    const int BL = B * Lmax;
+    const bool defer = diag_deferred();
+    const int nlb = (nbc * Lmax + 255) / 256;
+    hipLaunchKernelGGL(k_step_head<T>, dim3(nlb + (defer ? 3 : 0), B), dim3(256), 0, st, geo, omega.p, lastbc.p,
+                       gate, diag_args(dpend.p), defer ? 1 : 0);
+    if (q0_uniform()) {
+      SrcStep1<T, true> s1f{omega.p, nullptr, q.p, nullptr, lastbc.p, rhs_old.p, con1.p, con2.p, vfac.p,
+                            nullptr, (long)nw, (long)nq, (long)nbc, 0, first};
+      s1f.q0c = q0c.p;
+      launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geoL, s1f, DstField<T>{T1L.p, nw}, nullptr, 0}, geo.nm1,
+                2, BL, kStep1Row);
+    } else {
+      SrcStep1<T> s1{omega.p, nullptr, q.p, q0.p, lastbc.p, rhs_old.p, con1.p, con2.p, vfac.p,
+                     nullptr, (long)nw, (long)nq, (long)nbc, 0, first};
+      launch<0>(RowOp<T, SrcStep1<T>, DstField<T>>{geoL, s1, DstField<T>{T1L.p, nw}, nullptr, 0}, geo.nm1, 2, BL,
+                kStep1Row);
+    }
+    step1_levels_add();
+    // the correction for the new coarser edge values, on the bands only: the
+    // edge spectra, then one DST-I per band row and per band column
+    const int nch = (band_nmax - 1 + 63) / 64;
+    hipLaunchKernelGGL(k_band_coef<T>, dim3(4 * nch, BL), dim3(256), sizeof(double) * 3 * band_nmax, st, geoL,
+                       omega.p, Lmax, band_beta.p, band_w.p, band_wmax, band_nmax, band_C.p);
+    const BandBase<T> bb{band_C.p, omega.p, geoL, Lmax, band_w.p, band_wmax, band_nmax};
+    Geo gRb = geoL, gCb = geoL;
+    gRb.nm1 = 2 * band_wmax;  // sequences of the row pass: the bottom and top band rows
+    gCb.mm1 = 2 * band_wmax;  // sequences of the column pass: the left and right band columns
+    launch<0>(RowOp<T, SrcBandRow<T>, DstBandRow<T>>{gRb, SrcBandRow<T>{bb}, DstBandRow<T>{bb}, nullptr, 0},
+              gRb.nm1, 2, BL, kRow);
+    launch<1>(ColOp<T, SrcBandCol<T>, DstBandCol<T>>{gCb, SrcBandCol<T>{bb}, DstBandCol<T>{bb}, nullptr, 0},
+              gCb.mm1, 2, BL, kCol);
+  }
+  // omega += S (lam1i * S T1L) over every (case, level) item
+  void step1_levels_add() {
+    const int BL = B * Lmax;
+    launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{geoL, SrcField<T>{T1L.p, nw}, MidTab<T>{lam1i.p, nwp},
+                                                               DstField<T>{T2L.p, nw}, nullptr, 0},
+                geo.mm1, 2, BL, kStep1Col2);
+    launch<0>(RowOp<T, SrcField<T>, DstAdd<T>>{geoL, SrcField<T>{T2L.p, nw}, DstAdd<T>{omega.p, nw, T(1)}, nullptr,
+                                               0},
+              geo.nm1, 2, BL);
+  }
+
   // lastbc and STEP 1 of advance
   void enqueue_step1(int first) {
+    if (lev_batch && !base) {
+      enqueue_step1_levels(first);
+      return;
+    }
     const long FB = fbs(), QB = qbs(), BB = (long)Lmax * nbc;
     // freestream form: lastbc is side work of the first row pass (the
     // coarsest slot, which reads no level's lastbc that it writes)
@@ -711,7 +855,9 @@ struct Engine : GroupBase {
   // pass exists: the freestream form with lastbc as side work, and a
   // workgroup size of a multiple of 64 for the row's reductions.
   bool diag_deferred() const {
-    if (!side_diag || base || !q0_uniform() || Lmax < 2) return false;
+    if (!side_diag || base) return false;
+    if (lev_batch) return true;  // the head kernel of the level-batched step 1 writes it
+    if (!q0_uniform() || Lmax < 2) return false;
     if (!fs.spec_m) return true;  // the generic pass: 256 threads
     const int nt = ptile[kStep1Row] * fs.pm.n2 / 4;
     return nt % 64 == 0;
@@ -849,6 +995,8 @@ struct Engine : GroupBase {
     build_smear(h);
     upload_smear();
     cg_drop_graphs();  // they hold the old tables
+    cg_moved = true;   // A follows the body from now on: a_times in every CG product
+    cg_have_dense = false;
     Gm.alloc(std::max<long>(gmat_tot, 1));  // rot(reg(A^-1)) rows follow the new vertex set
     h.have_chol = false;
     RR.zero(st);
@@ -868,7 +1016,20 @@ struct Engine : GroupBase {
   // one CG iteration as a graph: [0] r -= alpha q, [1] the residual
   // recomputed from b - A x (every 50th iteration)
   hipGraphExec_t cg_exec[2] = {nullptr, nullptr};
-  static constexpr int kCgBatch = 8;  // iterations replayed between two read-backs
+  static constexpr int kCgBatch = 16;  // a_times form: iterations replayed between two read-backs
+  // When no point moves, A is the same every step: it is built once on the
+  // device (the columns of preprocess, build_columns) and each CG product is
+  // a dense GEMV (k_gemv_dense) instead of the a_times composition.
+  DevBuf<double> cgA, cgPart;
+  int cg_nslice = 1;
+  bool cg_have_dense = false;
+  bool cg_moved = false;  // move_body was called: A changes, a_times every product
+  bool cg_dense() const {
+    if (cg_moved) return false;
+    for (int b : cs[0].d.bdyno)
+      if (b > 0) return false;
+    return true;
+  }
 
   void cg_alloc() {
     if (cg_host) return;
@@ -894,20 +1055,59 @@ struct Engine : GroupBase {
     if (cg_host) (void)hipHostFree(cg_host);
     cg_host = nullptr;
   }
+  // The a_times form: one graph per iteration kind ([0] r -= alpha q, [1]
+  // r = b - A x).  The dense form: one graph of kCgDense iterations, whose
+  // kernels read the iteration count and the done flag on the device (the
+  // product A x of a residual recompute is skipped in other iterations).
+  static constexpr int kCgDense = 32;
   void cg_capture() {
     const int nf = cs[0].nf;
+    if (cg_have_dense) {
+      if (cg_exec[0]) return;
+      hipGraph_t gr = nullptr;
+      const dim3 gg((nf + 255) / 256, cg_nslice);
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < kCgDense; k++) {
+        hipLaunchKernelGGL(k_gemv_dense, gg, dim3(256), 0, st, nf, cgA.p, cg_d.p, cgPart.p, cg_nslice,
+                           (const CgState*)cg_dev.p, 0);
+        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, nf, cg_d.p, cg_q.p, fb.p, cg_dev.p,
+                           (const double*)cgPart.p, cg_nslice);
+        hipLaunchKernelGGL(k_gemv_dense, gg, dim3(256), 0, st, nf, cgA.p, (const double*)fb.p, cgPart.p, cg_nslice,
+                           (const CgState*)cg_dev.p, 1);
+        hipLaunchKernelGGL(k_cg_resid, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p, -1,
+                           (const double*)cgPart.p, cg_nslice);
+      }
+      HIPCHK(hipStreamEndCapture(st, &gr));
+      HIPCHK(hipGraphInstantiate(&cg_exec[0], gr, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(gr));
+      return;
+    }
     for (int v = 0; v < 2; v++) {
       if (cg_exec[v]) continue;
       hipGraph_t gr = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       a_times_one(cg_d.p, cg_q.p);
-      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, nf, cg_d.p, cg_q.p, fb.p, cg_dev.p);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, nf, cg_d.p, cg_q.p, fb.p, cg_dev.p,
+                         (const double*)nullptr, 1);
       if (v) a_times_one(fb.p, cg_q.p);
-      hipLaunchKernelGGL(k_cg_resid, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p, v);
+      hipLaunchKernelGGL(k_cg_resid, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p, v,
+                         (const double*)nullptr, 1);
       HIPCHK(hipStreamEndCapture(st, &gr));
       HIPCHK(hipGraphInstantiate(&cg_exec[v], gr, nullptr, nullptr, 0));
       HIPCHK(hipGraphDestroy(gr));
     }
+  }
+  // A of the current geometry on the device, once (the graphs of the
+  // a_times form are dropped)
+  void cg_build_dense() {
+    if (cg_have_dense || !cg_dense()) return;
+    const int nf = cs[0].nf;
+    cgA.alloc((size_t)nf * nf);
+    build_columns(0, cgA.p);
+    cg_nslice = (nf + kGemvCols - 1) / kGemvCols;  // at most kGemvCols columns per slice
+    cgPart.alloc((size_t)cg_nslice * nf);
+    cg_drop_graphs();
+    cg_have_dense = true;
   }
 
   // out = a_times(x) (src/operators.f90:444-462) for case 0: ainv(rot(reg(x)))
@@ -936,13 +1136,27 @@ struct Engine : GroupBase {
   int cg_solve(double cgtol, int cg_max_iter) {
     const int nf = cs[0].nf;
     if (nf == 0) return 0;
+    cg_build_dense();
     cg_capture();
     *cg_host = CgState{};
     cg_host->eps = cgtol * cgtol;
     cg_host->max_iter = cg_max_iter;
     HIPCHK(hipMemcpyAsync(cg_dev.p, cg_host, sizeof(CgState), hipMemcpyHostToDevice, st));
+    if (cg_have_dense) {
+      hipLaunchKernelGGL(k_gemv_dense, dim3((nf + 255) / 256, cg_nslice), dim3(256), 0, st, nf, cgA.p,
+                         (const double*)fb.p, cgPart.p, cg_nslice, (const CgState*)nullptr, 0);
+      hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p,
+                         (const double*)cgPart.p, cg_nslice);
+      CgState c = cg_read();
+      for (int guard = 0; !c.done && guard <= cg_max_iter; guard += kCgDense) {
+        HIPCHK(hipGraphLaunch(cg_exec[0], st));
+        c = cg_read();
+      }
+      return c.iter;
+    }
     a_times_one(fb.p, cg_q.p);
-    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(256), 0, st, nf, rhsf.p, cg_q.p, cg_r.p, cg_d.p, cg_dev.p,
+                       (const double*)nullptr, 1);
     CgState c = cg_read();
     for (int guard = 0; !c.done && guard <= cg_max_iter; guard += kCgBatch) {
       for (int k = 0; k < kCgBatch; k++) HIPCHK(hipGraphLaunch(cg_exec[(c.iter + k + 1) % 50 == 0], st));

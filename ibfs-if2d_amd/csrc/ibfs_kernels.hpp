@@ -268,6 +268,28 @@ struct DiagArgs {
 template <typename T>
 __device__ void diag_row(const Geo& g, const DiagArgs<T>& a, int b, double* red, int part);
 
+// The head of a level-batched step: lastbc of every level (as k_lastbc) in
+// workgroups x < ceil(nbc Lmax / 256), and with has_diag the previous step's
+// pending diagnostics row in three more (diag_row parts 0-2), whose inputs
+// no launch of the step changes before k_tail.  grid (.., B), 256 threads.
+template <typename T>
+__global__ __launch_bounds__(256) void k_step_head(Geo g, const T* __restrict__ om, T* __restrict__ lastbc,
+                                                   const int* __restrict__ Lc, DiagArgs<T> dg, int has_diag) {
+  const int nlb = (g.nbc * g.Lmax + 255) / 256, b = blockIdx.y;
+  if ((int)blockIdx.x >= nlb) {
+    __shared__ double red[16];
+    if (has_diag) diag_row<T>(g, dg, b, red, blockIdx.x - nlb);
+    return;
+  }
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= g.nbc * g.Lmax) return;
+  const int lev = t / g.nbc, p = t - lev * g.nbc;
+  const long fb = (long)b * g.Lmax;
+  T v = T(0);
+  if (has_coarser(g, Lc, b, lev)) v = getbc_at<T>(g, om + (fb + lev + 1) * g.nw, p + 1, T(0.25));
+  lastbc[(fb + lev) * g.nbc + p] = v;
+}
+
 // ---------------------------------------------------------------------------
 // pass operators (see dst_core.hpp for the load/store protocol)
 // ---------------------------------------------------------------------------
@@ -760,6 +782,116 @@ struct MidTab {
   __device__ T coef(const Geo& g, int b, int ii, int jj) const { return tab[b * tbs + pm_ix(g, ii, jj)]; }
   __device__ T apply(T c, T v) const { return c * v; }
 };
+// Level-batched step 1, the correction for the new coarser edge values.
+// Level k's right-hand side gets vfac_k apply_bc(get_bc(omega_{k+1}, 0.25))
+// (src/operators.f90:103-107,122-123) on its first and last interior rows
+// and columns; through S lam1i S that is the field
+//   d = beta (I + beta C^T C)^-1 e,  beta = dt / (2 Re Delta_k^2)
+// (4mn vfac_k = beta) with e the edge source.  C^T C separates in sines:
+// for the bottom edge with values e_t (t = 0..m-2 along x),
+//   d(ii, jj) = sum_k c_k rho_k^(jj+1) sin(pi k (ii+1) / m),
+//   c_k = (2/m) sum_t e_t sin(pi k (t+1) / m),
+//   rho_k + 1/rho_k = (1 + beta (lx_k + 2)) / beta, lx_k = 2 - 2 cos(pi k / m)
+// (exact but for the reflection at the far edge, rho^(2n-2)), and likewise
+// for the other three edges (top: jj -> n-2-jj; left/right: the roles of x
+// and y swapped).  rho_k < rho_1 < 1e-17^(1/w): only the w rows and columns
+// next to each edge are written (the host picks w per level).
+//
+// k_band_coef: the spectra of the four edges of every (case, level) item
+// with a coarser level.  Workgroup (edge, chunk) of item y: 64 modes, each
+// summed by 4 threads over a quarter of the edge; it writes
+// C[((item * 4 + edge) * wmax + d) * nmax + k-1] = (c_k / 2) rho_k^(d+1) for
+// d < w, the input of one DST-I (S y = 2 sum y_k sin) per band row or
+// column (SrcBand*, DstBand*).  grid (4 ceil((nmax-1)/64), items), 256
+// threads; LDS: the edge values and a sine table (3 nmax doubles).
+template <typename T>
+__global__ __launch_bounds__(256) void k_band_coef(Geo g, const T* __restrict__ om, int lm,
+                                                   const double* __restrict__ beta, const int* __restrict__ wlev,
+                                                   int wmax, int nmax, double* __restrict__ C) {
+  extern __shared__ double sh[];
+  __shared__ double red[256];
+  const int item = blockIdx.y, lev = item % lm;
+  if (lev == lm - 1) return;
+  const int nch = (nmax - 1 + 63) / 64, edge = blockIdx.x / nch, chunk = blockIdx.x - edge * nch;
+  const int N = edge < 2 ? g.m : g.n, w = wlev[lev];
+  if (chunk * 64 + 1 >= N) return;  // the whole workgroup is past this edge's modes
+  double* ev = sh;          // N-1 edge values
+  double* tab = sh + nmax;  // sin(pi s / N), s = 0..2N-1
+  const T* rc = om + (long)(item + 1) * g.nw;  // the coarser level's new (bulk) vorticity
+  const int base = edge == 0 ? g.bottom : edge == 1 ? g.top : edge == 2 ? g.left : g.right;
+  for (int t = threadIdx.x; t < N - 1; t += 256) ev[t] = double(getbc_sel<T>(g, rc, base + t + 2, T(0.25)));
+  for (int s = threadIdx.x; s < 2 * N; s += 256) tab[s] = sinpi((double)s / (double)N);
+  __syncthreads();
+  const int k = chunk * 64 + (threadIdx.x & 63) + 1, part = threadIdx.x >> 6;
+  const int q = (N - 1 + 3) / 4, t0 = part * q, t1 = min(N - 1, t0 + q);
+  double acc = 0.0;
+  if (k < N && t0 < t1) {
+    int idx = (int)(((long)k * (t0 + 1)) % (2 * N));
+    for (int t = t0; t < t1; t++) {
+      acc += ev[t] * tab[idx];
+      idx += k;
+      if (idx >= 2 * N) idx -= 2 * N;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (part != 0 || k >= N) return;
+  const int l = threadIdx.x;
+  const double half_c = ((red[l] + red[l + 64]) + (red[l + 128] + red[l + 192])) / N;
+  const double b = beta[item];
+  const double lx = 2.0 - 2.0 * cospi((double)k / (double)N);
+  const double th = (1.0 + b * (lx + 2.0)) / (2.0 * b);
+  const double rho = 1.0 / (th + sqrt(th * th - 1.0));
+  double pw = rho;
+  double* out = C + ((long)(item * 4 + edge) * wmax) * nmax + (k - 1);
+  for (int d = 0; d < w; d++) {
+    out[(long)d * nmax] = half_c * pw;
+    pw *= rho;
+  }
+}
+
+// The band passes: a row pass over 2 wmax sequences per item (bottom band
+// rows d, then top band rows d) and a column pass over 2 wmax sequences
+// (left band columns, then right), both with pass geometries whose sequence
+// count is 2 wmax; the loaders read C at (edge, d, k-1) and the stores add
+// the transformed row or column to omega at the band's place (gr: the
+// level's geometry).  Items of a case's coarsest level and d >= w load 0 and
+// store nothing.
+template <typename T>
+struct BandBase {
+  const double* C; T* om; Geo gr; int lm; const int* wlev; int wmax, nmax;
+  __device__ T coef(int b, int edge, int seq, int pos) const {
+    const int lev = b % lm, d = seq % wmax;
+    if (lev == lm - 1 || d >= wlev[lev]) return T(0);
+    return T(C[((long)(b * 4 + edge + seq / wmax) * wmax + d) * nmax + pos]);
+  }
+  __device__ void add(int b, int seq, int ii_or_jj, bool rows, T v) const {
+    const int lev = b % lm, d = seq % wmax, hi = seq / wmax;
+    if (lev == lm - 1 || d >= wlev[lev]) return;
+    int ii, jj;
+    if (rows) { ii = ii_or_jj; jj = hi ? gr.nm1 - 1 - d : d; }
+    else { jj = ii_or_jj; ii = hi ? gr.mm1 - 1 - d : d; }
+    T* f = om + (long)b * gr.nw + ii + (long)jj * gr.mm1;
+    *f = *f + v;
+  }
+};
+template <typename T>
+struct SrcBandRow : BandBase<T> {  // RowOp: src(g, b, position, sequence)
+  __device__ T operator()(const Geo&, int b, int pos, int seq) const { return this->coef(b, 0, seq, pos); }
+};
+template <typename T>
+struct DstBandRow : BandBase<T> {  // RowOp: dst(g, b, position, sequence, v)
+  __device__ void operator()(const Geo&, int b, int pos, int seq, T v) const { this->add(b, seq, pos, true, v); }
+};
+template <typename T>
+struct SrcBandCol : BandBase<T> {  // ColOp: src(g, b, sequence, position)
+  __device__ T operator()(const Geo&, int b, int seq, int pos) const { return this->coef(b, 2, seq, pos); }
+};
+template <typename T>
+struct DstBandCol : BandBase<T> {  // ColOp: dst(g, b, sequence, position, v)
+  __device__ void operator()(const Geo&, int b, int seq, int pos, T v) const { this->add(b, seq, pos, false, v); }
+};
+
 // a = a + scale * v (the step-1 increment; adjoint accumulations)
 template <typename T>
 struct DstAdd {
@@ -1407,6 +1539,21 @@ struct CgState {
 // round trip per chunk, and thread 0 sums the chunk); the sum is returned
 // to thread 0
 constexpr int kDotChunk = 1024;
+// a . b in a fixed order that does not depend on timing: each thread sums
+// its strided terms in index order, then the 256 partial sums are added in a
+// fixed tree (wave shuffles, then the four waves' sums in order); returned
+// to every thread.  256 threads.
+__device__ double dot_tree(int nf, const double* __restrict__ a, const double* __restrict__ b) {
+#pragma clang fp contract(off)
+  __shared__ double wsum[4];
+  double s = 0.0;
+  for (int f = threadIdx.x; f < nf; f += 256) s = s + a[f] * b[f];
+  for (int off = 32; off > 0; off >>= 1) s = s + __shfl_xor(s, off, 64);
+  __syncthreads();  // wsum is free (an earlier call may still read it)
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  return ((wsum[0] + wsum[1]) + (wsum[2] + wsum[3]));
+}
 __device__ double dot_in_order(int nf, const double* __restrict__ a, const double* __restrict__ b) {
 #pragma clang fp contract(off)
   __shared__ double sa[kDotChunk], sb[kDotChunk];
@@ -1426,17 +1573,66 @@ __device__ double dot_in_order(int nf, const double* __restrict__ a, const doubl
 }
 
 // r = b - Ax; d = r; delta_new = r . d; iter = 0 (src/operators.f90:407-413)
+constexpr int kGemvCols = 8;  // columns per slice of k_gemv_dense (nslice = ceil(nf / 8))
+// q = A x with a dense A (column-major, nf x nf) for the CG form when no
+// body moves: workgroup (r, s) takes 256 rows and the s-th of nslice column
+// slices, and writes its partial sums to part[s * nf + row]; the CG kernels
+// add the slices in slice order (form_q), so the product is deterministic.
+// grid (ceil(nf/256), nslice), 256 threads.
+// With cs set, the product is skipped once the CG loop is done and, with
+// only_resid, in every iteration but the ones that recompute the residual
+// from b - A x (every 50th), so one captured graph serves every iteration.
+static __global__ __launch_bounds__(256) void k_gemv_dense(int nf, const double* __restrict__ A,
+                                                    const double* __restrict__ x, double* __restrict__ part,
+                                                    int nslice, const CgState* __restrict__ cs, int only_resid) {
+  if (cs && (cs->done || (only_resid && cs->iter % 50 != 0))) return;
+  const int i = blockIdx.x * 256 + threadIdx.x, sl = blockIdx.y;
+  const int cw = (nf + nslice - 1) / nslice, j0 = sl * cw, j1 = min(nf, j0 + cw);
+  if (i >= nf) return;
+  // a slice is at most kGemvCols columns: every load of a thread is issued
+  // before the first product (out-of-range columns load column j0)
+  double a[kGemvCols], xv[kGemvCols];
+#pragma unroll
+  for (int u = 0; u < kGemvCols; u++) {
+    const int j = j0 + u < j1 ? j0 + u : j0;
+    a[u] = A[i + (long)j * nf];
+    xv[u] = x[j];
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int u = 0; u < kGemvCols; u++)
+    if (j0 + u < j1) acc += a[u] * xv[u];
+  part[(long)sl * nf + i] = acc;
+}
+// q = the sum of the slices of k_gemv_dense, in slice order (by the
+// threads of the calling single-workgroup CG kernel)
+__device__ __forceinline__ void form_q(int nf, const double* __restrict__ part, int nslice, double* __restrict__ q) {
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+    double acc = 0.0;
+#pragma unroll 8
+    for (int sl = 0; sl < nslice; sl++) acc += part[(long)sl * nf + f];
+    q[f] = acc;
+  }
+  __syncthreads();
+}
+
+// part != nullptr: Ax is first formed from the slices of k_gemv_dense
+// With part set (the dense form) the products come from k_gemv_dense's
+// slices and the dot products use the fixed tree (dot_tree); otherwise (the
+// a_times form) they are summed in index order like DOT_PRODUCT.
 static __global__ __launch_bounds__(256) void k_cg_start(int nf, const double* __restrict__ b,
-                                                  const double* __restrict__ Ax, double* __restrict__ r,
-                                                  double* __restrict__ d, CgState* __restrict__ cs) {
+                                                  double* __restrict__ Ax, double* __restrict__ r,
+                                                  double* __restrict__ d, CgState* __restrict__ cs,
+                                                  const double* __restrict__ part, int nslice) {
 #pragma clang fp contract(off)
+  if (part) form_q(nf, part, nslice, Ax);
   for (int f = threadIdx.x; f < nf; f += blockDim.x) {
     const double v = b[f] - Ax[f];
     r[f] = v;
     d[f] = v;
   }
   __syncthreads();
-  const double dn = dot_in_order(nf, r, d);
+  const double dn = part ? dot_tree(nf, r, d) : dot_in_order(nf, r, d);
   if (threadIdx.x == 0) {
     cs->delta_new = dn;
     cs->iter = 0;
@@ -1447,12 +1643,14 @@ static __global__ __launch_bounds__(256) void k_cg_start(int nf, const double* _
 // iter += 1; alpha = delta_new / (d . q); x = x + alpha d  (q = A d;
 // src/operators.f90:415-418)
 static __global__ __launch_bounds__(256) void k_cg_alpha(int nf, const double* __restrict__ d,
-                                                  const double* __restrict__ q, double* __restrict__ x,
-                                                  CgState* __restrict__ cs) {
+                                                  double* __restrict__ q, double* __restrict__ x,
+                                                  CgState* __restrict__ cs, const double* __restrict__ part,
+                                                  int nslice) {
 #pragma clang fp contract(off)
   __shared__ double alpha;
   if (cs->done) return;
-  const double dq = dot_in_order(nf, d, q);
+  if (part) form_q(nf, part, nslice, q);
+  const double dq = part ? dot_tree(nf, d, q) : dot_in_order(nf, d, q);
   if (threadIdx.x == 0) {
     alpha = cs->delta_new / dq;
     cs->alpha = alpha;
@@ -1466,16 +1664,21 @@ static __global__ __launch_bounds__(256) void k_cg_alpha(int nf, const double* _
 // otherwise r = r - alpha q (q = A d); then delta_old = delta_new,
 // delta_new = r . r, beta = delta_new / delta_old, d = r + beta d
 // (src/operators.f90:420-431)
+// recompute < 0: the device decides (iteration iter % 50 == 0, as cjgr)
 static __global__ __launch_bounds__(256) void k_cg_resid(int nf, const double* __restrict__ b,
-                                                  const double* __restrict__ q, double* __restrict__ r,
-                                                  double* __restrict__ d, CgState* __restrict__ cs, int recompute) {
+                                                  double* __restrict__ q, double* __restrict__ r,
+                                                  double* __restrict__ d, CgState* __restrict__ cs, int recompute,
+                                                  const double* __restrict__ part, int nslice) {
 #pragma clang fp contract(off)
   __shared__ double beta;
   if (cs->done) return;
+  const bool dense = recompute < 0;
+  if (dense) recompute = cs->iter % 50 == 0;
+  if (part && recompute) form_q(nf, part, nslice, q);  // the recomputed A x of this iteration
   const double alpha = cs->alpha;
   for (int f = threadIdx.x; f < nf; f += blockDim.x) r[f] = recompute ? b[f] - q[f] : r[f] - alpha * q[f];
   __syncthreads();
-  const double rr = dot_in_order(nf, r, r);
+  const double rr = dense ? dot_tree(nf, r, r) : dot_in_order(nf, r, r);
   if (threadIdx.x == 0) {
     const double dold = cs->delta_new;
     const double dnew = rr;

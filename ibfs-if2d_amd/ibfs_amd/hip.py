@@ -64,6 +64,7 @@ def _load():
     L.ibfs_case_set_actuators.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), _D]
     L.ibfs_op_dst_gemm.argtypes = [C.c_void_p, C.c_int, _D, _D]
     L.ibfs_time_copy.argtypes = [C.c_void_p, C.c_long, C.c_int, _D]
+    L.ibfs_time_step_kernel.argtypes = [C.c_void_p, C.c_int, _D, C.POINTER(C.c_int)]
     L.ibfs_tile_shapes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ibfs_last_error.restype = C.c_char_p
     L.ibfs_last_error.argtypes = [C.c_void_p]
@@ -82,6 +83,9 @@ def _load():
     L.ibfs_tangent_linear.argtypes = [C.c_void_p, _IP, C.c_int, C.c_int, _D, _D, C.c_double, C.c_double, _D, _D]
     L.ibfs_adjoint_gradient.argtypes = [C.c_void_p, _IP, C.c_int, C.c_int, C.c_int, _D, C.c_double, C.c_double,
                                         _D, _D]
+    L.ibfs_adjoint_forward.argtypes = [C.c_void_p, _IP, C.c_int, C.c_int, _D, _D]
+    L.ibfs_adjoint_reverse.argtypes = [C.c_void_p, _D, _D]
+    L.ibfs_step_vjp.argtypes = [C.c_void_p, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
     return L
 
 
@@ -128,6 +132,7 @@ class Solver:
         (self.m, self.n, self.L, self.nb, self.nf, self.nq, self.nw, self.nbc, self.nbody, self.bmin, self.bmax,
          self.nsmear) = list(d)
         self.dt = dt
+        self.delta = len_ / float(np.float32(m))  # src/grid.f90:64 (REAL(m))
         self.itime = 0
         self.ncols = 2 * self.nbody + self.L + 1
 
@@ -271,6 +276,39 @@ class Solver:
         self.itime = it.value
         return J, g
 
+    def adjoint_forward(self, nsteps, ckpt, ub=None):
+        """Forward sweep of an adjoint run: nsteps steps with motion = delta *
+        u_B, checkpoints every ckpt steps kept on the device for
+        adjoint_reverse; returns the (nsteps, rowlen) history rows."""
+        it = C.c_int(self.itime)
+        rl = self.rowlen if hasattr(self, "rowlen") else self.ncols
+        hist = np.zeros(nsteps * rl)
+        self._chk(_lib.ibfs_adjoint_forward(self._h, C.byref(it), nsteps, ckpt, _p(_f64(ub)), _p(hist)),
+                  "ibfs_adjoint_forward")
+        self.itime = it.value
+        return hist.reshape(nsteps, rl)
+
+    def adjoint_reverse(self, ghist):
+        """Reverse sweep of the last adjoint_forward: ghist = the cotangents of
+        its history rows (force columns and circulations; slip entries 0).
+        Returns dJ/du_B; the state stays at the forward sweep's end."""
+        g = np.zeros(self._nf_total())
+        self._chk(_lib.ibfs_adjoint_reverse(self._h, _p(_f64(np.ravel(ghist))), _p(g)), "ibfs_adjoint_reverse")
+        return g
+
+    def step_vjp(self, itime, omega, q, a_omega=None, a_rhs_old=None, a_q=None, a_row=None):
+        """Vector-Jacobian product of the step that makes itime (from the state
+        omega, q at its start; the factor of the handle; motion = 0 here, the
+        product does not depend on it): the cotangents of the step's outputs
+        (omega, rhs_old, q and its history row) give those of its inputs
+        (omega, rhs_old, q) and of its motion argument."""
+        W, Q = self.nw * self.L, self.nq * self.L
+        out = {"omega": np.zeros(W), "rhs_old": np.zeros(W), "q": np.zeros(Q), "motion": np.zeros(self.nf)}
+        self._chk(_lib.ibfs_step_vjp(self._h, int(itime), _p(_f64(omega)), _p(_f64(q)), _p(_f64(a_omega)),
+                                     _p(_f64(a_rhs_old)), _p(_f64(a_q)), _p(_f64(a_row)), _p(out["omega"]),
+                                     _p(out["rhs_old"]), _p(out["q"]), _p(out["motion"])), "ibfs_step_vjp")
+        return out
+
     def reset(self):
         """Back to rest at itime 0 (zero vorticity, fluxes, forces)."""
         z = lambda n: np.zeros(n)
@@ -297,6 +335,15 @@ class Solver:
         ng = C.c_int()
         self._chk(_lib.ibfs_tile_shapes(self._h, group, C.byref(ng), out), "ibfs_tile_shapes")
         return dict(zip(("row", "col", "col2", "step1_row", "step1_col2"), list(out)))
+
+    def time_step_kernel(self, nsteps=20):
+        """Mean in-step duration (us) of the step's dominant kernel (the fused
+        ctci column pair), from events on its own dispatches during nsteps
+        eager steps (the state advances), and the number of launches."""
+        us = C.c_double()
+        n = C.c_int()
+        self._chk(_lib.ibfs_time_step_kernel(self._h, nsteps, C.byref(us), C.byref(n)), "ibfs_time_step_kernel")
+        return us.value, n.value
 
     def time_copy(self, n, reps=200):
         """Mean device time (ms) of a streaming copy of n doubles."""
@@ -412,6 +459,8 @@ class Ensemble:
     objective = Solver.objective
     tangent = Solver.tangent
     adjoint = Solver.adjoint
+    adjoint_forward = Solver.adjoint_forward
+    adjoint_reverse = Solver.adjoint_reverse
     reset = Solver.reset
 
     def build_cholesky(self):

@@ -193,6 +193,33 @@ int ibfs_tangent_linear(ibfs_handle* h, int* itime, int nspin, int nwin, const d
  * forward states checkpointed every ckpt steps and recomputed per segment */
 int ibfs_adjoint_gradient(ibfs_handle* h, int* itime, int nspin, int nwin, int ckpt, const double* ub, double cd,
                           double cl, double* J, double* grad);
+/* The same adjoint for any functional of the force and circulation history
+ * (no counterpart in the reference, which has no adjoint; for callers that
+ * differentiate their own functional of the rows of write_force and
+ * write_circ, src/operators.f90:583-659, 662-699).
+ * ibfs_adjoint_forward: nsteps steps from the current state with
+ * motion = delta * u_B, the state checkpointed every ckpt steps in the
+ * handle's tape; hist (nsteps x rowlen, may be NULL) gets the history rows
+ * [fx of each body, fy of each body, circulation of each level, slip]
+ * (concatenated over cases).  ibfs_adjoint_reverse: the reverse sweep of
+ * the last forward sweep with ghist = dJ/d(hist) (the slip entries are not
+ * differentiated and must be 0); grad = dJ/du_B (Nf values per case); the
+ * state stays at the forward sweep's end.  IBFS_ESTATE without a forward
+ * sweep. */
+int ibfs_adjoint_forward(ibfs_handle* h, int* itime, int nsteps, int ckpt, const double* ub, double* hist);
+int ibfs_adjoint_reverse(ibfs_handle* h, const double* ghist, double* grad);
+/* Vector-Jacobian product of one step (advance, src/operators.f90:62-188)
+ * of a one-case handle, from the state omega
+ * (Lmax nw), q (Lmax nq) at the start of step itime (itime == 1 takes the
+ * Euler start), with the handle's Cholesky factor: the cotangents of the
+ * step's outputs (a_omega, a_rhs_old: Lmax nw; a_q: Lmax nq; a_row: one
+ * history row; NULL means 0) give those of its inputs g_omega, g_rhs_old,
+ * g_q and of its motion argument g_motion (Nf).  Outputs may be NULL.  The
+ * handle's state is not touched.  No counterpart in the reference: for
+ * callers that run their own reverse sweep. */
+int ibfs_step_vjp(ibfs_handle* h, int itime, const double* omega, const double* q, const double* a_omega,
+                  const double* a_rhs_old, const double* a_q, const double* a_row, double* g_omega,
+                  double* g_rhs_old, double* g_q, double* g_motion);
 
 /* Operator entry points, one per reference module procedure; used by the
  * parity tests.  Each copies its inputs in, runs the device kernels on the
@@ -248,6 +275,14 @@ int ibfs_time_dst_gemm(ibfs_handle* h, int axis, int ks, int reps, double* ms_pe
 /* Test entry (no counterpart in the reference): the 2-D DST-I of
  * src/myfft.f90:83-94 computed with the two dense passes above (FP64). */
 int ibfs_op_dst_gemm(ibfs_handle* h, int ks, const double* in, double* out);
+
+/* Measurement only (no counterpart in the reference): nsteps further steps
+ * of an FP64 one-group handle with the factor (the state advances; zero
+ * motion; the step counter is not updated), launched eagerly with a start
+ * and a stop event attached to every dispatch of the fused ctci column pair,
+ * the step's dominant kernel.  us_mean receives the mean duration of those
+ * dispatches inside the step (microseconds), launches their number. */
+int ibfs_time_step_kernel(ibfs_handle* h, int nsteps, double* us_mean, int* launches);
 
 /* Measurement only (no counterpart in the reference): a plain streaming copy
  * of n doubles, launched reps times; the reference point that calibrates

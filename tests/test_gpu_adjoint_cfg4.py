@@ -40,3 +40,37 @@ def test_cfg4_gradient_vs_finite_differences_and_tlm():
     print("tlm", dJ[0], "adjoint", g @ dirs[0])
     assert abs(dJ[0] - g @ dirs[0]) <= 1e-8 * abs(dJ[0])
     assert abs(Jt[0] - J[0]) <= 1e-12 * abs(J[0])
+
+
+def test_cfg4_nonlinear_force_functional():
+    """The same case with a functional that is not linear in the forces: J =
+    mean over the 400-step window of F_x^2 (ForceHistory, reverse sweep in
+    backward), against central differences on 4 random directions."""
+    import torch
+
+    from ibfs_amd import cases, hip
+    from ibfs_amd.adjoint import ForceHistory
+
+    s = hip.Solver(**cases.cylinder_384_adjoint())
+    s.build_cholesky()
+    ub0 = np.random.default_rng(0).normal(0.0, 0.05, s.nf)
+
+    def J(u):
+        fx = ForceHistory.apply(u, s, 2400, 50)[2000:, 0]
+        return (fx * fx).mean()
+
+    ub = torch.tensor(ub0, requires_grad=True)
+    J(ub).backward()
+    g = ub.grad.numpy().copy()
+    rng = np.random.default_rng(2)
+    eps = 2e-4
+    worst = 0.0
+    with torch.no_grad():
+        for k in range(4):
+            d = rng.standard_normal(s.nf)
+            d /= np.linalg.norm(d)
+            fd = (J(torch.tensor(ub0 + eps * d)) - J(torch.tensor(ub0 - eps * d))).item() / (2 * eps)
+            err = abs(fd - g @ d) / max(abs(fd), 1e-12)
+            worst = max(worst, err)
+            print(f"direction {k}: fd {fd:.10e} adjoint {g @ d:.10e} rel {err:.2e}")
+    assert worst < 1e-4

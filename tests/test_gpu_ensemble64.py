@@ -44,8 +44,12 @@ def factors():
     return out
 
 
-def test_mixed_ensemble_equals_single_case_runs(factors):
+def test_mixed_ensemble_equals_single_case_runs(factors, monkeypatch):
     from ibfs_amd import hip
+
+    # both sides solve step 1 level by level (a one- or two-case group would
+    # otherwise batch its levels, which rounds differently)
+    monkeypatch.setenv("IBFS_STEP1_LEVELS", "0")
 
     sl = slice_cases()
     ens = hip.Ensemble(sl, precision=hip.MIXED)

@@ -170,7 +170,9 @@ def test_host_moving_body_without_motion(host_cg_run, tmp_path):
     """bdyno = 1 (a moving body): every step calls the host's move_body and
     rebuilds the smearing tables on the device (src/operators.f90:142-144).
     With no motion prescribed, as shipped, the run is the fixed-body CG run
-    exactly."""
+    up to the CG tolerance: a moving body's CG applies a_times in every
+    product, the fixed body's multiplies by A built once, so the iterates
+    differ by rounding that CG carries to its stopping test (cgtol 1e-7)."""
     d = host_cg_run
     d2 = str(tmp_path)
     shutil.copytree(os.path.join(d, "input"), os.path.join(d2, "input"))
@@ -185,8 +187,15 @@ def test_host_moving_body_without_motion(host_cg_run, tmp_path):
     with open(gp, "w") as f:
         f.write("\n".join(out) + "\n")
     run_host(d2, 0, 10, stationary=False)
-    for fn in ("force.dat", "circ.dat", "slip.dat"):
-        assert np.array_equal(read_table(os.path.join(d, "output", fn)), read_table(os.path.join(d2, "output", fn))), fn
+    # forces on the force scale; the net circulation of this mirror-symmetric
+    # case is rounding noise (~1e-16), compared absolutely; slip on delta
+    delta = 4.0 / 32
+    for fn, tol in (("force.dat", None), ("circ.dat", 1e-12), ("slip.dat", 1e-6 * delta)):
+        a = read_table(os.path.join(d, "output", fn))
+        b = read_table(os.path.join(d2, "output", fn))
+        assert np.array_equal(a[:, 0], b[:, 0])
+        bound = 1e-5 * np.max(np.abs(a[:, 1:])) if tol is None else tol
+        assert np.max(np.abs(a[:, 1:] - b[:, 1:])) <= bound, fn
 
 
 def test_host_rebuilds_a_stale_factor(host_run, tmp_path):

@@ -236,3 +236,53 @@ def test_errors_are_reported():
     assert g.itime == 0
     h = g.advance(1)  # the handle still steps
     assert np.all(np.isfinite(h))
+
+
+def levels_batch(a):
+    """Whether the library batches step 1 over the levels of this case: every
+    level k >= 1 damps an edge source below 1e-17 within min(m,n)/4 cells,
+    kappa_k min(m,n)/4 >= ln 1e17 with kappa = acosh(1 + 1/(2 beta)),
+    beta = dt / (2 Re Delta_k^2), and the band of w_k = ceil(ln 1e17 / kappa_k)
+    cells each level with a coarser level carries stays under a quarter of the
+    grid."""
+    m, n, L = a["m"], a["n"], a["mgridlev"]
+    need, q = np.log(1e17), min(m, n) // 4
+    if L < 2:
+        return False
+    for k in range(L):
+        dk = a["len_"] / m * 2.0**k
+        kappa = np.arccosh(1.0 + 1.0 / (2.0 * (0.5 * a["dt"] / (a["re"] * dk * dk))))
+        if k >= 1 and kappa * q < need:
+            return False
+        if k + 1 < L and 4 * int(np.ceil(need / kappa)) >= min(m, n):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("name", ["cyl32", "naca48x40", "cyl40r5"])
+def test_level_batched_step1_equals_level_by_level(name, monkeypatch):
+    """Step 1 with the levels batched (the bulk solve of every level, then
+    each level's correction for the new coarser edge values on the bands next
+    to its edges) against the reference's level-by-level order
+    (IBFS_STEP1_LEVELS=0): the same step to rounding, over 20 steps.  cyl32's
+    finest band would reach a quarter of its 32-cell grid, so it keeps the
+    level-by-level order and the two runs are bit-identical."""
+    from ibfs_amd import hip
+
+    a = case_args(name)
+    batched = levels_batch(a)
+    assert batched == (name != "cyl32")
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_STEP1_LEVELS", "0")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(20), g0.advance(20)
+    s1, s0 = g1.get_state(), g0.get_state()
+    nb2 = 2 * g1.nbody
+    assert np.max(np.abs(h1[:, :nb2] - h0[:, :nb2])) < 1e-12 * np.max(np.abs(h0[:, :nb2]))
+    for k in range(g1.L):
+        w = slice(k * g1.nw, (k + 1) * g1.nw)
+        assert relerr(s1["omega"][w], s0["omega"][w]) < 1e-12, f"omega level {k + 1}"
+    # the batched form did run where it applies (the two forms round differently)
+    assert np.array_equal(s1["omega"], s0["omega"]) != batched
