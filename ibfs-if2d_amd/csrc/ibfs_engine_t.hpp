@@ -131,12 +131,20 @@ struct Engine : GroupBase {
   bool lev_batch = false;
   Geo geoL{};
   DevBuf<T> T1L, T2L;
-  // the band correction (k_band_coef, k_band_add): beta per item, the band
-  // width per level, the coefficients c_k rho_k^(d+1) per item, edge, d, k
-  DevBuf<double> band_beta, band_C;
+  // vort2flux with the levels batched (see v2f_can_batch): ringN ring
+  // entries per level, ringT the powers T^1..T^(Lmax-2) of the ring map,
+  // ringG / ringGp the closed-form tables along i / j, ringR the ring
+  // vectors of every (case, level), ringS a streamfunction-shaped field that
+  // holds the level-1 ring values of step 2 (its get_bc for the force
+  // right-hand side)
+  bool v2f_batch = false;
+  int ringN = 0;
+  DevBuf<double> ringT, ringG, ringGp, ringR, ringS, ringV, ringX, ringSin;
+  // the band correction (BandOp): beta per item, the band width per level
+  DevBuf<double> band_beta;
   DevBuf<int> band_w;
   std::vector<int> hband_w;
-  int band_wmax = 0, band_nmax = 0;
+  int band_wmax = 0;
 
   long fbs() const { return (long)Lmax * nw; }
   long tbs() const { return (long)Lmax * nwp; }  // case stride of lam1i (pair-major tables)
@@ -281,10 +289,76 @@ struct Engine : GroupBase {
       upvec(band_beta, hb, st);
       upvec(band_w, hband_w, st);
       band_wmax = *std::max_element(hband_w.begin(), hband_w.end());
-      band_nmax = std::max(m, n);
-      band_C.alloc((size_t)B * Lmax * 4 * band_wmax * band_nmax);
       HIPCHK(hipStreamSynchronize(st));
     }
+    v2f_batch = v2f_can_batch();
+    if (v2f_batch) build_ring();
+  }
+
+  // Whether vort2flux runs with the levels batched (the kernels after
+  // k_ring_gtab): the decomposition s_k = ctci(omega_k) + H(sbc_k) is exact
+  // for any size, so this asks only that the FP64 engine solve one or two
+  // cases with all levels (batches are throughput-bound), at most
+  // kRingMaxLev levels, at most 1 GiB of powers of T, and the ring vectors
+  // of the levels above the finest in 64 KiB of LDS (k_ring_chain);
+  // IBFS_V2F_LEVELS=0 switches it off.
+  bool v2f_can_batch() const {
+    const char* env = std::getenv("IBFS_V2F_LEVELS");
+    if (env && env[0] == '0') return false;
+    if constexpr (!std::is_same_v<T, double>) return false;
+    if (gate != nullptr || Lmax < 2 || B > 2 || Lmax > kRingMaxLev) return false;
+    const double nr = ring_count(m, n);
+    if ((Lmax - 1) * nr * 8.0 > 65536.0) return false;
+    return (double)std::max(Lmax - 2, 1) * nr * nr * 8.0 <= double(1L << 30);
+  }
+  // the tables of the batched vort2flux: g / (2N) along each axis, T from its
+  // four edge blocks (T[:, r0_e + c] = B_e A_e^T), and T's powers
+  void build_ring() {
+    ringN = ring_count(m, n);
+    const int NR = ringN;
+    if (!lev_batch) {
+      T1L.alloc((size_t)B * Lmax * nw);
+      T2L.alloc((size_t)B * Lmax * nw);
+      geoL = geo;
+      geoL.Lmax = 1;
+    }
+    ringG.alloc((size_t)(m - 1) * (n - 1));
+    hipLaunchKernelGGL(k_ring_gtab, dim3((unsigned)(((long)(m - 1) * (n - 1) + 255) / 256)), dim3(256), 0, st, m,
+                       n, ringG.p);
+    if (m != n) {
+      ringGp.alloc((size_t)(m - 1) * (n - 1));
+      hipLaunchKernelGGL(k_ring_gtab, dim3((unsigned)(((long)(m - 1) * (n - 1) + 255) / 256)), dim3(256), 0, st, n,
+                         m, ringGp.p);
+    }
+    ringT.alloc((size_t)std::max(Lmax - 2, 1) * NR * NR);
+    const int Kmax = std::max(m, n) - 1;
+    DevBuf<double> Bm, Am;
+    Bm.alloc((size_t)NR * Kmax);
+    Am.alloc((size_t)(Kmax / 2 + 2) * Kmax);
+    for (int e = 0; e < 4; e++) {
+      const int Nal = e < 2 ? m : n, K = Nal - 1, cnt = Nal / 2 + 1;
+      const double* G = (e < 2 || m == n) ? ringG.p : ringGp.p;
+      hipLaunchKernelGGL(k_ring_factors, dim3((unsigned)(((long)NR * K + 255) / 256)), dim3(256), 0, st, m, n, e, G,
+                         Bm.p, Am.p);
+      hipLaunchKernelGGL(k_gemm64, dim3((cnt + 15) / 16, (NR + 15) / 16), dim3(256), 0, st, NR, cnt, K, Bm.p, (long)K,
+                         1L, Am.p, 1L, (long)K, ringT.p + ring_edge0(m, n, e), (long)NR, 0);
+    }
+    for (int j = 2; j <= Lmax - 2; j++)  // T^j = T^(j-1) T
+      hipLaunchKernelGGL(k_gemm64, dim3((NR + 15) / 16, (NR + 15) / 16), dim3(256), 0, st, NR, NR, NR,
+                         ringT.p + (size_t)(j - 2) * NR * NR, (long)NR, 1L, ringT.p, (long)NR, 1L,
+                         ringT.p + (size_t)(j - 1) * NR * NR, (long)NR, 0);
+    ringR.alloc((size_t)B * Lmax * NR);
+    ringR.zero(st);
+    ringV.alloc((size_t)B * Lmax * NR);
+    ringV.zero(st);
+    ringX.alloc((size_t)B * Lmax * 4 * (std::max(m, n) - 1));
+    ringX.zero(st);
+    ringS.alloc((size_t)B * fbs());
+    ringS.zero(st);
+    ringSin.alloc((size_t)2 * m);  // sin(pi t / m), t < 2m
+    hipLaunchKernelGGL(k_sinpi_table, dim3((2 * m + 255) / 256), dim3(256), 0, st, m, ringSin.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));  // Bm, Am are freed on return
   }
 
   // Whether step 1 may run with the levels batched.  The reference solves
@@ -367,7 +441,7 @@ struct Engine : GroupBase {
     }
   }
   template <int NN, int P, class Op>
-  void launch_spec_col2(const Op& op, int nseq, int mult, int batch) {
+  void launch_spec_col2(const Op& op, int nseq, int mult, int batch, int ax = 1) {
     using V = op_value_t<Op>;
     if constexpr (p_fits<V>(NN, P)) {
       const int groups = (nseq + mult * P - 1) / (mult * P);
@@ -380,7 +454,7 @@ struct Engine : GroupBase {
         }
       }
       hipLaunchKernelGGL((spec::dst_col2_n<V, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st, op,
-                         fset<V>().stw_n.p);
+                         ax ? fset<V>().stw_n.p : fset<V>().stw_m.p);
     }
   }
   // Measurement only: with probe set (eager steps, never a captured graph),
@@ -445,16 +519,16 @@ struct Engine : GroupBase {
 
   // fused column pair (dst_col2) over nseq columns
   template <class Op>
-  void launch_col2(const Op& op, int nseq, int mult, int batch, int kind = kCol2) {
+  void launch_col2(const Op& op, int nseq, int mult, int batch, int kind = kCol2, int ax = 1) {
     using V = op_value_t<Op>;
     const FftSet<V>& F = fset<V>();
-    const int n2 = F.pn.n2;
+    const int n2 = ax ? F.pn.n2 : F.pm.n2;
     const int P = tile<V>(kind);
-    if (F.spec_n) {
+    if (ax ? F.spec_n : F.spec_m) {
       switch (n2 * 8 + P) {
-#define IBFS_SPEC_CASE(NN, PP)                           \
-  case NN * 8 + PP:                                      \
-    launch_spec_col2<NN, PP, Op>(op, nseq, mult, batch); \
+#define IBFS_SPEC_CASE(NN, PP)                               \
+  case NN * 8 + PP:                                          \
+    launch_spec_col2<NN, PP, Op>(op, nseq, mult, batch, ax); \
     return;
 #define IBFS_SPEC_SIZE(NN) IBFS_SPEC_CASE(NN, 1) IBFS_SPEC_CASE(NN, 2) IBFS_SPEC_CASE(NN, 4)
         IBFS_SPEC_SIZE(256)
@@ -471,7 +545,7 @@ struct Engine : GroupBase {
         default: break;
       }
     }
-    FftPlan<V> plan = F.pn;
+    FftPlan<V> plan = ax ? F.pn : F.pm;
     plan.nfft = generic_fanout(n2, nseq, mult, batch);
     const int groups = (nseq + mult * plan.nfft - 1) / (mult * plan.nfft);
     hipLaunchKernelGGL((dst_col2<V, kEmax, Op>), dim3(groups, batch), dim3(kThreads),
@@ -661,6 +735,58 @@ struct Engine : GroupBase {
                                                                 lev},
               geo.nm1, 2, batch);
   }
+  // vort2flux with the levels batched (see k_ring_gtab): the coarsified
+  // vorticity's row transform and column pair on every (case, level) item
+  // at once, the ring vectors of s0 on every level from the column pair's
+  // output, the chain R_k = sum_j T^j ring(s0_{k+j}), the edge spectra of
+  // Interp(R_{k+1}), then the last row transform with the bottom and top
+  // edge terms and a column pass that adds the left and right ones: on every
+  // level (all), or on level 0 only with the level-1 ring values in ringS for
+  // get_bc (step 2, which needs s_1 nowhere else).
+  void vort2flux_levels(bool all) {
+    if constexpr (std::is_same_v<T, double>) {
+      const int BL = B * Lmax, NR = ringN, Kx = std::max(m, n) - 1;
+      coarsify(omega.p, B, gate);
+      launch<0>(RowOp<T, SrcField<T>, DstField<T>>{geoL, SrcField<T>{omega.p, nw}, DstField<T>{T1L.p, nw}, nullptr,
+                                                   0},
+                geo.nm1, 2, BL);
+      launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{geoL, SrcField<T>{T1L.p, nw},
+                                                                   MidScale<T>{laminv.p}, DstField<T>{T2L.p, nw},
+                                                                   nullptr, 0},
+                  geo.mm1, 2, BL);
+      hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, B * (Lmax - 1)), dim3(256), 0, st,
+                         geo, T2L.p, Lmax, NR, ringSin.p, ringV.p);
+      const size_t lds = sizeof(double) * (Lmax - 1) * NR;
+      double* sring = all ? nullptr : ringS.p;
+      switch (Lmax - 2) {
+#define IBFS_RING_CHAIN(NP)                                                                                     \
+  case NP:                                                                                                      \
+    hipLaunchKernelGGL(k_ring_chain<NP>, dim3((NR + 3) / 4, B), dim3(256), lds, st, geo, fbs(), NR, ringV.p,     \
+                       ringT.p, ringR.p, sring);                                                                \
+    break;
+        IBFS_RING_CHAIN(0)
+        IBFS_RING_CHAIN(1)
+        IBFS_RING_CHAIN(2)
+        IBFS_RING_CHAIN(3)
+        IBFS_RING_CHAIN(4)
+        IBFS_RING_CHAIN(5)
+        IBFS_RING_CHAIN(6)
+#undef IBFS_RING_CHAIN
+        default: throw std::logic_error("vort2flux_levels: level count");
+      }
+      const ItemMap below{all ? Lmax - 1 : 1, Lmax}, out{all ? Lmax : 1, Lmax};
+      const int nbelow = all ? B * (Lmax - 1) : B, nout = all ? BL : B;
+      hipLaunchKernelGGL(k_ring_spectra, dim3(4 * ((Kx + 31) / 32), nbelow), dim3(256),
+                         sizeof(double) * (Kx + 2 * (Kx + 1)), st, geo, ringR.p, NR, below, Kx, ringX.p);
+      const double* Gy = m == n ? ringG.p : ringGp.p;
+      launch<0>(RowOp<T, SrcRingRow, DstRingItem>{geoL, SrcRingRow{T2L.p, ringX.p, ringG.p, out, Kx},
+                                                  DstRingItem{S.p, out, 0}, nullptr, 0},
+                geo.nm1, 2, nout);
+      launch<1>(ColOp<T, SrcRingCol, DstRingItem>{geoL, SrcRingCol{ringX.p, Gy, below, Kx},
+                                                  DstRingItem{S.p, below, 1}, nullptr, 0},
+                geo.mm1, 2, nbelow, kCol);
+    }
+  }
   // vort2flux streamfunctions of an FP32 engine with every level solved in
   // FP64 into S64 (steps 2 and 4: the force system and the stored fluxes)
   void vort2flux64(T* om, int batch, const int* gate) {
@@ -675,23 +801,38 @@ struct Engine : GroupBase {
     enqueue_step34();
   }
 
-  // STEP 1 with the levels batched (see levels_can_batch): the head kernel
-  // (lastbc of every level, and the previous step's pending diagnostics
-  // row), the bulk solve of every level with its old edge values (a row
+  // STEP 1 with the levels batched (see levels_can_batch): the previous
+  // step's pending diagnostics row (with lastbc of every level in the head
+  // kernel, or as side work of the first pass), the bulk solve of every
+  // level with its old edge values (a row
   // pass, the lam1i column pair and an adding row pass over all (case,
   // level) items), then every level's correction for the new edge values of
-  // its coarser level, from that level's bulk solve (k_band_coef,
-  // k_band_add)
+  // its coarser level, from that level's bulk solve (BandOp)
   void enqueue_step1_levels(int first) {
     const int BL = B * Lmax;
     const bool defer = diag_deferred();
-    const int nlb = (nbc * Lmax + 255) / 256;
-    hipLaunchKernelGGL(k_step_head<T>, dim3(nlb + (defer ? 3 : 0), B), dim3(256), 0, st, geo, omega.p, lastbc.p,
-                       gate, diag_args(dpend.p), defer ? 1 : 0);
+    // The freestream loader reads the edge values itself and takes the
+    // diagnostics row as side work when its workgroups are whole waves;
+    // otherwise the head kernel writes lastbc and the row first.
+    const int nt = fs.spec_m ? ptile[kStep1Row] * fs.pm.n2 / 4 : kThreads;
+    const bool headless = q0_uniform() && nt % 64 == 0;
+    if (!headless) {
+      const int nlb = (nbc * Lmax + 255) / 256;
+      hipLaunchKernelGGL(k_step_head<T>, dim3(nlb + (defer ? 3 : 0), B), dim3(256), 0, st, geo, omega.p, lastbc.p,
+                         gate, diag_args(dpend.p), defer ? 1 : 0);
+    }
     if (q0_uniform()) {
       SrcStep1<T, true> s1f{omega.p, nullptr, q.p, nullptr, lastbc.p, rhs_old.p, con1.p, con2.p, vfac.p,
                             nullptr, (long)nw, (long)nq, (long)nbc, 0, first};
       s1f.q0c = q0c.p;
+      if (headless) {
+        s1f.lev_items = Lmax;
+        s1f.gdiag = geo;
+        if (defer) {
+          s1f.has_diag = 1;
+          s1f.dg = diag_args(dpend.p);
+        }
+      }
       launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geoL, s1f, DstField<T>{T1L.p, nw}, nullptr, 0}, geo.nm1,
                 2, BL, kStep1Row);
     } else {
@@ -701,19 +842,12 @@ struct Engine : GroupBase {
                 kStep1Row);
     }
     step1_levels_add();
-    // the correction for the new coarser edge values, on the bands only: the
-    // edge spectra, then one DST-I per band row and per band column
-    const int nch = (band_nmax - 1 + 63) / 64;
-    hipLaunchKernelGGL(k_band_coef<T>, dim3(4 * nch, BL), dim3(256), sizeof(double) * 3 * band_nmax, st, geoL,
-                       omega.p, Lmax, band_beta.p, band_w.p, band_wmax, band_nmax, band_C.p);
-    const BandBase<T> bb{band_C.p, omega.p, geoL, Lmax, band_w.p, band_wmax, band_nmax};
-    Geo gRb = geoL, gCb = geoL;
-    gRb.nm1 = 2 * band_wmax;  // sequences of the row pass: the bottom and top band rows
-    gCb.mm1 = 2 * band_wmax;  // sequences of the column pass: the left and right band columns
-    launch<0>(RowOp<T, SrcBandRow<T>, DstBandRow<T>>{gRb, SrcBandRow<T>{bb}, DstBandRow<T>{bb}, nullptr, 0},
-              gRb.nm1, 2, BL, kRow);
-    launch<1>(ColOp<T, SrcBandCol<T>, DstBandCol<T>>{gCb, SrcBandCol<T>{bb}, DstBandCol<T>{bb}, nullptr, 0},
-              gCb.mm1, 2, BL, kCol);
+    // the correction for the new coarser edge values, on the bands only: one
+    // fused column pair per band depth and pair of opposite edges, the
+    // bottom and top bands first, then the left and right ones (the bands
+    // cross in the corners, so the two adds there are kept in order)
+    launch_col2(BandOp<T>{geoL, omega.p, Lmax, band_beta.p, band_w.p, 0, 2, m}, 2 * band_wmax, 2, BL, kCol2, 0);
+    launch_col2(BandOp<T>{geoL, omega.p, Lmax, band_beta.p, band_w.p, 2, 2, n}, 2 * band_wmax, 2, BL, kCol2, 1);
   }
   // omega += S (lam1i * S T1L) over every (case, level) item
   void step1_levels_add() {
@@ -803,7 +937,11 @@ struct Engine : GroupBase {
     const dim3 gall(std::max(1, (nf_max + nvtx_max + kForceRows - 1) / kForceRows), B);
     const auto force = [&](auto s1) {
       using TS = std::remove_const_t<std::remove_pointer_t<decltype(s1)>>;
+      // get_bc(s_2) reads level 1's ring only; the batched vort2flux keeps
+      // those values in ringS (level 1 of S is not completed in step 2)
       const TS* s2 = Lmax > 1 ? s1 + nw : nullptr;
+      if constexpr (std::is_same_v<TS, double> && std::is_same_v<T, double>)
+        if (v2f_batch) s2 = ringS.p + nw;
       if (!refine) {
         hipLaunchKernelGGL((k_force_solve<T, TS>), gall, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
                            gate, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 0);
@@ -820,6 +958,9 @@ struct Engine : GroupBase {
     if constexpr (kSolve64) {
       vort2flux64(omega.p, B, gate);
       force((const double*)S64.p);
+    } else if (v2f_batch) {
+      vort2flux_levels(false);
+      force((const T*)S.p);
     } else {
       vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
       force((const T*)S.p);
@@ -833,6 +974,7 @@ struct Engine : GroupBase {
     ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, tbs());
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
     if constexpr (kSolve64) vort2flux64(omega.p, B, gate);
+    else if (v2f_batch) vort2flux_levels(true);
     else vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
     // fluxes and diagnostics (src/operators.f90:184-186)
     const int ncurl = (nq + 256 * kTailFaces - 1) / (256 * kTailFaces) * Lmax;

@@ -619,11 +619,27 @@ struct SrcStep1 {
   // launch of the step before k_tail writes them.
   DiagArgs<T> dg{};
   int has_diag = 0;
+  // Level-batched form (lev_items = levels per case > 0): the items are the
+  // flattened (case, level) pairs of one field each, and the edge values of
+  // the freestream form come from the coarser level's old vorticity (the
+  // next item) through get_bc(., 0.25), as lastbc would hold them; zero on a
+  // case's coarsest level.  The diagnostics row is then side work of the
+  // items of level 0, written for case b / lev_items with the geometry gdiag
+  // of all levels.
+  int lev_items = 0;
+  Geo gdiag{};
   __host__ __device__ int side_groups_for(int nt) const {
     const int n = (side_count + nt - 1) / nt + (has_diag ? 3 : 0);
     return (n + 7) & ~7;
   }
   __device__ void side(const Geo& g, int b, int x, int t) const {
+    if (lev_items > 0) {
+      if (has_diag && x < 3 && b % lev_items == 0) {
+        __shared__ double red[16];
+        diag_row<T>(gdiag, dg, b / lev_items, red, x);
+      }
+      return;
+    }
     const int nl = (side_count + (int)blockDim.x - 1) / (int)blockDim.x;
     if (x >= nl) {  // the row's circulation, slip and forces, one workgroup each
       if (has_diag && x - nl < 3) {
@@ -690,7 +706,8 @@ struct SrcStep1 {
     const T* w = om + b * fbs;
     const T* qq = q + b * qbs;
     const T* bc = lastbc + b * bbs;
-    const T* rc = omc != nullptr ? omc + b * fbs : w;
+    const bool below = lev_items > 0 && b % lev_items != lev_items - 1;  // batched, with a coarser level
+    const T* rc = omc != nullptr ? omc + b * fbs : (below ? w + g.nw : w);
     const int e = ii + jj * g.mm1;
     const int i = ii + 2, j = jj + 2;
     G.C = w[e];
@@ -716,12 +733,14 @@ struct SrcStep1 {
     // around loads alone waits for nothing; the others are never selected
     G.bU = G.bD = G.bR = G.bL = G.lj = G.li = G.gj1 = G.gj2 = G.gi1 = G.gi2 = T(0);
     if (i == 2 || i == g.m || j == 2 || j == g.n) {
-      G.bU = bc[g.top + i - 1];
-      G.bD = bc[g.bottom + i - 1];
-      G.bR = bc[g.right + j - 1];
-      G.bL = bc[g.left + j - 1];
-      G.lj = bc[(j == g.n ? g.top : g.bottom) + i - 1];
-      G.li = bc[(i == g.m ? g.right : g.left) + j - 1];
+      if (lev_items == 0) {
+        G.bU = bc[g.top + i - 1];
+        G.bD = bc[g.bottom + i - 1];
+        G.bR = bc[g.right + j - 1];
+        G.bL = bc[g.left + j - 1];
+        G.lj = bc[(j == g.n ? g.top : g.bottom) + i - 1];
+        G.li = bc[(i == g.m ? g.right : g.left) + j - 1];
+      }
       int a1, a2;
       getbc_places(g, (j == g.n ? g.top : g.bottom) + i, a1, a2);
       G.gj1 = rc[a1];
@@ -736,9 +755,20 @@ struct SrcStep1 {
     const int c = b * g.Lmax + lev;
     const T u0 = q0c[c];
     const T fac = vfac[c];
+    const T gj = getbc_value(g, (j == n ? g.top : g.bottom) + i, G.gj1, G.gj2, T(0.25));
+    const T gi = getbc_value(g, (i == m ? g.right : g.left) + j, G.gi1, G.gi2, T(0.25));
+    // the edge values: lastbc, or in the batched form get_bc of the coarser
+    // level read here (gj, gi), on the line (gj) and column (gi) of the
+    // vertex; only the one of the vertex's own edge is used
+    T bU = G.bU, bD = G.bD, bR = G.bR, bL = G.bL, lj = G.lj, li = G.li;
+    if (lev_items > 0) {
+      const bool below = b % lev_items != lev_items - 1;
+      bU = bD = lj = below ? gj : T(0);
+      bR = bL = li = below ? gi : T(0);
+    }
     // OmU(i, j+1), OmU(i, j-1), OmV(i+1, j), OmV(i-1, j) of rot_nonlinear
-    const T nU = j == n ? G.bU : G.wu, nD = j == 2 ? G.bD : G.wd;
-    const T nR = i == m ? G.bR : G.wr, nL = i == 2 ? G.bL : G.wl;
+    const T nU = j == n ? bU : G.wu, nD = j == 2 ? bD : G.wd;
+    const T nR = i == m ? bR : G.wr, nL = i == 2 ? bL : G.wl;
     const T va_up = T(0.5) * (G.v_a + G.v_b + T(0) + T(0));  // vavg(i, j+1)
     const T va_c = T(0.5) * (G.v_c + G.v_d + T(0) + T(0));   // vavg(i, j)
     const T va_dn = T(0.5) * (G.v_e + G.v_f + T(0) + T(0));  // vavg(i, j-1)
@@ -754,10 +784,8 @@ struct SrcStep1 {
     // applybc_vec_sel, then applybc_add_sel when a coarser level exists
     const bool ej = j == 2 || j == n, ei = i == 2 || i == m;
     T bcs = T(0);
-    bcs = ej ? bcs + fac * G.lj : bcs;
-    bcs = ei ? bcs + fac * G.li : bcs;
-    const T gj = getbc_value(g, (j == n ? g.top : g.bottom) + i, G.gj1, G.gj2, T(0.25));
-    const T gi = getbc_value(g, (i == m ? g.right : g.left) + j, G.gi1, G.gi2, T(0.25));
+    bcs = ej ? bcs + fac * lj : bcs;
+    bcs = ei ? bcs + fac * li : bcs;
     T bcc = bcs;
     bcc = ej ? bcc + fac * gj : bcc;
     bcc = ei ? bcc + fac * gi : bcc;
@@ -797,99 +825,385 @@ struct MidTab {
 // and y swapped).  rho_k < rho_1 < 1e-17^(1/w): only the w rows and columns
 // next to each edge are written (the host picks w per level).
 //
-// k_band_coef: the spectra of the four edges of every (case, level) item
-// with a coarser level.  Workgroup (edge, chunk) of item y: 64 modes, each
-// summed by 4 threads over a quarter of the edge; it writes
-// C[((item * 4 + edge) * wmax + d) * nmax + k-1] = (c_k / 2) rho_k^(d+1) for
-// d < w, the input of one DST-I (S y = 2 sum y_k sin) per band row or
-// column (SrcBand*, DstBand*).  grid (4 ceil((nmax-1)/64), items), 256
-// threads; LDS: the edge values and a sine table (3 nmax doubles).
+// BandOp runs that as a fused column pair (dst_col2_n): for band depth d
+// the first DST gives the spectra 2 sum_t e_t sin(pi k (t+1) / N) = N c_k of
+// two opposite edge vectors (packed as one complex sequence), the middle
+// step scales them by rho_k^(d+1) / (2N), and the second DST gives the
+// correction on the two band lines at depth d, added into omega.  Sequence
+// q of a launch: d = q / eper, edge e0 + q % eper (0 bottom, 1 top, 2 left,
+// 3 right; one launch per axis, eper = 2: the row and column bands cross in
+// the corners, where two launches add in a fixed order).  The edge values come from the coarser
+// level's bulk solve; its own correction lies in its bands, w < min(m,n)/4
+// cells from its edges, and get_bc reads it on lines min(m,n)/4 cells in,
+// so the two never meet within the launch.  Items of a case's coarsest
+// level and depths d >= w load 0 and store nothing.
 template <typename T>
-__global__ __launch_bounds__(256) void k_band_coef(Geo g, const T* __restrict__ om, int lm,
-                                                   const double* __restrict__ beta, const int* __restrict__ wlev,
-                                                   int wmax, int nmax, double* __restrict__ C) {
+struct BandOp {
+  Geo gr;  // the level geometry
+  T* om; int lm; const double* beta; const int* wlev;
+  int e0, eper, N;
+  __device__ void place(int q, int& d, int& e) const {
+    d = q / eper;
+    e = e0 + q % eper;
+  }
+  __device__ bool on(int b, int d) const {
+    const int lev = b % lm;
+    return lev != lm - 1 && d < wlev[lev];
+  }
+  __device__ T load(int b, int grp, int s, int P, int idx) const {
+    int d, e;
+    place(grp * 2 * P + s, d, e);
+    if (!on(b, d)) return T(0);
+    const int base = e == 0 ? gr.bottom : e == 1 ? gr.top : e == 2 ? gr.left : gr.right;
+    return getbc_sel<T>(gr, om + (long)(b + 1) * gr.nw, base + idx + 2, T(0.25));
+  }
+  __device__ void mid_coef(int b, int grp, int p, int P, int k, T& ca, T& cb) const {
+    int d, e;
+    place(grp * 2 * P + 2 * p, d, e);
+    ca = cb = T(0);
+    if (!on(b, d)) return;
+    const double bt = beta[b];
+    const double lx = 2.0 - 2.0 * cospi((double)k / (double)N);
+    const double th = (1.0 + bt * (lx + 2.0)) / (2.0 * bt);
+    const double rho = 1.0 / (th + sqrt(th * th - 1.0));
+    double pw = rho;
+    for (int t = 0; t < d; t++) pw *= rho;
+    ca = cb = T(pw / (2.0 * N));
+  }
+  __device__ void mid_apply(int, int, int, int, int, T ya, T yb, T ca, T cb, T& va, T& vb) const {
+    va = ya * ca;
+    vb = yb * cb;
+  }
+  __device__ void mid(int b, int grp, int p, int P, int k, T ya, T yb, T& va, T& vb) const {
+    T ca, cb;
+    mid_coef(b, grp, p, P, k, ca, cb);
+    mid_apply(b, grp, p, P, k, ya, yb, ca, cb, va, vb);
+  }
+  __device__ void add(int b, int e, int d, int pos, T v) const {
+    const int ii = e < 2 ? pos : (e == 2 ? d : gr.mm1 - 1 - d);
+    const int jj = e < 2 ? (e == 0 ? d : gr.nm1 - 1 - d) : pos;
+    T* f = om + (long)b * gr.nw + ii + (long)jj * gr.mm1;
+    *f = *f + v;
+  }
+  __device__ void store(int b, int grp, int p, int P, int k, T ya, T yb) const {
+    int d, e;
+    place(grp * 2 * P + 2 * p, d, e);
+    if (!on(b, d)) return;
+    add(b, e, d, k - 1, ya);
+    add(b, e + 1, d, k - 1, yb);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// vort2flux with the levels batched (src/operators.f90:191-240).  The
+// reference solves s_k = ctci(omega_k + apply_bc(get_bc(s_{k+1}, 1), 1))
+// coarse to fine.  ctci is the inverse of the 5-point operator C^T C, so
+//   s_k = s0_k + H(sbc_k),  s0_k = ctci(omega_k),  H(x) = ctci(apply_bc(0, x, 1)),
+// with H(x) the discrete harmonic extension of the edge values x.  get_bc
+// reads s_{k+1} only on its ring: the lines j = n/4, 3n/4 and i = m/4, 3m/4
+// between those corners, R_{k+1} = ring(s_{k+1}), sbc_k = Interp(R_{k+1})
+// (the averages of get_bc).  So
+//   R_k = ring(s0_k) + T R_{k+1},   T = ring . H . Interp,
+//   R_k = sum_{j >= 0} T^j ring(s0_{k+j})  (T^0 = I, up to the coarsest level),
+// one product with each power of T, which depends on m and n only.  Every
+// level's s0 is one batched ctci, the ring vectors of all levels are one
+// kernel, and H(Interp(R_{k+1})) is added in closed form.  H separates in
+// sines: for bottom edge values x_b (and top values x_t at row n-2-jj),
+//   H(ii, jj) = sum_k (DST(x_b)_k g_k(jj) + DST(x_t)_k g_k(n-2-jj)) / (2m)
+//               * 2 sin(pi k (ii+1) / m),
+//   g_k(jj) = (rho^(jj+1) - rho^(2n-1-jj)) / (1 - rho^(2n)),
+//   rho + 1/rho = 2 + lx_k,  lx_k = 4 sin^2(pi k / (2m)),
+// the 1-D solve across the grid with Dirichlet ends; left and right edges
+// swap the axes.  DST(x) = 2 sum_t x_t sin(pi k (t+1) / N).
+//
+// Ring layout: [bottom line (m/2+1), top line (m/2+1), left (n/2+1),
+// right (n/2+1)], each from its first corner; entry c of the bottom line is
+// s(m/4-1+c, n/4-1) (0-based interior indices), of the left line
+// s(m/4-1, n/4-1+c).  Corners appear on two lines.
+__host__ __device__ __forceinline__ int ring_count(int m, int n) { return 2 * (m / 2 + 1) + 2 * (n / 2 + 1); }
+__host__ __device__ __forceinline__ void ring_pos(int m, int n, int p, int& ii, int& jj) {
+  const int nb = m / 2 + 1, nl = n / 2 + 1;
+  if (p < nb) { ii = m / 4 - 1 + p; jj = n / 4 - 1; }
+  else if (p < 2 * nb) { ii = m / 4 - 1 + (p - nb); jj = 3 * n / 4 - 1; }
+  else if (p < 2 * nb + nl) { ii = m / 4 - 1; jj = n / 4 - 1 + (p - 2 * nb); }
+  else { ii = 3 * m / 4 - 1; jj = n / 4 - 1 + (p - 2 * nb - nl); }
+}
+__host__ __device__ __forceinline__ int ring_edge0(int m, int n, int e) {
+  const int nb = m / 2 + 1, nl = n / 2 + 1;
+  return e == 0 ? 0 : e == 1 ? nb : e == 2 ? 2 * nb : 2 * nb + nl;
+}
+// Interp(R) at interior place t (0-based, t = 0..N-2) of edge e (0 bottom,
+// 1 top, 2 left, 3 right): get_bc's value at 1-based place i = t+1 of that
+// edge, taken from the ring vector R (the same two terms in the same order)
+__device__ __forceinline__ double ring_interp(const double* __restrict__ R, int m, int n, int e, int t) {
+  const int r0 = ring_edge0(m, n, e), i = t + 1;
+  return (i & 1) ? 0.5 * (R[r0 + (i + 1) / 2] + R[r0 + (i - 1) / 2]) : R[r0 + i / 2];
+}
+
+// tab[t] = sin(pi t / N), t < 2N
+__global__ void k_sinpi_table(int N, double* __restrict__ tab) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 2 * N) tab[t] = sinpi((double)t / (double)N);
+}
+// g_k(jj) / (2 Nal) in G[jj * (Nal-1) + k-1]: modes k = 1..Nal-1 along the
+// edge, jj = 0..Nac-2 across.  Grid over (Nal-1)(Nac-1) entries.
+__global__ void k_ring_gtab(int Nal, int Nac, double* __restrict__ G) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long K = Nal - 1;
+  if (e >= K * (Nac - 1)) return;
+  const int jj = (int)(e / K), k = (int)(e - jj * K) + 1;
+  const double sn = sinpi((double)k / (2.0 * Nal));
+  const double lx = 4.0 * sn * sn;
+  const double rho = 0.5 * (2.0 + lx - sqrt(lx * (lx + 4.0)));
+  const double g = (pow(rho, (double)(jj + 1)) - pow(rho, (double)(2 * Nac - 1 - jj))) / (1.0 - pow(rho, 2.0 * Nac));
+  G[e] = g / (2.0 * Nal);
+}
+// The two factors of T's block of edge e: B[p][k-1] = G(k, d_p) 2 sin(pi k
+// (a_p+1) / Nal) for ring point p (d_p its distance from edge e across the
+// grid, a_p its place along it), and A[c][k-1] = DST of Interp(unit c) on
+// edge e = 2 sum_i w_i sin(pi k i / Nal) over get_bc's places i = 2c-1, 2c,
+// 2c+1 inside 1..Nal-1 (weights 1/2, 1, 1/2).  T[p][r0_e + c] =
+// sum_k B[p][k] A[c][k].  Grid over NR (Nal-1) entries; A in the first
+// (Nal/2+1)(Nal-1).
+__global__ void k_ring_factors(int m, int n, int e, const double* __restrict__ G, double* __restrict__ Bm,
+                               double* __restrict__ Am) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool xe = e < 2;
+  const int Nal = xe ? m : n, Nac = xe ? n : m, K = Nal - 1, NR = ring_count(m, n);
+  if (t < (long)NR * K) {
+    const int p = (int)(t / K), k = (int)(t - (long)p * K) + 1;
+    int ii, jj;
+    ring_pos(m, n, p, ii, jj);
+    const int along = xe ? ii : jj, across = xe ? jj : ii;
+    const int d = (e == 0 || e == 2) ? across : Nac - 2 - across;
+    Bm[t] = G[(long)d * K + k - 1] * 2.0 * sinpi((double)k * (along + 1) / (double)Nal);
+  }
+  if (t < (long)(Nal / 2 + 1) * K) {
+    const int c = (int)(t / K), k = (int)(t - (long)c * K) + 1;
+    double a = 0.0;
+    for (int i = 2 * c - 1; i <= 2 * c + 1; i++)
+      if (i >= 1 && i <= Nal - 1) a += (i == 2 * c ? 1.0 : 0.5) * 2.0 * sinpi((double)k * i / (double)Nal);
+    Am[t] = a;
+  }
+}
+// C[i][j] = sum_k X[i xi + k xk] Y[k yk + j yj] (+ C when acc), 16 x 16
+// tiles through LDS; set-up only (the powers of T).  grid (ceil(nj/16),
+// ceil(ni/16)), 256 threads.
+__global__ __launch_bounds__(256) void k_gemm64(int ni, int nj, int nk, const double* __restrict__ X, long xi,
+                                                long xk, const double* __restrict__ Y, long yk, long yj,
+                                                double* __restrict__ C, long ci, int acc) {
+  __shared__ double xs[16][17], ys[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int i = blockIdx.y * 16 + ty, j = blockIdx.x * 16 + tx;
+  double sum = 0.0;
+  for (int k0 = 0; k0 < nk; k0 += 16) {
+    const int kx = k0 + tx, ky = k0 + ty;
+    xs[ty][tx] = (i < ni && kx < nk) ? X[i * xi + kx * xk] : 0.0;
+    ys[ty][tx] = (ky < nk && j < nj) ? Y[ky * yk + j * yj] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) sum += xs[ty][kk] * ys[kk][tx];
+    __syncthreads();
+  }
+  if (i < ni && j < nj) C[i * ci + j] = (acc ? C[i * ci + j] : 0.0) + sum;
+}
+
+// Batch index -> (case, level) item: items of levels 0..per-1 of each case
+// (per = lm: all of them).
+struct ItemMap {
+  int per, lm;
+  __host__ __device__ int operator()(int b) const { return (b / per) * lm + b % per; }
+};
+
+// The ring vectors of s0_k = DST_i(Y_k) for k = 1..L-1 of case b, from Y_k
+// (the column pair's output, s0 before its last transform along i):
+// V[(b lm + k) NR + p].  s0(ii, jj) = 2 sum_kk Y(kk-1, jj) sin(pi kk (ii+1)/m);
+// the left and right lines (ii+1 = m/4, 3m/4) take one wave per row jj of
+// the ring, the bottom and top lines (rows n/4-1, 3n/4-1) one wave per 8
+// places.  sintab[t] = sin(pi t / m), t < 2m.  grid (ceil(tasks/4),
+// B (lm-1)), 256 threads.
+constexpr int kRingRowOut = 8;
+__host__ __device__ __forceinline__ int ring_fromY_tasks(int m, int n) {
+  return (n / 2 + 1) + 2 * ((m / 2 + 1 + kRingRowOut - 1) / kRingRowOut);
+}
+__device__ __forceinline__ double wave_sum(double v);
+__global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restrict__ Y, int lm, int NR,
+                                                    const double* __restrict__ sintab, double* __restrict__ V) {
+  const int b = blockIdx.y / (lm - 1), k = 1 + blockIdx.y % (lm - 1);
+  const double* y = Y + ((long)b * lm + k) * g.nw;
+  const int m = g.m, n = g.n, K = g.mm1, m2 = 2 * m;
+  const int nb = m / 2 + 1, nl = n / 2 + 1, nrt = (nb + kRingRowOut - 1) / kRingRowOut;
+  const int task = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  double* out = V + ((long)b * lm + k) * NR;
+  if (task < nl) {
+    const int jj = n / 4 - 1 + task;
+    double aL = 0.0, aR = 0.0;
+    for (int kk = 1 + lane; kk <= K; kk += 64) {
+      const double yv = y[(kk - 1) + (long)jj * K];
+      aL += yv * sintab[(kk * (m / 4)) % m2];
+      aR += yv * sintab[(kk * (3 * m / 4)) % m2];
+    }
+    aL = wave_sum(aL);
+    aR = wave_sum(aR);
+    if (lane == 0) {
+      out[2 * nb + task] = 2.0 * aL;
+      out[2 * nb + nl + task] = 2.0 * aR;
+    }
+  } else if (task < nl + 2 * nrt) {
+    const int t2 = task - nl, line = t2 / nrt, c0 = (t2 - line * nrt) * kRingRowOut;
+    const int jj = line == 0 ? n / 4 - 1 : 3 * n / 4 - 1;
+    double a[kRingRowOut];
+#pragma unroll
+    for (int u = 0; u < kRingRowOut; u++) a[u] = 0.0;
+    for (int kk = 1 + lane; kk <= K; kk += 64) {
+      const double yv = y[(kk - 1) + (long)jj * K];
+#pragma unroll
+      for (int u = 0; u < kRingRowOut; u++) {
+        const int c = c0 + u < nb ? c0 + u : nb - 1;
+        a[u] += yv * sintab[(kk * (m / 4 + c)) % m2];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRingRowOut; u++) {
+      const double t = wave_sum(a[u]);
+      if (lane == 0 && c0 + u < nb) out[line * nb + c0 + u] = 2.0 * t;
+    }
+  }
+}
+
+// R_k = V_k + sum_{j=1}^{L-1-k} T^j V_{k+j} for k = 1..L-1 of case b
+// (L = NP + 2 levels, Tp: T^1..T^NP, NR x NR each) into Rb (layout of V).
+// Workgroup: four ring rows p, one per wave, with the case's V_1..V_{L-1}
+// staged in LDS; each wave keeps 4 NP loads of its rows in flight.  With
+// sring, R_1 is also written at the ring places of level 1 of sring (a field
+// in the streamfunction's layout) for get_bc(s_1).  grid (ceil(NR/4), B).
+constexpr int kRingMaxLev = 8;
+template <int NP>
+__global__ __launch_bounds__(256) void k_ring_chain(Geo g, long fbs, int NR, const double* __restrict__ V,
+                                                    const double* __restrict__ Tp, double* __restrict__ Rb,
+                                                    double* __restrict__ sring) {
+  constexpr int L = NP + 2, U = 4;
+  extern __shared__ double v[];
+  const int b = blockIdx.y;
+  const double* vb = V + ((long)b * L + 1) * NR;
+  for (int e = threadIdx.x; e < (L - 1) * NR; e += 256) v[e] = vb[e];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= NR) return;
+  double acc[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) acc[k] = 0.0;
+  if constexpr (NP > 0) {
+    for (int q0 = 0; q0 < NR; q0 += 64 * U) {
+      double t[NP][U];
+#pragma unroll
+      for (int j = 0; j < NP; j++)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int q = q0 + lane + 64 * u;
+          t[j][u] = q < NR ? Tp[(long)j * NR * NR + (long)p * NR + q] : 0.0;
+        }
+#pragma unroll
+      for (int j = 0; j < NP; j++)  // T^(j+1) V_{k+j+1}
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int q = q0 + lane + 64 * u;
+          if (q < NR) {
+#pragma unroll
+            for (int k = 1; k + j + 1 <= L - 1; k++) acc[k] += t[j][u] * v[(k + j) * NR + q];
+          }
+        }
+    }
+  }
+#pragma unroll
+  for (int k = 1; k <= L - 1; k++) {
+    const double a = wave_sum(acc[k]);
+    if (lane == 0) {
+      const double r = v[(k - 1) * NR + p] + a;
+      Rb[((long)b * L + k) * NR + p] = r;
+      if (sring != nullptr && k == 1) {
+        int ii, jj;
+        ring_pos(g.m, g.n, p, ii, jj);
+        sring[b * fbs + g.nw + ii + (long)jj * g.mm1] = r;
+      }
+    }
+  }
+}
+
+// The edge spectra of sbc_k = Interp(R_{k+1}) for the items of im:
+// X[((item 4 + e) Kx) + kk-1] = DST(x_e)_kk = 2 sum_t x_e(t) sin(pi kk (t+1)
+// / N_e), e = 0 bottom, 1 top (N = m), 2 left, 3 right (N = n).  Workgroup
+// (edge, 32 modes) of one item: 8 partial sums per mode over t = part mod 8,
+// the edge vector and the sine table in LDS (N-1 + 2N doubles), summed in
+// a fixed order.  grid (4 ceil(Kx / 32), items), 256 threads.
+__global__ __launch_bounds__(256) void k_ring_spectra(Geo g, const double* __restrict__ Rb, int NR, ItemMap im,
+                                                      int Kx, double* __restrict__ X) {
   extern __shared__ double sh[];
   __shared__ double red[256];
-  const int item = blockIdx.y, lev = item % lm;
-  if (lev == lm - 1) return;
-  const int nch = (nmax - 1 + 63) / 64, edge = blockIdx.x / nch, chunk = blockIdx.x - edge * nch;
-  const int N = edge < 2 ? g.m : g.n, w = wlev[lev];
-  if (chunk * 64 + 1 >= N) return;  // the whole workgroup is past this edge's modes
-  double* ev = sh;          // N-1 edge values
-  double* tab = sh + nmax;  // sin(pi s / N), s = 0..2N-1
-  const T* rc = om + (long)(item + 1) * g.nw;  // the coarser level's new (bulk) vorticity
-  const int base = edge == 0 ? g.bottom : edge == 1 ? g.top : edge == 2 ? g.left : g.right;
-  for (int t = threadIdx.x; t < N - 1; t += 256) ev[t] = double(getbc_sel<T>(g, rc, base + t + 2, T(0.25)));
-  for (int s = threadIdx.x; s < 2 * N; s += 256) tab[s] = sinpi((double)s / (double)N);
+  const int item = im(blockIdx.y);
+  const int nch = (Kx + 31) / 32, e = blockIdx.x / nch, ch = blockIdx.x - e * nch;
+  const int N = e < 2 ? g.m : g.n, K = N - 1;
+  if (ch * 32 >= K) return;  // the whole workgroup is past this edge's modes
+  double* xs = sh;
+  double* st = sh + K;
+  const double* R = Rb + (long)(item + 1) * NR;
+  for (int t = threadIdx.x; t < K; t += 256) xs[t] = ring_interp(R, g.m, g.n, e, t);
+  for (int t = threadIdx.x; t < 2 * N; t += 256) st[t] = sinpi((double)t / (double)N);
   __syncthreads();
-  const int k = chunk * 64 + (threadIdx.x & 63) + 1, part = threadIdx.x >> 6;
-  const int q = (N - 1 + 3) / 4, t0 = part * q, t1 = min(N - 1, t0 + q);
+  const int kk = ch * 32 + (threadIdx.x & 31) + 1, part = threadIdx.x >> 5;
   double acc = 0.0;
-  if (k < N && t0 < t1) {
-    int idx = (int)(((long)k * (t0 + 1)) % (2 * N));
-    for (int t = t0; t < t1; t++) {
-      acc += ev[t] * tab[idx];
-      idx += k;
+  if (kk <= K) {
+    int idx = (kk * (part + 1)) % (2 * N);
+    const int step = (kk * 8) % (2 * N);
+    for (int t = part; t < K; t += 8) {
+      acc += xs[t] * st[idx];
+      idx += step;
       if (idx >= 2 * N) idx -= 2 * N;
     }
   }
   red[threadIdx.x] = acc;
   __syncthreads();
-  if (part != 0 || k >= N) return;
-  const int l = threadIdx.x;
-  const double half_c = ((red[l] + red[l + 64]) + (red[l + 128] + red[l + 192])) / N;
-  const double b = beta[item];
-  const double lx = 2.0 - 2.0 * cospi((double)k / (double)N);
-  const double th = (1.0 + b * (lx + 2.0)) / (2.0 * b);
-  const double rho = 1.0 / (th + sqrt(th * th - 1.0));
-  double pw = rho;
-  double* out = C + ((long)(item * 4 + edge) * wmax) * nmax + (k - 1);
-  for (int d = 0; d < w; d++) {
-    out[(long)d * nmax] = half_c * pw;
-    pw *= rho;
+  if (part == 0 && kk <= K) {
+    const int l = threadIdx.x;
+    const double sum = ((red[l] + red[l + 32]) + (red[l + 64] + red[l + 96])) +
+                       ((red[l + 128] + red[l + 160]) + (red[l + 192] + red[l + 224]));
+    X[((long)item * 4 + e) * Kx + kk - 1] = 2.0 * sum;
   }
 }
 
-// The band passes: a row pass over 2 wmax sequences per item (bottom band
-// rows d, then top band rows d) and a column pass over 2 wmax sequences
-// (left band columns, then right), both with pass geometries whose sequence
-// count is 2 wmax; the loaders read C at (edge, d, k-1) and the stores add
-// the transformed row or column to omega at the band's place (gr: the
-// level's geometry).  Items of a case's coarsest level and d >= w load 0 and
-// store nothing.
-template <typename T>
-struct BandBase {
-  const double* C; T* om; Geo gr; int lm; const int* wlev; int wmax, nmax;
-  __device__ T coef(int b, int edge, int seq, int pos) const {
-    const int lev = b % lm, d = seq % wmax;
-    if (lev == lm - 1 || d >= wlev[lev]) return T(0);
-    return T(C[((long)(b * 4 + edge + seq / wmax) * wmax + d) * nmax + pos]);
-  }
-  __device__ void add(int b, int seq, int ii_or_jj, bool rows, T v) const {
-    const int lev = b % lm, d = seq % wmax, hi = seq / wmax;
-    if (lev == lm - 1 || d >= wlev[lev]) return;
-    int ii, jj;
-    if (rows) { ii = ii_or_jj; jj = hi ? gr.nm1 - 1 - d : d; }
-    else { jj = ii_or_jj; ii = hi ? gr.mm1 - 1 - d : d; }
-    T* f = om + (long)b * gr.nw + ii + (long)jj * gr.mm1;
-    *f = *f + v;
+// The last transform of the batched ctci with the bottom and top edge terms
+// (a RowOp loader, batch index b -> item im(b)): the spectral row jj of s_k
+// is Y(kk-1, jj) + (X_b(kk) G(kk, jj) + X_t(kk) G(kk, n-2-jj)), G = g / (2m)
+// along i; nothing is added on a case's coarsest level.
+struct SrcRingRow {
+  const double* Y; const double* X; const double* G; ItemMap im; int Kx;
+  __device__ double operator()(const Geo& g, int b, int ii, int jj) const {
+    const int item = im(b);
+    const double y = Y[(long)item * g.nw + ii + (long)jj * g.mm1];
+    if (item % im.lm == im.lm - 1) return y;
+    const double* xb = X + (long)item * 4 * Kx;
+    const long K = g.mm1;
+    return y + (xb[ii] * G[jj * K + ii] + xb[Kx + ii] * G[(g.nm1 - 1 - jj) * K + ii]);
   }
 };
-template <typename T>
-struct SrcBandRow : BandBase<T> {  // RowOp: src(g, b, position, sequence)
-  __device__ T operator()(const Geo&, int b, int pos, int seq) const { return this->coef(b, 0, seq, pos); }
+// The left and right edge terms as a column pass (ColOp loader): column ii
+// of H along j has the spectrum X_l(l) G'(l, ii) + X_r(l) G'(l, m-2-ii),
+// G' = g / (2n) along j.
+struct SrcRingCol {
+  const double* X; const double* G; ItemMap im; int Kx;
+  __device__ double operator()(const Geo& g, int b, int ii, int l1) const {
+    const double* xl = X + ((long)im(b) * 4 + 2) * Kx;
+    const long K = g.nm1;
+    return xl[l1] * G[ii * K + l1] + xl[Kx + l1] * G[(g.mm1 - 1 - ii) * K + l1];
+  }
 };
-template <typename T>
-struct DstBandRow : BandBase<T> {  // RowOp: dst(g, b, position, sequence, v)
-  __device__ void operator()(const Geo&, int b, int pos, int seq, T v) const { this->add(b, seq, pos, true, v); }
-};
-template <typename T>
-struct SrcBandCol : BandBase<T> {  // ColOp: src(g, b, sequence, position)
-  __device__ T operator()(const Geo&, int b, int seq, int pos) const { return this->coef(b, 2, seq, pos); }
-};
-template <typename T>
-struct DstBandCol : BandBase<T> {  // ColOp: dst(g, b, sequence, position, v)
-  __device__ void operator()(const Geo&, int b, int seq, int pos, T v) const { this->add(b, seq, pos, false, v); }
+// a = v or a += v on item im(b)
+struct DstRingItem {
+  double* a; ItemMap im; int add;
+  __device__ void operator()(const Geo& g, int b, int ii, int jj, double v) const {
+    double* f = a + (long)im(b) * g.nw + ii + (long)jj * g.mm1;
+    *f = add ? *f + v : v;
+  }
 };
 
 // a = a + scale * v (the step-1 increment; adjoint accumulations)

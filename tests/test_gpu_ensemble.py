@@ -27,13 +27,15 @@ def ens_cases():
 
 
 def test_ensemble_matches_single_case_runs(monkeypatch):
-    """Both sides run step 1 level by level (IBFS_STEP1_LEVELS=0): a one- or
-    two-case group otherwise batches its levels, which sums the coarser edge
-    term separately and so rounds differently (that form is held to the
-    oracle by the parity tests)."""
+    """Both sides run step 1 and vort2flux level by level
+    (IBFS_STEP1_LEVELS=0, IBFS_V2F_LEVELS=0): a one- or two-case group
+    otherwise batches its levels, which sums the edge terms separately and so
+    rounds differently (those forms are held to the oracle by the parity
+    tests)."""
     from ibfs_amd import hip
 
     monkeypatch.setenv("IBFS_STEP1_LEVELS", "0")
+    monkeypatch.setenv("IBFS_V2F_LEVELS", "0")
 
     cases = ens_cases()
     ens = hip.Ensemble(cases)

@@ -286,3 +286,53 @@ def test_level_batched_step1_equals_level_by_level(name, monkeypatch):
         assert relerr(s1["omega"][w], s0["omega"][w]) < 1e-12, f"omega level {k + 1}"
     # the batched form did run where it applies (the two forms round differently)
     assert np.array_equal(s1["omega"], s0["omega"]) != batched
+
+
+# one-case grids with three to five levels (the batched vort2flux applies
+# two or three powers of its ring map there), square and oblong
+V2F_GRIDS = {
+    "cyl64L4": dict(m=64, n=64, mgridlev=4, dt=0.01, re=100.0, len_=4.0, offx=2.0, offy=2.0),
+    "rect64x48L5": dict(m=64, n=48, mgridlev=5, dt=0.01, re=150.0, len_=4.0, offx=2.0, offy=1.5),
+}
+
+
+def v2f_args(name):
+    if name in CASES:
+        return case_args(name)
+    from ibfs_amd import geometry
+
+    a = dict(V2F_GRIDS[name])
+    xb, yb, bd = geometry.circle(a["len_"] / a["m"], 1.0, 0.0, 0.0)
+    return dict(a, xb=xb, yb=yb, bdyno=bd)
+
+
+@pytest.mark.parametrize("name", ["cyl32", "naca48x40", "cyl64L4", "rect64x48L5"])
+def test_level_batched_vort2flux_equals_level_by_level(name, monkeypatch):
+    """vort2flux with every level's ctci in one batch, the edge values of the
+    levels from the ring vectors R_k = sum_j T^j ring(ctci(omega_{k+j})) and
+    the edge terms added in closed form, against the reference's coarse-to-
+    fine order (IBFS_V2F_LEVELS=0): the same steps to rounding over 20 steps,
+    forces, circulation, vorticity and fluxes on every level."""
+    from ibfs_amd import hip
+
+    a = v2f_args(name)
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_V2F_LEVELS", "0")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(20), g0.advance(20)
+    s1, s0 = g1.get_state(), g0.get_state()
+    nb2 = 2 * g1.nbody
+    assert np.max(np.abs(h1[:, :nb2] - h0[:, :nb2])) < 1e-12 * np.max(np.abs(h0[:, :nb2]))
+    # circulation: against the sum of |omega| dx^2 (a symmetric wake's is ~0)
+    circ = slice(nb2, nb2 + g1.L)
+    cscale = max(np.sum(np.abs(s0["omega"][k * g1.nw:(k + 1) * g1.nw])) * (a["len_"] / a["m"] * 2**k) ** 2
+                 for k in range(g1.L))
+    assert np.max(np.abs(h1[:, circ] - h0[:, circ])) < 1e-12 * cscale
+    for k in range(g1.L):
+        w = slice(k * g1.nw, (k + 1) * g1.nw)
+        assert relerr(s1["omega"][w], s0["omega"][w]) < 1e-12, f"omega level {k + 1}"
+        v = slice(k * g1.nq, (k + 1) * g1.nq)
+        assert relerr(s1["q"][v], s0["q"][v]) < 1e-12, f"q level {k + 1}"
+    assert not np.array_equal(s1["q"], s0["q"])  # the two forms did run (they round differently)
