@@ -811,7 +811,7 @@ __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, lon
 }
 }  // namespace
 
-int ibfs_time_step_kernel(ibfs_handle* h, int nsteps, double* us_mean, int* launches) {
+int ibfs_time_step_kernel(ibfs_handle* h, int nsteps, double* us_mean, int* launches, int* fields) {
   if (!h || nsteps < 1 || !us_mean) return IBFS_EINVAL;
   return guarded(h, [&] {
     Engine<double>* E = h->e64();
@@ -845,6 +845,7 @@ int ibfs_time_step_kernel(ibfs_handle* h, int nsteps, double* us_mean, int* laun
     }
     *us_mean = pr.next ? 1e3 * sum / (double)pr.next : 0.0;
     if (launches) *launches = (int)pr.next;
+    if (fields) *fields = (int)pr.fields;
   });
 }
 

@@ -129,6 +129,10 @@ struct Engine : GroupBase {
   // items (geoL: one level per item), then every level's correction for the
   // new coarser edge values in another.  T1L, T2L: one field per item.
   bool lev_batch = false;
+  // the freestream step-1 source of the level-batched form runs as its own
+  // elementwise kernel (k_src_eval) before a plain row pass, instead of as
+  // the loader of that pass; IBFS_STEP1_SPLIT=0 keeps it in the loader
+  bool src_split = true;
   Geo geoL{};
   DevBuf<T> T1L, T2L;
   // vort2flux with the levels batched (see v2f_can_batch): ringN ring
@@ -275,6 +279,7 @@ struct Engine : GroupBase {
     HIPCHK(hipStreamSynchronize(st));
     tune();
     lev_batch = levels_can_batch();
+    if (const char* sp = std::getenv("IBFS_STEP1_SPLIT")) src_split = sp[0] != '0';
     if (lev_batch) {
       geoL = geo;
       geoL.Lmax = 1;
@@ -355,8 +360,9 @@ struct Engine : GroupBase {
     ringX.zero(st);
     ringS.alloc((size_t)B * fbs());
     ringS.zero(st);
-    ringSin.alloc((size_t)2 * m);  // sin(pi t / m), t < 2m
+    ringSin.alloc((size_t)2 * (m + n));  // sin(pi t / m), t < 2m, then sin(pi t / n), t < 2n
     hipLaunchKernelGGL(k_sinpi_table, dim3((2 * m + 255) / 256), dim3(256), 0, st, m, ringSin.p);
+    hipLaunchKernelGGL(k_sinpi_table, dim3((2 * n + 255) / 256), dim3(256), 0, st, n, ringSin.p + 2 * m);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));  // Bm, Am are freed on return
   }
@@ -448,6 +454,7 @@ struct Engine : GroupBase {
       if constexpr (std::is_same_v<Op, Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>>) {
         if (probe && probe->next < probe->ev.size()) {  // measurement: events on the dispatch itself
           auto& e = probe->ev[probe->next++];
+          probe->fields += batch;
           hipExtLaunchKernelGGL((spec::dst_col2_n<V, NN, P, Op>), dim3(groups, batch), dim3(P * NN / 4), 0, st,
                                 e.first, e.second, 0, op, fset<V>().stw_n.p);
           return;
@@ -460,10 +467,13 @@ struct Engine : GroupBase {
   // Measurement only: with probe set (eager steps, never a captured graph),
   // every launch of the ctci column pair (the step's dominant kernel) carries
   // a start and a stop event on its own dispatch, so their difference is the
-  // kernel's duration inside the step.
+  // kernel's duration inside the step; fields counts the W fields those
+  // launches transform (their grid.y: one per level when the levels are
+  // batched).
   struct Probe {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     size_t next = 0;
+    long fields = 0;
   };
   Probe* probe = nullptr;
 
@@ -754,7 +764,8 @@ struct Engine : GroupBase {
                                                                    MidScale<T>{laminv.p}, DstField<T>{T2L.p, nw},
                                                                    nullptr, 0},
                   geo.mm1, 2, BL);
-      hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, B * (Lmax - 1)), dim3(256), 0, st,
+      hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, B * (Lmax - 1)), dim3(256),
+                         sizeof(double) * 2 * m, st,
                          geo, T2L.p, Lmax, NR, ringSin.p, ringV.p);
       const size_t lds = sizeof(double) * (Lmax - 1) * NR;
       double* sring = all ? nullptr : ringS.p;
@@ -777,7 +788,8 @@ struct Engine : GroupBase {
       const ItemMap below{all ? Lmax - 1 : 1, Lmax}, out{all ? Lmax : 1, Lmax};
       const int nbelow = all ? B * (Lmax - 1) : B, nout = all ? BL : B;
       hipLaunchKernelGGL(k_ring_spectra, dim3(4 * ((Kx + 31) / 32), nbelow), dim3(256),
-                         sizeof(double) * (Kx + 2 * (Kx + 1)), st, geo, ringR.p, NR, below, Kx, ringX.p);
+                         sizeof(double) * (Kx + 2 * (Kx + 1)), st, geo, ringR.p, NR, below, Kx, ringSin.p,
+                         ringSin.p + 2 * m, ringX.p);
       const double* Gy = m == n ? ringG.p : ringGp.p;
       launch<0>(RowOp<T, SrcRingRow, DstRingItem>{geoL, SrcRingRow{T2L.p, ringX.p, ringG.p, out, Kx},
                                                   DstRingItem{S.p, out, 0}, nullptr, 0},
@@ -815,7 +827,7 @@ struct Engine : GroupBase {
     // diagnostics row as side work when its workgroups are whole waves;
     // otherwise the head kernel writes lastbc and the row first.
     const int nt = fs.spec_m ? ptile[kStep1Row] * fs.pm.n2 / 4 : kThreads;
-    const bool headless = q0_uniform() && nt % 64 == 0;
+    const bool headless = q0_uniform() && (src_split || nt % 64 == 0);
     if (!headless) {
       const int nlb = (nbc * Lmax + 255) / 256;
       hipLaunchKernelGGL(k_step_head<T>, dim3(nlb + (defer ? 3 : 0), B), dim3(256), 0, st, geo, omega.p, lastbc.p,
@@ -833,8 +845,19 @@ struct Engine : GroupBase {
           s1f.dg = diag_args(dpend.p);
         }
       }
-      launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geoL, s1f, DstField<T>{T1L.p, nw}, nullptr, 0}, geo.nm1,
-                2, BL, kStep1Row);
+      if (src_split) {
+        // the source of every level as its own kernel (into T2L), then a
+        // plain row pass
+        const int sg = s1f.side_groups_for(256);
+        hipLaunchKernelGGL((k_src_eval<T, SrcStep1<T, true>>), dim3(sg + (nw + 255) / 256, BL), dim3(256), 0, st,
+                           geoL, s1f, T2L.p, sg);
+        launch<0>(RowOp<T, SrcField<T>, DstField<T>>{geoL, SrcField<T>{T2L.p, nw}, DstField<T>{T1L.p, nw}, nullptr,
+                                                     0},
+                  geo.nm1, 2, BL);
+      } else {
+        launch<0>(RowOp<T, SrcStep1<T, true>, DstField<T>>{geoL, s1f, DstField<T>{T1L.p, nw}, nullptr, 0},
+                  geo.nm1, 2, BL, kStep1Row);
+      }
     } else {
       SrcStep1<T> s1{omega.p, nullptr, q.p, q0.p, lastbc.p, rhs_old.p, con1.p, con2.p, vfac.p,
                      nullptr, (long)nw, (long)nq, (long)nbc, 0, first};
@@ -943,8 +966,13 @@ struct Engine : GroupBase {
       if constexpr (std::is_same_v<TS, double> && std::is_same_v<T, double>)
         if (v2f_batch) s2 = ringS.p + nw;
       if (!refine) {
+        // the right-hand side once per case, then the rows of [A^-1; G]
+        if constexpr (std::is_same_v<TS, T>)
+          hipLaunchKernelGGL((k_regT_curl<T>), dim3((std::max(nf_max, 1) + 255) / 256, B), dim3(256), 0, st, geo, sm,
+                             s1, s2, FB, motion.p, rhsf.p, 0L, gate, -1);
         hipLaunchKernelGGL((k_force_solve<T, TS>), gall, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
-                           gate, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr, 0);
+                           gate, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr,
+                           std::is_same_v<TS, T> ? 3 : 0);
       } else {  // one FP64 refinement sweep: fb += A^-1 (rhsf - A fb)
         const dim3 gf(std::max(1, (nf_max + kForceRows - 1) / kForceRows), B);
         hipLaunchKernelGGL((k_force_solve<T, TS>), gf, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,

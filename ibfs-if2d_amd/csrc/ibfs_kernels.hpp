@@ -1020,48 +1020,114 @@ struct ItemMap {
 // V[(b lm + k) NR + p].  s0(ii, jj) = 2 sum_kk Y(kk-1, jj) sin(pi kk (ii+1)/m);
 // the left and right lines (ii+1 = m/4, 3m/4) take one wave per row jj of
 // the ring, the bottom and top lines (rows n/4-1, 3n/4-1) one wave per 8
-// places.  sintab[t] = sin(pi t / m), t < 2m.  grid (ceil(tasks/4),
-// B (lm-1)), 256 threads.
+// places.  sintab[t] = sin(pi t / m), t < 2m, staged in LDS (2m doubles of
+// dynamic LDS).  grid (ceil(tasks/4), B (lm-1)), 256 threads.
 constexpr int kRingRowOut = 8;
 __host__ __device__ __forceinline__ int ring_fromY_tasks(int m, int n) {
   return (n / 2 + 1) + 2 * ((m / 2 + 1 + kRingRowOut - 1) / kRingRowOut);
 }
 __device__ __forceinline__ double wave_sum(double v);
+// Stages n doubles from src into LDS dst with every thread's loads issued
+// before its stores (kStageU per thread per round), so that the staging
+// costs one memory round trip per round rather than one per element.
+constexpr int kStageU = 16;
+template <class F>
+__device__ __forceinline__ void stage_lds(double* dst, int n, F&& load) {
+  for (int base = 0; base < n; base += 256 * kStageU) {
+    double r[kStageU];
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {
+      const int e = base + threadIdx.x + 256 * u;
+      r[u] = e < n ? load(e) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {
+      const int e = base + threadIdx.x + 256 * u;
+      if (e < n) dst[e] = r[u];
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restrict__ Y, int lm, int NR,
                                                     const double* __restrict__ sintab, double* __restrict__ V) {
+  extern __shared__ double sn[];  // sin(pi t / m), t < 2m
   const int b = blockIdx.y / (lm - 1), k = 1 + blockIdx.y % (lm - 1);
   const double* y = Y + ((long)b * lm + k) * g.nw;
   const int m = g.m, n = g.n, K = g.mm1, m2 = 2 * m;
   const int nb = m / 2 + 1, nl = n / 2 + 1, nrt = (nb + kRingRowOut - 1) / kRingRowOut;
   const int task = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   double* out = V + ((long)b * lm + k) * NR;
-  if (task < nl) {
-    const int jj = n / 4 - 1 + task;
-    double aL = 0.0, aR = 0.0;
-    for (int kk = 1 + lane; kk <= K; kk += 64) {
-      const double yv = y[(kk - 1) + (long)jj * K];
-      aL += yv * sintab[(kk * (m / 4)) % m2];
-      aR += yv * sintab[(kk * (3 * m / 4)) % m2];
+  const bool live = task < nl + 2 * nrt;  // whole waves
+  const bool lr = task < nl;
+  const int t2 = task - nl, line = lr ? 0 : t2 / nrt, c0 = lr ? 0 : (t2 - line * nrt) * kRingRowOut;
+  const int jj = !live ? 0 : lr ? n / 4 - 1 + task : (line == 0 ? n / 4 - 1 : 3 * n / 4 - 1);
+  // this lane's first modes kk = 1 + lane + 64 u, loaded before the table
+  // is staged
+  constexpr int UY = 8;
+  double yv[UY];
+#pragma unroll
+  for (int u = 0; u < UY; u++) {
+    const int kk = 1 + lane + 64 * u;
+    yv[u] = (live && kk <= K) ? y[(kk - 1) + (long)jj * K] : 0.0;
+  }
+  stage_lds(sn, m2, [&](int e) { return sintab[e]; });
+  __syncthreads();
+  if (!live) return;
+  // the sine places kk c mod 2m advance by 64 c mod 2m from one u to the
+  // next (no division in the loop)
+  double a[kRingRowOut];
+#pragma unroll
+  for (int f = 0; f < kRingRowOut; f++) a[f] = 0.0;
+  if (lr) {
+    int fL = ((1 + lane) * (m / 4)) % m2, fR = ((1 + lane) * (3 * m / 4)) % m2;
+    const int sL = (64 * (m / 4)) % m2, sR = (64 * (3 * m / 4)) % m2;
+    for (int k0 = 0; k0 < K; k0 += 64 * UY) {
+      if (k0 > 0) {
+#pragma unroll
+        for (int u = 0; u < UY; u++) {
+          const int kk = k0 + 1 + lane + 64 * u;
+          yv[u] = kk <= K ? y[(kk - 1) + (long)jj * K] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UY; u++) {
+        a[0] += yv[u] * sn[fL];
+        a[1] += yv[u] * sn[fR];
+        fL += sL;
+        fL -= fL >= m2 ? m2 : 0;
+        fR += sR;
+        fR -= fR >= m2 ? m2 : 0;
+      }
     }
-    aL = wave_sum(aL);
-    aR = wave_sum(aR);
+    const double aL = wave_sum(a[0]), aR = wave_sum(a[1]);
     if (lane == 0) {
       out[2 * nb + task] = 2.0 * aL;
       out[2 * nb + nl + task] = 2.0 * aR;
     }
-  } else if (task < nl + 2 * nrt) {
-    const int t2 = task - nl, line = t2 / nrt, c0 = (t2 - line * nrt) * kRingRowOut;
-    const int jj = line == 0 ? n / 4 - 1 : 3 * n / 4 - 1;
-    double a[kRingRowOut];
+  } else {
+    int fr[kRingRowOut], st[kRingRowOut];
 #pragma unroll
-    for (int u = 0; u < kRingRowOut; u++) a[u] = 0.0;
-    for (int kk = 1 + lane; kk <= K; kk += 64) {
-      const double yv = y[(kk - 1) + (long)jj * K];
+    for (int f = 0; f < kRingRowOut; f++) {
+      const int c = m / 4 + min(c0 + f, nb - 1);
+      fr[f] = ((1 + lane) * c) % m2;
+      st[f] = (64 * c) % m2;
+    }
+    for (int k0 = 0; k0 < K; k0 += 64 * UY) {
+      if (k0 > 0) {
 #pragma unroll
-      for (int u = 0; u < kRingRowOut; u++) {
-        const int c = c0 + u < nb ? c0 + u : nb - 1;
-        a[u] += yv * sintab[(kk * (m / 4 + c)) % m2];
+        for (int u = 0; u < UY; u++) {
+          const int kk = k0 + 1 + lane + 64 * u;
+          yv[u] = kk <= K ? y[(kk - 1) + (long)jj * K] : 0.0;
+        }
       }
+#pragma unroll
+      for (int u = 0; u < UY; u++)
+#pragma unroll
+        for (int f = 0; f < kRingRowOut; f++) {
+          a[f] += yv[u] * sn[fr[f]];
+          fr[f] += st[f];
+          fr[f] -= fr[f] >= m2 ? m2 : 0;
+        }
     }
 #pragma unroll
     for (int u = 0; u < kRingRowOut; u++) {
@@ -1074,35 +1140,45 @@ __global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restr
 // R_k = V_k + sum_{j=1}^{L-1-k} T^j V_{k+j} for k = 1..L-1 of case b
 // (L = NP + 2 levels, Tp: T^1..T^NP, NR x NR each) into Rb (layout of V).
 // Workgroup: four ring rows p, one per wave, with the case's V_1..V_{L-1}
-// staged in LDS; each wave keeps 4 NP loads of its rows in flight.  With
-// sring, R_1 is also written at the ring places of level 1 of sring (a field
-// in the streamfunction's layout) for get_bc(s_1).  grid (ceil(NR/4), B).
+// staged in LDS.  The powers are the kernel's whole HBM traffic (NP NR^2
+// doubles), so each wave issues the loads of its first kChainU entries per
+// lane of every power before it stages V and waits at the barrier, and
+// keeps NP kChainU loads in flight after that.  With sring, R_1 is also
+// written at the ring places of level 1 of sring (a field in the
+// streamfunction's layout) for get_bc(s_1).  grid (ceil(NR/4), B).
 constexpr int kRingMaxLev = 8;
+constexpr int kChainU = 8;
 template <int NP>
 __global__ __launch_bounds__(256) void k_ring_chain(Geo g, long fbs, int NR, const double* __restrict__ V,
                                                     const double* __restrict__ Tp, double* __restrict__ Rb,
                                                     double* __restrict__ sring) {
-  constexpr int L = NP + 2, U = 4;
+  constexpr int L = NP + 2, U = kChainU;
   extern __shared__ double v[];
   const int b = blockIdx.y;
   const double* vb = V + ((long)b * L + 1) * NR;
-  for (int e = threadIdx.x; e < (L - 1) * NR; e += 256) v[e] = vb[e];
-  __syncthreads();
   const int lane = threadIdx.x & 63, p = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= NR) return;
+  const bool live = p < NR;
+  const int pr = live ? p : NR - 1;  // a row that exists: no branch holds the loads back
   double acc[L];
 #pragma unroll
   for (int k = 0; k < L; k++) acc[k] = 0.0;
-  if constexpr (NP > 0) {
+  if constexpr (NP == 0) {
+    stage_lds(v, (L - 1) * NR, [&](int e) { return vb[e]; });
+    __syncthreads();
+  } else {
+    double t[NP][U];
     for (int q0 = 0; q0 < NR; q0 += 64 * U) {
-      double t[NP][U];
 #pragma unroll
       for (int j = 0; j < NP; j++)
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const int q = q0 + lane + 64 * u;
-          t[j][u] = q < NR ? Tp[(long)j * NR * NR + (long)p * NR + q] : 0.0;
+          t[j][u] = q < NR ? Tp[(long)j * NR * NR + (long)pr * NR + q] : 0.0;
         }
+      if (q0 == 0) {  // uniform over the workgroup
+        stage_lds(v, (L - 1) * NR, [&](int e) { return vb[e]; });
+        __syncthreads();
+      }
 #pragma unroll
       for (int j = 0; j < NP; j++)  // T^(j+1) V_{k+j+1}
 #pragma unroll
@@ -1115,6 +1191,7 @@ __global__ __launch_bounds__(256) void k_ring_chain(Geo g, long fbs, int NR, con
         }
     }
   }
+  if (!live) return;
 #pragma unroll
   for (int k = 1; k <= L - 1; k++) {
     const double a = wave_sum(acc[k]);
@@ -1135,9 +1212,11 @@ __global__ __launch_bounds__(256) void k_ring_chain(Geo g, long fbs, int NR, con
 // / N_e), e = 0 bottom, 1 top (N = m), 2 left, 3 right (N = n).  Workgroup
 // (edge, 32 modes) of one item: 8 partial sums per mode over t = part mod 8,
 // the edge vector and the sine table in LDS (N-1 + 2N doubles), summed in
-// a fixed order.  grid (4 ceil(Kx / 32), items), 256 threads.
+// a fixed order.  sin_m[t] = sin(pi t / m), t < 2m, and sin_n likewise.
+// grid (4 ceil(Kx / 32), items), 256 threads.
 __global__ __launch_bounds__(256) void k_ring_spectra(Geo g, const double* __restrict__ Rb, int NR, ItemMap im,
-                                                      int Kx, double* __restrict__ X) {
+                                                      int Kx, const double* __restrict__ sin_m,
+                                                      const double* __restrict__ sin_n, double* __restrict__ X) {
   extern __shared__ double sh[];
   __shared__ double red[256];
   const int item = im(blockIdx.y);
@@ -1147,8 +1226,9 @@ __global__ __launch_bounds__(256) void k_ring_spectra(Geo g, const double* __res
   double* xs = sh;
   double* st = sh + K;
   const double* R = Rb + (long)(item + 1) * NR;
-  for (int t = threadIdx.x; t < K; t += 256) xs[t] = ring_interp(R, g.m, g.n, e, t);
-  for (int t = threadIdx.x; t < 2 * N; t += 256) st[t] = sinpi((double)t / (double)N);
+  const double* sn = e < 2 ? sin_m : sin_n;
+  stage_lds(st, 2 * N, [&](int t) { return sn[t]; });
+  stage_lds(xs, K, [&](int t) { return ring_interp(R, g.m, g.n, e, t); });
   __syncthreads();
   const int kk = ch * 32 + (threadIdx.x & 31) + 1, part = threadIdx.x >> 5;
   double acc = 0.0;
@@ -1219,6 +1299,42 @@ struct DstAdd {
     a[b * bs + ii + jj * g.mm1] = old + scale * v;
   }
 };
+
+// The value a row-pass loader Src would hand the transform, evaluated at
+// every vertex of item blockIdx.y and written to out (same layout), with the
+// loader's side store (post) and its side work (side, in the first
+// side_groups workgroups) done as the pass would do them.  The level-batched
+// step 1 runs its source this way: one vertex per thread keeps every load of
+// the stencil in flight with few registers, and the row pass that follows
+// reads one value per element.  grid (side_groups + ceil(nw / 256), items),
+// 256 threads.
+template <typename T, class Src>
+__global__ __launch_bounds__(256) void k_src_eval(Geo g, Src src, T* __restrict__ out, int side_groups) {
+  const int b = blockIdx.y;
+  int x = blockIdx.x;
+  if constexpr (src_has_side<Src>::value) {
+    if (x < side_groups) {
+      src.side(g, b, x, threadIdx.x);
+      return;
+    }
+    x -= side_groups;
+  }
+  const int e = x * 256 + threadIdx.x;
+  if (e >= g.nw) return;
+  const int jj = e / g.mm1, ii = e - jj * g.mm1;
+  T aux = T(0), v;
+  if constexpr (src_gathered<Src>::value) {
+    typename src_gathered<Src>::type G;
+    src.gather(g, b, ii, jj, G);
+    v = src.combine(g, b, ii, jj, G, aux);
+  } else if constexpr (src_has_post<Src>::value) {
+    v = src.value(g, b, ii, jj, aux);
+  } else {
+    v = src(g, b, ii, jj);
+  }
+  if constexpr (src_has_post<Src>::value) src.post(g, b, ii, jj, aux);
+  out[(long)b * g.nw + e] = v;
+}
 
 // ---------------------------------------------------------------------------
 // small kernels (grid.y = case unless stated)
@@ -1735,6 +1851,9 @@ static __global__ void k_make_g(Smear sm, int ci, const double* __restrict__ Ai,
 //           refined form).
 //   mode 2: the refinement sweep with the residual res = rhsf - A fb:
 //           fb += A^-1 res and rr = G (rhsf + res).
+//   mode 3: as mode 0 with rhsf read from memory (written by k_regT_curl
+//           beforehand), so that the right-hand side is evaluated once per
+//           case instead of once per workgroup.
 // Latency: the right-hand side takes two forces per thread and pass, whose
 // gathers are independent; each wave holds kRowsPerWave rows and walks the
 // columns once, so the row loads of one column block are all in flight
@@ -1800,12 +1919,14 @@ __global__ __launch_bounds__(kForceThreads) void k_force_solve(Geo g, Smear sm, 
         if (f1 < nf) rhsf[c.foff + f1] = v1;
       }
     }
-  } else {
+  } else if (mode == 2) {
     for (int f = tid; f < nf; f += NT) {
       const double r = res[c.foff + f];
       vec[f] = r;
       vec[nf + f] = rhsf[c.foff + f] + r;
     }
+  } else {
+    for (int f = tid; f < nf; f += NT) vec[f] = rhsf[c.foff + f];
   }
   __syncthreads();
   // the sums keep the column order k = lane, lane+64, ... of a plain loop

@@ -64,7 +64,7 @@ def _load():
     L.ibfs_case_set_actuators.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), _D]
     L.ibfs_op_dst_gemm.argtypes = [C.c_void_p, C.c_int, _D, _D]
     L.ibfs_time_copy.argtypes = [C.c_void_p, C.c_long, C.c_int, _D]
-    L.ibfs_time_step_kernel.argtypes = [C.c_void_p, C.c_int, _D, C.POINTER(C.c_int)]
+    L.ibfs_time_step_kernel.argtypes = [C.c_void_p, C.c_int, _D, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ibfs_tile_shapes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ibfs_last_error.restype = C.c_char_p
     L.ibfs_last_error.argtypes = [C.c_void_p]
@@ -339,11 +339,15 @@ class Solver:
     def time_step_kernel(self, nsteps=20):
         """Mean in-step duration (us) of the step's dominant kernel (the fused
         ctci column pair), from events on its own dispatches during nsteps
-        eager steps (the state advances), and the number of launches."""
+        eager steps (the state advances), the number of launches and the
+        number of W fields those launches transform (one per level and case
+        of each launch)."""
         us = C.c_double()
         n = C.c_int()
-        self._chk(_lib.ibfs_time_step_kernel(self._h, nsteps, C.byref(us), C.byref(n)), "ibfs_time_step_kernel")
-        return us.value, n.value
+        f = C.c_int()
+        self._chk(_lib.ibfs_time_step_kernel(self._h, nsteps, C.byref(us), C.byref(n), C.byref(f)),
+                  "ibfs_time_step_kernel")
+        return us.value, n.value, f.value
 
     def time_copy(self, n, reps=200):
         """Mean device time (ms) of a streaming copy of n doubles."""

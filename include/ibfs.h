@@ -281,8 +281,10 @@ int ibfs_op_dst_gemm(ibfs_handle* h, int ks, const double* in, double* out);
  * motion; the step counter is not updated), launched eagerly with a start
  * and a stop event attached to every dispatch of the fused ctci column pair,
  * the step's dominant kernel.  us_mean receives the mean duration of those
- * dispatches inside the step (microseconds), launches their number. */
-int ibfs_time_step_kernel(ibfs_handle* h, int nsteps, double* us_mean, int* launches);
+ * dispatches inside the step (microseconds), launches their number and
+ * fields (may be NULL) the number of W fields they transform in total (one
+ * per level and case of each launch). */
+int ibfs_time_step_kernel(ibfs_handle* h, int nsteps, double* us_mean, int* launches, int* fields);
 
 /* Measurement only (no counterpart in the reference): a plain streaming copy
  * of n doubles, launched reps times; the reference point that calibrates
