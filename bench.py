@@ -428,6 +428,7 @@ def main(argv=None):
                      "copy_same_size_us": t_copy_w * 1e3, "measured_copy_GBs": copy_gbs},
         "step_roofline": {"bytes_per_step": B, "achieved_GBs": B * steps_s / 1e9,
                           "frac": B * steps_s / 1e9 / HBM_PEAK_GBS},
+        "tile_shapes": {"single_field": g.tile_shapes(0), "level_batched": g.level_tile_shapes(0)},
         "preprocess_s": t_pre,
         "chosen": chosen,
         "adjoint": adjoint,

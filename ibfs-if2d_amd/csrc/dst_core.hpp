@@ -106,19 +106,29 @@ struct FftPlan {
   int nfft;                // FFTs per workgroup, set per launch (nfft * n2 <= EMAX)
 };
 
+// complex product with explicit fused multiply-adds, so that every kernel
+// that forms it (whatever the compiler would contract around it) rounds it
+// the same way: the register-blocked and LDS passes stay bit-identical
 template <typename T>
 __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return {fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
 }
 
-// In-register DFT of radix R (forward, exp(-2 pi i q r / R)).
+// In-register DFT of radix R (forward, exp(-2 pi i q r / R)).  The radix-2
+// and radix-4 butterflies only add and subtract, and are kept so: no product
+// computed before them (a loader's value, a spectral factor, a twiddle) may
+// be fused into their additions, which would round differently in kernels
+// whose inputs come from registers than in those whose inputs pass through
+// LDS.
 template <typename T, int R>
 __device__ __forceinline__ void dft_small(cplx<T>* v) {
   if constexpr (R == 2) {
+#pragma clang fp contract(off)
     cplx<T> a = v[0], b = v[1];
     v[0] = {a.re + b.re, a.im + b.im};
     v[1] = {a.re - b.re, a.im - b.im};
   } else if constexpr (R == 4) {
+#pragma clang fp contract(off)
     cplx<T> t0 = {v[0].re + v[2].re, v[0].im + v[2].im};
     cplx<T> t1 = {v[0].re - v[2].re, v[0].im - v[2].im};
     cplx<T> t2 = {v[1].re + v[3].re, v[1].im + v[3].im};

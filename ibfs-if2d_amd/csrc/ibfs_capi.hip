@@ -876,6 +876,12 @@ int ibfs_tile_shapes(const ibfs_handle* h, int group, int* ngroups, int* shapes)
   return IBFS_OK;
 }
 
+int ibfs_level_tile_shapes(const ibfs_handle* h, int group, int* shapes) {
+  if (!h || group < 0 || !shapes || group >= (int)h->groups.size()) return IBFS_EINVAL;
+  h->groups[group]->level_tile_shapes(shapes);
+  return IBFS_OK;
+}
+
 extern "C++" {
 namespace {
 // mean duration (ms) of reps launches of f on the handle's stream

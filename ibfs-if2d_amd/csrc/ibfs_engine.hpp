@@ -282,6 +282,7 @@ struct GroupBase {
   // FFTs per workgroup chosen by the tuner: row, column, column pair,
   // step-1 row, step-1 column pair (0: generic pass)
   virtual void tile_shapes(int* out) const = 0;
+  virtual void level_tile_shapes(int* out) const = 0;
 };
 
 }  // namespace ibfs

@@ -1,5 +1,6 @@
 // ibfs_engine_t.hpp -- Engine<T>: one group of equal-size cases on the device.
 #pragma once
+#include "dst_wave.hpp"
 #include "ibfs_engine.hpp"
 
 namespace ibfs {
@@ -145,7 +146,7 @@ struct Engine : GroupBase {
   int ringN = 0;
   DevBuf<double> ringT, ringG, ringGp, ringR, ringS, ringV, ringX, ringSin;
   // the band correction (BandOp): beta per item, the band width per level
-  DevBuf<double> band_beta;
+  DevBuf<double> band_beta, band_rho_m, band_rho_n;
   DevBuf<int> band_w;
   std::vector<int> hband_w;
   int band_wmax = 0;
@@ -293,11 +294,20 @@ struct Engine : GroupBase {
         }
       upvec(band_beta, hb, st);
       upvec(band_w, hband_w, st);
+      const int BL = B * Lmax;
+      band_rho_m.alloc((size_t)BL * (m - 1));
+      band_rho_n.alloc((size_t)BL * (n - 1));
+      hipLaunchKernelGGL(k_band_rho, dim3((unsigned)(((long)BL * (m - 1) + 255) / 256)), dim3(256), 0, st,
+                         band_beta.p, m, BL, band_rho_m.p);
+      hipLaunchKernelGGL(k_band_rho, dim3((unsigned)(((long)BL * (n - 1) + 255) / 256)), dim3(256), 0, st,
+                         band_beta.p, n, BL, band_rho_n.p);
+      HIPCHK(hipGetLastError());
       band_wmax = *std::max_element(hband_w.begin(), hband_w.end());
       HIPCHK(hipStreamSynchronize(st));
     }
     v2f_batch = v2f_can_batch();
     if (v2f_batch) build_ring();
+    if (lev_batch || v2f_batch) tune_levels();
   }
 
   // Whether vort2flux runs with the levels batched (the kernels after
@@ -412,6 +422,13 @@ struct Engine : GroupBase {
       out[k] = (rowtype ? fs.spec_m : fs.spec_n) ? ptile[k] : 0;
     }
   }
+  void level_tile_shapes(int* out) const override {
+    for (int k = 0; k < kKinds; k++) {  // 0: no level-batched launch of this kind, or the generic pass
+      const bool rowtype = k == kRow || k == kStep1Row;
+      const bool used = lev_tiles && k != kStep1Row;
+      out[k] = used && (rowtype ? fs.spec_m : fs.spec_n) ? ptileL[k] : 0;
+    }
+  }
 
   // ------------------------------------------------------------------ passes
   // Tile shapes: FFTs per workgroup (P) of the size-specialised kernels, per
@@ -419,6 +436,14 @@ struct Engine : GroupBase {
   // group's batch size (tune()).  Results do not depend on P.
   enum Kind { kRow = 0, kCol = 1, kCol2 = 2, kStep1Row = 3, kStep1Col2 = 4, kKinds = 5 };
   int ptile[kKinds] = {1, 1, 1, 1, 1};
+  // tile shapes of the level-batched launches (batch B Lmax), tuned on that
+  // batch when the step batches its levels (lev_tiles)
+  int ptileL[kKinds] = {1, 1, 1, 1, 1};
+  bool lev_tiles = false;
+  // tile code of the register-blocked kernels (dst_wave.hpp): one FFT per
+  // 64 / TPF lanes, one wave per workgroup; bit-identical to P = 1
+  static constexpr int kWaveTile = 8;
+  static constexpr bool wave_size(int n2) { return n2 == 256 || n2 == 512 || n2 == 1024; }
   template <typename V = T>
   static constexpr bool p_fits(int n2, int P) {
     return P * n2 / 4 <= 1024 && (long)P * n2 * (long)sizeof(cplx<V>) <= 65536;
@@ -431,8 +456,9 @@ struct Engine : GroupBase {
     else return fs64;
   }
   template <typename V>
-  int tile(int kind) const {
-    return std::is_same_v<V, T> ? ptile[kind] : 1;
+  int tile(int kind, int batch) const {
+    if (!std::is_same_v<V, T>) return 1;
+    return (lev_tiles && batch == B * Lmax && batch != B) ? ptileL[kind] : ptile[kind];
   }
 
   template <int NN, int P, int AXIS, class Op>
@@ -477,6 +503,32 @@ struct Engine : GroupBase {
   };
   Probe* probe = nullptr;
 
+  // the register-blocked passes: one wave per workgroup, 64 / (N2 / 16)
+  // FFTs per wave
+  template <int NN, int AXIS, class Op>
+  void launch_wave(const Op& op, const cplx<op_value_t<Op>>* stw, int nseq, int mult, int batch) {
+    using V = op_value_t<Op>;
+    constexpr int P = 64 / (NN / 16);
+    const int groups = (nseq + mult * P - 1) / (mult * P);
+    hipLaunchKernelGGL((wave::dst_pass_w<V, NN, 1, AXIS, Op>), dim3(groups, batch), dim3(64), 0, st, op, stw);
+  }
+  template <int NN, class Op>
+  void launch_wave_col2(const Op& op, const cplx<op_value_t<Op>>* stw, int nseq, int mult, int batch) {
+    using V = op_value_t<Op>;
+    constexpr int P = 64 / (NN / 16);
+    const int groups = (nseq + mult * P - 1) / (mult * P);
+    if constexpr (std::is_same_v<Op, Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>>) {
+      if (probe && probe->next < probe->ev.size()) {  // measurement: events on the dispatch itself
+        auto& e = probe->ev[probe->next++];
+        probe->fields += batch;
+        hipExtLaunchKernelGGL((wave::dst_col2_w<V, NN, 1, Op>), dim3(groups, batch), dim3(64), 0, st, e.first,
+                              e.second, 0, op, stw);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((wave::dst_col2_w<V, NN, 1, Op>), dim3(groups, batch), dim3(64), 0, st, op, stw);
+  }
+
   // Launch one pass over nseq sequences along the other axis; mult = 2 when
   // a workgroup's P FFTs take 2P sequences of one field, 1 when they pair
   // two fields (X + iY).  kind selects the tile shape.
@@ -487,7 +539,18 @@ struct Engine : GroupBase {
     const int n2 = AXIS == 0 ? F.pm.n2 : F.pn.n2;
     const bool sp = AXIS == 0 ? F.spec_m : F.spec_n;
     const cplx<V>* stw = AXIS == 0 ? F.stw_m.p : F.stw_n.p;
-    const int P = tile<V>(kind);
+    int P = tile<V>(kind, batch);
+    if (P == kWaveTile) {
+      if constexpr (wave::kOpOk<Op>) {
+        switch (n2) {
+          case 256: launch_wave<256, AXIS>(op, stw, nseq, mult, batch); return;
+          case 512: launch_wave<512, AXIS>(op, stw, nseq, mult, batch); return;
+          case 1024: launch_wave<1024, AXIS>(op, stw, nseq, mult, batch); return;
+          default: break;
+        }
+      }
+      P = 1;  // the same results on the LDS pass
+    }
     if (sp) {
       switch (n2 * 8 + P) {
 #define IBFS_SPEC_CASE(NN, PP)                                 \
@@ -533,7 +596,19 @@ struct Engine : GroupBase {
     using V = op_value_t<Op>;
     const FftSet<V>& F = fset<V>();
     const int n2 = ax ? F.pn.n2 : F.pm.n2;
-    const int P = tile<V>(kind);
+    int P = tile<V>(kind, batch);
+    if (P == kWaveTile) {
+      if constexpr (wave::kOpOk<Op>) {
+        const cplx<V>* stw = ax ? F.stw_n.p : F.stw_m.p;
+        switch (n2) {
+          case 256: launch_wave_col2<256>(op, stw, nseq, mult, batch); return;
+          case 512: launch_wave_col2<512>(op, stw, nseq, mult, batch); return;
+          case 1024: launch_wave_col2<1024>(op, stw, nseq, mult, batch); return;
+          default: break;
+        }
+      }
+      P = 1;
+    }
     if (ax ? F.spec_n : F.spec_m) {
       switch (n2 * 8 + P) {
 #define IBFS_SPEC_CASE(NN, PP)                               \
@@ -570,11 +645,32 @@ struct Engine : GroupBase {
   // (so clock ramp-up and noise hit all of them), and keeps the fastest.  The choice is cached
   // per (kind, m, n, element size, batch).  IBFS_TILE=P forces the largest
   // fitting value <= P instead.
+  // IBFS_TILE: one forced shape for every kind, or five comma-separated
+  // ones (row, column, column pair, step-1 row, step-1 column pair; 0 = tune);
+  // IBFS_TILE_LEVELS likewise for the level-batched launches (default: as
+  // IBFS_TILE)
+  static std::array<int, kKinds> forced_tiles(const char* var = "IBFS_TILE") {
+    std::array<int, kKinds> f{};
+    const char* env = std::getenv(var);
+    if (!env && std::strcmp(var, "IBFS_TILE") != 0) env = std::getenv("IBFS_TILE");
+    if (!env) return f;
+    const char* c = env;
+    for (int k = 0; k < kKinds; k++) {
+      f[k] = std::atoi(c);
+      const char* nx = std::strchr(c, ',');
+      if (!nx) {
+        if (k == 0)
+          for (int j = 1; j < kKinds; j++) f[j] = f[0];
+        break;
+      }
+      c = nx + 1;
+    }
+    return f;
+  }
   void tune() {
     static std::mutex mu;
     static std::map<std::array<int, 5>, int> cache;
-    const char* force = std::getenv("IBFS_TILE");
-    const int fp = force ? std::atoi(force) : 0;
+    const std::array<int, kKinds> ftile = forced_tiles();
     DevBuf<T> qz, bz;
     bool scratch = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -585,6 +681,8 @@ struct Engine : GroupBase {
       std::vector<int> cand;
       for (int P : {1, 2, 4})
         if (p_fits<T>(n2, P)) cand.push_back(P);
+      if (wave_size(n2) && k != kStep1Row) cand.push_back(kWaveTile);  // its loader is not register-blocked
+      const int fp = ftile[k];
       if (fp > 0) {
         int best = cand[0];
         for (int P : cand)
@@ -664,6 +762,101 @@ struct Engine : GroupBase {
       ptile[k] = best;
       std::lock_guard<std::mutex> lk(mu);
       cache[key] = best;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  // Tile shapes of the launches that transform every level of the cases at
+  // once (batch B Lmax, one field per item): the kinds those launches use,
+  // timed as tune() does on that batch (T1L in, T2L out).
+  void tune_levels() {
+    static std::mutex mu;
+    static std::map<std::array<int, 5>, int> cache;
+    const int BL = B * Lmax;
+    lev_tiles = true;
+    for (int k = 0; k < kKinds; k++) ptileL[k] = ptile[k];
+    const std::array<int, kKinds> ftile = forced_tiles("IBFS_TILE_LEVELS");
+    T1L.zero(st);
+    T2L.zero(st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    for (int k : {(int)kRow, (int)kCol, (int)kCol2, (int)kStep1Col2}) {
+      const bool rowtype = k == kRow;
+      const int n2 = rowtype ? fs.pm.n2 : fs.pn.n2;
+      if (!(rowtype ? fs.spec_m : fs.spec_n)) continue;
+      std::vector<int> cand;
+      for (int P : {1, 2, 4})
+        if (p_fits<T>(n2, P)) cand.push_back(P);
+      if (wave_size(n2)) cand.push_back(kWaveTile);
+      const int fp = ftile[k];
+      if (fp > 0) {
+        int best = cand[0];
+        for (int P : cand)
+          if (P <= fp) best = P;
+        ptileL[k] = best;
+        continue;
+      }
+      const std::array<int, 5> key{k, m, n, (int)sizeof(T), BL};
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+          ptileL[k] = it->second;
+          continue;
+        }
+      }
+      if (!e0) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+      }
+      auto run = [&] {
+        switch (k) {
+          case kRow:
+            launch<0>(RowOp<T, SrcField<T>, DstField<T>>{geoL, SrcField<T>{T1L.p, nw}, DstField<T>{T2L.p, nw},
+                                                         nullptr, 0},
+                      geo.nm1, 2, BL, kRow);
+            break;
+          case kCol:
+            launch<1>(ColOp<T, SrcField<T>, DstField<T>>{geoL, SrcField<T>{T1L.p, nw}, DstField<T>{T2L.p, nw},
+                                                         nullptr, 0},
+                      geo.mm1, 2, BL, kCol);
+            break;
+          case kCol2:
+            launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{geoL, SrcField<T>{T1L.p, nw},
+                                                                          MidScale<T>{laminv.p},
+                                                                          DstField<T>{T2L.p, nw}, nullptr, 0},
+                        geo.mm1, 2, BL, kCol2);
+            break;
+          default:
+            launch_col2(Col2Op<T, SrcField<T>, MidTab<T>, DstField<T>>{geoL, SrcField<T>{T1L.p, nw},
+                                                                       MidTab<T>{lam1i.p, nwp},
+                                                                       DstField<T>{T2L.p, nw}, nullptr, 0},
+                        geo.mm1, 2, BL, kStep1Col2);
+            break;
+        }
+      };
+      std::vector<float> tmin(cand.size(), 1e30f);
+      for (int round = 0; round < 3; round++)
+        for (size_t ci = 0; ci < cand.size(); ci++) {
+          ptileL[k] = cand[ci];
+          run();
+          run();
+          HIPCHK(hipEventRecord(e0, st));
+          constexpr int reps = 20;
+          for (int r = 0; r < reps; r++) run();
+          HIPCHK(hipEventRecord(e1, st));
+          HIPCHK(hipStreamSynchronize(st));
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          tmin[ci] = std::min(tmin[ci], ms);
+        }
+      size_t bi = 0;
+      for (size_t ci = 1; ci < cand.size(); ci++)
+        if (tmin[ci] < tmin[bi]) bi = ci;
+      ptileL[k] = cand[bi];
+      std::lock_guard<std::mutex> lk(mu);
+      cache[key] = cand[bi];
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -869,8 +1062,8 @@ struct Engine : GroupBase {
     // fused column pair per band depth and pair of opposite edges, the
     // bottom and top bands first, then the left and right ones (the bands
     // cross in the corners, so the two adds there are kept in order)
-    launch_col2(BandOp<T>{geoL, omega.p, Lmax, band_beta.p, band_w.p, 0, 2, m}, 2 * band_wmax, 2, BL, kCol2, 0);
-    launch_col2(BandOp<T>{geoL, omega.p, Lmax, band_beta.p, band_w.p, 2, 2, n}, 2 * band_wmax, 2, BL, kCol2, 1);
+    launch_col2(BandOp<T>{geoL, omega.p, Lmax, band_rho_m.p, band_w.p, 0, 2, m}, 2 * band_wmax, 2, BL, kCol2, 0);
+    launch_col2(BandOp<T>{geoL, omega.p, Lmax, band_rho_n.p, band_w.p, 2, 2, n}, 2 * band_wmax, 2, BL, kCol2, 1);
   }
   // omega += S (lam1i * S T1L) over every (case, level) item
   void step1_levels_add() {

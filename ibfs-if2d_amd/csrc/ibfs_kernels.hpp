@@ -388,6 +388,11 @@ struct dst_has_pre : std::false_type {};
 template <class D>
 struct dst_has_pre<D, std::void_t<decltype(&D::pre)>> : std::true_type {};
 // Src::post present (a loader with a side store)?
+// Src::kNoWave: keep this loader off the register-blocked passes
+template <class S, class = void>
+struct src_no_wave : std::false_type {};
+template <class S>
+struct src_no_wave<S, std::void_t<decltype(S::kNoWave)>> : std::bool_constant<S::kNoWave> {};
 template <class S, class = void>
 struct src_has_post : std::false_type {};
 template <class S>
@@ -413,6 +418,7 @@ struct src_gathered<S, std::void_t<typename S::Gathered>> {
 // Cases whose level count is <= lev are skipped (Lc == nullptr: none).
 template <typename T, class Src, class Dst>
 struct RowOp {
+  static constexpr bool kNoWave = src_no_wave<Src>::value;
   Geo g; Src src; Dst dst; const int* Lc; int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int row = gr * 2 * P + s;
@@ -491,6 +497,7 @@ struct RowOp {
 // single-field column pass: columns ii
 template <typename T, class Src, class Dst>
 struct ColOp {
+  static constexpr bool kNoWave = src_no_wave<Src>::value;
   Geo g; Src src; Dst dst; const int* Lc; int lev;
   __device__ T load(int b, int gr, int s, int P, int idx) const {
     const int col = gr * 2 * P + s;
@@ -837,10 +844,22 @@ struct MidTab {
 // cells from its edges, and get_bc reads it on lines min(m,n)/4 cells in,
 // so the two never meet within the launch.  Items of a case's coarsest
 // level and depths d >= w load 0 and store nothing.
+// rho_k of every (case, level) item b for an axis of N cells:
+// rho[b (N-1) + k-1], k = 1..N-1, from beta[b] (set-up; BandOp reads the
+// table, so every pass kernel that runs it gets the same factors)
+__global__ void k_band_rho(const double* __restrict__ beta, int N, int items, double* __restrict__ rho) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)items * (N - 1)) return;
+  const int b = (int)(t / (N - 1)), k = (int)(t - (long)b * (N - 1)) + 1;
+  const double bt = beta[b];
+  const double lx = fma(-2.0, cospi((double)k / (double)N), 2.0);
+  const double th = fma(bt, lx + 2.0, 1.0) / (2.0 * bt);
+  rho[t] = 1.0 / (th + sqrt(fma(th, th, -1.0)));
+}
 template <typename T>
 struct BandOp {
   Geo gr;  // the level geometry
-  T* om; int lm; const double* beta; const int* wlev;
+  T* om; int lm; const double* rho; const int* wlev;  // rho: k_band_rho's table for this axis
   int e0, eper, N;
   __device__ void place(int q, int& d, int& e) const {
     d = q / eper;
@@ -862,12 +881,9 @@ struct BandOp {
     place(grp * 2 * P + 2 * p, d, e);
     ca = cb = T(0);
     if (!on(b, d)) return;
-    const double bt = beta[b];
-    const double lx = 2.0 - 2.0 * cospi((double)k / (double)N);
-    const double th = (1.0 + bt * (lx + 2.0)) / (2.0 * bt);
-    const double rho = 1.0 / (th + sqrt(th * th - 1.0));
-    double pw = rho;
-    for (int t = 0; t < d; t++) pw *= rho;
+    const double r = rho[(long)b * (N - 1) + k - 1];
+    double pw = r;
+    for (int t = 0; t < d; t++) pw *= r;
     ca = cb = T(pw / (2.0 * N));
   }
   __device__ void mid_apply(int, int, int, int, int, T ya, T yb, T ca, T cb, T& va, T& vb) const {
@@ -1233,13 +1249,34 @@ __global__ __launch_bounds__(256) void k_ring_spectra(Geo g, const double* __res
   const int kk = ch * 32 + (threadIdx.x & 31) + 1, part = threadIdx.x >> 5;
   double acc = 0.0;
   if (kk <= K) {
-    int idx = (kk * (part + 1)) % (2 * N);
-    const int step = (kk * 8) % (2 * N);
-    for (int t = part; t < K; t += 8) {
-      acc += xs[t] * st[idx];
-      idx += step;
-      if (idx >= 2 * N) idx -= 2 * N;
+    // terms t = part, part + 8, ... in four interleaved chains (t mod 32), so
+    // that four LDS reads and FMAs are in flight; the sine place kk (t+1)
+    // mod 2N advances by 8 kk mod 2N per term
+    const int N2 = 2 * N;
+    const int step = (kk * 8) % N2;
+    int idx = (kk * (part + 1)) % N2;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int t = part;
+    for (; t + 24 < K; t += 32) {
+      int i1 = idx + step;
+      i1 -= i1 >= N2 ? N2 : 0;
+      int i2 = i1 + step;
+      i2 -= i2 >= N2 ? N2 : 0;
+      int i3 = i2 + step;
+      i3 -= i3 >= N2 ? N2 : 0;
+      a0 += xs[t] * st[idx];
+      a1 += xs[t + 8] * st[i1];
+      a2 += xs[t + 16] * st[i2];
+      a3 += xs[t + 24] * st[i3];
+      idx = i3 + step;
+      idx -= idx >= N2 ? N2 : 0;
     }
+    for (; t < K; t += 8) {
+      a0 += xs[t] * st[idx];
+      idx += step;
+      idx -= idx >= N2 ? N2 : 0;
+    }
+    acc = (a0 + a1) + (a2 + a3);
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -1256,6 +1293,7 @@ __global__ __launch_bounds__(256) void k_ring_spectra(Geo g, const double* __res
 // is Y(kk-1, jj) + (X_b(kk) G(kk, jj) + X_t(kk) G(kk, n-2-jj)), G = g / (2m)
 // along i; nothing is added on a case's coarsest level.
 struct SrcRingRow {
+  static constexpr bool kNoWave = true;  // five loads per element
   const double* Y; const double* X; const double* G; ItemMap im; int Kx;
   __device__ double operator()(const Geo& g, int b, int ii, int jj) const {
     const int item = im(b);
@@ -1263,18 +1301,20 @@ struct SrcRingRow {
     if (item % im.lm == im.lm - 1) return y;
     const double* xb = X + (long)item * 4 * Kx;
     const long K = g.mm1;
-    return y + (xb[ii] * G[jj * K + ii] + xb[Kx + ii] * G[(g.nm1 - 1 - jj) * K + ii]);
+    // explicit fma: the same rounding in every pass kernel that inlines it
+    return y + fma(xb[ii], G[jj * K + ii], xb[Kx + ii] * G[(g.nm1 - 1 - jj) * K + ii]);
   }
 };
 // The left and right edge terms as a column pass (ColOp loader): column ii
 // of H along j has the spectrum X_l(l) G'(l, ii) + X_r(l) G'(l, m-2-ii),
 // G' = g / (2n) along j.
 struct SrcRingCol {
+  static constexpr bool kNoWave = true;  // four loads per element
   const double* X; const double* G; ItemMap im; int Kx;
   __device__ double operator()(const Geo& g, int b, int ii, int l1) const {
     const double* xl = X + ((long)im(b) * 4 + 2) * Kx;
     const long K = g.nm1;
-    return xl[l1] * G[ii * K + l1] + xl[Kx + l1] * G[(g.mm1 - 1 - ii) * K + l1];
+    return fma(xl[l1], G[ii * K + l1], xl[Kx + l1] * G[(g.mm1 - 1 - ii) * K + l1]);
   }
 };
 // a = v or a += v on item im(b)

@@ -66,6 +66,7 @@ def _load():
     L.ibfs_time_copy.argtypes = [C.c_void_p, C.c_long, C.c_int, _D]
     L.ibfs_time_step_kernel.argtypes = [C.c_void_p, C.c_int, _D, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.ibfs_tile_shapes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.ibfs_level_tile_shapes.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
     L.ibfs_last_error.restype = C.c_char_p
     L.ibfs_last_error.argtypes = [C.c_void_p]
     L.ibfs_destroy.argtypes = [C.c_void_p]
@@ -336,6 +337,13 @@ class Solver:
         self._chk(_lib.ibfs_tile_shapes(self._h, group, C.byref(ng), out), "ibfs_tile_shapes")
         return dict(zip(("row", "col", "col2", "step1_row", "step1_col2"), list(out)))
 
+    def level_tile_shapes(self, group=0):
+        """The shapes of the launches that transform every level at once
+        (0: no such launch of that kind); 8 is the register-blocked kernel."""
+        out = (C.c_int * 5)()
+        self._chk(_lib.ibfs_level_tile_shapes(self._h, group, out), "ibfs_level_tile_shapes")
+        return dict(zip(("row", "col", "col2", "step1_row", "step1_col2"), list(out)))
+
     def time_step_kernel(self, nsteps=20):
         """Mean in-step duration (us) of the step's dominant kernel (the fused
         ctci column pair), from events on its own dispatches during nsteps
@@ -456,6 +464,7 @@ class Ensemble:
     __del__ = Solver.__del__
     _chk = Solver._chk
     tile_shapes = Solver.tile_shapes
+    level_tile_shapes = Solver.level_tile_shapes
     ngroups = Solver.ngroups
     time_dst_pass = Solver.time_dst_pass
     _ncases = Solver._ncases

@@ -248,11 +248,18 @@ int ibfs_last_advance_ms(const ibfs_handle* h, double* ms);
  * the autotuner chose for group `group` of the handle (groups = cases of one
  * grid size), for the row pass, column pass, fused column pair, step-1 row
  * pass and step-1 column pair (0: the generic pass of a size without a
- * specialised kernel).  *ngroups (may be NULL) receives the group count.
- * The tuner times every fitting shape on the group's batch when the handle
- * is created; IBFS_TILE=P in the environment forces P instead.  Results do
- * not depend on the tile shape. */
+ * specialised kernel; 8: the register-blocked kernel of FFT lengths 256,
+ * 512 and 1024, one wave per workgroup).  *ngroups (may be NULL) receives
+ * the group count.  The tuner times every fitting shape on the group's
+ * batch when the handle is created; IBFS_TILE=P in the environment forces
+ * the largest fitting shape <= P instead.  Results do not depend on the
+ * tile shape. */
 int ibfs_tile_shapes(const ibfs_handle* h, int group, int* ngroups, int* shapes);
+
+/* The same five shapes for the launches that transform every level of the
+ * group's cases at once (one or two cases with all levels), tuned on that
+ * batch; 0 for a kind without such a launch. */
+int ibfs_level_tile_shapes(const ibfs_handle* h, int group, int* shapes);
 
 /* Measurement only (no counterpart in the reference): launches one 1-D
  * DST-I pass over a W field per case of the handle's first group (one field

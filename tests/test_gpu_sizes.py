@@ -131,6 +131,51 @@ def test_tile_shapes_do_not_change_results(monkeypatch):
             assert np.array_equal(st[k], base[k]), k
 
 
+@pytest.mark.parametrize("m", [128, 256, 512])
+def test_register_blocked_passes_do_not_change_results(monkeypatch, m):
+    """The register-blocked kernels (tile code 8: FFT lengths 2m = 256, 512,
+    1024, 16 values per lane, two radix-4 stages per LDS exchange) run the
+    butterflies of the LDS passes with the same twiddles, so forcing them
+    (IBFS_TILE=8; the step-1 loader stays on P = 4), forcing P = 1, and the
+    tuned choice give identical states after 3 steps of a 4-level cylinder,
+    whose level-batched launches are tuned on their own batch."""
+    from ibfs_amd import hip
+
+    runs = {}
+    for tile in ("1", "8", None):
+        if tile is None:
+            monkeypatch.delenv("IBFS_TILE", raising=False)
+        else:
+            monkeypatch.setenv("IBFS_TILE", tile)
+        _, g = solver_pair(m, 4, 4.0, 2.0, 2.0, "circle", 0.5 * 4.0 / m, 100.0)
+        shapes, lshapes = g.tile_shapes(), g.level_tile_shapes()
+        if tile == "8":
+            assert shapes["row"] == 8 and shapes["col2"] == 8 and shapes["step1_row"] == 4
+        g.build_cholesky()
+        g.advance(3)
+        runs[tile] = (shapes, lshapes, g.get_state())
+    base = runs["1"][2]
+    for tile, (shapes, lshapes, st) in runs.items():
+        print("m", m, "tile", tile, shapes, "levels", lshapes,
+              {k: f"{relerr(st[k], base[k]):.1e}" for k in base if k != "q0"})
+    for tile, (shapes, lshapes, st) in runs.items():
+        for k in base:
+            assert np.array_equal(st[k], base[k]), k
+
+
+@pytest.mark.parametrize("m", [128, 256, 512])
+def test_register_blocked_transforms(monkeypatch, m, rng):
+    """The register-blocked kernels forced on (IBFS_TILE=8) for the operator
+    entry points: dst, ctci and ainv against the oracle."""
+    monkeypatch.setenv("IBFS_TILE", "8")
+    o, g = solver_pair(m, 3, 4.0, 2.0, 2.0, "circle", 0.004, 100.0)
+    assert g.tile_shapes()["row"] == 8
+    x = rng.standard_normal(o.nw)
+    errs = (relerr(g.dst(x), o.dst(x)), relerr(g.ctci(x), o.ctci(x)), relerr(g.ainv(x), o.ainv(x)))
+    print("m", m, "dst %.1e ctci %.1e ainv %.1e" % errs)
+    assert errs[0] < 1e-13 and errs[1] < 1e-12 and errs[2] < 1e-12
+
+
 def test_tandem_1024_six_levels():
     """The large case (1024^2 x 6 levels, dx 0.01, two d = 1 cylinders 3 d
     apart, 628 points, Re 200, dt 0.0015): 2 FP64 steps with the device
