@@ -627,6 +627,7 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_col2_n(Op op, const cplx<T>* _
   constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
   constexpr int NQ = (P * LEN + NT - 1) / NT;  // (FFT, k) places per thread
   constexpr bool kCoef = op_has_mid_coef<Op>::value;
+  constexpr bool kPre = op_store_pre<Op>::value;
   constexpr bool DB = kTwoBuf<T, N2, P>;
   __shared__ cplx<T> z[(DB ? 2 : 1) * P * N2];
   cplx<T>* const z2 = DB ? z + P * N2 : z;
@@ -644,6 +645,19 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_col2_n(Op op, const cplx<T>* _
       if (e < P * LEN) {
         const int k = e / P + 1, p = e - (k - 1) * P;
         op.mid_coef(b, g, p, P, k, ca[q], cb[q]);
+      }
+    }
+  }
+  // old values of a read-modify-write destination, read with the inputs
+  T oa[kPre ? NQ : 1], ob[kPre ? NQ : 1];
+  if constexpr (kPre) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int e = t + q * NT;
+      oa[q] = ob[q] = T(0);
+      if (e < P * LEN) {
+        const int k = e / P + 1, p = e - (k - 1) * P;
+        op.store_pre(b, g, p, P, k, oa[q], ob[q]);
       }
     }
   }
@@ -687,7 +701,8 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_col2_n(Op op, const cplx<T>* _
     if (e < P * LEN) {
       const int k = e / P + 1, p = e - (k - 1) * P;
       const cplx<T> Z = zr[p * N2 + k];
-      op.store(b, g, p, P, k, -Z.im, Z.re);
+      if constexpr (kPre) op.store_put(b, g, p, P, k, -Z.im, Z.re, oa[q], ob[q]);
+      else op.store(b, g, p, P, k, -Z.im, Z.re);
     }
   }
 }

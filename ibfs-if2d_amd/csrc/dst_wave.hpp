@@ -263,6 +263,7 @@ template <typename T, int N2, int W, class Op>
 __global__ __launch_bounds__(64 * W) void dst_col2_w(Op op, const cplx<T>* __restrict__ tw) {
   constexpr int TPF = N2 / 16, P = W * 64 / TPF;
   constexpr bool kCoef = op_has_mid_coef<Op>::value;
+  constexpr bool kPre = op_store_pre<Op>::value;
   __shared__ cplx<T> zs[P * kLdsPerFft<N2>];
   const int lane = threadIdx.x & 63, p = threadIdx.x / TPF, t = threadIdx.x % TPF;
   const int fl = lane - t;
@@ -276,6 +277,15 @@ __global__ __launch_bounds__(64 * W) void dst_col2_w(Op op, const cplx<T>* __res
       const int k = t + TPF * q;
       ca[q] = cb[q] = T(0);
       if (k >= 1) op.mid_coef(b, g, p, P, k, ca[q], cb[q]);
+    }
+  }
+  T oa[kPre ? 8 : 1], ob[kPre ? 8 : 1];
+  if constexpr (kPre) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = t + TPF * q;
+      oa[q] = ob[q] = T(0);
+      if (k >= 1) op.store_pre(b, g, p, P, k, oa[q], ob[q]);
     }
   }
   Tw<T, N2> w;
@@ -299,7 +309,10 @@ __global__ __launch_bounds__(64 * W) void dst_col2_w(Op op, const cplx<T>* __res
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int k = t + TPF * q;
-    if (k >= 1) op.store(b, g, p, P, k, -v[q].im, v[q].re);
+    if (k >= 1) {
+      if constexpr (kPre) op.store_put(b, g, p, P, k, -v[q].im, v[q].re, oa[q], ob[q]);
+      else op.store(b, g, p, P, k, -v[q].im, v[q].re);
+    }
   }
 }
 

@@ -1041,9 +1041,10 @@ struct Engine : GroupBase {
       if (src_split) {
         // the source of every level as its own kernel (into T2L), then a
         // plain row pass
-        const int sg = s1f.side_groups_for(256);
-        hipLaunchKernelGGL((k_src_eval<T, SrcStep1<T, true>>), dim3(sg + (nw + 255) / 256, BL), dim3(256), 0, st,
-                           geoL, s1f, T2L.p, sg);
+        const int sg = s1f.side_groups_for(256);  // a multiple of 8
+        const int eg = ((nw + 255) / 256 + 7) & ~7;
+        hipLaunchKernelGGL((k_src_eval<T, SrcStep1<T, true>>), dim3(sg + eg, BL), dim3(256), 0, st, geoL, s1f, T2L.p,
+                           sg);
         launch<0>(RowOp<T, SrcField<T>, DstField<T>>{geoL, SrcField<T>{T2L.p, nw}, DstField<T>{T1L.p, nw}, nullptr,
                                                      0},
                   geo.nm1, 2, BL);

@@ -908,6 +908,29 @@ struct BandOp {
     add(b, e, d, k - 1, ya);
     add(b, e + 1, d, k - 1, yb);
   }
+  // the add in two parts: the old band values read with the inputs, then
+  // old + correction (no other workgroup of the launch writes them)
+  static constexpr bool kStorePre = true;
+  __device__ T* at(int b, int e, int d, int pos) const {
+    const int ii = e < 2 ? pos : (e == 2 ? d : gr.mm1 - 1 - d);
+    const int jj = e < 2 ? (e == 0 ? d : gr.nm1 - 1 - d) : pos;
+    return om + (long)b * gr.nw + ii + (long)jj * gr.mm1;
+  }
+  __device__ void store_pre(int b, int grp, int p, int P, int k, T& oa, T& ob) const {
+    int d, e;
+    place(grp * 2 * P + 2 * p, d, e);
+    oa = ob = T(0);
+    if (!on(b, d)) return;
+    oa = *at(b, e, d, k - 1);
+    ob = *at(b, e + 1, d, k - 1);
+  }
+  __device__ void store_put(int b, int grp, int p, int P, int k, T ya, T yb, T oa, T ob) const {
+    int d, e;
+    place(grp * 2 * P + 2 * p, d, e);
+    if (!on(b, d)) return;
+    *at(b, e, d, k - 1) = oa + ya;
+    *at(b, e + 1, d, k - 1) = ob + yb;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -1157,18 +1180,19 @@ __global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restr
 // (L = NP + 2 levels, Tp: T^1..T^NP, NR x NR each) into Rb (layout of V).
 // Workgroup: four ring rows p, one per wave, with the case's V_1..V_{L-1}
 // staged in LDS.  The powers are the kernel's whole HBM traffic (NP NR^2
-// doubles), so each wave issues the loads of its first kChainU entries per
-// lane of every power before it stages V and waits at the barrier, and
-// keeps NP kChainU loads in flight after that.  With sring, R_1 is also
+// doubles), so each wave issues the loads of its first U entries per lane of
+// every power before it stages V and waits at the barrier, and keeps NP U
+// loads in flight after that (U = 17 covers a 1028-entry ring, 512^2, in
+// one round trip while NP <= 3).  With sring, R_1 is also
 // written at the ring places of level 1 of sring (a field in the
 // streamfunction's layout) for get_bc(s_1).  grid (ceil(NR/4), B).
 constexpr int kRingMaxLev = 8;
-constexpr int kChainU = 8;
+constexpr int chain_u(int np) { return np <= 3 ? 17 : 8; }
 template <int NP>
 __global__ __launch_bounds__(256) void k_ring_chain(Geo g, long fbs, int NR, const double* __restrict__ V,
                                                     const double* __restrict__ Tp, double* __restrict__ Rb,
                                                     double* __restrict__ sring) {
-  constexpr int L = NP + 2, U = kChainU;
+  constexpr int L = NP + 2, U = chain_u(NP);
   extern __shared__ double v[];
   const int b = blockIdx.y;
   const double* vb = V + ((long)b * L + 1) * NR;
@@ -1346,8 +1370,11 @@ struct DstAdd {
 // side_groups workgroups) done as the pass would do them.  The level-batched
 // step 1 runs its source this way: one vertex per thread keeps every load of
 // the stencil in flight with few registers, and the row pass that follows
-// reads one value per element.  grid (side_groups + ceil(nw / 256), items),
-// 256 threads.
+// reads one value per element.  The element blocks run in XCD-contiguous
+// order (gridDim.x - side_groups is a multiple of 8, surplus blocks exit), so
+// the rows a block's stencil shares with its neighbours stay in one L2.
+// grid (side_groups + ceil(nw / 256) rounded up to a multiple of 8, items),
+// 256 threads; side_groups a multiple of 8.
 template <typename T, class Src>
 __global__ __launch_bounds__(256) void k_src_eval(Geo g, Src src, T* __restrict__ out, int side_groups) {
   const int b = blockIdx.y;
@@ -1359,6 +1386,7 @@ __global__ __launch_bounds__(256) void k_src_eval(Geo g, Src src, T* __restrict_
     }
     x -= side_groups;
   }
+  x = xcd_contiguous(x, (int)gridDim.x - side_groups);
   const int e = x * 256 + threadIdx.x;
   if (e >= g.nw) return;
   const int jj = e / g.mm1, ii = e - jj * g.mm1;
