@@ -316,5 +316,163 @@ __global__ __launch_bounds__(64 * W) void dst_col2_w(Op op, const cplx<T>* __res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Column passes with staged loads and stores.  The register-blocked FFT reads
+// its 16 places per lane at 16 different rows when the sequences are
+// columns, so one wave's loads touch one cache line per lane for 16 bytes of
+// it.  Here the four waves of a workgroup (P = 4 64 / TPF FFTs, 2P adjacent
+// columns) first load the 2P columns together, P pairs of adjacent values
+// per row (64 bytes for 2m = 1024, a whole 128-byte line for 512), into LDS,
+// each wave then takes its places from there; the outputs go back through
+// LDS the same way.  The staging area is the FFTs' exchange area (it fits:
+// 2P (N-1) values against P (N2 + N2/16) complex ones), so it costs three
+// workgroup barriers and no extra LDS.  Any loader (several loads per
+// element included) and any store; not the prefetched read-modify-write
+// stores (store_pre), which run on the other passes.
+template <class Op>
+constexpr bool kStagedOk = !op_side<Op>::value && !op_gather<Op>::value && !op_load_post<Op>::value &&
+                           !op_store_pre<Op>::value;
+constexpr int kStagedWaves = 4;
+
+// the 2P sequences of the workgroup, rows 0..LEN-1, into stage[s LEN + r]
+template <typename T, int N2, class Op>
+__device__ __forceinline__ void staged_load(const Op& op, T* stage, int b, int g) {
+  constexpr int TPF = N2 / 16, P = kStagedWaves * 64 / TPF, LEN = N2 / 2 - 1, NT = kStagedWaves * 64;
+  constexpr int NE = P * LEN, NL = (NE + NT - 1) / NT;
+  T xa[NL], xb[NL];
+#pragma unroll
+  for (int i = 0; i < NL; i++) {  // element e: pair e mod P of row e / P (consecutive threads: one row)
+    const int e = threadIdx.x + NT * i;
+    const int ec = e < NE ? e : NE - 1;
+    const int pr = ec % P, r = ec / P;
+    xa[i] = op.load(b, g, 2 * pr, P, r);
+    xb[i] = op.load(b, g, 2 * pr + 1, P, r);
+  }
+#pragma unroll
+  for (int i = 0; i < NL; i++) {
+    const int e = threadIdx.x + NT * i;
+    if (e < NE) {
+      const int pr = e % P, r = e / P;
+      stage[(2 * pr) * LEN + r] = xa[i];
+      stage[(2 * pr + 1) * LEN + r] = xb[i];
+    }
+  }
+}
+// places t + TPF q (q < 8) of FFT p from the staging area, then the mirror
+template <typename T, int N2>
+__device__ __forceinline__ void staged_take(const T* stage, cplx<T>* v, int p, int t, int fl) {
+  constexpr int TPF = N2 / 16, LEN = N2 / 2 - 1;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int idx = t + TPF * q - 1;
+    const int ic = idx < 0 ? 0 : idx;
+    const T a = stage[(2 * p) * LEN + ic], c = stage[(2 * p + 1) * LEN + ic];
+    v[q] = idx < 0 ? cplx<T>{T(0), T(0)} : cplx<T>{a, c};
+  }
+  mirror_upper<T, TPF>(v, t, fl);
+}
+// outputs Y(k) = (-Im Z_k, Re Z_k) of FFT p at the lane's places, into the
+// staging area (row k-1 of sequences 2p, 2p+1)
+template <typename T, int N2>
+__device__ __forceinline__ void staged_put(T* stage, const cplx<T>* v, int p, int t) {
+  constexpr int TPF = N2 / 16, LEN = N2 / 2 - 1;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int k = t + TPF * q;
+    if (k >= 1) {
+      stage[(2 * p) * LEN + k - 1] = -v[q].im;
+      stage[(2 * p + 1) * LEN + k - 1] = v[q].re;
+    }
+  }
+}
+template <typename T, int N2, class Op>
+__device__ __forceinline__ void staged_store(const Op& op, const T* stage, int b, int g) {
+  constexpr int TPF = N2 / 16, P = kStagedWaves * 64 / TPF, LEN = N2 / 2 - 1, NT = kStagedWaves * 64;
+  constexpr int NE = P * LEN, NL = (NE + NT - 1) / NT;
+#pragma unroll
+  for (int i = 0; i < NL; i++) {
+    const int e = threadIdx.x + NT * i;
+    if (e < NE) {
+      const int pr = e % P, r = e / P;
+      op.store(b, g, pr, P, r + 1, stage[(2 * pr) * LEN + r], stage[(2 * pr + 1) * LEN + r]);
+    }
+  }
+}
+
+// one DST-I along the columns (AXIS 1 pass)
+template <typename T, int N2, class Op>
+__global__ __launch_bounds__(kStagedWaves * 64) void dst_col_ws(Op op, const cplx<T>* __restrict__ tw) {
+  constexpr int TPF = N2 / 16, P = kStagedWaves * 64 / TPF;
+  static_assert(2 * P * (N2 / 2 - 1) * sizeof(T) <= P * kLdsPerFft<N2> * sizeof(cplx<T>), "staging area");
+  __shared__ cplx<T> zs[P * kLdsPerFft<N2>];
+  T* const stage = reinterpret_cast<T*>(zs);
+  const int lane = threadIdx.x & 63, p = threadIdx.x / TPF, t = threadIdx.x % TPF;
+  const int fl = lane - t;
+  const int b = blockIdx.y;
+  int g = blockIdx.x;
+  if ((gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  Tw<T, N2> w;
+  w.load(tw, t);
+  staged_load<T, N2, Op>(op, stage, b, g);
+  __syncthreads();
+  cplx<T> v[16];
+  staged_take<T, N2>(stage, v, p, t, fl);
+  __syncthreads();  // the exchange areas overlap other FFTs' staged columns
+  fft<T, N2>(v, zs + p * kLdsPerFft<N2>, w, t);
+  __syncthreads();
+  staged_put<T, N2>(stage, v, p, t);
+  __syncthreads();
+  staged_store<T, N2, Op>(op, stage, b, g);
+}
+
+// the fused column pair (dst_col2 of dst_core.hpp) with staged columns
+template <typename T, int N2, class Op>
+__global__ __launch_bounds__(kStagedWaves * 64) void dst_col2_ws(Op op, const cplx<T>* __restrict__ tw) {
+  constexpr int TPF = N2 / 16, P = kStagedWaves * 64 / TPF;
+  static_assert(2 * P * (N2 / 2 - 1) * sizeof(T) <= P * kLdsPerFft<N2> * sizeof(cplx<T>), "staging area");
+  constexpr bool kCoef = op_has_mid_coef<Op>::value;
+  __shared__ cplx<T> zs[P * kLdsPerFft<N2>];
+  T* const stage = reinterpret_cast<T*>(zs);
+  const int lane = threadIdx.x & 63, p = threadIdx.x / TPF, t = threadIdx.x % TPF;
+  const int fl = lane - t;
+  const int b = blockIdx.y;
+  int g = blockIdx.x;
+  if ((gridDim.x & 7) == 0) g = (g & 7) * (gridDim.x >> 3) + (g >> 3);
+  T ca[kCoef ? 8 : 1], cb[kCoef ? 8 : 1];
+  if constexpr (kCoef) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = t + TPF * q;
+      ca[q] = cb[q] = T(0);
+      if (k >= 1) op.mid_coef(b, g, p, P, k, ca[q], cb[q]);
+    }
+  }
+  Tw<T, N2> w;
+  w.load(tw, t);
+  staged_load<T, N2, Op>(op, stage, b, g);
+  __syncthreads();
+  cplx<T> v[16];
+  staged_take<T, N2>(stage, v, p, t, fl);
+  __syncthreads();
+  cplx<T>* zf = zs + p * kLdsPerFft<N2>;
+  fft<T, N2>(v, zf, w, t);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int k = t + TPF * q;
+    T va = T(0), vb = T(0);
+    if (k >= 1) {
+      if constexpr (kCoef) op.mid_apply(b, g, p, P, k, -v[q].im, v[q].re, ca[q], cb[q], va, vb);
+      else op.mid(b, g, p, P, k, -v[q].im, v[q].re, va, vb);
+    }
+    v[q] = {va, vb};
+  }
+  mirror_upper<T, TPF>(v, t, fl);
+  fft<T, N2>(v, zf, w, t);
+  __syncthreads();
+  staged_put<T, N2>(stage, v, p, t);
+  __syncthreads();
+  staged_store<T, N2, Op>(op, stage, b, g);
+}
+
 }  // namespace wave
 }  // namespace ibfs

@@ -443,6 +443,9 @@ struct Engine : GroupBase {
   // tile code of the register-blocked kernels (dst_wave.hpp): one FFT per
   // 64 / TPF lanes, one wave per workgroup; bit-identical to P = 1
   static constexpr int kWaveTile = 8;
+  // tile code of the column passes with staged loads and stores
+  // (dst_col_ws, dst_col2_ws): four waves, four FFTs at 2m = 1024
+  static constexpr int kStagedTile = 16;
   static constexpr bool wave_size(int n2) { return n2 == 256 || n2 == 512 || n2 == 1024; }
   template <typename V = T>
   static constexpr bool p_fits(int n2, int P) {
@@ -529,6 +532,34 @@ struct Engine : GroupBase {
     hipLaunchKernelGGL((wave::dst_col2_w<V, NN, 1, Op>), dim3(groups, batch), dim3(64), 0, st, op, stw);
   }
 
+  // the staged column passes: four waves per workgroup, 4 64 / (N2 / 16)
+  // FFTs per workgroup
+  template <int NN, class Op>
+  void launch_staged(const Op& op, const cplx<op_value_t<Op>>* stw, int nseq, int mult, int batch) {
+    using V = op_value_t<Op>;
+    constexpr int P = wave::kStagedWaves * 64 / (NN / 16);
+    const int groups = (nseq + mult * P - 1) / (mult * P);
+    hipLaunchKernelGGL((wave::dst_col_ws<V, NN, Op>), dim3(groups, batch), dim3(wave::kStagedWaves * 64), 0, st, op,
+                       stw);
+  }
+  template <int NN, class Op>
+  void launch_staged_col2(const Op& op, const cplx<op_value_t<Op>>* stw, int nseq, int mult, int batch) {
+    using V = op_value_t<Op>;
+    constexpr int P = wave::kStagedWaves * 64 / (NN / 16);
+    const int groups = (nseq + mult * P - 1) / (mult * P);
+    if constexpr (std::is_same_v<Op, Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>>) {
+      if (probe && probe->next < probe->ev.size()) {  // measurement: events on the dispatch itself
+        auto& e = probe->ev[probe->next++];
+        probe->fields += batch;
+        hipExtLaunchKernelGGL((wave::dst_col2_ws<V, NN, Op>), dim3(groups, batch), dim3(wave::kStagedWaves * 64), 0,
+                              st, e.first, e.second, 0, op, stw);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((wave::dst_col2_ws<V, NN, Op>), dim3(groups, batch), dim3(wave::kStagedWaves * 64), 0, st,
+                       op, stw);
+  }
+
   // Launch one pass over nseq sequences along the other axis; mult = 2 when
   // a workgroup's P FFTs take 2P sequences of one field, 1 when they pair
   // two fields (X + iY).  kind selects the tile shape.
@@ -540,6 +571,17 @@ struct Engine : GroupBase {
     const bool sp = AXIS == 0 ? F.spec_m : F.spec_n;
     const cplx<V>* stw = AXIS == 0 ? F.stw_m.p : F.stw_n.p;
     int P = tile<V>(kind, batch);
+    if (P == kStagedTile) {
+      if constexpr (AXIS == 1 && wave::kStagedOk<Op>) {
+        switch (n2) {
+          case 256: launch_staged<256>(op, stw, nseq, mult, batch); return;
+          case 512: launch_staged<512>(op, stw, nseq, mult, batch); return;
+          case 1024: launch_staged<1024>(op, stw, nseq, mult, batch); return;
+          default: break;
+        }
+      }
+      P = kWaveTile;
+    }
     if (P == kWaveTile) {
       if constexpr (wave::kOpOk<Op>) {
         switch (n2) {
@@ -597,9 +639,22 @@ struct Engine : GroupBase {
     const FftSet<V>& F = fset<V>();
     const int n2 = ax ? F.pn.n2 : F.pm.n2;
     int P = tile<V>(kind, batch);
+    const cplx<V>* stw = ax ? F.stw_n.p : F.stw_m.p;
+    if (P == kStagedTile) {
+      if constexpr (wave::kStagedOk<Op>) {
+        if (ax == 1) {
+          switch (n2) {
+            case 256: launch_staged_col2<256>(op, stw, nseq, mult, batch); return;
+            case 512: launch_staged_col2<512>(op, stw, nseq, mult, batch); return;
+            case 1024: launch_staged_col2<1024>(op, stw, nseq, mult, batch); return;
+            default: break;
+          }
+        }
+      }
+      P = kWaveTile;
+    }
     if (P == kWaveTile) {
       if constexpr (wave::kOpOk<Op>) {
-        const cplx<V>* stw = ax ? F.stw_n.p : F.stw_m.p;
         switch (n2) {
           case 256: launch_wave_col2<256>(op, stw, nseq, mult, batch); return;
           case 512: launch_wave_col2<512>(op, stw, nseq, mult, batch); return;
@@ -682,6 +737,7 @@ struct Engine : GroupBase {
       for (int P : {1, 2, 4})
         if (p_fits<T>(n2, P)) cand.push_back(P);
       if (wave_size(n2) && k != kStep1Row) cand.push_back(kWaveTile);  // its loader is not register-blocked
+      if (wave_size(n2) && !rowtype) cand.push_back(kStagedTile);
       const int fp = ftile[k];
       if (fp > 0) {
         int best = cand[0];
@@ -789,6 +845,7 @@ struct Engine : GroupBase {
       for (int P : {1, 2, 4})
         if (p_fits<T>(n2, P)) cand.push_back(P);
       if (wave_size(n2)) cand.push_back(kWaveTile);
+      if (wave_size(n2) && !rowtype) cand.push_back(kStagedTile);
       const int fp = ftile[k];
       if (fp > 0) {
         int best = cand[0];
@@ -1420,10 +1477,13 @@ struct Engine : GroupBase {
     cg_host = nullptr;
   }
   // The a_times form: one graph per iteration kind ([0] r -= alpha q, [1]
-  // r = b - A x).  The dense form: one graph of kCgDense iterations, whose
-  // kernels read the iteration count and the done flag on the device (the
-  // product A x of a residual recompute is skipped in other iterations).
-  static constexpr int kCgDense = 32;
+  // r = b - A x).  The dense form: one graph of kCgDense = 50 iterations
+  // aligned with cjgr's residual recompute: 49 iterations of the product
+  // A d and the fused iteration kernel (k_cg_iter), then the 50th with the
+  // product A d, alpha, the product A x and the recomputed residual.  The
+  // kernels read the done flag on the device, and every replay starts at an
+  // iteration count that is a multiple of 50.
+  static constexpr int kCgDense = 50;
   void cg_capture() {
     const int nf = cs[0].nf;
     if (cg_have_dense) {
@@ -1434,6 +1494,11 @@ struct Engine : GroupBase {
       for (int k = 0; k < kCgDense; k++) {
         hipLaunchKernelGGL(k_gemv_dense, gg, dim3(256), 0, st, nf, cgA.p, cg_d.p, cgPart.p, cg_nslice,
                            (const CgState*)cg_dev.p, 0);
+        if (k + 1 < kCgDense) {
+          hipLaunchKernelGGL(k_cg_iter, dim3(1), dim3(256), 0, st, nf, cg_d.p, cg_q.p, fb.p, cg_r.p, cg_dev.p,
+                             (const double*)cgPart.p, cg_nslice);
+          continue;
+        }
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, st, nf, cg_d.p, cg_q.p, fb.p, cg_dev.p,
                            (const double*)cgPart.p, cg_nslice);
         hipLaunchKernelGGL(k_gemv_dense, gg, dim3(256), 0, st, nf, cgA.p, (const double*)fb.p, cgPart.p, cg_nslice,

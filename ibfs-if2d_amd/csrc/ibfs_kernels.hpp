@@ -2195,6 +2195,46 @@ static __global__ __launch_bounds__(256) void k_cg_resid(int nf, const double* _
   for (int f = threadIdx.x; f < nf; f += blockDim.x) d[f] = r[f] + beta * d[f];
 }
 
+// One CG iteration that does not recompute the residual, after q = A d
+// (k_gemv_dense slices): k_cg_alpha and then k_cg_resid with r = r - alpha q,
+// in one workgroup, with the same arithmetic in the same order, so the
+// iterate is bit-identical to the two kernels'.  The dense form's graph runs
+// it for 49 of every 50 iterations (iter % 50 != 0 after the increment);
+// every 50th runs the two kernels with the recomputed residual.
+static __global__ __launch_bounds__(256) void k_cg_iter(int nf, double* __restrict__ d, double* __restrict__ q,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 CgState* __restrict__ cs, const double* __restrict__ part,
+                                                 int nslice) {
+#pragma clang fp contract(off)
+  __shared__ double alpha, beta;
+  if (cs->done) return;
+  form_q(nf, part, nslice, q);
+  const double dq = dot_tree(nf, d, q);
+  if (threadIdx.x == 0) {
+    alpha = cs->delta_new / dq;
+    cs->alpha = alpha;
+    cs->iter = cs->iter + 1;
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+    x[f] = x[f] + alpha * d[f];
+    r[f] = r[f] - alpha * q[f];
+  }
+  __syncthreads();
+  const double rr = dot_tree(nf, r, r);
+  if (threadIdx.x == 0) {
+    const double dold = cs->delta_new;
+    const double dnew = rr;
+    cs->delta_old = dold;
+    cs->delta_new = dnew;
+    beta = dnew / dold;
+    cs->beta = beta;
+    cs->done = !(cs->iter < cs->max_iter && dnew > cs->eps);
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) d[f] = r[f] + beta * d[f];
+}
+
 // ---------------------------------------------------------------------------
 // tail of the step: fluxes, circulation and diagnostics
 // ---------------------------------------------------------------------------

@@ -134,15 +134,16 @@ def test_tile_shapes_do_not_change_results(monkeypatch):
 @pytest.mark.parametrize("m", [128, 256, 512])
 def test_register_blocked_passes_do_not_change_results(monkeypatch, m):
     """The register-blocked kernels (tile code 8: FFT lengths 2m = 256, 512,
-    1024, 16 values per lane, two radix-4 stages per LDS exchange) run the
-    butterflies of the LDS passes with the same twiddles, so forcing them
-    (IBFS_TILE=8; the step-1 loader stays on P = 4), forcing P = 1, and the
-    tuned choice give identical states after 3 steps of a 4-level cylinder,
-    whose level-batched launches are tuned on their own batch."""
+    1024, 16 values per lane, two radix-4 stages per LDS exchange; tile code
+    16: the column passes with staged loads and stores) run the butterflies
+    of the LDS passes with the same twiddles, so forcing them (IBFS_TILE=8 or
+    16; the step-1 loader stays on P = 4), forcing P = 1, and the tuned
+    choice give identical states after 3 steps of a 4-level cylinder, whose
+    level-batched launches are tuned on their own batch."""
     from ibfs_amd import hip
 
     runs = {}
-    for tile in ("1", "8", None):
+    for tile in ("1", "8", "16", None):
         if tile is None:
             monkeypatch.delenv("IBFS_TILE", raising=False)
         else:
@@ -151,6 +152,8 @@ def test_register_blocked_passes_do_not_change_results(monkeypatch, m):
         shapes, lshapes = g.tile_shapes(), g.level_tile_shapes()
         if tile == "8":
             assert shapes["row"] == 8 and shapes["col2"] == 8 and shapes["step1_row"] == 4
+        if tile == "16":
+            assert shapes["row"] == 8 and shapes["col"] == 16 and shapes["col2"] == 16 and lshapes["col2"] == 16
         g.build_cholesky()
         g.advance(3)
         runs[tile] = (shapes, lshapes, g.get_state())
@@ -163,13 +166,15 @@ def test_register_blocked_passes_do_not_change_results(monkeypatch, m):
             assert np.array_equal(st[k], base[k]), k
 
 
+@pytest.mark.parametrize("tile", ["8", "16"])
 @pytest.mark.parametrize("m", [128, 256, 512])
-def test_register_blocked_transforms(monkeypatch, m, rng):
-    """The register-blocked kernels forced on (IBFS_TILE=8) for the operator
-    entry points: dst, ctci and ainv against the oracle."""
-    monkeypatch.setenv("IBFS_TILE", "8")
+def test_register_blocked_transforms(monkeypatch, m, tile, rng):
+    """The register-blocked kernels forced on (IBFS_TILE=8, or 16 for the
+    staged column passes) for the operator entry points: dst, ctci and ainv
+    against the oracle."""
+    monkeypatch.setenv("IBFS_TILE", tile)
     o, g = solver_pair(m, 3, 4.0, 2.0, 2.0, "circle", 0.004, 100.0)
-    assert g.tile_shapes()["row"] == 8
+    assert g.tile_shapes()["row"] == 8 and g.tile_shapes()["col2"] == int(tile)
     x = rng.standard_normal(o.nw)
     errs = (relerr(g.dst(x), o.dst(x)), relerr(g.ctci(x), o.ctci(x)), relerr(g.ainv(x), o.ainv(x)))
     print("m", m, "dst %.1e ctci %.1e ainv %.1e" % errs)
