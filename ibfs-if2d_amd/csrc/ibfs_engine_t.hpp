@@ -143,6 +143,12 @@ struct Engine : GroupBase {
   // holds the level-1 ring values of step 2 (its get_bc for the force
   // right-hand side)
   bool v2f_batch = false;
+  // the rows swin[0..1] and columns swin[2..3] of level 0 that hold every
+  // vertex the force stencils read (upload_smear); swin_ok when there are
+  // such vertices and no stencil face lies on the domain edge.
+  // IBFS_S_WINDOW=0: step 2 transforms the whole of level 0.
+  bool swin_ok = false;
+  int swin[4] = {0, 0, 0, 0};
   int ringN = 0;
   DevBuf<double> ringT, ringG, ringGp, ringR, ringS, ringV, ringX, ringSin;
   // the band correction (BandOp): beta per item, the band width per level
@@ -150,6 +156,11 @@ struct Engine : GroupBase {
   DevBuf<int> band_w;
   std::vector<int> hband_w;
   int band_wmax = 0;
+  // the real-space form of the band correction (k_band_direct): H_d(x) per
+  // item, axis and depth, |x| <= band_R; IBFS_BAND_FFT=1 keeps BandOp
+  bool band_direct = false;
+  int band_R = 0;
+  DevBuf<double> band_h;
 
   long fbs() const { return (long)Lmax * nw; }
   long tbs() const { return (long)Lmax * nwp; }  // case stride of lam1i (pair-major tables)
@@ -303,6 +314,8 @@ struct Engine : GroupBase {
                          band_beta.p, n, BL, band_rho_n.p);
       HIPCHK(hipGetLastError());
       band_wmax = *std::max_element(hband_w.begin(), hband_w.end());
+      const char* bf = std::getenv("IBFS_BAND_FFT");
+      band_direct = !(bf && bf[0] == '1') && build_band_kernels(hb);
       HIPCHK(hipStreamSynchronize(st));
     }
     v2f_batch = v2f_can_batch();
@@ -314,8 +327,7 @@ struct Engine : GroupBase {
   // k_ring_gtab): the decomposition s_k = ctci(omega_k) + H(sbc_k) is exact
   // for any size, so this asks only that the FP64 engine solve one or two
   // cases with all levels (batches are throughput-bound), at most
-  // kRingMaxLev levels, at most 1 GiB of powers of T, and the ring vectors
-  // of the levels above the finest in 64 KiB of LDS (k_ring_chain);
+  // kRingMaxLev levels and at most 1 GiB of powers of T;
   // IBFS_V2F_LEVELS=0 switches it off.
   bool v2f_can_batch() const {
     const char* env = std::getenv("IBFS_V2F_LEVELS");
@@ -323,7 +335,6 @@ struct Engine : GroupBase {
     if constexpr (!std::is_same_v<T, double>) return false;
     if (gate != nullptr || Lmax < 2 || B > 2 || Lmax > kRingMaxLev) return false;
     const double nr = ring_count(m, n);
-    if ((Lmax - 1) * nr * 8.0 > 65536.0) return false;
     return (double)std::max(Lmax - 2, 1) * nr * nr * 8.0 <= double(1L << 30);
   }
   // the tables of the batched vort2flux: g / (2N) along each axis, T from its
@@ -398,6 +409,59 @@ struct Engine : GroupBase {
   // each edge of level k, w_k = ceil(ln(1e17) / kappa_k) for the level's own
   // beta (the dropped terms are below 1e-17 of the edge source), which must
   // stay under a quarter of the grid (hband_w; filled here).
+  // The kernels of k_band_direct, in long double from the same rho_k
+  // formula as k_band_rho: H_d(x) = (1/2N) sum_{k=0}^{2N-1} rho_k^(d+1)
+  // cos(pi k x / N) for x <= X = min(N, 64), and the reach R = the largest
+  // over items, axes and depths d < w of the smallest R_d with
+  // 2 sum_{R_d < x <= X} |H_d(x)| < 1e-17 (H_d(X) itself below 1e-20, so
+  // that the rest of the period adds nothing).  Declines (the spectral
+  // BandOp form runs) when R would pass kBandRmax or a depth's table
+  // would not fit the kernel's LDS.
+  bool build_band_kernels(const std::vector<double>& beta) {
+    const int BL = B * Lmax;
+    if (band_wmax > kBandWmax || band_wmax * (kBandRmax + 1) > kBandHmax) return false;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<long double> tab((size_t)BL * 2 * band_wmax * 65, 0.0L);
+    int R = 0;
+    for (int axis = 0; axis < 2; axis++) {
+      const int N = axis == 0 ? m : n, P = 2 * N, X = std::min(N, 64);
+      std::vector<long double> cs2(P), rho(P);
+      for (int j = 0; j < P; j++) cs2[j] = cosl(pi * j / N);
+      for (int b = 0; b < BL; b++) {
+        const int lev = b % Lmax;
+        if (lev == Lmax - 1) continue;
+        const long double bt = beta[b];
+        for (int k = 0; k < P; k++) {
+          const long double lx = 2.0L - 2.0L * cs2[k];
+          const long double th = (1.0L + bt * (lx + 2.0L)) / (2.0L * bt);
+          rho[k] = 1.0L / (th + sqrtl(th * th - 1.0L));
+        }
+        const int w = hband_w[lev];
+        std::vector<long double> pw(rho);
+        for (int d = 0; d < w; d++) {
+          long double* h = &tab[(((size_t)b * 2 + axis) * band_wmax + d) * 65];
+          for (int xx = 0; xx <= X; xx++) {
+            long double acc = 0.0L;
+            for (int k = 0; k < P; k++) acc += pw[k] * cs2[(size_t)((long)k * xx % P)];
+            h[xx] = acc / P;
+          }
+          if (fabsl(h[X]) > 1e-20L) return false;
+          long double tail = 0.0L;
+          int rd = X;
+          while (rd > 0 && tail + 2.0L * fabsl(h[rd]) < 1e-17L) tail += 2.0L * fabsl(h[rd--]);
+          R = std::max(R, rd);
+          for (int k = 0; k < P; k++) pw[k] *= rho[k];
+        }
+      }
+    }
+    if (R > kBandRmax) return false;
+    band_R = R;
+    std::vector<double> hh((size_t)BL * 2 * band_wmax * (R + 1), 0.0);
+    for (size_t r = 0; r < (size_t)BL * 2 * band_wmax; r++)
+      for (int xx = 0; xx <= R; xx++) hh[r * (R + 1) + xx] = (double)tab[r * 65 + xx];
+    upvec(band_h, hh, st);
+    return true;
+  }
   bool levels_can_batch() {
     const char* env = std::getenv("IBFS_STEP1_LEVELS");
     if (env && env[0] == '0') return false;
@@ -1017,13 +1081,12 @@ struct Engine : GroupBase {
       hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, B * (Lmax - 1)), dim3(256),
                          sizeof(double) * 2 * m, st,
                          geo, T2L.p, Lmax, NR, ringSin.p, ringV.p);
-      const size_t lds = sizeof(double) * (Lmax - 1) * NR;
       double* sring = all ? nullptr : ringS.p;
       switch (Lmax - 2) {
-#define IBFS_RING_CHAIN(NP)                                                                                     \
-  case NP:                                                                                                      \
-    hipLaunchKernelGGL(k_ring_chain<NP>, dim3((NR + 3) / 4, B), dim3(256), lds, st, geo, fbs(), NR, ringV.p,     \
-                       ringT.p, ringR.p, sring);                                                                \
+#define IBFS_RING_CHAIN(NP)                                                                                   \
+  case NP:                                                                                                    \
+    hipLaunchKernelGGL(k_ring_chain<NP>, dim3(NR, B), dim3(256), 0, st, geo, fbs(), NR, ringV.p, ringT.p,     \
+                       ringR.p, sring);                                                                       \
     break;
         IBFS_RING_CHAIN(0)
         IBFS_RING_CHAIN(1)
@@ -1041,6 +1104,24 @@ struct Engine : GroupBase {
                          sizeof(double) * (Kx + 2 * (Kx + 1)), st, geo, ringR.p, NR, below, Kx, ringSin.p,
                          ringSin.p + 2 * m, ringX.p);
       const double* Gy = m == n ? ringG.p : ringGp.p;
+      if (!all && swin_ok) {
+        // step 2 reads s_1 only on the vertices of the force stencils: the
+        // last two transforms of level 0 on the rows and the columns of
+        // their bounding rectangle (the rest of s_1 is rewritten in step 4
+        // before anything reads it)
+        Geo gr = geoL, gc = geoL;
+        gr.nm1 = swin[1] - swin[0] + 1;
+        gc.mm1 = swin[3] - swin[2] + 1;
+        launch<0>(RowOp<T, SrcRowWin<SrcRingRow>, DstRowWin<DstRingItem>>{
+                      gr, {SrcRingRow{T2L.p, ringX.p, ringG.p, out, Kx}, geoL, swin[0]},
+                      {DstRingItem{S.p, out, 0}, geoL, swin[0]}, nullptr, 0},
+                  gr.nm1, 2, nout);
+        launch<1>(ColOp<T, SrcColWin<SrcRingCol>, DstColWin<DstRingItem>>{
+                      gc, {SrcRingCol{ringX.p, Gy, below, Kx}, geoL, swin[2]},
+                      {DstRingItem{S.p, below, 1}, geoL, swin[2], swin[0], swin[1]}, nullptr, 0},
+                  gc.mm1, 2, nbelow, kCol);
+        return;
+      }
       launch<0>(RowOp<T, SrcRingRow, DstRingItem>{geoL, SrcRingRow{T2L.p, ringX.p, ringG.p, out, Kx},
                                                   DstRingItem{S.p, out, 0}, nullptr, 0},
                 geo.nm1, 2, nout);
@@ -1120,6 +1201,12 @@ struct Engine : GroupBase {
     // fused column pair per band depth and pair of opposite edges, the
     // bottom and top bands first, then the left and right ones (the bands
     // cross in the corners, so the two adds there are kept in order)
+    if (band_direct) {
+      const int nbt = (m - 1 + 63) / 64, nlr = (n - 1 + 15) / 16;
+      hipLaunchKernelGGL(k_band_direct<T>, dim3(2 * nbt + 2 * nlr, BL), dim3(256), 0, st, geoL, omega.p, Lmax,
+                         band_w.p, band_h.p, band_wmax, band_R, nbt, nlr);
+      return;
+    }
     launch_col2(BandOp<T>{geoL, omega.p, Lmax, band_rho_m.p, band_w.p, 0, 2, m}, 2 * band_wmax, 2, BL, kCol2, 0);
     launch_col2(BandOp<T>{geoL, omega.p, Lmax, band_rho_n.p, band_w.p, 2, 2, n}, 2 * band_wmax, 2, BL, kCol2, 1);
   }
@@ -1344,6 +1431,33 @@ struct Engine : GroupBase {
       hof += c.ncol;
       nf_max = std::max(nf_max, c.nf);
       nvtx_max = std::max(nvtx_max, h.nvtx);
+    }
+    {
+      const char* env = std::getenv("IBFS_S_WINDOW");
+      swin_ok = !(env && env[0] == '0') && !ria.empty();
+      swin[0] = swin[2] = std::max(m, n);
+      swin[1] = swin[3] = -1;
+      for (const std::vector<int>* v : {&ria, &rib})
+        for (int a : *v) {
+          if (a < 0) {
+            swin_ok = false;
+            break;
+          }
+          const int ii = a % (m - 1), jj = a / (m - 1);
+          swin[0] = std::min(swin[0], jj);
+          swin[1] = std::max(swin[1], jj);
+          swin[2] = std::min(swin[2], ii);
+          swin[3] = std::max(swin[3], ii);
+        }
+      if (swin[1] < swin[0] || swin[1] >= n - 1 || swin[3] >= m - 1) swin_ok = false;
+      // sequences 2p and 2p+1 of a pass share one complex FFT, whose
+      // rounding mixes the two: the window starts on an even row and column
+      // and ends on an odd one (or the last), so that it pairs them as the
+      // whole-field pass does
+      swin[0] &= ~1;
+      swin[2] &= ~1;
+      swin[1] = std::min(swin[1] | 1, n - 2);
+      swin[3] = std::min(swin[3] | 1, m - 2);
     }
     nf_tot = foff;
     mat_tot = loff;

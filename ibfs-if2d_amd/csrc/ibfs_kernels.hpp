@@ -933,6 +933,112 @@ struct BandOp {
   }
 };
 
+// The same band correction in real space.  With the edge values e_t
+// (t = 0..N-2) and their odd extension of period 2N (^e(-1) = ^e(N-1) = 0,
+// ^e(-2-t) = ^e(2N-2-t) = -e_t), the correction at depth d, position pos is
+//   v = (2/N) sum_{k=1}^{N-1} rho_k^(d+1) sum_t e_t sin(pi k (t+1)/N) sin(pi k (pos+1)/N)
+//     = sum over one period of ^e(pos - x) H_d(x),
+//   H_d(x) = (1/2N) sum_{k=0}^{2N-1} rho_k^(d+1) cos(pi k x / N)
+// (the k = 0 and k = N terms, absent from the sine series, add a constant and
+// an alternating part to H_d whose sums against one period of ^e vanish).
+// rho_k is analytic in cos(pi k / N), so H_d decays geometrically in |x|;
+// the host tabulates H_d(x) for |x| <= R, beyond which the dropped terms sum
+// below 1e-17 (the tolerance of the band depth), and v is a (2R+1)-term sum.
+// One launch for all four bands: a workgroup takes 64 positions of a bottom
+// or top band (all depths) or 16 positions of a left or right band, stages
+// ^e over its positions and their reach, and adds v on its band vertices.
+// A corner vertex lies in two bands; the bottom or top workgroup adds both
+// terms there, its own first (the order of the spectral form's two
+// launches), and the left and right workgroups skip it.  The edge values
+// come from the coarser level, whose own bands are w < min(m,n)/4 cells from
+// its edges and so never meet the get_bc lines (min(m,n)/4 cells in).
+// hk[((b 2 + axis) wmax + d) (R+1) + x], axis 0 for the bottom and top edges
+// (N = m), 1 for the left and right ones (N = n); rows d >= w are zero.
+// grid (2 ceil((m-1)/64) + 2 ceil((n-1)/16), items), 256 threads.
+constexpr int kBandRmax = 32;    // the widest kernel the direct form takes
+constexpr int kBandWmax = 31;    // the deepest band it takes
+constexpr int kBandHmax = 1024;  // w (R+1) of one axis, staged in LDS
+template <typename T>
+__device__ __forceinline__ double band_ehat(const Geo& g, const T* __restrict__ omc, int base, int N, int t) {
+  const int P = 2 * N;
+  int tt = t % P;
+  tt += tt < 0 ? P : 0;
+  if (tt == N - 1 || tt == P - 1) return 0.0;
+  const bool neg = tt >= N;
+  const int s = neg ? P - 2 - tt : tt;
+  const double v = (double)getbc_sel<T>(g, omc, base + s + 2, T(0.25));
+  return neg ? -v : v;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, int lm, const int* __restrict__ wlev,
+                                                     const double* __restrict__ hk, int wmax, int R, int nbt,
+                                                     int nlr) {
+  __shared__ double es[64 + 2 * kBandRmax];
+  __shared__ double hs[kBandHmax];
+  // a bottom or top workgroup at a corner: the left and right edges' values
+  // next to its band rows, and the left/right kernels
+  __shared__ double cs[2][kBandWmax + 2 * kBandRmax];
+  __shared__ double hc[kBandHmax];
+  const int b = blockIdx.y, lev = b % lm;
+  if (lev == lm - 1) return;  // a case's coarsest level has no coarser edge values
+  const int w = wlev[lev];
+  int x = blockIdx.x, e, chunk;
+  if (x < 2 * nbt) {
+    e = x / nbt;
+    chunk = x - e * nbt;
+  } else {
+    x -= 2 * nbt;
+    e = 2 + x / nlr;
+    chunk = x - (e - 2) * nlr;
+  }
+  const bool horiz = e < 2;
+  const int N = horiz ? g.m : g.n, PC = horiz ? 64 : 16, p0 = chunk * PC;
+  const int base = e == 0 ? g.bottom : e == 1 ? g.top : e == 2 ? g.left : g.right;
+  const T* omc = om + (long)(b + 1) * g.nw;
+  const double* H = hk + (long)(b * 2 + (horiz ? 0 : 1)) * wmax * (R + 1);
+  // corners this workgroup owns: positions < w (left) or > m-2-w (right)
+  const bool cl = horiz && p0 < w, cr = horiz && p0 + PC - 1 > g.mm1 - 1 - w;
+  const int jlo = e == 0 ? 0 : g.nm1 - w;  // the band rows of a bottom or top workgroup
+  const int nc = w + 2 * R;
+  for (int i = threadIdx.x; i < PC + 2 * R; i += 256) es[i] = band_ehat<T>(g, omc, base, N, p0 - R + i);
+  for (int i = threadIdx.x; i < w * (R + 1); i += 256) hs[i] = H[i];
+  if (cl || cr) {
+    const double* Hc = hk + (long)(b * 2 + 1) * wmax * (R + 1);
+    for (int i = threadIdx.x; i < w * (R + 1); i += 256) hc[i] = Hc[i];
+    for (int i = threadIdx.x; i < 2 * nc; i += 256) {
+      const int side = i / nc, t = jlo - R + (i - side * nc);
+      if (side == 0 ? cl : cr) cs[side][i - side * nc] = band_ehat<T>(g, omc, side == 0 ? g.left : g.right, g.n, t);
+    }
+  }
+  __syncthreads();
+  const int lp = horiz ? threadIdx.x & 63 : threadIdx.x >> 4;
+  const int dl = horiz ? threadIdx.x >> 6 : threadIdx.x & 15, DL = horiz ? 4 : 16;
+  const int pos = p0 + lp;
+  if (pos > N - 2) return;
+  if (!horiz && (pos < w || pos > N - 2 - w)) return;  // a corner: the bottom or top workgroup's
+  for (int d = dl; d < w; d += DL) {
+    const double* hd = hs + d * (R + 1);
+    const double* ec = es + lp + R;
+    double v = hd[0] * ec[0];
+    for (int k = 1; k <= R; k++) v = fma(hd[k], ec[-k] + ec[k], v);
+    const int ii = horiz ? pos : (e == 2 ? d : g.mm1 - 1 - d);
+    const int jj = horiz ? (e == 0 ? d : g.nm1 - 1 - d) : pos;
+    T* f = om + (long)b * g.nw + ii + (long)jj * g.mm1;
+    T out = *f + T(v);
+    if (horiz && (ii < w || ii > g.mm1 - 1 - w)) {
+      // the left or right band's term at this corner vertex: depth ii (or
+      // mm1-1-ii) of that band, position jj along its edge
+      const bool left = ii < w;
+      const double* hl = hc + (left ? ii : g.mm1 - 1 - ii) * (R + 1);
+      const double* el = cs[left ? 0 : 1] + (jj - jlo) + R;
+      double u = hl[0] * el[0];
+      for (int k = 1; k <= R; k++) u = fma(hl[k], el[-k] + el[k], u);
+      out = out + T(u);
+    }
+    *f = out;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // vort2flux with the levels batched (src/operators.f90:191-240).  The
 // reference solves s_k = ctci(omega_k + apply_bc(get_bc(s_{k+1}, 1), 1))
@@ -1178,71 +1284,67 @@ __global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restr
 
 // R_k = V_k + sum_{j=1}^{L-1-k} T^j V_{k+j} for k = 1..L-1 of case b
 // (L = NP + 2 levels, Tp: T^1..T^NP, NR x NR each) into Rb (layout of V).
-// Workgroup: four ring rows p, one per wave, with the case's V_1..V_{L-1}
-// staged in LDS.  The powers are the kernel's whole HBM traffic (NP NR^2
-// doubles), so each wave issues the loads of its first U entries per lane of
-// every power before it stages V and waits at the barrier, and keeps NP U
-// loads in flight after that (U = 17 covers a 1028-entry ring, 512^2, in
-// one round trip while NP <= 3).  With sring, R_1 is also
-// written at the ring places of level 1 of sring (a field in the
-// streamfunction's layout) for get_bc(s_1).  grid (ceil(NR/4), B).
+// Workgroup: one ring row p, its columns split over the four waves.  Each
+// lane loads its U columns of every power and of V_1..V_{L-1} before any
+// product (a 1028-entry ring, 512^2, is one chunk: one memory round trip),
+// and the four waves' sums are added in a fixed order.  The powers are the
+// kernel's whole HBM traffic (NP NR^2 doubles); one row per workgroup keeps
+// sixteen waves of loads in flight per CU.  With sring, R_1 is also written
+// at the ring places of level 1 of sring (a field in the streamfunction's
+// layout) for get_bc(s_1).  grid (NR, B), 256 threads.
 constexpr int kRingMaxLev = 8;
-constexpr int chain_u(int np) { return np <= 3 ? 17 : 8; }
+constexpr int chain_u(int np) { return np <= 3 ? 5 : 3; }
 template <int NP>
 __global__ __launch_bounds__(256) void k_ring_chain(Geo g, long fbs, int NR, const double* __restrict__ V,
                                                     const double* __restrict__ Tp, double* __restrict__ Rb,
                                                     double* __restrict__ sring) {
-  constexpr int L = NP + 2, U = chain_u(NP);
-  extern __shared__ double v[];
-  const int b = blockIdx.y;
+  constexpr int L = NP + 2, U = chain_u(NP), NJ = NP > 0 ? NP : 1;
+  __shared__ double red[L][4];
+  const int b = blockIdx.y, p = blockIdx.x;
   const double* vb = V + ((long)b * L + 1) * NR;
-  const int lane = threadIdx.x & 63, p = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const bool live = p < NR;
-  const int pr = live ? p : NR - 1;  // a row that exists: no branch holds the loads back
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double acc[L];
 #pragma unroll
   for (int k = 0; k < L; k++) acc[k] = 0.0;
-  if constexpr (NP == 0) {
-    stage_lds(v, (L - 1) * NR, [&](int e) { return vb[e]; });
-    __syncthreads();
-  } else {
-    double t[NP][U];
-    for (int q0 = 0; q0 < NR; q0 += 64 * U) {
+  if constexpr (NP > 0) {
+    for (int q0 = 0; q0 < NR; q0 += 256 * U) {
+      double t[NJ][U], vv[L - 1][U];
 #pragma unroll
       for (int j = 0; j < NP; j++)
 #pragma unroll
         for (int u = 0; u < U; u++) {
-          const int q = q0 + lane + 64 * u;
-          t[j][u] = q < NR ? Tp[(long)j * NR * NR + (long)pr * NR + q] : 0.0;
+          const int q = q0 + threadIdx.x + 256 * u;
+          t[j][u] = q < NR ? Tp[(long)j * NR * NR + (long)p * NR + q] : 0.0;
         }
-      if (q0 == 0) {  // uniform over the workgroup
-        stage_lds(v, (L - 1) * NR, [&](int e) { return vb[e]; });
-        __syncthreads();
-      }
+#pragma unroll
+      for (int k = 1; k < L; k++)
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int q = q0 + threadIdx.x + 256 * u;
+          vv[k - 1][u] = q < NR ? vb[(k - 1) * NR + q] : 0.0;
+        }
 #pragma unroll
       for (int j = 0; j < NP; j++)  // T^(j+1) V_{k+j+1}
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int q = q0 + lane + 64 * u;
-          if (q < NR) {
+        for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int k = 1; k + j + 1 <= L - 1; k++) acc[k] += t[j][u] * v[(k + j) * NR + q];
-          }
-        }
+          for (int k = 1; k + j + 1 <= L - 1; k++) acc[k] += t[j][u] * vv[k + j][u];
     }
   }
-  if (!live) return;
 #pragma unroll
-  for (int k = 1; k <= L - 1; k++) {
+  for (int k = 1; k < L; k++) {
     const double a = wave_sum(acc[k]);
-    if (lane == 0) {
-      const double r = v[(k - 1) * NR + p] + a;
-      Rb[((long)b * L + k) * NR + p] = r;
-      if (sring != nullptr && k == 1) {
-        int ii, jj;
-        ring_pos(g.m, g.n, p, ii, jj);
-        sring[b * fbs + g.nw + ii + (long)jj * g.mm1] = r;
-      }
+    if (lane == 0) red[k][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < L - 1) {
+    const int k = threadIdx.x + 1;
+    const double r = vb[(k - 1) * NR + p] + ((red[k][0] + red[k][1]) + (red[k][2] + red[k][3]));
+    Rb[((long)b * L + k) * NR + p] = r;
+    if (sring != nullptr && k == 1) {
+      int ii, jj;
+      ring_pos(g.m, g.n, p, ii, jj);
+      sring[b * fbs + g.nw + ii + (long)jj * g.mm1] = r;
     }
   }
 }
@@ -1347,6 +1449,37 @@ struct DstRingItem {
   __device__ void operator()(const Geo& g, int b, int ii, int jj, double v) const {
     double* f = a + (long)im(b) * g.nw + ii + (long)jj * g.mm1;
     *f = add ? *f + v : v;
+  }
+};
+
+// A window of a pass: the rows j0 .. j0 + rows - 1 of a row pass (its Geo
+// has nm1 = rows) or the columns i0 .. i0 + cols - 1 of a column pass (mm1 =
+// cols), with the loader and the store of the full field (geometry gf) at
+// the full field's places.  A column window keeps the rows j0 .. j1 of its
+// output.  Each row or column is transformed as in the full pass, so the
+// values in the window are the full pass's.
+template <class S>
+struct SrcRowWin {
+  static constexpr bool kNoWave = true;
+  S s; Geo gf; int j0;
+  __device__ double operator()(const Geo&, int b, int ii, int jj) const { return s(gf, b, ii, jj + j0); }
+};
+template <class D>
+struct DstRowWin {
+  D d; Geo gf; int j0;
+  __device__ void operator()(const Geo&, int b, int ii, int jj, double v) const { d(gf, b, ii, jj + j0, v); }
+};
+template <class S>
+struct SrcColWin {
+  static constexpr bool kNoWave = true;
+  S s; Geo gf; int i0;
+  __device__ double operator()(const Geo&, int b, int ii, int l1) const { return s(gf, b, ii + i0, l1); }
+};
+template <class D>
+struct DstColWin {
+  D d; Geo gf; int i0, j0, j1;
+  __device__ void operator()(const Geo&, int b, int ii, int jj, double v) const {
+    if (jj >= j0 && jj <= j1) d(gf, b, ii + i0, jj, v);
   }
 };
 

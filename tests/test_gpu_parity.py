@@ -336,3 +336,51 @@ def test_level_batched_vort2flux_equals_level_by_level(name, monkeypatch):
         v = slice(k * g1.nq, (k + 1) * g1.nq)
         assert relerr(s1["q"][v], s0["q"][v]) < 1e-12, f"q level {k + 1}"
     assert not np.array_equal(s1["q"], s0["q"])  # the two forms did run (they round differently)
+
+
+@pytest.mark.parametrize("name", ["naca48x40", "cyl40r5"])
+def test_band_correction_real_space_equals_spectral(name, monkeypatch):
+    """The band correction of the batched step 1 as short sums with the
+    tabulated real-space kernels H_d (one launch, the default) against its
+    spectral form, a fused DST-I column pair per band depth and pair of
+    opposite edges (IBFS_BAND_FFT=1): the same steps to rounding over 20
+    steps (the dropped tail of H_d is below 1e-17 of the edge values)."""
+    from ibfs_amd import hip
+
+    a = case_args(name)
+    assert levels_batch(a)
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_BAND_FFT", "1")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(20), g0.advance(20)
+    s1, s0 = g1.get_state(), g0.get_state()
+    nb2 = 2 * g1.nbody
+    assert np.max(np.abs(h1[:, :nb2] - h0[:, :nb2])) < 1e-13 * np.max(np.abs(h0[:, :nb2]))
+    for k in range(g1.L):
+        w = slice(k * g1.nw, (k + 1) * g1.nw)
+        assert relerr(s1["omega"][w], s0["omega"][w]) < 1e-13, f"omega level {k + 1}"
+    assert not np.array_equal(s1["omega"], s0["omega"])  # the two forms did run
+
+
+@pytest.mark.parametrize("name", ["naca48x40", "cyl64L4"])
+def test_step2_window_is_bit_identical(name, monkeypatch):
+    """Step 2 transforms level 0's streamfunction only on the rows and
+    columns of the rectangle that holds every vertex the force stencils read
+    (the row and column transforms there are the full pass's): the state and
+    the history after 10 steps equal those of the whole-field passes
+    (IBFS_S_WINDOW=0) bit for bit."""
+    from ibfs_amd import hip
+
+    a = v2f_args(name)
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_S_WINDOW", "0")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(10), g0.advance(10)
+    s1, s0 = g1.get_state(), g0.get_state()
+    assert np.array_equal(h1, h0)
+    for k in s0:
+        assert np.array_equal(s1[k], s0[k]), k
