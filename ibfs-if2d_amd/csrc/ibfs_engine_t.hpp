@@ -134,6 +134,9 @@ struct Engine : GroupBase {
   // elementwise kernel (k_src_eval) before a plain row pass, instead of as
   // the loader of that pass; IBFS_STEP1_SPLIT=0 keeps it in the loader
   bool src_split = true;
+  // the two-level coarsify launches in LDS tiles; IBFS_COARSIFY_TILES=0
+  // keeps one place per thread (the same values)
+  bool coarsify_tiles = true;
   Geo geoL{};
   DevBuf<T> T1L, T2L;
   // vort2flux with the levels batched (see v2f_can_batch): ringN ring
@@ -292,6 +295,7 @@ struct Engine : GroupBase {
     tune();
     lev_batch = levels_can_batch();
     if (const char* sp = std::getenv("IBFS_STEP1_SPLIT")) src_split = sp[0] != '0';
+    if (const char* ct = std::getenv("IBFS_COARSIFY_TILES")) coarsify_tiles = ct[0] != '0';
     if (lev_batch) {
       geoL = geo;
       geoL.Lmax = 1;
@@ -1013,11 +1017,18 @@ struct Engine : GroupBase {
     row_pass(SrcField<T>{t2, nw}, dst, batch, gate, lev);
   }
 
-  // the in-place coarsify chain of vort2flux (src/operators.f90:224-226)
-  // (two levels per launch, k_coarsify2)
+  // the in-place coarsify chain of vort2flux (src/operators.f90:224-226):
+  // two levels per launch, in LDS tiles (k_coarsify2t) or one place per
+  // thread (k_coarsify2, also the single last level of an even level count)
   void coarsify(T* om, int batch, const int* gate) {
     const int wcnt = (m / 2 - 1) * (n / 2 - 1), nblk = (wcnt + 255) / 256;
     for (int lev = 1; lev < Lmax; lev += 2) {
+      if (lev + 1 < Lmax && coarsify_tiles) {
+        const int ntx = (m / 2 - 1 + kCoTX - 1) / kCoTX, nty = (n / 2 - 1 + kCoTY - 1) / kCoTY;
+        hipLaunchKernelGGL(k_coarsify2t<T>, dim3(ntx * nty, batch), dim3(256), 0, st, geo, om, fbs(), gate, lev,
+                           ntx);
+        continue;
+      }
       const int nlev = lev + 1 < Lmax ? 2 : 1;
       hipLaunchKernelGGL(k_coarsify2<T>, dim3(nblk * nlev, batch), dim3(256), 0, st, geo, om, fbs(), gate, lev,
                          nblk);

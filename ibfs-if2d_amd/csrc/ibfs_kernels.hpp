@@ -958,6 +958,7 @@ struct BandOp {
 constexpr int kBandRmax = 32;    // the widest kernel the direct form takes
 constexpr int kBandWmax = 31;    // the deepest band it takes
 constexpr int kBandHmax = 1024;  // w (R+1) of one axis, staged in LDS
+constexpr int kBandPer = (kBandWmax + 3) / 4;  // depths per thread of a bottom or top workgroup
 template <typename T>
 __device__ __forceinline__ double band_ehat(const Geo& g, const T* __restrict__ omc, int base, int N, int t) {
   const int P = 2 * N;
@@ -1000,6 +1001,24 @@ __global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, 
   const bool cl = horiz && p0 < w, cr = horiz && p0 + PC - 1 > g.mm1 - 1 - w;
   const int jlo = e == 0 ? 0 : g.nm1 - w;  // the band rows of a bottom or top workgroup
   const int nc = w + 2 * R;
+  // this thread's band vertices (depths dl, dl + DL, ...) and their old
+  // values, loaded with the staging loads below rather than after the sums
+  const int lp = horiz ? threadIdx.x & 63 : threadIdx.x >> 4;
+  const int dl = horiz ? threadIdx.x >> 6 : threadIdx.x & 15, DL = horiz ? 4 : 16;
+  const int pos = p0 + lp;
+  // a left or right workgroup leaves the corners to the bottom or top one
+  const bool act = pos <= N - 2 && (horiz || (pos >= w && pos <= N - 2 - w));
+  T old[kBandPer];
+#pragma unroll
+  for (int i = 0; i < kBandPer; i++) {
+    const int d = dl + DL * i;
+    old[i] = T(0);
+    if (act && d < w) {
+      const int ii = horiz ? pos : (e == 2 ? d : g.mm1 - 1 - d);
+      const int jj = horiz ? (e == 0 ? d : g.nm1 - 1 - d) : pos;
+      old[i] = om[(long)b * g.nw + ii + (long)jj * g.mm1];
+    }
+  }
   for (int i = threadIdx.x; i < PC + 2 * R; i += 256) es[i] = band_ehat<T>(g, omc, base, N, p0 - R + i);
   for (int i = threadIdx.x; i < w * (R + 1); i += 256) hs[i] = H[i];
   if (cl || cr) {
@@ -1011,20 +1030,18 @@ __global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, 
     }
   }
   __syncthreads();
-  const int lp = horiz ? threadIdx.x & 63 : threadIdx.x >> 4;
-  const int dl = horiz ? threadIdx.x >> 6 : threadIdx.x & 15, DL = horiz ? 4 : 16;
-  const int pos = p0 + lp;
-  if (pos > N - 2) return;
-  if (!horiz && (pos < w || pos > N - 2 - w)) return;  // a corner: the bottom or top workgroup's
-  for (int d = dl; d < w; d += DL) {
+  if (!act) return;
+#pragma unroll
+  for (int i = 0; i < kBandPer; i++) {
+    const int d = dl + DL * i;
+    if (d >= w) break;
     const double* hd = hs + d * (R + 1);
     const double* ec = es + lp + R;
     double v = hd[0] * ec[0];
     for (int k = 1; k <= R; k++) v = fma(hd[k], ec[-k] + ec[k], v);
     const int ii = horiz ? pos : (e == 2 ? d : g.mm1 - 1 - d);
     const int jj = horiz ? (e == 0 ? d : g.nm1 - 1 - d) : pos;
-    T* f = om + (long)b * g.nw + ii + (long)jj * g.mm1;
-    T out = *f + T(v);
+    T out = old[i] + T(v);
     if (horiz && (ii < w || ii > g.mm1 - 1 - w)) {
       // the left or right band's term at this corner vertex: depth ii (or
       // mm1-1-ii) of that band, position jj along its edge
@@ -1035,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, 
       for (int k = 1; k <= R; k++) u = fma(hl[k], el[-k] + el[k], u);
       out = out + T(u);
     }
-    *f = out;
+    om[(long)b * g.nw + ii + (long)jj * g.mm1] = out;
   }
 }
 
@@ -1587,6 +1604,91 @@ __global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ 
 // those entries are being overwritten by the first half of the grid.
 // grid.x = 2*nblk (nblk = ceil((m/2-1)(n/2-1)/256)) or nblk when
 // lev+1 == Lmax.
+// The same two levels per launch in LDS tiles.  Workgroup (tx, ty) takes a
+// kCoTX x kCoTY block of level lev+1's centre.  It stages the level lev
+// places its stencils read ((2 kCoTX + 1) x (2 kCoTY + 1)) and, when those
+// places meet level lev's own centre, the level lev-1 block they are
+// coarsified from ((4 kCoTX + 3) x (4 kCoTY + 3)), with every global load
+// issued before the first store into LDS.  It then recomputes the level lev
+// values inside that centre (restrict_at, k_coarsify's expression), writes
+// those of its own (2 kCoTX) x (2 kCoTY) places to level lev (each centre
+// place is owned by exactly one workgroup), and coarsifies its level lev+1
+// block from the staged values.  Every value is the one k_coarsify computes,
+// so the result is bit-identical to two k_coarsify launches.  Stored level
+// lev values inside the centre are never read (they are being overwritten
+// by other workgroups): those staging loads read place (2, 2) instead, and
+// the recomputed value replaces them.  grid (ntx nty, B), 256 threads;
+// lev + 1 < Lmax.
+constexpr int kCoTX = 32, kCoTY = 8;
+constexpr int kCoMX = 2 * kCoTX + 1, kCoMY = 2 * kCoTY + 1, kCoFX = 4 * kCoTX + 3, kCoFY = 4 * kCoTY + 3;
+constexpr int kCoFU = (kCoFX * kCoFY + 255) / 256, kCoMU = (kCoMX * kCoMY + 255) / 256;
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarsify2t(Geo g, T* __restrict__ om, long bs, const int* __restrict__ Lc,
+                                                    int lev, int ntx) {
+  __shared__ T fs[kCoFY][kCoFX];
+  __shared__ T ms[kCoMY][kCoMX];
+  const int b = blockIdx.y;
+  if (!active(Lc, b, lev)) return;
+  const bool up = active(Lc, b, lev + 1);
+  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int i0 = g.m / 4 + 2 + tx * kCoTX, j0 = g.n / 4 + 2 + ty * kCoTY;
+  const int ma = 2 * i0 - g.m / 2 - 2, mc = 2 * j0 - g.n / 2 - 2;  // level lev place of ms[0][0]
+  const int fa = 2 * ma - g.m / 2 - 2, fc = 2 * mc - g.n / 2 - 2;  // level lev-1 place of fs[0][0]
+  const int clo_x = g.m / 4 + 2, chi_x = 3 * g.m / 4, clo_y = g.n / 4 + 2, chi_y = 3 * g.n / 4;
+  auto centre = [&](int a, int c) { return a >= clo_x && a <= chi_x && c >= clo_y && c <= chi_y; };
+  // does this block's level lev region meet level lev's centre?
+  const bool deep = ma + kCoMX - 1 >= clo_x && ma <= chi_x && mc + kCoMY - 1 >= clo_y && mc <= chi_y;
+  const T* rF = om + b * bs + (long)(lev - 1) * g.nw;
+  const T* rM = om + b * bs + (long)lev * g.nw;
+  T vf[kCoFU], vm[kCoMU];
+#pragma unroll
+  for (int u = 0; u < kCoMU; u++) {
+    const int e = threadIdx.x + 256 * u, y = e / kCoMX, x = e - y * kCoMX;
+    int a = min(ma + x, g.m), c = min(mc + y, g.n);
+    if (centre(a, c)) a = c = 2;
+    vm[u] = e < kCoMX * kCoMY ? rM[wix(g, a, c)] : T(0);
+  }
+  if (deep) {
+#pragma unroll
+    for (int u = 0; u < kCoFU; u++) {
+      const int e = threadIdx.x + 256 * u, y = e / kCoFX, x = e - y * kCoFX;
+      const int a = min(max(fa + x, 2), g.m), c = min(max(fc + y, 2), g.n);
+      vf[u] = e < kCoFX * kCoFY ? rF[wix(g, a, c)] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kCoFU; u++) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < kCoFX * kCoFY) (&fs[0][0])[e] = vf[u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kCoMU; u++) {
+    const int e = threadIdx.x + 256 * u;
+    if (e < kCoMX * kCoMY) (&ms[0][0])[e] = vm[u];
+  }
+  __syncthreads();
+  if (deep) {
+    T* wM = om + b * bs + (long)lev * g.nw;
+    auto Fl = [&](int x, int y) -> T { return fs[y][x]; };
+#pragma unroll
+    for (int u = 0; u < kCoMU; u++) {
+      const int e = threadIdx.x + 256 * u, y = e / kCoMX, x = e - y * kCoMX;
+      if (e < kCoMX * kCoMY && centre(ma + x, mc + y)) {
+        const T v = restrict_at<T>(Fl, 2 * x + 1, 2 * y + 1);
+        ms[y][x] = v;
+        if (x >= 1 && x <= 2 * kCoTX && y >= 1 && y <= 2 * kCoTY) wM[wix(g, ma + x, mc + y)] = v;
+      }
+    }
+    __syncthreads();
+  }
+  if (!up) return;
+  const int li = threadIdx.x % kCoTX, lj = threadIdx.x / kCoTX;
+  const int i = i0 + li, j = j0 + lj;
+  if (i > chi_x || j > chi_y) return;
+  auto Ml = [&](int x, int y) -> T { return ms[y][x]; };
+  om[b * bs + (long)(lev + 1) * g.nw + wix(g, i, j)] = restrict_at<T>(Ml, 1 + 2 * li, 1 + 2 * lj);
+}
+
 template <typename T>
 __global__ void k_coarsify2(Geo g, T* __restrict__ om, long bs, const int* __restrict__ Lc, int lev, int nblk) {
   const int wi = g.m / 2 - 1, wj = g.n / 2 - 1;

@@ -364,6 +364,29 @@ def test_band_correction_real_space_equals_spectral(name, monkeypatch):
     assert not np.array_equal(s1["omega"], s0["omega"])  # the two forms did run
 
 
+@pytest.mark.parametrize("name", ["rect64x48L5", "cyl64L4", "cyl40r5"])
+def test_coarsify_tiles_are_bit_identical(name, monkeypatch):
+    """The coarsify chain two levels per launch in LDS tiles (the default)
+    against one place per thread (IBFS_COARSIFY_TILES=0): both evaluate
+    every coarsified value with the same expression from the same inputs, so
+    the states and histories after 10 steps are equal bit for bit (five
+    levels: two tiled launches; four: one tiled launch and the last level on
+    its own; three: one tiled launch)."""
+    from ibfs_amd import hip
+
+    a = v2f_args(name)
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_COARSIFY_TILES", "0")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(10), g0.advance(10)
+    s1, s0 = g1.get_state(), g0.get_state()
+    assert np.array_equal(h1, h0)
+    for k in s0:
+        assert np.array_equal(s1[k], s0[k]), k
+
+
 @pytest.mark.parametrize("name", ["naca48x40", "cyl64L4"])
 def test_step2_window_is_bit_identical(name, monkeypatch):
     """Step 2 transforms level 0's streamfunction only on the rows and

@@ -104,6 +104,35 @@ def test_airfoil_512_five_levels():
     run_pair(o, g, 2, [0, 130, 521])
 
 
+def test_step_switches_at_512_five_levels(monkeypatch):
+    """The headline case under the switches that select equivalent forms:
+    the coarsify chain in LDS tiles or one place per thread, and step 2's
+    level-0 transforms on the force-stencil rectangle or the whole field,
+    give bit-identical states after 3 steps; the band correction in real
+    space or as spectral column pairs agrees to rounding."""
+    from ibfs_amd import hip
+
+    def run(**env):
+        for k in ("IBFS_COARSIFY_TILES", "IBFS_S_WINDOW", "IBFS_BAND_FFT"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        _, g = solver_pair(512, 5, 4.0, 1.0, 2.0, "naca", 0.002, 300.0)
+        g.build_cholesky()
+        h = g.advance(3)
+        return h, g.get_state()
+
+    h, st = run()
+    for env in ({"IBFS_COARSIFY_TILES": "0"}, {"IBFS_S_WINDOW": "0"}):
+        h1, st1 = run(**env)
+        assert np.array_equal(h1, h), env
+        for k in st:
+            assert np.array_equal(st1[k], st[k]), (env, k)
+    h1, st1 = run(IBFS_BAND_FFT="1")
+    for k in ("omega", "q", "fb"):
+        assert relerr(st1[k], st[k]) < 1e-13, k
+
+
 def test_tile_shapes_do_not_change_results(monkeypatch):
     """The tuner's choice of FFTs per workgroup (1, 2 or 4) must not change a
     single bit: sequences 2p and 2p+1 share FFT p whatever the shape.  Forced
