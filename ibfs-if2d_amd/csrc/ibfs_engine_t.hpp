@@ -152,6 +152,12 @@ struct Engine : GroupBase {
   // IBFS_S_WINDOW=0: step 2 transforms the whole of level 0.
   bool swin_ok = false;
   int swin[4] = {0, 0, 0, 0};
+  // step 3 in real space (k_ainv_local), per case: taken (ainv_loc), the
+  // target box, the kernel K(a, b) for 0 <= a, b <= ainv_R; set in
+  // upload_smear.  IBFS_STEP3_LOCAL=0 keeps the spectral form everywhere.
+  DevBuf<int> ainv_loc, ainv_box;
+  DevBuf<double> ainv_K;
+  int ainv_nloc = 0, ainv_R = 0, ainv_tiles = 0;
   int ringN = 0;
   DevBuf<double> ringT, ringG, ringGp, ringR, ringS, ringV, ringX, ringSin;
   // the band correction (BandOp): beta per item, the band width per level
@@ -1347,14 +1353,22 @@ struct Engine : GroupBase {
   // STEPS 3 and 4, fluxes and diagnostics
   void enqueue_step34() {
     const long FB = fbs(), QB = qbs();
-    // STEP 3 (src/operators.f90:174)
-    ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB}, T2.p, T3.p, B, nullptr, 0, lam1i.p, tbs());
+    // STEP 3 (src/operators.f90:174): in real space for the cases whose
+    // active vertices are far enough from the edges (k_ainv_local), as the
+    // spectral ctci for the others
+    const int nloc = base == nullptr ? ainv_nloc : 0;
+    if (nloc < B)
+      ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB, nloc > 0 ? ainv_loc.p : nullptr}, T2.p, T3.p, B,
+                nullptr, 0, lam1i.p, tbs());
+    if (nloc > 0)
+      hipLaunchKernelGGL(k_ainv_local<T>, dim3(ainv_tiles, B), dim3(256), 0, st, geo, omega.p, FB, (const T*)RR.p,
+                         (long)nw, ainv_box.p, ainv_K.p, ainv_loc.p, ainv_R);
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
     if constexpr (kSolve64) vort2flux64(omega.p, B, gate);
     else if (v2f_batch) vort2flux_levels(true);
     else vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
     // fluxes and diagnostics (src/operators.f90:184-186)
-    const int ncurl = (nq + 256 * kTailFaces - 1) / (256 * kTailFaces) * Lmax;
+    const int ncurl = tail_groups(geo) * Lmax;
     const bool defer = diag_deferred();
     hipLaunchKernelGGL(k_tail<T>, dim3(ncurl + nchunk * Lmax, B), dim3(256), 0, st, geo, S.p, q.p, omega.p, FB, QB,
                        partial.p, nchunk, gate, kSolve64 ? (const double*)S64.p : nullptr,
@@ -1390,6 +1404,90 @@ struct Engine : GroupBase {
     hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, diag_args(dpend.p, 1));
   }
 
+  // The tables of k_ainv_local: for each case the kernel of ainv on level 1
+  // in long double (the closed form over kx, the sum over ky) for
+  // 0 <= a, b <= R, and the box of its active vertices widened by R.  The
+  // reach R comes from a bound rather than from the tabulated values, whose
+  // last digits are rounding: every term of the ky sum has r_ky <= r_0 and
+  // sqrt(A_ky^2 - 4 beta^2) >= that of ky = 0, so |K(a, b)| <= r_0^|a| / s_0,
+  // and likewise in b (the lattice kernel is symmetric), so the places with
+  // max(|a|, |b|) = t (8t of them) add at most 8t r_0^t / s_0; R is the
+  // smallest with the sum of that over t > R below 1e-17.  A case is taken
+  // when its active box keeps R cells from every edge and R <= kAinvRmax.
+  // One R (the largest over the cases taken) for the launch.
+  // Tangent-linear engines (base set) keep the spectral form
+  // (enqueue_step34), so that their step stays the exact transpose of the
+  // adjoint's.
+  void ainv_tables(const std::vector<int>& vid) {
+    ainv_nloc = 0;
+    const char* env = std::getenv("IBFS_STEP3_LOCAL");
+    if (env && env[0] == '0') return;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<std::vector<long double>> Ks(B);
+    std::vector<int> Rb(B, -1), loc(B, 0), boxv(4 * B, 0);
+    std::vector<long double> cy(2 * n);
+    for (int k = 0; k < 2 * n; k++) cy[k] = cosl(pi * k / n);
+    for (int b = 0; b < B; b++) {
+      const CaseHost& c = cs[b];
+      const CaseSm& h = hcs[b];
+      if (h.nvtx == 0) continue;
+      const long double bt = 0.5L * c.d.dt / (c.d.re * c.delta * c.delta);
+      const long double A0 = 1.0L + 2.0L * bt, s0 = sqrtl(A0 * A0 - 4.0L * bt * bt), r0 = (A0 - s0) / (2.0L * bt);
+      int R = 0;
+      for (;; R++) {
+        long double tail = 0.0L, pw = powl(r0, (long double)(R + 1));
+        for (int t = R + 1; t < R + 400 && pw > 1e-40L; t++, pw *= r0) tail += 8.0L * t * pw / s0;
+        if (tail < 1e-17L || R > kAinvRmax) break;
+      }
+      if (R > kAinvRmax) continue;
+      std::vector<long double> Kt((size_t)(R + 1) * (R + 1), 0.0L);
+      for (int k = 0; k < 2 * n; k++) {
+        const long double A = 1.0L + bt * (4.0L - 2.0L * cy[k]), sq = sqrtl(A * A - 4.0L * bt * bt);
+        const long double r = (A - sq) / (2.0L * bt);
+        long double ra = 1.0L / sq;
+        for (int aa = 0; aa <= R; aa++) {
+          for (int bb = 0; bb <= R; bb++) Kt[(size_t)bb * (R + 1) + aa] += ra * cy[(size_t)((long)k * bb % (2 * n))];
+          ra *= r;
+        }
+      }
+      for (long double& v : Kt) v /= 2.0L * n;
+      int i0 = m, i1 = -1, j0 = n, j1 = -1;
+      for (int e = h.vbase; e < h.vbase + h.nvtx; e++) {
+        const int a = vid[e], ii = a % (m - 1), jj = a / (m - 1);
+        i0 = std::min(i0, ii), i1 = std::max(i1, ii), j0 = std::min(j0, jj), j1 = std::max(j1, jj);
+      }
+      if (i0 < R || j0 < R || i1 > m - 2 - R || j1 > n - 2 - R) continue;
+      Ks[b] = std::move(Kt);
+      Rb[b] = R;
+      loc[b] = 1;
+      boxv[4 * b] = i0, boxv[4 * b + 1] = j0, boxv[4 * b + 2] = i1, boxv[4 * b + 3] = j1;
+    }
+    int R = 0, tiles = 0, nloc = 0;
+    for (int b = 0; b < B; b++)
+      if (loc[b]) R = std::max(R, Rb[b]), nloc++;
+    if (nloc == 0) return;
+    // the target box at the launch's R, inside the field (places past the
+    // case's own reach get exact zeros)
+    for (int b = 0; b < B; b++)
+      if (loc[b]) {
+        const int i0 = std::max(boxv[4 * b] - R, 0), j0 = std::max(boxv[4 * b + 1] - R, 0);
+        const int i1 = std::min(boxv[4 * b + 2] + R, m - 2), j1 = std::min(boxv[4 * b + 3] + R, n - 2);
+        boxv[4 * b] = i0, boxv[4 * b + 1] = j0, boxv[4 * b + 2] = i1 - i0 + 1, boxv[4 * b + 3] = j1 - j0 + 1;
+        tiles = std::max(tiles, ((boxv[4 * b + 2] + 7) / 8) * ((boxv[4 * b + 3] + 7) / 8));
+      }
+    std::vector<double> kh((size_t)B * (R + 1) * (R + 1), 0.0);
+    for (int b = 0; b < B; b++)
+      if (loc[b])
+        for (int bb = 0; bb <= Rb[b]; bb++)
+          for (int a = 0; a <= Rb[b]; a++)
+            kh[((size_t)b * (R + 1) + bb) * (R + 1) + a] = (double)Ks[b][(size_t)bb * (Rb[b] + 1) + a];
+    upvec(ainv_K, kh, st);
+    upvec(ainv_loc, loc, st);
+    upvec(ainv_box, boxv, st);
+    ainv_R = R;
+    ainv_tiles = tiles;
+    ainv_nloc = nloc;
+  }
   // the compact smearing tables of every case, concatenated, on the device
   // (and the per-case offsets of the force system that depend on them)
   void upload_smear() {
@@ -1470,6 +1568,7 @@ struct Engine : GroupBase {
       swin[1] = std::min(swin[1] | 1, n - 2);
       swin[3] = std::min(swin[3] | 1, m - 2);
     }
+    ainv_tables(vid);
     nf_tot = foff;
     mat_tot = loff;
     gmat_tot = goff;

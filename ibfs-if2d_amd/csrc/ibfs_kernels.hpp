@@ -371,15 +371,20 @@ struct DstField {
 // destinations also offer pre (the old value, read by the size-specialised
 // passes together with their inputs) and put (the store given that value),
 // which give the same result as operator().
+// (skip: per batch entry, nonzero leaves that entry unwritten)
 template <typename T>
 struct DstSub {
-  T* a; long bs;
+  T* a; long bs; const int* skip = nullptr;
   __device__ void operator()(const Geo& g, int b, int ii, int jj, T v) const {
+    if (skip != nullptr && skip[b]) return;
     const long e = b * bs + ii + jj * g.mm1;
     a[e] = a[e] - v;
   }
   __device__ T pre(const Geo& g, int b, int ii, int jj) const { return a[b * bs + ii + jj * g.mm1]; }
-  __device__ void put(const Geo& g, int b, int ii, int jj, T v, T old) const { a[b * bs + ii + jj * g.mm1] = old - v; }
+  __device__ void put(const Geo& g, int b, int ii, int jj, T v, T old) const {
+    if (skip != nullptr && skip[b]) return;
+    a[b * bs + ii + jj * g.mm1] = old - v;
+  }
 };
 
 // Dst::pre present?
@@ -1604,6 +1609,71 @@ __global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ 
 // those entries are being overwritten by the first half of the grid.
 // grid.x = 2*nblk (nblk = ceil((m/2-1)(n/2-1)/256)) or nblk when
 // lev+1 == Lmax.
+// STEP 3 near the bodies (src/operators.f90:174): omega_1 -= ainv(rr) in
+// real space.  ainv = S diag(lam1i) S / (4mn) inverts the implicit viscous
+// operator 1 + beta L (L the 5-point stencil, beta = dt / (2 Re Delta^2));
+// written over one period of the sine form's odd extension its kernel is
+// translation invariant plus images in the domain's edges,
+//   K(a, b) = (1/2n) sum_{ky=0}^{2n-1} cos(pi ky b / n) r_ky^|a| / sqrt(A_ky^2 - 4 beta^2),
+//   A_ky = 1 + beta (4 - 2 cos(pi ky / n)),  r_ky = (A_ky - sqrt(A_ky^2 - 4 beta^2)) / (2 beta)
+// (the sum over kx in closed form: the lattice Green's function of a
+// three-point operator, whose periodic images at distance 2m are below
+// 1e-300), and it decays geometrically in |a| and |b|.  rr is zero but on
+// the active vertices, so omega changes only within R cells of them: the
+// host tabulates K for 0 <= a, b <= R (the dropped tail below 1e-17 of a unit
+// source) and takes a case this way only when its active vertices are at
+// least R cells from every edge (then every image term is past R).
+// Workgroup: an 8 x 8 block of the case's target box (the active box
+// widened by R, inside the field); it stages rr over the block and R around
+// it (zero outside the field), and K; each of the four waves sums every
+// fourth row of the (2R+1)^2 window for the 64 places, and the four partial
+// sums are added in a fixed order.  R is the largest reach of the cases
+// taken; a case's table is zero past its own reach.
+// om: level 1 of case b at b bs; rr (one level per case) at b rbs.
+// box[b] = {i0, j0, nx, ny} (target box), K per case ((R+1)^2, row b = |dy|),
+// loc[b] != 0 for the cases taken.  grid (max tiles, B), 256 threads.
+constexpr int kAinvRmax = 20;
+constexpr int kAinvW = 8 + 2 * kAinvRmax;
+template <typename T>
+__global__ __launch_bounds__(256) void k_ainv_local(Geo g, T* __restrict__ om, long bs, const T* __restrict__ rr,
+                                                    long rbs, const int* __restrict__ box,
+                                                    const double* __restrict__ K, const int* __restrict__ loc, int R) {
+  __shared__ double rs[kAinvW * kAinvW];
+  __shared__ double ks[(kAinvRmax + 1) * (kAinvRmax + 1)];
+  __shared__ double red[4][64];
+  const int b = blockIdx.y;
+  if (!loc[b]) return;
+  const int bi0 = box[4 * b], bj0 = box[4 * b + 1], nx = box[4 * b + 2], ny = box[4 * b + 3];
+  const int ntx = (nx + 7) / 8, tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  if (ty * 8 >= ny) return;
+  const int x0 = bi0 + tx * 8, y0 = bj0 + ty * 8, W = 8 + 2 * R, R1 = R + 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lx = lane & 7, ly = lane >> 3;
+  const bool mine = w == 0 && tx * 8 + lx < nx && ty * 8 + ly < ny;
+  T* o = om + b * bs + (x0 + lx) + (long)(y0 + ly) * g.mm1;
+  const T old = mine ? *o : T(0);
+  const T* rb = rr + b * rbs;
+  for (int e = threadIdx.x; e < W * W; e += 256) {
+    const int y = e / W, x = e - y * W, ii = x0 - R + x, jj = y0 - R + y;
+    rs[e] = (ii >= 0 && ii < g.mm1 && jj >= 0 && jj < g.nm1) ? (double)rb[ii + (long)jj * g.mm1] : 0.0;
+  }
+  for (int e = threadIdx.x; e < R1 * R1; e += 256) ks[e] = K[(long)b * R1 * R1 + e];
+  __syncthreads();
+  // wave w takes the rows dy = w mod 4, so a case's sums do not depend on
+  // the launch's R (rows and places past the case's own reach add exact
+  // zeros): a case in a batch gives its single-case result bit for bit
+  double acc = 0.0;
+  for (int dy = -R + ((w + R) & 3); dy <= R; dy += 4) {
+    const double* kr = ks + (dy < 0 ? -dy : dy) * R1;
+    const double* rrow = rs + (ly + R - dy) * W + lx + R;
+    double a = kr[0] * rrow[0];
+    for (int dx = 1; dx <= R; dx++) a = fma(kr[dx], rrow[-dx] + rrow[dx], a);
+    acc += a;
+  }
+  red[w][lane] = acc;
+  __syncthreads();
+  if (mine) *o = old - T((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+}
+
 // The same two levels per launch in LDS tiles.  Workgroup (tx, ty) takes a
 // kCoTX x kCoTY block of level lev+1's centre.  It stages the level lev
 // places its stencils read ((2 kCoTX + 1) x (2 kCoTY + 1)) and, when those
@@ -2493,13 +2563,17 @@ __device__ __forceinline__ bool curl_interior(const Geo& g, int f, int& ia, int&
 }
 
 // One launch after the last streamfunction of the step is final:
-//  * workgroups x < ncurl = ceil(nq/(256*kTailFaces))*Lmax: q = curl(s) on
-//    level x / ceil(nq/(256*kTailFaces)) (src/operators.f90:236-238, as
-//    k_curl_levels), kTailFaces faces per thread whose interior loads are
-//    all issued before the first is used; with s1d set, the streamfunctions
-//    of every level are s1d (FP64).  These groups run in XCD-contiguous
-//    order, so the streamfunction rows shared by neighbouring face rows
-//    stay in one L2;
+//  * workgroups x < ncurl = ceil((m+1)(n+1)/(256*kTailPos))*Lmax: q =
+//    curl(s) on level x / ceil(...) (src/operators.f90:236-238, as
+//    k_curl_levels).  A thread takes kTailPos places p = (i-1) + (j-1)(m+1),
+//    i <= m+1, j <= n+1, and at each the u face (i, j) (j <= n) and the v
+//    face (i, j) (i <= m): s(i, j), s(i, j+1) and s(i+1, j) serve both, so
+//    a workgroup reads two rows of s once (u and v faces of one row used to
+//    be far apart in the grid and read those rows twice).  Every interior
+//    load is issued before the first is used; faces on the domain edge take
+//    curl_face.  With s1d set, the streamfunctions of every level are s1d
+//    (FP64).  These groups run in XCD-contiguous order, so the rows shared
+//    by neighbouring groups stay in one L2;
 //  * the next nchunk*Lmax: partial sums of omega for write_circ
 //    (src/operators.f90:675-684), chunk (x-ncurl) % nchunk of level
 //    (x-ncurl) / nchunk, at partial[((b*Lmax)+lev)*nchunk + chunk].
@@ -2507,29 +2581,43 @@ __device__ __forceinline__ bool curl_interior(const Geo& g, int f, int& ia, int&
 // row (slot counter[b] - 1) is left to the next launch that writes pending
 // rows (diag_row).
 // grid (ncurl + nchunk*Lmax, B), 256 threads.
-constexpr int kTailFaces = 4;
+constexpr int kTailPos = 2;
+__host__ __device__ __forceinline__ int tail_groups(const Geo& g) {
+  return ((g.m + 1) * (g.n + 1) + 256 * kTailPos - 1) / (256 * kTailPos);
+}
 template <typename T, typename TS, typename TC = T>
-__device__ __forceinline__ void tail_curl(const Geo& g, int f0, const TS* __restrict__ sl, const TC* __restrict__ sc,
+__device__ __forceinline__ void tail_curl(const Geo& g, int p0, const TS* __restrict__ sl, const TC* __restrict__ sc,
                                           T* __restrict__ ql) {
-  int ia[kTailFaces], ib[kTailFaces];
-  bool in[kTailFaces];
-  TS va[kTailFaces], vb[kTailFaces];
+  const int m = g.m, n = g.n, nu = (m + 1) * n, np = (m + 1) * (n + 1);
+  int ic[kTailPos], iu[kTailPos], ir[kTailPos];
+  bool hu[kTailPos], hv[kTailPos], inu[kTailPos], inv[kTailPos];
 #pragma unroll
-  for (int u = 0; u < kTailFaces; u++) {
-    const int f = f0 + 256 * u;
-    in[u] = f < g.nq && curl_interior(g, f, ia[u], ib[u]);
-    if (!in[u]) ia[u] = ib[u] = 0;
+  for (int u = 0; u < kTailPos; u++) {
+    const int p = p0 + 256 * u, i = p % (m + 1) + 1, j = p / (m + 1) + 1;
+    hu[u] = p < np && j <= n;
+    hv[u] = p < np && i <= m;
+    inu[u] = hu[u] && i > 1 && i < m + 1 && j > 1 && j < n;
+    inv[u] = hv[u] && j > 1 && j < n + 1 && i > 1 && i < m;
+    const bool c = inu[u] || inv[u];
+    ic[u] = c ? wix(g, i, j) : 0;
+    iu[u] = inu[u] ? wix(g, i, j + 1) : 0;
+    ir[u] = inv[u] ? wix(g, i + 1, j) : 0;
+  }
+  TS vc[kTailPos], vu[kTailPos], vr[kTailPos];
+#pragma unroll
+  for (int u = 0; u < kTailPos; u++) {
+    vc[u] = sl[ic[u]];
+    vu[u] = sl[iu[u]];
+    vr[u] = sl[ir[u]];
   }
 #pragma unroll
-  for (int u = 0; u < kTailFaces; u++) {
-    va[u] = sl[ia[u]];
-    vb[u] = sl[ib[u]];
-  }
-#pragma unroll
-  for (int u = 0; u < kTailFaces; u++) {
-    const int f = f0 + 256 * u;
-    if (f >= g.nq) break;
-    ql[f] = in[u] ? T(va[u] - vb[u]) : T(curl_face<TS, TC>(g, f, sl, sc));
+  for (int u = 0; u < kTailPos; u++) {
+    const int p = p0 + 256 * u, i = p % (m + 1) + 1, j = p / (m + 1) + 1;
+    if (hu[u]) ql[p] = inu[u] ? T(vu[u] - vc[u]) : T(curl_face<TS, TC>(g, p, sl, sc));
+    if (hv[u]) {
+      const int fv = nu + (i - 1) + (j - 1) * m;
+      ql[fv] = inv[u] ? T(vc[u] - vr[u]) : T(curl_face<TS, TC>(g, fv, sl, sc));
+    }
   }
 }
 
@@ -2545,10 +2633,10 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
     pend[b] = 1;
     counter[b] += 1;
   }
-  const int ncb = (g.nq + 256 * kTailFaces - 1) / (256 * kTailFaces), ncurl = ncb * g.Lmax;
+  const int ncb = tail_groups(g), ncurl = ncb * g.Lmax;
   if ((int)blockIdx.x < ncurl) {
     const int x = xcd_contiguous(blockIdx.x, ncurl);
-    const int lev = x / ncb, f0 = (x - lev * ncb) * 256 * kTailFaces + tid;
+    const int lev = x / ncb, f0 = (x - lev * ncb) * 256 * kTailPos + tid;
     if (!active(Lc, b, lev)) return;
     const bool hc = has_coarser(g, Lc, b, lev);
     if (s1d != nullptr) {  // every level solved in FP64: the differences in FP64, rounded once

@@ -364,6 +364,34 @@ def test_band_correction_real_space_equals_spectral(name, monkeypatch):
     assert not np.array_equal(s1["omega"], s0["omega"])  # the two forms did run
 
 
+@pytest.mark.parametrize("name", ["cyl40r5", "cyl64L4", "rect64x48L5"])
+def test_step3_real_space_equals_spectral(name, monkeypatch):
+    """Step 3 (omega_1 -= ainv(rot(reg(fb)))) as a short sum with the
+    tabulated real-space kernel of ainv around the active vertices (the
+    default where they keep the kernel's reach from the edges) against the
+    spectral ctci (IBFS_STEP3_LOCAL=0): the same 20 steps to rounding (the
+    kernel's dropped tail is below 1e-17 of a unit source)."""
+    from ibfs_amd import hip
+
+    a = v2f_args(name)
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_STEP3_LOCAL", "0")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(20), g0.advance(20)
+    s1, s0 = g1.get_state(), g0.get_state()
+    nb2 = 2 * g1.nbody
+    assert np.max(np.abs(h1[:, :nb2] - h0[:, :nb2])) < 1e-13 * np.max(np.abs(h0[:, :nb2]))
+    for k in range(g1.L):
+        w = slice(k * g1.nw, (k + 1) * g1.nw)
+        assert relerr(s1["omega"][w], s0["omega"][w]) < 1e-13, f"omega level {k + 1}"
+        v = slice(k * g1.nq, (k + 1) * g1.nq)
+        assert relerr(s1["q"][v], s0["q"][v]) < 1e-13, f"q level {k + 1}"
+    assert relerr(s1["fb"], s0["fb"]) < 1e-12
+    assert not np.array_equal(s1["omega"], s0["omega"])  # the real-space form did run
+
+
 @pytest.mark.parametrize("name", ["rect64x48L5", "cyl64L4", "cyl40r5"])
 def test_coarsify_tiles_are_bit_identical(name, monkeypatch):
     """The coarsify chain two levels per launch in LDS tiles (the default)
