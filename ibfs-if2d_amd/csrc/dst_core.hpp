@@ -561,25 +561,17 @@ __device__ __forceinline__ void load_seqs(const Op& op, cplx<T>* z, int b, int g
   }
 }
 
+// One workgroup's part of a size-specialised pass: group g of ng (before
+// the XCD-contiguous remap), LDS buffer z of (DB ? 2 : 1) P N2 entries.
 template <typename T, int N2, int P, int AXIS, class Op>
-__global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* __restrict__ tw) {
+__device__ __forceinline__ void dst_pass_body(const Op& op, const cplx<T>* __restrict__ tw, cplx<T>* z, int g,
+                                              int ng) {
   constexpr int N = N2 / 2, LEN = N - 1, NT = P * N2 / 4;
   constexpr int NQ = (P * LEN + NT - 1) / NT;  // outputs (FFT, k) per thread
   constexpr bool kPre = op_store_pre<Op>::value;
   constexpr bool DB = kTwoBuf<T, N2, P>;
-  __shared__ cplx<T> z[(DB ? 2 : 1) * P * N2];
   const int t = threadIdx.x;
   const int b = blockIdx.y;
-  int g = blockIdx.x, ng = gridDim.x;
-  if constexpr (op_side<Op>::value) {
-    const int sg = op.side_groups_for(NT);
-    if (g < sg) {
-      op.side(b, g, t);
-      return;
-    }
-    g -= sg;
-    ng -= sg;
-  }
   if ((ng & 7) == 0) g = (g & 7) * (ng >> 3) + (g >> 3);
   auto out_at = [&](int e, int& p, int& k) {
     if (AXIS == 0) { p = e / LEN; k = e - p * LEN + 1; }
@@ -619,6 +611,38 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
       else op.store(b, g, p, P, k, -Z.im, Z.re);
     }
   }
+}
+
+template <typename T, int N2, int P, int AXIS, class Op>
+__global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* __restrict__ tw) {
+  constexpr int NT = P * N2 / 4;
+  constexpr bool DB = kTwoBuf<T, N2, P>;
+  __shared__ cplx<T> z[(DB ? 2 : 1) * P * N2];
+  int g = blockIdx.x, ng = gridDim.x;
+  if constexpr (op_side<Op>::value) {
+    const int sg = op.side_groups_for(NT);
+    if (g < sg) {
+      op.side(blockIdx.y, g, threadIdx.x);
+      return;
+    }
+    g -= sg;
+    ng -= sg;
+  }
+  dst_pass_body<T, N2, P, AXIS, Op>(op, tw, z, g, ng);
+}
+
+// Two independent passes of one FFT length in one launch: workgroups
+// x < ga run the row pass A, the others the column pass B (ops without
+// side work; they must write disjoint data).  Latency-bound pairs of small
+// passes then cost one launch and one load-transform-store chain.
+template <typename T, int N2, int P, class OpA, class OpB>
+__global__ __launch_bounds__(P * N2 / 4) void dst_pass_dual(OpA a, OpB bq, const cplx<T>* __restrict__ twa,
+                                                            const cplx<T>* __restrict__ twb, int ga) {
+  constexpr bool DB = kTwoBuf<T, N2, P>;
+  __shared__ cplx<T> z[(DB ? 2 : 1) * P * N2];
+  const int x = blockIdx.x;
+  if (x < ga) dst_pass_body<T, N2, P, 0, OpA>(a, twa, z, x, ga);
+  else dst_pass_body<T, N2, P, 1, OpB>(bq, twb, z, x - ga, (int)gridDim.x - ga);
 }
 
 // fused column pair (see dst_col2) at a compile-time size

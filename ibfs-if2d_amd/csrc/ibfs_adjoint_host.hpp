@@ -114,7 +114,7 @@ struct AdjointRunner {
     else
       AFB.zero(st);
     hipLaunchKernelGGL(k_regT_curl<T>, grid(E.nf_max), dim3(256), 0, st, g, E.sm, U.p, (const T*)nullptr,
-                       (long)nw, AFB.p, Wv.p, 0L, Lc, -1);
+                       (long)nw, AFB.p, Wv.p, 0L, Lc, -1, (const T*)nullptr);
     const dim3 g4((E.nf_max + 3) / 4, B);
     hipLaunchKernelGGL(k_trmv_lower, g4, dim3(256), 0, st, E.sm, E.Linv.p, Wv.p, Y.p);
     hipLaunchKernelGGL(k_trmv_upper_t, g4, dim3(256), 0, st, E.sm, E.Linv.p, Y.p, Z.p, 0);

@@ -1169,7 +1169,8 @@ int ibfs_op_a_times(ibfs_handle* h, const double* x, double* out) {
                  0);
     E->vort2flux(VB.p, SB.p, nullptr, 1, E->Lc.p, t1.p, t2.p);
     hipLaunchKernelGGL(k_regT_curl<double>, dim3((nf + 255) / 256, 1), dim3(256), 0, st, E->geo, E->sm, SB.p,
-                       E->Lmax > 1 ? SB.p + nw : nullptr, 0L, (const double*)nullptr, o.p, (long)nf, E->Lc.p, 0);
+                       E->Lmax > 1 ? SB.p + nw : nullptr, 0L, (const double*)nullptr, o.p, (long)nf, E->Lc.p, 0,
+                       (const double*)nullptr);
     o.download(out, nf, st);
     HIPCHK(hipStreamSynchronize(st));
   });

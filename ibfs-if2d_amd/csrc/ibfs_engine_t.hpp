@@ -637,6 +637,45 @@ struct Engine : GroupBase {
   // Launch one pass over nseq sequences along the other axis; mult = 2 when
   // a workgroup's P FFTs take 2P sequences of one field, 1 when they pair
   // two fields (X + iY).  kind selects the tile shape.
+  // A row pass and a column pass of the same FFT length in one launch
+  // (dst_pass_dual), two sequences per FFT, when both kinds run the same
+  // size-specialised LDS pass at this batch; false (nothing launched)
+  // otherwise.
+  template <class OpA, class OpB>
+  bool launch_dual(const OpA& a, int nseqa, const OpB& bq, int nseqb, int batch) {
+    using V = op_value_t<OpA>;
+    static_assert(std::is_same_v<V, op_value_t<OpB>>, "launch_dual: one value type");
+    const FftSet<V>& F = fset<V>();
+    if (!F.spec_m || !F.spec_n || F.pm.n2 != F.pn.n2) return false;
+    const int P = tile<V>(0, batch);
+    if (P != tile<V>(kCol, batch) || (P != 1 && P != 2 && P != 4)) return false;
+    const int ga = (nseqa + 2 * P - 1) / (2 * P), gb = (nseqb + 2 * P - 1) / (2 * P);
+    switch (F.pm.n2 * 8 + P) {
+#define IBFS_DUAL_CASE(NN, PP)                                                                                 \
+  case NN * 8 + PP:                                                                                            \
+    if constexpr (p_fits<V>(NN, PP)) {                                                                         \
+      hipLaunchKernelGGL((spec::dst_pass_dual<V, NN, PP, OpA, OpB>), dim3(ga + gb, batch), dim3(PP * NN / 4), 0, \
+                         st, a, bq, F.stw_m.p, F.stw_n.p, ga);                                                 \
+      return true;                                                                                             \
+    }                                                                                                          \
+    return false;
+#define IBFS_DUAL_SIZE(NN) IBFS_DUAL_CASE(NN, 1) IBFS_DUAL_CASE(NN, 2) IBFS_DUAL_CASE(NN, 4)
+      IBFS_DUAL_SIZE(256)
+      IBFS_DUAL_SIZE(400)
+      IBFS_DUAL_SIZE(512)
+      IBFS_DUAL_SIZE(600)
+      IBFS_DUAL_SIZE(640)
+      IBFS_DUAL_SIZE(768)
+      IBFS_DUAL_SIZE(1024)
+      IBFS_DUAL_SIZE(1200)
+      IBFS_DUAL_SIZE(2048)
+#undef IBFS_DUAL_SIZE
+#undef IBFS_DUAL_CASE
+      default: return false;
+    }
+  }
+  // step 2 on the force-stencil rectangle (see vort2flux_levels)
+  bool step2_window() const { return swin_ok && !refine; }
   template <int AXIS, class Op>
   void launch(const Op& op, int nseq, int mult, int batch, int kind = AXIS) {
     using V = op_value_t<Op>;
@@ -1121,22 +1160,28 @@ struct Engine : GroupBase {
                          sizeof(double) * (Kx + 2 * (Kx + 1)), st, geo, ringR.p, NR, below, Kx, ringSin.p,
                          ringSin.p + 2 * m, ringX.p);
       const double* Gy = m == n ? ringG.p : ringGp.p;
-      if (!all && swin_ok) {
+      if (!all && step2_window()) {
         // step 2 reads s_1 only on the vertices of the force stencils: the
         // last two transforms of level 0 on the rows and the columns of
         // their bounding rectangle (the rest of s_1 is rewritten in step 4
-        // before anything reads it)
+        // before anything reads it).  The row pass writes S, the column pass
+        // the left/right terms into level 0 of ringS, which the force
+        // right-hand side adds (the same sum as the column pass adding into
+        // S), so the two passes are independent: one launch when both are
+        // the same size-specialised pass.
         Geo gr = geoL, gc = geoL;
         gr.nm1 = swin[1] - swin[0] + 1;
         gc.mm1 = swin[3] - swin[2] + 1;
-        launch<0>(RowOp<T, SrcRowWin<SrcRingRow>, DstRowWin<DstRingItem>>{
-                      gr, {SrcRingRow{T2L.p, ringX.p, ringG.p, out, Kx}, geoL, swin[0]},
-                      {DstRingItem{S.p, out, 0}, geoL, swin[0]}, nullptr, 0},
-                  gr.nm1, 2, nout);
-        launch<1>(ColOp<T, SrcColWin<SrcRingCol>, DstColWin<DstRingItem>>{
-                      gc, {SrcRingCol{ringX.p, Gy, below, Kx}, geoL, swin[2]},
-                      {DstRingItem{S.p, below, 1}, geoL, swin[2], swin[0], swin[1]}, nullptr, 0},
-                  gc.mm1, 2, nbelow, kCol);
+        const RowOp<T, SrcRowWin<SrcRingRow>, DstRowWin<DstRingItem>> rop{
+            gr, {SrcRingRow{T2L.p, ringX.p, ringG.p, out, Kx}, geoL, swin[0]},
+            {DstRingItem{S.p, out, 0}, geoL, swin[0]}, nullptr, 0};
+        const ColOp<T, SrcColWin<SrcRingCol>, DstColWin<DstRingItem>> cop{
+            gc, {SrcRingCol{ringX.p, Gy, below, Kx}, geoL, swin[2]},
+            {DstRingItem{ringS.p, below, 0}, geoL, swin[2], swin[0], swin[1]}, nullptr, 0};
+        if (!launch_dual(rop, gr.nm1, cop, gc.mm1, nout)) {
+          launch<0>(rop, gr.nm1, 2, nout);
+          launch<1>(cop, gc.mm1, 2, nbelow, kCol);
+        }
         return;
       }
       launch<0>(RowOp<T, SrcRingRow, DstRingItem>{geoL, SrcRingRow{T2L.p, ringX.p, ringG.p, out, Kx},
@@ -1318,13 +1363,16 @@ struct Engine : GroupBase {
       // get_bc(s_2) reads level 1's ring only; the batched vort2flux keeps
       // those values in ringS (level 1 of S is not completed in step 2)
       const TS* s2 = Lmax > 1 ? s1 + nw : nullptr;
-      if constexpr (std::is_same_v<TS, double> && std::is_same_v<T, double>)
+      const TS* s1c = nullptr;  // the left/right terms of s_1 on the force-stencil rectangle
+      if constexpr (std::is_same_v<TS, double> && std::is_same_v<T, double>) {
         if (v2f_batch) s2 = ringS.p + nw;
+        if (v2f_batch && step2_window()) s1c = ringS.p;
+      }
       if (!refine) {
         // the right-hand side once per case, then the rows of [A^-1; G]
         if constexpr (std::is_same_v<TS, T>)
           hipLaunchKernelGGL((k_regT_curl<T>), dim3((std::max(nf_max, 1) + 255) / 256, B), dim3(256), 0, st, geo, sm,
-                             s1, s2, FB, motion.p, rhsf.p, 0L, gate, -1);
+                             s1, s2, FB, motion.p, rhsf.p, 0L, gate, -1, s1c);
         hipLaunchKernelGGL((k_force_solve<T, TS>), gall, dim3(kForceThreads), lds, st, geo, sm, s1, s2, FB, motion.p,
                            gate, Ainv.p, Gm.p, fb.p, RR.p, (long)nw, rhsf.p, (const double*)nullptr,
                            std::is_same_v<TS, T> ? 3 : 0);
@@ -1772,7 +1820,8 @@ struct Engine : GroupBase {
     ctci_like(SrcField<T>{cgRR.p, nw}, true, DstField<T>{cgVB.p, fbs()}, cgT1.p, cgT2.p, 1, nullptr, 0, lam1i.p, 0);
     vort2flux(cgVB.p, cgSB.p, nullptr, 1, gate, cgT1.p, cgT2.p);
     hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf + 255) / 256, 1), dim3(256), 0, st, geo, sm, cgSB.p,
-                       Lmax > 1 ? cgSB.p + nw : nullptr, fbs(), (const double*)nullptr, out, (long)nf, gate, 0);
+                       Lmax > 1 ? cgSB.p + nw : nullptr, fbs(), (const double*)nullptr, out, (long)nf, gate, 0,
+                       (const T*)nullptr);
   }
 
   CgState cg_read() {
@@ -1829,7 +1878,8 @@ struct Engine : GroupBase {
     // STEP 2 (src/operators.f90:155-164): rhsf = regT(q_1) - motion
     vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
     hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf_max + 255) / 256, B), dim3(256), 0, st, geo, sm, S.p,
-                       Lmax > 1 ? S.p + nw : nullptr, fbs(), (const double*)motion.p, rhsf.p, 0L, gate, -1);
+                       Lmax > 1 ? S.p + nw : nullptr, fbs(), (const double*)motion.p, rhsf.p, 0L, gate, -1,
+                       (const T*)nullptr);
     const int it = cg_solve(cgtol, cg_max_iter);
     // the step-3 source rot(reg(fb)) on the active vertices
     hipLaunchKernelGGL((k_rotreg<T, double>), dim3((nvtx_max + 255) / 256, B), dim3(256), 0, st, sm, fb.p, 0L, RR.p,
@@ -2044,7 +2094,7 @@ struct Engine : GroupBase {
       vort2flux(VB.p, SB.p, nullptr, bb, gate.p, t1.p, t2.p);
       hipLaunchKernelGGL(k_regT_curl<T>, dim3((nf + 255) / 256, bb), dim3(256), 0, st, geo, sm, SB.p,
                          Lmax > 1 ? SB.p + nw : nullptr, bsL, (const double*)nullptr, acol + (long)c0 * nf,
-                         (long)nf, gate.p, c);
+                         (long)nf, gate.p, c, (const T*)nullptr);
       HIPCHK(hipStreamSynchronize(st));  // hF is reused
     }
   }

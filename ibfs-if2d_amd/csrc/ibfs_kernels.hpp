@@ -1652,11 +1652,29 @@ __global__ __launch_bounds__(256) void k_ainv_local(Geo g, T* __restrict__ om, l
   T* o = om + b * bs + (x0 + lx) + (long)(y0 + ly) * g.mm1;
   const T old = mine ? *o : T(0);
   const T* rb = rr + b * rbs;
-  for (int e = threadIdx.x; e < W * W; e += 256) {
-    const int y = e / W, x = e - y * W, ii = x0 - R + x, jj = y0 - R + y;
-    rs[e] = (ii >= 0 && ii < g.mm1 && jj >= 0 && jj < g.nm1) ? (double)rb[ii + (long)jj * g.mm1] : 0.0;
+  // every staging load of the thread is issued before the first store
+  constexpr int U = (kAinvW * kAinvW + 255) / 256, UK = ((kAinvRmax + 1) * (kAinvRmax + 1) + 255) / 256;
+  double v[U], kv[UK];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int e = threadIdx.x + 256 * u, y = e / W, x = e - y * W, ii = x0 - R + x, jj = y0 - R + y;
+    v[u] = (e < W * W && ii >= 0 && ii < g.mm1 && jj >= 0 && jj < g.nm1) ? (double)rb[ii + (long)jj * g.mm1] : 0.0;
   }
-  for (int e = threadIdx.x; e < R1 * R1; e += 256) ks[e] = K[(long)b * R1 * R1 + e];
+#pragma unroll
+  for (int u = 0; u < UK; u++) {
+    const int e = threadIdx.x + 256 * u;
+    kv[u] = e < R1 * R1 ? K[(long)b * R1 * R1 + e] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int e = threadIdx.x + 256 * u;
+    if (e < W * W) rs[e] = v[u];
+  }
+#pragma unroll
+  for (int u = 0; u < UK; u++) {
+    const int e = threadIdx.x + 256 * u;
+    if (e < R1 * R1) ks[e] = kv[u];
+  }
   __syncthreads();
   // wave w takes the rows dy = w mod 4, so a case's sums do not depend on
   // the launch's R (rows and places past the case's own reach add exact
@@ -1962,11 +1980,13 @@ __device__ __forceinline__ double reg_face(const Smear& sm, const CaseSm& c, int
 template <typename T, typename TC = T>
 __device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, const CaseSm& c, int f,
                                                const T* __restrict__ s1, const TC* __restrict__ sc,
-                                               const double* __restrict__ sub) {
+                                               const double* __restrict__ sub, const T* __restrict__ s1c = nullptr) {
   // Interior faces take curl = s1[ia] - s1[ib] from the table, so the K
   // entries' index, weight and field loads are each issued together; faces
   // on the domain edge (ia < 0, only for bodies touching the edge) fall back
-  // to curl_face with its get_bc terms.  Same values as curl_face.
+  // to curl_face with its get_bc terms.  Same values as curl_face.  With s1c
+  // the streamfunction is s1 + s1c, added here (interior faces only: the
+  // caller passes s1c only when no stencil face lies on the edge).
   double s = 0.0;
   if (sm.f9_w != nullptr) {  // padded: the slots past a force's entries add +0
     const int base = c.foff + f;
@@ -1983,6 +2003,19 @@ __device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, co
     for (int k = 0; k < kFrcK; k++) {
       va[k] = s1[ia[k] < 0 ? 0 : ia[k]];
       vb[k] = s1[ib[k] < 0 ? 0 : ib[k]];
+    }
+    if (s1c != nullptr) {
+      T ca[kFrcK], cb[kFrcK];
+#pragma unroll
+      for (int k = 0; k < kFrcK; k++) {
+        ca[k] = s1c[ia[k] < 0 ? 0 : ia[k]];
+        cb[k] = s1c[ib[k] < 0 ? 0 : ib[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < kFrcK; k++) {
+        va[k] = va[k] + ca[k];
+        vb[k] = vb[k] + cb[k];
+      }
     }
 #pragma unroll
     for (int k = 0; k < kFrcK; k++) {
@@ -2009,6 +2042,13 @@ __device__ __forceinline__ double regT_curl_at(const Geo& g, const Smear& sm, co
     for (int k = 0; k < kFrcK; k++) {
       va[k] = s1[ia[k] < 0 ? 0 : ia[k]];
       vb[k] = s1[ib[k] < 0 ? 0 : ib[k]];
+    }
+    if (s1c != nullptr) {
+#pragma unroll
+      for (int k = 0; k < kFrcK; k++) {
+        va[k] = va[k] + s1c[ia[k] < 0 ? 0 : ia[k]];
+        vb[k] = vb[k] + s1c[ib[k] < 0 ? 0 : ib[k]];
+      }
     }
 #pragma unroll
     for (int k = 0; k < kFrcK; k++) {
@@ -2063,11 +2103,12 @@ __global__ void k_regT(Smear sm, const T* __restrict__ h, double* __restrict__ h
 // out = regT(curl(s1, get_bc(s2))) - sub (sub may be null): the right-hand
 // side of the force system (src/operators.f90:163-164), and the columns of
 // a_times (src/operators.f90:459-461) for batches of one case.  s2 is used
-// only for cases with more than one level.  grid.x over max nf.
+// only for cases with more than one level; s1c (may be null) is added to
+// s1 (the same layout).  grid.x over max nf.
 template <typename T>
 __global__ void k_regT_curl(Geo g, Smear sm, const T* __restrict__ s1, const T* __restrict__ s2, long bs,
                             const double* __restrict__ sub, double* __restrict__ out, long obs,
-                            const int* __restrict__ Lc, int one_case) {
+                            const int* __restrict__ Lc, int one_case, const T* __restrict__ s1c) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   const int ci = one_case >= 0 ? one_case : b;
@@ -2075,7 +2116,7 @@ __global__ void k_regT_curl(Geo g, Smear sm, const T* __restrict__ s1, const T* 
   if (f >= c.nf) return;
   const T* sc = (s2 != nullptr && levels(g, Lc, b) > 1) ? s2 + b * bs : nullptr;  // Lc indexed by batch entry
   const long o = obs > 0 ? b * obs + f : c.foff + f;
-  out[o] = regT_curl_at<T>(g, sm, c, f, s1 + b * bs, sc, sub);
+  out[o] = regT_curl_at<T>(g, sm, c, f, s1 + b * bs, sc, sub, s1c != nullptr ? s1c + b * bs : nullptr);
 }
 
 // motion_eff = motion_in - regT(q0_1) (src/operators.f90:155), per case;
