@@ -1409,7 +1409,8 @@ struct Engine : GroupBase {
       ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB, nloc > 0 ? ainv_loc.p : nullptr}, T2.p, T3.p, B,
                 nullptr, 0, lam1i.p, tbs());
     if (nloc > 0)
-      hipLaunchKernelGGL(k_ainv_local<T>, dim3(ainv_tiles, B), dim3(256), 0, st, geo, omega.p, FB, (const T*)RR.p,
+      hipLaunchKernelGGL(k_ainv_local<T>, dim3(ainv_tiles, B), dim3(64 * kAinvWaves), 0, st, geo, omega.p, FB,
+                         (const T*)RR.p,
                          (long)nw, ainv_box.p, ainv_K.p, ainv_loc.p, ainv_R);
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
     if constexpr (kSolve64) vort2flux64(omega.p, B, gate);

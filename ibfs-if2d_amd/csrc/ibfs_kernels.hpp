@@ -1625,22 +1625,26 @@ __global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ 
 // least R cells from every edge (then every image term is past R).
 // Workgroup: an 8 x 8 block of the case's target box (the active box
 // widened by R, inside the field); it stages rr over the block and R around
-// it (zero outside the field), and K; each of the four waves sums every
-// fourth row of the (2R+1)^2 window for the 64 places, and the four partial
-// sums are added in a fixed order.  R is the largest reach of the cases
+// it (zero outside the field), and K; each of the sixteen waves sums every
+// sixteenth row of the (2R+1)^2 window for the 64 places (short chains: the
+// sums are latency-bound), and the partial sums are added in a fixed tree.  R is the largest reach of the cases
 // taken; a case's table is zero past its own reach.
 // om: level 1 of case b at b bs; rr (one level per case) at b rbs.
 // box[b] = {i0, j0, nx, ny} (target box), K per case ((R+1)^2, row b = |dy|),
-// loc[b] != 0 for the cases taken.  grid (max tiles, B), 256 threads.
+// loc[b] != 0 for the cases taken.  grid (max tiles, B), 64 kAinvWaves threads.
 constexpr int kAinvRmax = 20;
 constexpr int kAinvW = 8 + 2 * kAinvRmax;
+constexpr int kAinvWaves = 16;  // waves per workgroup; wave w sums the rows dy = w mod 16
 template <typename T>
-__global__ __launch_bounds__(256) void k_ainv_local(Geo g, T* __restrict__ om, long bs, const T* __restrict__ rr,
-                                                    long rbs, const int* __restrict__ box,
-                                                    const double* __restrict__ K, const int* __restrict__ loc, int R) {
+__global__ __launch_bounds__(64 * kAinvWaves) void k_ainv_local(Geo g, T* __restrict__ om, long bs,
+                                                                const T* __restrict__ rr, long rbs,
+                                                                const int* __restrict__ box,
+                                                                const double* __restrict__ K,
+                                                                const int* __restrict__ loc, int R) {
+  constexpr int NT = 64 * kAinvWaves;
   __shared__ double rs[kAinvW * kAinvW];
   __shared__ double ks[(kAinvRmax + 1) * (kAinvRmax + 1)];
-  __shared__ double red[4][64];
+  __shared__ double red[kAinvWaves][64];
   const int b = blockIdx.y;
   if (!loc[b]) return;
   const int bi0 = box[4 * b], bj0 = box[4 * b + 1], nx = box[4 * b + 2], ny = box[4 * b + 3];
@@ -1653,43 +1657,59 @@ __global__ __launch_bounds__(256) void k_ainv_local(Geo g, T* __restrict__ om, l
   const T old = mine ? *o : T(0);
   const T* rb = rr + b * rbs;
   // every staging load of the thread is issued before the first store
-  constexpr int U = (kAinvW * kAinvW + 255) / 256, UK = ((kAinvRmax + 1) * (kAinvRmax + 1) + 255) / 256;
+  constexpr int U = (kAinvW * kAinvW + NT - 1) / NT, UK = ((kAinvRmax + 1) * (kAinvRmax + 1) + NT - 1) / NT;
   double v[U], kv[UK];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const int e = threadIdx.x + 256 * u, y = e / W, x = e - y * W, ii = x0 - R + x, jj = y0 - R + y;
+    const int e = threadIdx.x + NT * u, y = e / W, x = e - y * W, ii = x0 - R + x, jj = y0 - R + y;
     v[u] = (e < W * W && ii >= 0 && ii < g.mm1 && jj >= 0 && jj < g.nm1) ? (double)rb[ii + (long)jj * g.mm1] : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < UK; u++) {
-    const int e = threadIdx.x + 256 * u;
+    const int e = threadIdx.x + NT * u;
     kv[u] = e < R1 * R1 ? K[(long)b * R1 * R1 + e] : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const int e = threadIdx.x + 256 * u;
+    const int e = threadIdx.x + NT * u;
     if (e < W * W) rs[e] = v[u];
   }
 #pragma unroll
   for (int u = 0; u < UK; u++) {
-    const int e = threadIdx.x + 256 * u;
+    const int e = threadIdx.x + NT * u;
     if (e < R1 * R1) ks[e] = kv[u];
   }
   __syncthreads();
-  // wave w takes the rows dy = w mod 4, so a case's sums do not depend on
-  // the launch's R (rows and places past the case's own reach add exact
-  // zeros): a case in a batch gives its single-case result bit for bit
+  // wave w takes the rows dy = w mod 16 (two at most while R <= 15), so a
+  // case's sums do not depend on the launch's R (rows and places past the
+  // case's own reach add exact zeros): a case in a batch gives its
+  // single-case result bit for bit.  Even and odd dx go to two chains.
   double acc = 0.0;
-  for (int dy = -R + ((w + R) & 3); dy <= R; dy += 4) {
+  for (int dy = -R + ((w + R) & (kAinvWaves - 1)); dy <= R; dy += kAinvWaves) {
     const double* kr = ks + (dy < 0 ? -dy : dy) * R1;
     const double* rrow = rs + (ly + R - dy) * W + lx + R;
-    double a = kr[0] * rrow[0];
-    for (int dx = 1; dx <= R; dx++) a = fma(kr[dx], rrow[-dx] + rrow[dx], a);
-    acc += a;
+    double a0 = kr[0] * rrow[0], a1 = 0.0;
+    int dx = 1;
+#pragma unroll 4
+    for (; dx + 1 <= R; dx += 2) {
+      a1 = fma(kr[dx], rrow[-dx] + rrow[dx], a1);
+      a0 = fma(kr[dx + 1], rrow[-dx - 1] + rrow[dx + 1], a0);
+    }
+    if (dx <= R) a1 = fma(kr[dx], rrow[-dx] + rrow[dx], a1);
+    acc += a0 + a1;
   }
   red[w][lane] = acc;
   __syncthreads();
-  if (mine) *o = old - T((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+  if (mine) {
+    double p[kAinvWaves];
+#pragma unroll
+    for (int x = 0; x < kAinvWaves; x++) p[x] = red[x][lane];
+#pragma unroll
+    for (int h = kAinvWaves / 2; h > 0; h /= 2)
+#pragma unroll
+      for (int x = 0; x < h; x++) p[x] = p[x] + p[x + h];
+    *o = old - T(p[0]);
+  }
 }
 
 // The same two levels per launch in LDS tiles.  Workgroup (tx, ty) takes a
