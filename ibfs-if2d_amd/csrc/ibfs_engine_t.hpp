@@ -1264,8 +1264,8 @@ struct Engine : GroupBase {
     // bottom and top bands first, then the left and right ones (the bands
     // cross in the corners, so the two adds there are kept in order)
     if (band_direct) {
-      const int nbt = (m - 1 + 63) / 64, nlr = (n - 1 + 15) / 16;
-      hipLaunchKernelGGL(k_band_direct<T>, dim3(2 * nbt + 2 * nlr, BL), dim3(256), 0, st, geoL, omega.p, Lmax,
+      const int nbt = (m - 1 + 63) / 64, nlr = (n - 1 + 63) / 64;
+      hipLaunchKernelGGL(k_band_direct<T>, dim3(2 * nbt + 2 * nlr, BL), dim3(64 * kBandDL), 0, st, geoL, omega.p, Lmax,
                          band_w.p, band_h.p, band_wmax, band_R, nbt, nlr);
       return;
     }

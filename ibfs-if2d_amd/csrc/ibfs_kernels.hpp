@@ -949,9 +949,10 @@ struct BandOp {
 // rho_k is analytic in cos(pi k / N), so H_d decays geometrically in |x|;
 // the host tabulates H_d(x) for |x| <= R, beyond which the dropped terms sum
 // below 1e-17 (the tolerance of the band depth), and v is a (2R+1)-term sum.
-// One launch for all four bands: a workgroup takes 64 positions of a bottom
-// or top band (all depths) or 16 positions of a left or right band, stages
-// ^e over its positions and their reach, and adds v on its band vertices.
+// One launch for all four bands: a workgroup takes 64 positions of a band
+// and all its depths (16 depth lanes), stages ^e over its positions and
+// their reach, and adds v on its band vertices (each sum in two chains, even
+// and odd offsets: the sums are latency-bound).
 // A corner vertex lies in two bands; the bottom or top workgroup adds both
 // terms there, its own first (the order of the spectral form's two
 // launches), and the left and right workgroups skip it.  The edge values
@@ -959,11 +960,12 @@ struct BandOp {
 // its edges and so never meet the get_bc lines (min(m,n)/4 cells in).
 // hk[((b 2 + axis) wmax + d) (R+1) + x], axis 0 for the bottom and top edges
 // (N = m), 1 for the left and right ones (N = n); rows d >= w are zero.
-// grid (2 ceil((m-1)/64) + 2 ceil((n-1)/16), items), 256 threads.
+// grid (2 ceil((m-1)/64) + 2 ceil((n-1)/64), items), 64 kBandDL threads.
 constexpr int kBandRmax = 32;    // the widest kernel the direct form takes
 constexpr int kBandWmax = 31;    // the deepest band it takes
 constexpr int kBandHmax = 1024;  // w (R+1) of one axis, staged in LDS
-constexpr int kBandPer = (kBandWmax + 3) / 4;  // depths per thread of a bottom or top workgroup
+constexpr int kBandDL = 16;                                  // depth lanes of a workgroup
+constexpr int kBandPer = (kBandWmax + kBandDL - 1) / kBandDL;  // depths per thread
 template <typename T>
 __device__ __forceinline__ double band_ehat(const Geo& g, const T* __restrict__ omc, int base, int N, int t) {
   const int P = 2 * N;
@@ -976,7 +978,8 @@ __device__ __forceinline__ double band_ehat(const Geo& g, const T* __restrict__ 
   return neg ? -v : v;
 }
 template <typename T>
-__global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, int lm, const int* __restrict__ wlev,
+__global__ __launch_bounds__(64 * kBandDL) void k_band_direct(Geo g, T* __restrict__ om, int lm,
+                                                              const int* __restrict__ wlev,
                                                      const double* __restrict__ hk, int wmax, int R, int nbt,
                                                      int nlr) {
   __shared__ double es[64 + 2 * kBandRmax];
@@ -998,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, 
     chunk = x - (e - 2) * nlr;
   }
   const bool horiz = e < 2;
-  const int N = horiz ? g.m : g.n, PC = horiz ? 64 : 16, p0 = chunk * PC;
+  const int N = horiz ? g.m : g.n, PC = 64, p0 = chunk * PC;
   const int base = e == 0 ? g.bottom : e == 1 ? g.top : e == 2 ? g.left : g.right;
   const T* omc = om + (long)(b + 1) * g.nw;
   const double* H = hk + (long)(b * 2 + (horiz ? 0 : 1)) * wmax * (R + 1);
@@ -1008,8 +1011,10 @@ __global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, 
   const int nc = w + 2 * R;
   // this thread's band vertices (depths dl, dl + DL, ...) and their old
   // values, loaded with the staging loads below rather than after the sums
-  const int lp = horiz ? threadIdx.x & 63 : threadIdx.x >> 4;
-  const int dl = horiz ? threadIdx.x >> 6 : threadIdx.x & 15, DL = horiz ? 4 : 16;
+  // bottom/top: consecutive lanes take consecutive positions (one band row);
+  // left/right: consecutive lanes take consecutive depths (one grid row)
+  const int lp = horiz ? threadIdx.x & 63 : threadIdx.x / kBandDL;
+  const int dl = horiz ? threadIdx.x >> 6 : threadIdx.x % kBandDL, DL = kBandDL;
   const int pos = p0 + lp;
   // a left or right workgroup leaves the corners to the bottom or top one
   const bool act = pos <= N - 2 && (horiz || (pos >= w && pos <= N - 2 - w));
@@ -1024,12 +1029,12 @@ __global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, 
       old[i] = om[(long)b * g.nw + ii + (long)jj * g.mm1];
     }
   }
-  for (int i = threadIdx.x; i < PC + 2 * R; i += 256) es[i] = band_ehat<T>(g, omc, base, N, p0 - R + i);
-  for (int i = threadIdx.x; i < w * (R + 1); i += 256) hs[i] = H[i];
+  for (int i = threadIdx.x; i < PC + 2 * R; i += 64 * kBandDL) es[i] = band_ehat<T>(g, omc, base, N, p0 - R + i);
+  for (int i = threadIdx.x; i < w * (R + 1); i += 64 * kBandDL) hs[i] = H[i];
   if (cl || cr) {
     const double* Hc = hk + (long)(b * 2 + 1) * wmax * (R + 1);
-    for (int i = threadIdx.x; i < w * (R + 1); i += 256) hc[i] = Hc[i];
-    for (int i = threadIdx.x; i < 2 * nc; i += 256) {
+    for (int i = threadIdx.x; i < w * (R + 1); i += 64 * kBandDL) hc[i] = Hc[i];
+    for (int i = threadIdx.x; i < 2 * nc; i += 64 * kBandDL) {
       const int side = i / nc, t = jlo - R + (i - side * nc);
       if (side == 0 ? cl : cr) cs[side][i - side * nc] = band_ehat<T>(g, omc, side == 0 ? g.left : g.right, g.n, t);
     }
@@ -1042,8 +1047,15 @@ __global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, 
     if (d >= w) break;
     const double* hd = hs + d * (R + 1);
     const double* ec = es + lp + R;
-    double v = hd[0] * ec[0];
-    for (int k = 1; k <= R; k++) v = fma(hd[k], ec[-k] + ec[k], v);
+    double v0 = hd[0] * ec[0], v1 = 0.0;  // even and odd k in two chains
+    int k = 1;
+#pragma unroll 4
+    for (; k + 1 <= R; k += 2) {
+      v1 = fma(hd[k], ec[-k] + ec[k], v1);
+      v0 = fma(hd[k + 1], ec[-k - 1] + ec[k + 1], v0);
+    }
+    if (k <= R) v1 = fma(hd[k], ec[-k] + ec[k], v1);
+    const double v = v0 + v1;
     const int ii = horiz ? pos : (e == 2 ? d : g.mm1 - 1 - d);
     const int jj = horiz ? (e == 0 ? d : g.nm1 - 1 - d) : pos;
     T out = old[i] + T(v);
@@ -1053,9 +1065,15 @@ __global__ __launch_bounds__(256) void k_band_direct(Geo g, T* __restrict__ om, 
       const bool left = ii < w;
       const double* hl = hc + (left ? ii : g.mm1 - 1 - ii) * (R + 1);
       const double* el = cs[left ? 0 : 1] + (jj - jlo) + R;
-      double u = hl[0] * el[0];
-      for (int k = 1; k <= R; k++) u = fma(hl[k], el[-k] + el[k], u);
-      out = out + T(u);
+      double u0 = hl[0] * el[0], u1 = 0.0;
+      int kc = 1;
+#pragma unroll 4
+      for (; kc + 1 <= R; kc += 2) {
+        u1 = fma(hl[kc], el[-kc] + el[kc], u1);
+        u0 = fma(hl[kc + 1], el[-kc - 1] + el[kc + 1], u0);
+      }
+      if (kc <= R) u1 = fma(hl[kc], el[-kc] + el[kc], u1);
+      out = out + T(u0 + u1);
     }
     om[(long)b * g.nw + ii + (long)jj * g.mm1] = out;
   }
