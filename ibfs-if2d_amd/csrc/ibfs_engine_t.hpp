@@ -137,6 +137,9 @@ struct Engine : GroupBase {
   // the two-level coarsify launches in LDS tiles; IBFS_COARSIFY_TILES=0
   // keeps one place per thread (the same values)
   bool coarsify_tiles = true;
+  // the ring vectors' bottom and top lines by one FFT per item
+  // (k_ring_fromY_fft); IBFS_RING_FFT=0 keeps the direct sums
+  bool ring_fft_lines = true;
   Geo geoL{};
   DevBuf<T> T1L, T2L;
   // vort2flux with the levels batched (see v2f_can_batch): ringN ring
@@ -302,6 +305,7 @@ struct Engine : GroupBase {
     lev_batch = levels_can_batch();
     if (const char* sp = std::getenv("IBFS_STEP1_SPLIT")) src_split = sp[0] != '0';
     if (const char* ct = std::getenv("IBFS_COARSIFY_TILES")) coarsify_tiles = ct[0] != '0';
+    if (const char* rf = std::getenv("IBFS_RING_FFT")) ring_fft_lines = rf[0] != '0';
     if (lev_batch) {
       geoL = geo;
       geoL.Lmax = 1;
@@ -1123,6 +1127,36 @@ struct Engine : GroupBase {
   // edge terms and a column pass that adds the left and right ones: on every
   // level (all), or on level 0 only with the level-1 ring values in ringS for
   // get_bc (step 2, which needs s_1 nowhere else).
+  // the ring vectors of s0 on the levels below the finest, from the column
+  // pair's output T2L: the bottom and top lines by one FFT per item where
+  // the row length has a size-specialised pass of 256 threads
+  // (k_ring_fromY_fft), by direct sums otherwise (k_ring_fromY)
+  void ring_fromY() {
+    const int NR = ringN, n2 = fset<double>().pm.n2;
+    const dim3 blk(256);
+    const size_t lds = sizeof(double) * 2 * m;
+    const int nlrg = (n / 2 + 1 + 3) / 4;
+    const cplx<double>* tw = fset<double>().stw_m.p;
+    if (fset<double>().spec_m && ring_fft_lines) {
+      switch (n2) {
+        case 256:
+          hipLaunchKernelGGL(k_ring_fromY_fft<256>, dim3(nlrg + 1, B * (Lmax - 1)), blk, lds, st, geo, T2L.p, Lmax,
+                             NR, ringSin.p, ringV.p, tw, nlrg);
+          return;
+        case 512:
+          hipLaunchKernelGGL(k_ring_fromY_fft<512>, dim3(nlrg + 1, B * (Lmax - 1)), blk, lds, st, geo, T2L.p, Lmax,
+                             NR, ringSin.p, ringV.p, tw, nlrg);
+          return;
+        case 1024:
+          hipLaunchKernelGGL(k_ring_fromY_fft<1024>, dim3(nlrg + 1, B * (Lmax - 1)), blk, lds, st, geo, T2L.p, Lmax,
+                             NR, ringSin.p, ringV.p, tw, nlrg);
+          return;
+        default: break;
+      }
+    }
+    hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, B * (Lmax - 1)), blk, lds, st, geo, T2L.p,
+                       Lmax, NR, ringSin.p, ringV.p);
+  }
   void vort2flux_levels(bool all) {
     if constexpr (std::is_same_v<T, double>) {
       const int BL = B * Lmax, NR = ringN, Kx = std::max(m, n) - 1;
@@ -1134,9 +1168,7 @@ struct Engine : GroupBase {
                                                                    MidScale<T>{laminv.p}, DstField<T>{T2L.p, nw},
                                                                    nullptr, 0},
                   geo.mm1, 2, BL);
-      hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, B * (Lmax - 1)), dim3(256),
-                         sizeof(double) * 2 * m, st,
-                         geo, T2L.p, Lmax, NR, ringSin.p, ringV.p);
+      ring_fromY();
       double* sring = all ? nullptr : ringS.p;
       switch (Lmax - 2) {
 #define IBFS_RING_CHAIN(NP)                                                                                   \

@@ -1233,8 +1233,9 @@ __device__ __forceinline__ void stage_lds(double* dst, int n, F&& load) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restrict__ Y, int lm, int NR,
-                                                    const double* __restrict__ sintab, double* __restrict__ V) {
+__device__ __forceinline__ void ring_fromY_waves(const Geo& g, const double* __restrict__ Y, int lm, int NR,
+                                                 const double* __restrict__ sintab, double* __restrict__ V,
+                                                 bool lr_only) {
   extern __shared__ double sn[];  // sin(pi t / m), t < 2m
   const int b = blockIdx.y / (lm - 1), k = 1 + blockIdx.y % (lm - 1);
   const double* y = Y + ((long)b * lm + k) * g.nw;
@@ -1242,7 +1243,7 @@ __global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restr
   const int nb = m / 2 + 1, nl = n / 2 + 1, nrt = (nb + kRingRowOut - 1) / kRingRowOut;
   const int task = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   double* out = V + ((long)b * lm + k) * NR;
-  const bool live = task < nl + 2 * nrt;  // whole waves
+  const bool live = task < (lr_only ? nl : nl + 2 * nrt);  // whole waves
   const bool lr = task < nl;
   const int t2 = task - nl, line = lr ? 0 : t2 / nrt, c0 = lr ? 0 : (t2 - line * nrt) * kRingRowOut;
   const int jj = !live ? 0 : lr ? n / 4 - 1 + task : (line == 0 ? n / 4 - 1 : 3 * n / 4 - 1);
@@ -1320,6 +1321,57 @@ __global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restr
       if (lane == 0 && c0 + u < nb) out[line * nb + c0 + u] = 2.0 * t;
     }
   }
+}
+__global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restrict__ Y, int lm, int NR,
+                                                    const double* __restrict__ sintab, double* __restrict__ V) {
+  ring_fromY_waves(g, Y, lm, NR, sintab, V, false);
+}
+
+// The bottom and top lines of the ring vectors as one DST-I of the two rows
+// n/4-1 and 3n/4-1 of Y_k (row 0 and row 1 of a two-row pass): every place
+// of the two rows in one FFT, in place of 2 ceil((m/2+1)/8) waves of direct
+// sums whose sine lookups conflict in LDS banks.  The item is the batch
+// index as in k_ring_fromY; the outputs at ii = m/4-1 .. 3m/4-1 go to V at
+// row nb + ii - (m/4-1), nb = m/2+1.
+struct SrcRingLines {
+  static constexpr bool kNoWave = true;
+  const double* Y; int lm; Geo gf;
+  __device__ double operator()(const Geo&, int b, int ii, int row) const {
+    const int c = b / (lm - 1), k = 1 + b % (lm - 1);
+    const int jj = row == 0 ? gf.n / 4 - 1 : 3 * gf.n / 4 - 1;
+    return Y[((long)c * lm + k) * gf.nw + ii + (long)jj * gf.mm1];
+  }
+};
+struct DstRingLines {
+  double* V; int lm, NR; Geo gf;
+  __device__ void operator()(const Geo&, int b, int ii, int row, double v) const {
+    const int nb = gf.m / 2 + 1, pos = ii - (gf.m / 4 - 1);
+    if (pos < 0 || pos >= nb) return;
+    const int c = b / (lm - 1), k = 1 + b % (lm - 1);
+    V[((long)c * lm + k) * NR + row * nb + pos] = v;
+  }
+};
+// k_ring_fromY with the bottom and top lines from one FFT per item: the
+// workgroups x < nlrg take the left and right lines (one wave per ring row,
+// as k_ring_fromY), workgroup nlrg the two-row DST-I (P = 1024 / N2 FFTs of
+// the size-specialised pass, the second and later ones of zeros).
+// grid (nlrg + 1, B (lm-1)), 256 threads, 2m doubles of dynamic LDS.
+template <int N2>
+__global__ __launch_bounds__(256) void k_ring_fromY_fft(Geo g, const double* __restrict__ Y, int lm, int NR,
+                                                        const double* __restrict__ sintab, double* __restrict__ V,
+                                                        const cplx<double>* __restrict__ tw, int nlrg) {
+  constexpr int P = 1024 / N2;
+  static_assert(P * N2 / 4 == 256, "k_ring_fromY_fft: one pass per workgroup of 256 threads");
+  if ((int)blockIdx.x >= nlrg) {
+    __shared__ cplx<double> z[(spec::kTwoBuf<double, N2, P> ? 2 : 1) * P * N2];
+    Geo g2 = g;
+    g2.nm1 = 2;  // the pass's two rows
+    const RowOp<double, SrcRingLines, DstRingLines> op{g2, SrcRingLines{Y, lm, g}, DstRingLines{V, lm, NR, g},
+                                                       nullptr, 0};
+    spec::dst_pass_body<double, N2, P, 0, RowOp<double, SrcRingLines, DstRingLines>>(op, tw, z, 0, 1);
+    return;
+  }
+  ring_fromY_waves(g, Y, lm, NR, sintab, V, true);
 }
 
 // R_k = V_k + sum_{j=1}^{L-1-k} T^j V_{k+j} for k = 1..L-1 of case b
