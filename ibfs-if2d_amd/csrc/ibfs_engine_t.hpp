@@ -137,8 +137,9 @@ struct Engine : GroupBase {
   // the two-level coarsify launches in LDS tiles; IBFS_COARSIFY_TILES=0
   // keeps one place per thread (the same values)
   bool coarsify_tiles = true;
-  // the ring vectors' bottom and top lines by one FFT per item
-  // (k_ring_fromY_fft); IBFS_RING_FFT=0 keeps the direct sums
+  // the ring vectors' bottom and top lines and the edge spectra by FFT
+  // (k_ring_fromY_fft, k_ring_spectra_fft); IBFS_RING_FFT=0 keeps the
+  // direct sums
   bool ring_fft_lines = true;
   Geo geoL{};
   DevBuf<T> T1L, T2L;
@@ -1157,6 +1158,34 @@ struct Engine : GroupBase {
     hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, B * (Lmax - 1)), blk, lds, st, geo, T2L.p,
                        Lmax, NR, ringSin.p, ringV.p);
   }
+  // the edge spectra of Interp(R_{k+1}) for the items of im: by FFT on
+  // square grids whose row length has a size-specialised pass of 256
+  // threads (k_ring_spectra_fft), by direct sums otherwise (k_ring_spectra)
+  void ring_spectra(ItemMap im, int nitems, int Kx) {
+    const int NR = ringN, n2 = fset<double>().pm.n2;
+    const cplx<double>* tw = fset<double>().stw_m.p;
+    if (m == n && fset<double>().spec_m && ring_fft_lines) {
+      const int P = n2 > 0 ? 1024 / n2 : 0, groups = P > 0 ? (4 + 2 * P - 1) / (2 * P) : 0;
+      switch (n2) {
+        case 256:
+          hipLaunchKernelGGL(k_ring_spectra_fft<256>, dim3(groups, nitems), dim3(256), 0, st, geo, ringR.p, NR, im,
+                             Kx, ringX.p, tw);
+          return;
+        case 512:
+          hipLaunchKernelGGL(k_ring_spectra_fft<512>, dim3(groups, nitems), dim3(256), 0, st, geo, ringR.p, NR, im,
+                             Kx, ringX.p, tw);
+          return;
+        case 1024:
+          hipLaunchKernelGGL(k_ring_spectra_fft<1024>, dim3(groups, nitems), dim3(256), 0, st, geo, ringR.p, NR, im,
+                             Kx, ringX.p, tw);
+          return;
+        default: break;
+      }
+    }
+    hipLaunchKernelGGL(k_ring_spectra, dim3(4 * ((Kx + 31) / 32), nitems), dim3(256),
+                       sizeof(double) * (Kx + 2 * (Kx + 1)), st, geo, ringR.p, NR, im, Kx, ringSin.p,
+                       ringSin.p + 2 * m, ringX.p);
+  }
   void vort2flux_levels(bool all) {
     if constexpr (std::is_same_v<T, double>) {
       const int BL = B * Lmax, NR = ringN, Kx = std::max(m, n) - 1;
@@ -1188,9 +1217,7 @@ struct Engine : GroupBase {
       }
       const ItemMap below{all ? Lmax - 1 : 1, Lmax}, out{all ? Lmax : 1, Lmax};
       const int nbelow = all ? B * (Lmax - 1) : B, nout = all ? BL : B;
-      hipLaunchKernelGGL(k_ring_spectra, dim3(4 * ((Kx + 31) / 32), nbelow), dim3(256),
-                         sizeof(double) * (Kx + 2 * (Kx + 1)), st, geo, ringR.p, NR, below, Kx, ringSin.p,
-                         ringSin.p + 2 * m, ringX.p);
+      ring_spectra(below, nbelow, Kx);
       const double* Gy = m == n ? ringG.p : ringGp.p;
       if (!all && step2_window()) {
         // step 2 reads s_1 only on the vertices of the force stencils: the
@@ -1431,12 +1458,14 @@ struct Engine : GroupBase {
   }
 
   // STEPS 3 and 4, fluxes and diagnostics
-  void enqueue_step34() {
+  // (local3: step 3 may take the real-space form; the CG form keeps the
+  // spectral one, whose iterate near the stopping test follows rounding)
+  void enqueue_step34(bool local3 = true) {
     const long FB = fbs(), QB = qbs();
     // STEP 3 (src/operators.f90:174): in real space for the cases whose
     // active vertices are far enough from the edges (k_ainv_local), as the
     // spectral ctci for the others
-    const int nloc = base == nullptr ? ainv_nloc : 0;
+    const int nloc = base == nullptr && local3 ? ainv_nloc : 0;
     if (nloc < B)
       ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB, nloc > 0 ? ainv_loc.p : nullptr}, T2.p, T3.p, B,
                 nullptr, 0, lam1i.p, tbs());
@@ -1917,7 +1946,7 @@ struct Engine : GroupBase {
     // the step-3 source rot(reg(fb)) on the active vertices
     hipLaunchKernelGGL((k_rotreg<T, double>), dim3((nvtx_max + 255) / 256, B), dim3(256), 0, st, sm, fb.p, 0L, RR.p,
                        (long)nw, -1);
-    enqueue_step34();
+    enqueue_step34(false);
     return it;
   }
 

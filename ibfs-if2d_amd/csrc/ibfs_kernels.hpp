@@ -1506,6 +1506,40 @@ __global__ __launch_bounds__(256) void k_ring_spectra(Geo g, const double* __res
   }
 }
 
+// The edge spectra of k_ring_spectra by FFT: the four edge vectors
+// x_e = Interp(R_{k+1}) of an item are the four "rows" of a pass (bottom and
+// top in one complex FFT, left and right in the next), each transformed in
+// one DST-I, for square grids (one FFT length for all four edges).  The
+// same values as the direct sums but for rounding.  Item im(b) of batch
+// entry b; grid (ceil(4 / (2P)), items), 256 threads, P = 1024 / N2.
+struct SrcRingEdges {
+  static constexpr bool kNoWave = true;
+  const double* Rb; int NR; ItemMap im; Geo gf;
+  __device__ double operator()(const Geo&, int b, int t, int e) const {
+    return ring_interp(Rb + (long)(im(b) + 1) * NR, gf.m, gf.n, e, t);
+  }
+};
+struct DstRingEdges {
+  double* X; ItemMap im; int Kx;
+  __device__ void operator()(const Geo&, int b, int kk1, int e, double v) const {
+    X[((long)im(b) * 4 + e) * Kx + kk1] = v;
+  }
+};
+template <int N2>
+__global__ __launch_bounds__(256) void k_ring_spectra_fft(Geo g, const double* __restrict__ Rb, int NR, ItemMap im,
+                                                          int Kx, double* __restrict__ X,
+                                                          const cplx<double>* __restrict__ tw) {
+  constexpr int P = 1024 / N2;
+  static_assert(P * N2 / 4 == 256, "k_ring_spectra_fft: one pass per workgroup of 256 threads");
+  __shared__ cplx<double> z[(spec::kTwoBuf<double, N2, P> ? 2 : 1) * P * N2];
+  Geo g4 = g;
+  g4.nm1 = 4;  // the four edges
+  const RowOp<double, SrcRingEdges, DstRingEdges> op{g4, SrcRingEdges{Rb, NR, im, g}, DstRingEdges{X, im, Kx},
+                                                     nullptr, 0};
+  spec::dst_pass_body<double, N2, P, 0, RowOp<double, SrcRingEdges, DstRingEdges>>(op, tw, z, blockIdx.x,
+                                                                                     gridDim.x);
+}
+
 // The last transform of the batched ctci with the bottom and top edge terms
 // (a RowOp loader, batch index b -> item im(b)): the spectral row jj of s_k
 // is Y(kk-1, jj) + (X_b(kk) G(kk, jj) + X_t(kk) G(kk, n-2-jj)), G = g / (2m)
