@@ -637,12 +637,17 @@ __global__ __launch_bounds__(P * N2 / 4) void dst_pass_n(Op op, const cplx<T>* _
 // passes then cost one launch and one load-transform-store chain.
 template <typename T, int N2, int P, class OpA, class OpB>
 __global__ __launch_bounds__(P * N2 / 4) void dst_pass_dual(OpA a, OpB bq, const cplx<T>* __restrict__ twa,
-                                                            const cplx<T>* __restrict__ twb, int ga) {
+                                                            const cplx<T>* __restrict__ twb, int ga, int nya,
+                                                            int nyb) {
   constexpr bool DB = kTwoBuf<T, N2, P>;
   __shared__ cplx<T> z[(DB ? 2 : 1) * P * N2];
   const int x = blockIdx.x;
-  if (x < ga) dst_pass_body<T, N2, P, 0, OpA>(a, twa, z, x, ga);
-  else dst_pass_body<T, N2, P, 1, OpB>(bq, twb, z, x - ga, (int)gridDim.x - ga);
+  // batch entries blockIdx.y < nya of pass A, < nyb of pass B
+  if (x < ga) {
+    if ((int)blockIdx.y < nya) dst_pass_body<T, N2, P, 0, OpA>(a, twa, z, x, ga);
+  } else if ((int)blockIdx.y < nyb) {
+    dst_pass_body<T, N2, P, 1, OpB>(bq, twb, z, x - ga, (int)gridDim.x - ga);
+  }
 }
 
 // fused column pair (see dst_col2) at a compile-time size

@@ -141,6 +141,9 @@ struct Engine : GroupBase {
   // (k_ring_fromY_fft, k_ring_spectra_fft); IBFS_RING_FFT=0 keeps the
   // direct sums
   bool ring_fft_lines = true;
+  // step 4's batched vort2flux keeps the left/right terms apart from S (in
+  // ringS) for the tail to add; IBFS_SPLIT_S=0 adds them into S
+  bool s_split = true;
   Geo geoL{};
   DevBuf<T> T1L, T2L;
   // vort2flux with the levels batched (see v2f_can_batch): ringN ring
@@ -307,6 +310,7 @@ struct Engine : GroupBase {
     if (const char* sp = std::getenv("IBFS_STEP1_SPLIT")) src_split = sp[0] != '0';
     if (const char* ct = std::getenv("IBFS_COARSIFY_TILES")) coarsify_tiles = ct[0] != '0';
     if (const char* rf = std::getenv("IBFS_RING_FFT")) ring_fft_lines = rf[0] != '0';
+    if (const char* sp = std::getenv("IBFS_SPLIT_S")) s_split = sp[0] != '0';
     if (lev_batch) {
       geoL = geo;
       geoL.Lmax = 1;
@@ -642,25 +646,31 @@ struct Engine : GroupBase {
   // Launch one pass over nseq sequences along the other axis; mult = 2 when
   // a workgroup's P FFTs take 2P sequences of one field, 1 when they pair
   // two fields (X + iY).  kind selects the tile shape.
-  // A row pass and a column pass of the same FFT length in one launch
-  // (dst_pass_dual), two sequences per FFT, when both kinds run the same
-  // size-specialised LDS pass at this batch; false (nothing launched)
-  // otherwise.
+  // A row pass (batch nya) and a column pass (batch nyb) of the same FFT
+  // length in one launch (dst_pass_dual), two sequences per FFT, when the
+  // row pass runs a size-specialised LDS pass with P = 1, 2 or 4 FFTs per
+  // workgroup at its batch (the column pass takes the same P: the results do
+  // not depend on it); false (nothing launched) otherwise.
   template <class OpA, class OpB>
-  bool launch_dual(const OpA& a, int nseqa, const OpB& bq, int nseqb, int batch) {
+  bool launch_dual(const OpA& a, int nseqa, const OpB& bq, int nseqb, int nya, int nyb) {
     using V = op_value_t<OpA>;
     static_assert(std::is_same_v<V, op_value_t<OpB>>, "launch_dual: one value type");
     const FftSet<V>& F = fset<V>();
     if (!F.spec_m || !F.spec_n || F.pm.n2 != F.pn.n2) return false;
-    const int P = tile<V>(0, batch);
-    if (P != tile<V>(kCol, batch) || (P != 1 && P != 2 && P != 4)) return false;
+    int P = tile<V>(0, nya);
+    if (P == kWaveTile || P == kStagedTile) {
+      if (wave::kOpOk<OpA> && wave_size(F.pm.n2)) return false;  // the row pass is register-blocked
+      P = 1;
+    }
+    if (P != 1 && P != 2 && P != 4) return false;
     const int ga = (nseqa + 2 * P - 1) / (2 * P), gb = (nseqb + 2 * P - 1) / (2 * P);
+    const int batch = std::max(nya, nyb);
     switch (F.pm.n2 * 8 + P) {
 #define IBFS_DUAL_CASE(NN, PP)                                                                                 \
   case NN * 8 + PP:                                                                                            \
     if constexpr (p_fits<V>(NN, PP)) {                                                                         \
       hipLaunchKernelGGL((spec::dst_pass_dual<V, NN, PP, OpA, OpB>), dim3(ga + gb, batch), dim3(PP * NN / 4), 0, \
-                         st, a, bq, F.stw_m.p, F.stw_n.p, ga);                                                 \
+                         st, a, bq, F.stw_m.p, F.stw_n.p, ga, nya, nyb);                                       \
       return true;                                                                                             \
     }                                                                                                          \
     return false;
@@ -1237,9 +1247,24 @@ struct Engine : GroupBase {
         const ColOp<T, SrcColWin<SrcRingCol>, DstColWin<DstRingItem>> cop{
             gc, {SrcRingCol{ringX.p, Gy, below, Kx}, geoL, swin[2]},
             {DstRingItem{ringS.p, below, 0}, geoL, swin[2], swin[0], swin[1]}, nullptr, 0};
-        if (!launch_dual(rop, gr.nm1, cop, gc.mm1, nout)) {
+        if (!launch_dual(rop, gr.nm1, cop, gc.mm1, nout, nbelow)) {
           launch<0>(rop, gr.nm1, 2, nout);
           launch<1>(cop, gc.mm1, 2, nbelow, kCol);
+        }
+        return;
+      }
+      if (all && s_split) {
+        // step 4: the last row pass writes S and the left/right column pass
+        // writes its terms into ringS (free after step 2), which the tail
+        // adds (the same sum s = S + C as the column pass adding into S);
+        // the two passes are then independent: one launch where it can be
+        const RowOp<T, SrcRingRow, DstRingItem> rop{geoL, SrcRingRow{T2L.p, ringX.p, ringG.p, out, Kx},
+                                                    DstRingItem{S.p, out, 0}, nullptr, 0};
+        const ColOp<T, SrcRingCol, DstRingItem> cop{geoL, SrcRingCol{ringX.p, Gy, below, Kx},
+                                                    DstRingItem{ringS.p, below, 0}, nullptr, 0};
+        if (!launch_dual(rop, geo.nm1, cop, geo.mm1, nout, nbelow)) {
+          launch<0>(rop, geo.nm1, 2, nout);
+          launch<1>(cop, geo.mm1, 2, nbelow, kCol);
         }
         return;
       }
@@ -1480,9 +1505,12 @@ struct Engine : GroupBase {
     // fluxes and diagnostics (src/operators.f90:184-186)
     const int ncurl = tail_groups(geo) * Lmax;
     const bool defer = diag_deferred();
+    const T* tailC = nullptr;  // the left/right terms kept apart by the batched vort2flux
+    if constexpr (std::is_same_v<T, double>)
+      if (v2f_batch && s_split) tailC = ringS.p;
     hipLaunchKernelGGL(k_tail<T>, dim3(ncurl + nchunk * Lmax, B), dim3(256), 0, st, geo, S.p, q.p, omega.p, FB, QB,
                        partial.p, nchunk, gate, kSolve64 ? (const double*)S64.p : nullptr,
-                       defer ? dpend.p : (int*)nullptr, counter.p);
+                       defer ? dpend.p : (int*)nullptr, counter.p, tailC);
     if (!defer)
       hipLaunchKernelGGL(k_diag<T>, dim3(B), dim3(1024), 0, st, geo, diag_args(nullptr));
   }

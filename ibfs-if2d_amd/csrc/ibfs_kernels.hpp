@@ -1987,6 +1987,58 @@ __device__ __forceinline__ T curl_face(const Geo& g, int f, const T* __restrict_
   }
 }
 
+// curl_face of the streamfunction s + c (c may be null) with the coarser
+// level's sc + cc (cc may be null): every place's two parts are added before
+// get_bc's averages, so the values are curl_face's of the summed fields.
+template <typename T, typename TC = T>
+__device__ __forceinline__ T curl_face_sum(const Geo& g, int f, const T* __restrict__ s, const T* __restrict__ c,
+                                           const TC* __restrict__ sc, const TC* __restrict__ cc) {
+  const int m = g.m, n = g.n;
+  auto X = [&](int i, int j) -> T {
+    const int e = wix(g, i, j);
+    return c ? T(s[e] + c[e]) : s[e];
+  };
+  auto R1 = [&](int a, int b) -> TC {
+    const int e = (a - 1) + (b - 1) * g.mm1;
+    return cc ? TC(sc[e] + cc[e]) : sc[e];
+  };
+  auto B = [&](int pos) -> T {
+    if (!sc) return T(0);
+    TC v;
+    if (pos > g.bottom) {  // getbc_at's places and averages
+      const bool top = pos > g.top;
+      const int i0 = pos - (top ? g.top : g.bottom) - 1;
+      const int jb = top ? 3 * n / 4 : n / 4;
+      if ((i0 & 1) == 0) v = R1(m / 4 + i0 / 2, jb);
+      else v = TC(0.5) * (R1(m / 4 + (i0 + 1) / 2, jb) + R1(m / 4 - 1 + (i0 + 1) / 2, jb));
+    } else {
+      const bool right = pos > g.right;
+      const int j0 = pos - (right ? g.right : g.left) - 1;
+      const int ib = right ? 3 * m / 4 : m / 4;
+      if ((j0 & 1) == 0) v = R1(ib, n / 4 + j0 / 2);
+      else v = TC(0.5) * (R1(ib, n / 4 + (j0 + 1) / 2) + R1(ib, n / 4 - 1 + (j0 + 1) / 2));
+    }
+    return T(v * TC(1));
+  };
+  const int nu = (m + 1) * n;
+  if (f < nu) {
+    const int i = f % (m + 1) + 1, j = f / (m + 1) + 1;
+    if (i == 1) return B(g.left + j + 1) - B(g.left + j);
+    if (i == m + 1) return B(g.right + j + 1) - B(g.right + j);
+    if (j == 1) return X(i, 2) - B(g.bottom + i);
+    if (j == n) return B(g.top + i) - X(i, n);
+    return X(i, j + 1) - X(i, j);
+  } else {
+    const int fv = f - nu;
+    const int i = fv % m + 1, j = fv / m + 1;
+    if (j == 1) return B(g.bottom + i) - B(g.bottom + i + 1);
+    if (j == n + 1) return B(g.top + i) - B(g.top + i + 1);
+    if (i == 1) return -X(2, j) + B(g.left + j);
+    if (i == m) return -B(g.right + j) + X(m, j);
+    return X(i, j) - X(i + 1, j);
+  }
+}
+
 // vel = curl(s_lev, get_bc(s_{lev+1})) over all faces (zero edges on a case's
 // coarsest level).  grid.x over nq; s/sc stride bs, vel stride vbs
 template <typename T>
@@ -2744,7 +2796,8 @@ __device__ __forceinline__ bool curl_interior(const Geo& g, int f, int& ia, int&
 //    (x-ncurl) / nchunk, at partial[((b*Lmax)+lev)*nchunk + chunk].
 // With pend set, pend[b] = 1 and counter[b] += 1: the step's diagnostics
 // row (slot counter[b] - 1) is left to the next launch that writes pending
-// rows (diag_row).
+// rows (diag_row).  With C set (same layout as s), the streamfunction of
+// every level but a case's coarsest is s + C.
 // grid (ncurl + nchunk*Lmax, B), 256 threads.
 constexpr int kTailPos = 2;
 __host__ __device__ __forceinline__ int tail_groups(const Geo& g) {
@@ -2752,7 +2805,8 @@ __host__ __device__ __forceinline__ int tail_groups(const Geo& g) {
 }
 template <typename T, typename TS, typename TC = T>
 __device__ __forceinline__ void tail_curl(const Geo& g, int p0, const TS* __restrict__ sl, const TC* __restrict__ sc,
-                                          T* __restrict__ ql) {
+                                          T* __restrict__ ql, const TS* __restrict__ cl = nullptr,
+                                          const TC* __restrict__ ccs = nullptr) {
   const int m = g.m, n = g.n, nu = (m + 1) * n, np = (m + 1) * (n + 1);
   int ic[kTailPos], iu[kTailPos], ir[kTailPos];
   bool hu[kTailPos], hv[kTailPos], inu[kTailPos], inv[kTailPos];
@@ -2775,13 +2829,33 @@ __device__ __forceinline__ void tail_curl(const Geo& g, int p0, const TS* __rest
     vu[u] = sl[iu[u]];
     vr[u] = sl[ir[u]];
   }
+  if (cl != nullptr) {  // the streamfunction is sl + cl: the parts added first
+    TS c0[kTailPos], c1[kTailPos], c2[kTailPos];
+#pragma unroll
+    for (int u = 0; u < kTailPos; u++) {
+      c0[u] = cl[ic[u]];
+      c1[u] = cl[iu[u]];
+      c2[u] = cl[ir[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kTailPos; u++) {
+      vc[u] = vc[u] + c0[u];
+      vu[u] = vu[u] + c1[u];
+      vr[u] = vr[u] + c2[u];
+    }
+  }
 #pragma unroll
   for (int u = 0; u < kTailPos; u++) {
     const int p = p0 + 256 * u, i = p % (m + 1) + 1, j = p / (m + 1) + 1;
-    if (hu[u]) ql[p] = inu[u] ? T(vu[u] - vc[u]) : T(curl_face<TS, TC>(g, p, sl, sc));
+    if (hu[u])
+      ql[p] = inu[u] ? T(vu[u] - vc[u])
+                     : T(cl != nullptr || ccs != nullptr ? curl_face_sum<TS, TC>(g, p, sl, cl, sc, ccs)
+                                                         : curl_face<TS, TC>(g, p, sl, sc));
     if (hv[u]) {
       const int fv = nu + (i - 1) + (j - 1) * m;
-      ql[fv] = inv[u] ? T(vc[u] - vr[u]) : T(curl_face<TS, TC>(g, fv, sl, sc));
+      ql[fv] = inv[u] ? T(vc[u] - vr[u])
+                      : T(cl != nullptr || ccs != nullptr ? curl_face_sum<TS, TC>(g, fv, sl, cl, sc, ccs)
+                                                          : curl_face<TS, TC>(g, fv, sl, sc));
     }
   }
 }
@@ -2791,7 +2865,7 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
                                               const T* __restrict__ om, long bs, long qbs,
                                               double* __restrict__ partial, int nchunk, const int* __restrict__ Lc,
                                               const double* __restrict__ s1d, int* __restrict__ pend,
-                                              int* __restrict__ counter) {
+                                              int* __restrict__ counter, const T* __restrict__ C) {
   __shared__ double red[4];
   const int tid = threadIdx.x, b = blockIdx.y;
   if (pend != nullptr && blockIdx.x == 0 && tid == 0) {  // the row is deferred to the next pass
@@ -2809,7 +2883,12 @@ __global__ __launch_bounds__(256) void k_tail(Geo g, const T* __restrict__ s, T*
       tail_curl<T, double, double>(g, f0, sl, hc ? sl + g.nw : nullptr, q + b * qbs + (long)lev * g.nq);
     } else {
       const T* sl = s + b * bs + (long)lev * g.nw;
-      tail_curl<T, T>(g, f0, sl, hc ? sl + g.nw : nullptr, q + b * qbs + (long)lev * g.nq);
+      // C: the left/right terms of every level but a case's coarsest, kept
+      // apart from s (the same sum s + C)
+      const int L = levels(g, Lc, b);
+      const T* cl = (C != nullptr && lev + 1 < L) ? C + b * bs + (long)lev * g.nw : nullptr;
+      const T* ccs = (C != nullptr && lev + 2 < L) ? C + b * bs + (long)(lev + 1) * g.nw : nullptr;
+      tail_curl<T, T>(g, f0, sl, hc ? sl + g.nw : nullptr, q + b * qbs + (long)lev * g.nq, cl, ccs);
     }
     return;
   }

@@ -392,6 +392,30 @@ def test_step3_real_space_equals_spectral(name, monkeypatch):
     assert not np.array_equal(s1["omega"], s0["omega"])  # the real-space form did run
 
 
+@pytest.mark.parametrize("name", ["rect64x48L5", "cyl64L4"])
+def test_split_streamfunction_is_bit_identical(name, monkeypatch):
+    """Step 4's batched vort2flux keeps the left/right edge terms apart from
+    the streamfunction (the last row and column passes independent, in one
+    launch where both are the same size-specialised pass) and the tail adds
+    them, before get_bc's averages at the edges too: the same sums as the
+    column pass adding into the streamfunction (IBFS_SPLIT_S=0), so the
+    states and histories after 10 steps are equal bit for bit (square and
+    oblong grids)."""
+    from ibfs_amd import hip
+
+    a = v2f_args(name)
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_SPLIT_S", "0")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(10), g0.advance(10)
+    s1, s0 = g1.get_state(), g0.get_state()
+    assert np.array_equal(h1, h0)
+    for k in s0:
+        assert np.array_equal(s1[k], s0[k]), k
+
+
 @pytest.mark.parametrize("name", ["rect64x48L5", "cyl64L4", "cyl40r5"])
 def test_coarsify_tiles_are_bit_identical(name, monkeypatch):
     """The coarsify chain two levels per launch in LDS tiles (the default)

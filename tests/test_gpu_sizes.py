@@ -106,16 +106,18 @@ def test_airfoil_512_five_levels():
 
 def test_step_switches_at_512_five_levels(monkeypatch):
     """The headline case under the switches that select equivalent forms:
-    the coarsify chain in LDS tiles or one place per thread, and step 2's
+    the coarsify chain in LDS tiles or one place per thread, step 2's
     level-0 transforms on the force-stencil rectangle or the whole field,
-    give bit-identical states after 3 steps; the band correction in real
+    and step 4's left/right terms kept apart for the tail or added into the
+    streamfunction, give bit-identical states after 3 steps; the band correction in real
     space or as spectral column pairs, step 3 in real space or as the
     spectral ctci, and the ring vectors' bottom and top lines by one FFT or
     by direct sums, agree to rounding."""
     from ibfs_amd import hip
 
     def run(**env):
-        for k in ("IBFS_COARSIFY_TILES", "IBFS_S_WINDOW", "IBFS_BAND_FFT", "IBFS_STEP3_LOCAL", "IBFS_RING_FFT"):
+        for k in ("IBFS_COARSIFY_TILES", "IBFS_S_WINDOW", "IBFS_BAND_FFT", "IBFS_STEP3_LOCAL", "IBFS_RING_FFT",
+                  "IBFS_SPLIT_S"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -125,7 +127,7 @@ def test_step_switches_at_512_five_levels(monkeypatch):
         return h, g.get_state()
 
     h, st = run()
-    for env in ({"IBFS_COARSIFY_TILES": "0"}, {"IBFS_S_WINDOW": "0"}):
+    for env in ({"IBFS_COARSIFY_TILES": "0"}, {"IBFS_S_WINDOW": "0"}, {"IBFS_SPLIT_S": "0"}):
         h1, st1 = run(**env)
         assert np.array_equal(h1, h), env
         for k in st:
