@@ -1632,10 +1632,12 @@ struct DstAdd {
 // reads one value per element.  The element blocks run in XCD-contiguous
 // order (gridDim.x - side_groups is a multiple of 8, surplus blocks exit), so
 // the rows a block's stencil shares with its neighbours stay in one L2.
-// grid (side_groups + ceil(nw / 256) rounded up to a multiple of 8, items),
-// 256 threads; side_groups a multiple of 8.
+// Batch entries without level lev (per-case level counts Lc, null: all)
+// get zeros.  grid (side_groups + ceil(nw / 256) rounded up to a multiple of
+// 8, items), 256 threads; side_groups a multiple of 8.
 template <typename T, class Src>
-__global__ __launch_bounds__(256) void k_src_eval(Geo g, Src src, T* __restrict__ out, int side_groups) {
+__global__ __launch_bounds__(256) void k_src_eval(Geo g, Src src, T* __restrict__ out, int side_groups,
+                                                  const int* __restrict__ Lc, int lev) {
   const int b = blockIdx.y;
   int x = blockIdx.x;
   if constexpr (src_has_side<Src>::value) {
@@ -1649,6 +1651,10 @@ __global__ __launch_bounds__(256) void k_src_eval(Geo g, Src src, T* __restrict_
   const int e = x * 256 + threadIdx.x;
   if (e >= g.nw) return;
   const int jj = e / g.mm1, ii = e - jj * g.mm1;
+  // a case without level lev (Lc) gets zeros and no side store, as the row
+  // pass's loader gives it; the value is evaluated either way (every place
+  // it reads exists)
+  const bool act = active(Lc, b, lev);
   T aux = T(0), v;
   if constexpr (src_gathered<Src>::value) {
     typename src_gathered<Src>::type G;
@@ -1659,8 +1665,9 @@ __global__ __launch_bounds__(256) void k_src_eval(Geo g, Src src, T* __restrict_
   } else {
     v = src(g, b, ii, jj);
   }
-  if constexpr (src_has_post<Src>::value) src.post(g, b, ii, jj, aux);
-  out[(long)b * g.nw + e] = v;
+  if constexpr (src_has_post<Src>::value)
+    if (act) src.post(g, b, ii, jj, aux);
+  out[(long)b * g.nw + e] = act ? v : T(0);
 }
 
 // ---------------------------------------------------------------------------
