@@ -166,6 +166,17 @@ struct Engine : GroupBase {
   DevBuf<int> ainv_loc, ainv_box;
   DevBuf<double> ainv_K;
   int ainv_nloc = 0, ainv_R = 0, ainv_tiles = 0;
+  // step 4's coarsify after step 3 in real space: between step 2's coarsify
+  // and step 4's only level 1's box around the body changed, so each
+  // two-level tile launch runs only the tiles whose places have a stencil
+  // reaching a changed place (co_tiles: per tile launch, per case {tx0, ty0,
+  // ntx, nty}; the whole grid for a case on the spectral step 3; co_grid:
+  // each launch's largest tile count).  The other places would be
+  // recomputed from unchanged inputs: the result is bit-identical.
+  // IBFS_COARSIFY_INC=0 runs the whole chain.
+  bool coarsify_inc = true;
+  DevBuf<int> co_tiles;
+  std::vector<int> co_grid;
   int ringN = 0;
   DevBuf<double> ringT, ringG, ringGp, ringR, ringS, ringV, ringX, ringSin;
   // the band correction (BandOp): beta per item, the band width per level
@@ -310,6 +321,7 @@ struct Engine : GroupBase {
     lev_batch = levels_can_batch();
     if (const char* sp = std::getenv("IBFS_STEP1_SPLIT")) src_split = sp[0] != '0';
     if (const char* ct = std::getenv("IBFS_COARSIFY_TILES")) coarsify_tiles = ct[0] != '0';
+    if (const char* ci = std::getenv("IBFS_COARSIFY_INC")) coarsify_inc = ci[0] != '0';
     if (const char* rf = std::getenv("IBFS_RING_FFT")) ring_fft_lines = rf[0] != '0';
     if (const char* sp = std::getenv("IBFS_SPLIT_S")) s_split = sp[0] != '0';
     if (lev_batch) {
@@ -1080,14 +1092,21 @@ struct Engine : GroupBase {
 
   // the in-place coarsify chain of vort2flux (src/operators.f90:224-226):
   // two levels per launch, in LDS tiles (k_coarsify2t) or one place per
-  // thread (k_coarsify2, also the single last level of an even level count)
-  void coarsify(T* om, int batch, const int* gate) {
+  // thread (k_coarsify2, also the single last level of an even level count).
+  // inc: step 4 after step 3 in real space (the tiles of co_tiles only)
+  void coarsify(T* om, int batch, const int* gate, bool inc = false) {
     const int wcnt = (m / 2 - 1) * (n / 2 - 1), nblk = (wcnt + 255) / 256;
-    for (int lev = 1; lev < Lmax; lev += 2) {
+    inc = inc && coarsify_inc && coarsify_tiles && batch == B && !co_grid.empty();
+    for (int lev = 1, t = 0; lev < Lmax; lev += 2) {
       if (lev + 1 < Lmax && coarsify_tiles) {
         const int ntx = (m / 2 - 1 + kCoTX - 1) / kCoTX, nty = (n / 2 - 1 + kCoTY - 1) / kCoTY;
-        hipLaunchKernelGGL(k_coarsify2t<T>, dim3(ntx * nty, batch), dim3(256), 0, st, geo, om, fbs(), gate, lev,
-                           ntx);
+        if (!inc)
+          hipLaunchKernelGGL(k_coarsify2t<T>, dim3(ntx * nty, batch), dim3(256), 0, st, geo, om, fbs(), gate, lev,
+                             ntx, (const int*)nullptr);
+        else if (co_grid[t] > 0)
+          hipLaunchKernelGGL(k_coarsify2t<T>, dim3(co_grid[t], batch), dim3(256), 0, st, geo, om, fbs(), gate, lev,
+                             ntx, (const int*)co_tiles.p + 4L * B * t);
+        t++;
         continue;
       }
       const int nlev = lev + 1 < Lmax ? 2 : 1;
@@ -1099,9 +1118,9 @@ struct Engine : GroupBase {
   // vort2flux (src/operators.f90:191-240) on a batch of multi-level fields om
   // (stride Lmax*nw); streamfunctions go to s (same layout); when vel is set,
   // curl(s) is written on every level (stride Lmax*nq).
-  void vort2flux(T* om, T* s, T* vel, int batch, const int* gate, T* t1, T* t2) {
+  void vort2flux(T* om, T* s, T* vel, int batch, const int* gate, T* t1, T* t2, bool inc = false) {
     const long bs = fbs();
-    coarsify(om, batch, gate);
+    coarsify(om, batch, gate, inc);
     for (int lev = Lmax - 1; lev >= 0; lev--) {
       T* sc = (lev + 1 < Lmax) ? s + (long)(lev + 1) * nw : nullptr;
       ctci_like(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, false, DstField<T>{s + (long)lev * nw, bs}, t1,
@@ -1197,10 +1216,10 @@ struct Engine : GroupBase {
                        sizeof(double) * (Kx + 2 * (Kx + 1)), st, geo, ringR.p, NR, im, Kx, ringSin.p,
                        ringSin.p + 2 * m, ringX.p);
   }
-  void vort2flux_levels(bool all) {
+  void vort2flux_levels(bool all, bool inc = false) {
     if constexpr (std::is_same_v<T, double>) {
       const int BL = B * Lmax, NR = ringN, Kx = std::max(m, n) - 1;
-      coarsify(omega.p, B, gate);
+      coarsify(omega.p, B, gate, inc);
       launch<0>(RowOp<T, SrcField<T>, DstField<T>>{geoL, SrcField<T>{omega.p, nw}, DstField<T>{T1L.p, nw}, nullptr,
                                                    0},
                 geo.nm1, 2, BL);
@@ -1279,8 +1298,8 @@ struct Engine : GroupBase {
   }
   // vort2flux streamfunctions of an FP32 engine with every level solved in
   // FP64 into S64 (steps 2 and 4: the force system and the stored fluxes)
-  void vort2flux64(T* om, int batch, const int* gate) {
-    coarsify(om, batch, gate);
+  void vort2flux64(T* om, int batch, const int* gate, bool inc = false) {
+    coarsify(om, batch, gate, inc);
     for (int lev = Lmax - 1; lev >= 0; lev--)
       ctci64<double>(om, (lev + 1 < Lmax) ? S64.p + (long)(lev + 1) * nw : nullptr, batch, gate, lev);
   }
@@ -1513,9 +1532,12 @@ struct Engine : GroupBase {
                          (const T*)RR.p,
                          (long)nw, ainv_box.p, ainv_K.p, ainv_loc.p, ainv_R);
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
-    if constexpr (kSolve64) vort2flux64(omega.p, B, gate);
-    else if (v2f_batch) vort2flux_levels(true);
-    else vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
+    // (the coarsify chain only where step 3 changed its inputs when some
+    // case took the real-space form)
+    const bool inc = nloc > 0;
+    if constexpr (kSolve64) vort2flux64(omega.p, B, gate, inc);
+    else if (v2f_batch) vort2flux_levels(true, inc);
+    else vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p, inc);
     // fluxes and diagnostics (src/operators.f90:184-186)
     const int ncurl = tail_groups(geo) * Lmax;
     const bool defer = diag_deferred();
@@ -1572,6 +1594,7 @@ struct Engine : GroupBase {
   // adjoint's.
   void ainv_tables(const std::vector<int>& vid) {
     ainv_nloc = 0;
+    co_grid.clear();
     const char* env = std::getenv("IBFS_STEP3_LOCAL");
     if (env && env[0] == '0') return;
     const long double pi = 3.141592653589793238462643383279502884L;
@@ -1639,6 +1662,51 @@ struct Engine : GroupBase {
     ainv_R = R;
     ainv_tiles = tiles;
     ainv_nloc = nloc;
+    coarsify_tables(loc, boxv);
+  }
+  // co_tiles and co_grid from the target boxes of step 3 (boxv, interior
+  // indices {i0, j0, nx, ny}; vertex i = index + 2).  Level k's centre place
+  // (i, j) reads level k-1's places 2i - m/2 - 2 .. 2i - m/2 (and likewise
+  // in j), so the changed range [lo, hi] of level k-1 makes level k's
+  // [ceil((lo + m/2) / 2), floor((hi + m/2 + 2) / 2)] within the centre
+  // [m/4 + 2, 3m/4].  A tile launch over levels lev and lev+1 runs the tiles
+  // of level lev+1's range: a level lev place p is written by the tile of
+  // level lev+1's place floor((p + m/2 + 1) / 2), whose stencil holds p, so
+  // those tiles hold every changed place of level lev too.
+  void coarsify_tables(const std::vector<int>& loc, const std::vector<int>& boxv) {
+    const int ntx = (m / 2 - 1 + kCoTX - 1) / kCoTX, nty = (n / 2 - 1 + kCoTY - 1) / kCoTY;
+    int nlaunch = 0;
+    for (int lev = 1; lev + 1 < Lmax; lev += 2) nlaunch++;
+    if (nlaunch == 0) return;
+    std::vector<int> tv((size_t)4 * B * nlaunch, 0);
+    co_grid.assign(nlaunch, 0);
+    for (int b = 0; b < B; b++) {
+      int r[4] = {0, 0, 0, 0};  // the changed range of the current level {ilo, ihi, jlo, jhi}
+      bool any = loc[b] != 0;
+      if (any) {
+        r[0] = boxv[4 * b] + 2, r[1] = boxv[4 * b] + boxv[4 * b + 2] + 1;
+        r[2] = boxv[4 * b + 1] + 2, r[3] = boxv[4 * b + 1] + boxv[4 * b + 3] + 1;
+      }
+      for (int lev = 1, t = 0; lev < Lmax; lev++) {
+        if (any) {
+          r[0] = std::max((r[0] + m / 2 + 1) / 2, m / 4 + 2), r[1] = std::min((r[1] + m / 2 + 2) / 2, 3 * m / 4);
+          r[2] = std::max((r[2] + n / 2 + 1) / 2, n / 4 + 2), r[3] = std::min((r[3] + n / 2 + 2) / 2, 3 * n / 4);
+          any = r[0] <= r[1] && r[2] <= r[3];
+        }
+        if (lev % 2 == 0 && t < nlaunch) {  // the launch over levels lev-1 and lev: level lev's tiles
+          int* o = tv.data() + 4 * ((size_t)t * B + b);
+          if (!loc[b]) {
+            o[0] = o[1] = 0, o[2] = ntx, o[3] = nty;
+          } else if (any) {
+            o[0] = (r[0] - (m / 4 + 2)) / kCoTX, o[1] = (r[2] - (n / 4 + 2)) / kCoTY;
+            o[2] = (r[1] - (m / 4 + 2)) / kCoTX - o[0] + 1, o[3] = (r[3] - (n / 4 + 2)) / kCoTY - o[1] + 1;
+          }
+          co_grid[t] = std::max(co_grid[t], o[2] * o[3]);
+          t++;
+        }
+      }
+    }
+    upvec(co_tiles, tv, st);
   }
   // the compact smearing tables of every case, concatenated, on the device
   // (and the per-case offsets of the force system that depend on them)

@@ -1837,19 +1837,29 @@ __global__ __launch_bounds__(64 * kAinvWaves) void k_ainv_local(Geo g, T* __rest
 // lev values inside the centre are never read (they are being overwritten
 // by other workgroups): those staging loads read place (2, 2) instead, and
 // the recomputed value replaces them.  grid (ntx nty, B), 256 threads;
-// lev + 1 < Lmax.
+// lev + 1 < Lmax.  With tiles set, case b runs only the tiles
+// tiles[4b..4b+3] = {tx0, ty0, ntx_b, nty_b} (grid.x = the largest
+// ntx_b nty_b): the places whose values can differ from those already
+// stored (the rest would be recomputed from unchanged inputs).
 constexpr int kCoTX = 32, kCoTY = 8;
 constexpr int kCoMX = 2 * kCoTX + 1, kCoMY = 2 * kCoTY + 1, kCoFX = 4 * kCoTX + 3, kCoFY = 4 * kCoTY + 3;
 constexpr int kCoFU = (kCoFX * kCoFY + 255) / 256, kCoMU = (kCoMX * kCoMY + 255) / 256;
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarsify2t(Geo g, T* __restrict__ om, long bs, const int* __restrict__ Lc,
-                                                    int lev, int ntx) {
+                                                    int lev, int ntx, const int* __restrict__ tiles) {
   __shared__ T fs[kCoFY][kCoFX];
   __shared__ T ms[kCoMY][kCoMX];
   const int b = blockIdx.y;
   if (!active(Lc, b, lev)) return;
   const bool up = active(Lc, b, lev + 1);
-  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  if (tiles != nullptr) {  // the case's own rectangle of tiles
+    const int* tb = tiles + 4 * b;
+    const int nx = tb[2];
+    if ((int)blockIdx.x >= nx * tb[3]) return;
+    tx = tb[0] + (int)blockIdx.x % nx;
+    ty = tb[1] + (int)blockIdx.x / nx;
+  }
   const int i0 = g.m / 4 + 2 + tx * kCoTX, j0 = g.n / 4 + 2 + ty * kCoTY;
   const int ma = 2 * i0 - g.m / 2 - 2, mc = 2 * j0 - g.n / 2 - 2;  // level lev place of ms[0][0]
   const int fa = 2 * ma - g.m / 2 - 2, fc = 2 * mc - g.n / 2 - 2;  // level lev-1 place of fs[0][0]

@@ -55,20 +55,23 @@ def test_ensemble_matches_single_case_runs(monkeypatch):
             assert np.array_equal(es[key], ss[key]), f"case {i} {key}"
 
 
-def test_step1_source_split_in_an_uneven_batch(monkeypatch):
-    """Step 1 level by level on an FP64 batch of uneven cases (2 to 5
-    levels): the freestream source evaluated by its own kernel into a scratch
-    field and read by a plain row pass (the default) or inside the row pass's
-    loader (IBFS_STEP1_SPLIT=0) is the same arithmetic, and a case without
-    the level being solved gets zeros from either form, so the states after
-    6 steps are identical."""
+@pytest.mark.parametrize("switch", ["IBFS_STEP1_SPLIT", "IBFS_COARSIFY_INC"])
+def test_equivalent_forms_in_an_uneven_batch(monkeypatch, switch):
+    """An FP64 batch of uneven cases (2 to 5 levels, step 1 and vort2flux
+    level by level), each switch on (the default) and off, for 6 steps:
+    IBFS_STEP1_SPLIT, the freestream source evaluated by its own kernel into
+    a scratch field and read by a plain row pass, or inside the row pass's
+    loader (the same arithmetic; a case without the level being solved gets
+    zeros from either form); IBFS_COARSIFY_INC, step 4's coarsify on the
+    tiles whose inputs step 3 changed (each case its own rectangle), or on
+    every tile.  The states are identical."""
     from ibfs_amd import hip
 
     monkeypatch.setenv("IBFS_STEP1_LEVELS", "0")
     monkeypatch.setenv("IBFS_V2F_LEVELS", "0")
     runs = []
-    for split in ("1", "0"):
-        monkeypatch.setenv("IBFS_STEP1_SPLIT", split)
+    for on in ("1", "0"):
+        monkeypatch.setenv(switch, on)
         ens = hip.Ensemble(ens_cases())
         ens.build_cholesky()
         hist = ens.advance(6)

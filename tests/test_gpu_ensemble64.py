@@ -97,6 +97,33 @@ def test_mixed_ensemble_within_1e5_of_fp64(factors):
     print("worst per field kind", worst)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_step4_coarsify_on_changed_tiles(factors, monkeypatch, precision):
+    """Step 4's coarsify on the tiles whose inputs step 3 changed (each case
+    of a group its own rectangle, from its own body box and level count) or
+    on every tile (IBFS_COARSIFY_INC=0): identical states after 5 steps, in
+    the FP64 and the mixed ensemble of the slice (groups of 1 to 3 cases of
+    one grid size)."""
+    from ibfs_amd import hip
+
+    prec = hip.MIXED if precision == "mixed" else hip.PLAIN_FP64
+    sl = slice_cases()
+    runs = []
+    for on in ("1", "0"):
+        monkeypatch.setenv("IBFS_COARSIFY_INC", on)
+        ens = hip.Ensemble(sl, precision=prec)
+        for i, (cm, cv) in enumerate(factors):
+            ens.set_case_cholesky(i, cm, cv)
+        hist = ens.advance(5)
+        runs.append((hist, [ens.get_state(i) for i in range(len(sl))]))
+        ens.close()
+    (h1, s1), (h0, s0) = runs
+    assert np.array_equal(h1, h0)
+    for i in range(len(sl)):
+        for key in ("omega", "q", "rhs_old", "fb"):
+            assert np.array_equal(s1[i][key], s0[i][key]), f"case {SLICE[i]} {key}"
+
+
 @pytest.mark.parametrize("m", [256, 320, 384, 512])
 def test_fp64_case_of_each_size_matches_oracle(m, factors):
     from ibfs_amd import hip

@@ -106,7 +106,8 @@ def test_airfoil_512_five_levels():
 
 def test_step_switches_at_512_five_levels(monkeypatch):
     """The headline case under the switches that select equivalent forms:
-    the coarsify chain in LDS tiles or one place per thread, step 2's
+    the coarsify chain in LDS tiles or one place per thread, step 4's
+    coarsify on the tiles step 3 can change or on every tile, step 2's
     level-0 transforms on the force-stencil rectangle or the whole field,
     and step 4's left/right terms kept apart for the tail or added into the
     streamfunction, give bit-identical states after 3 steps; the band correction in real
@@ -116,8 +117,8 @@ def test_step_switches_at_512_five_levels(monkeypatch):
     from ibfs_amd import hip
 
     def run(**env):
-        for k in ("IBFS_COARSIFY_TILES", "IBFS_S_WINDOW", "IBFS_BAND_FFT", "IBFS_STEP3_LOCAL", "IBFS_RING_FFT",
-                  "IBFS_SPLIT_S"):
+        for k in ("IBFS_COARSIFY_TILES", "IBFS_COARSIFY_INC", "IBFS_S_WINDOW", "IBFS_BAND_FFT", "IBFS_STEP3_LOCAL",
+                  "IBFS_RING_FFT", "IBFS_SPLIT_S"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -127,7 +128,8 @@ def test_step_switches_at_512_five_levels(monkeypatch):
         return h, g.get_state()
 
     h, st = run()
-    for env in ({"IBFS_COARSIFY_TILES": "0"}, {"IBFS_S_WINDOW": "0"}, {"IBFS_SPLIT_S": "0"}):
+    for env in ({"IBFS_COARSIFY_TILES": "0"}, {"IBFS_COARSIFY_INC": "0"}, {"IBFS_S_WINDOW": "0"},
+                {"IBFS_SPLIT_S": "0"}):
         h1, st1 = run(**env)
         assert np.array_equal(h1, h), env
         for k in st:
