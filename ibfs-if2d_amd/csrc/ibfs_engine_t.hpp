@@ -179,6 +179,11 @@ struct Engine : GroupBase {
   std::vector<int> co_grid;
   int ringN = 0;
   DevBuf<double> ringT, ringG, ringGp, ringR, ringS, ringV, ringX, ringSin;
+  // the ring chain on the representatives' rows of T's powers (k_ring_chain
+  // with sym; set in build_ring when T commutes with the ring's symmetries);
+  // IBFS_RING_SYM=0 reads every row
+  DevBuf<int> ringSym;
+  bool ring_sym_on = false;
   // the band correction (BandOp): beta per item, the band width per level
   DevBuf<double> band_beta, band_rho_m, band_rho_n;
   DevBuf<int> band_w;
@@ -369,6 +374,52 @@ struct Engine : GroupBase {
     const double nr = ring_count(m, n);
     return (double)std::max(Lmax - 2, 1) * nr * nr * 8.0 <= double(1L << 30);
   }
+  // The orbits of the ring places under ring_sym (8 symmetries on square
+  // grids, 4 otherwise): each place's representative (the smallest place of
+  // its orbit) and a symmetry that maps the representative to it, and the
+  // places ordered orbit by orbit.  Taken only when the built T commutes
+  // with every symmetry to 1e-12 of its largest entry (it does up to the
+  // rounding of its GEMMs; the chain then uses the representative's row,
+  // the same matrix to rounding).
+  void ring_sym_tables() {
+    ring_sym_on = false;
+    const char* env = std::getenv("IBFS_RING_SYM");
+    if ((env && env[0] == '0') || Lmax < 3) return;
+    const int NR = ringN, ng = m == n ? 8 : 4;
+    std::vector<int> perm((size_t)ng * NR);
+    for (int gi = 0; gi < ng; gi++)
+      for (int p = 0; p < NR; p++) perm[(size_t)gi * NR + p] = ring_sym(m, n, gi, p);
+    std::vector<double> Th((size_t)NR * NR);
+    HIPCHK(hipMemcpyAsync(Th.data(), ringT.p, sizeof(double) * Th.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double mx = 0.0, worst = 0.0;
+    for (double v : Th) mx = std::max(mx, std::fabs(v));
+    for (int gi = 1; gi < ng; gi++) {
+      const int* pg = perm.data() + (size_t)gi * NR;
+      for (int r = 0; r < NR; r++) {
+        const double* tr = Th.data() + (size_t)r * NR;
+        const double* tg = Th.data() + (size_t)pg[r] * NR;
+        for (int c = 0; c < NR; c++) worst = std::max(worst, std::fabs(tg[pg[c]] - tr[c]));
+      }
+    }
+    if (!(worst <= 1e-12 * mx)) return;
+    std::vector<int> tab((size_t)3 * NR), ord(NR);
+    for (int p = 0; p < NR; p++) {
+      int r = p;
+      for (int gi = 1; gi < ng; gi++) r = std::min(r, perm[(size_t)gi * NR + p]);
+      int gp = -1;
+      for (int gi = 0; gi < ng && gp < 0; gi++)
+        if (perm[(size_t)gi * NR + r] == p) gp = gi;
+      if (gp < 0) return;  // not a group action: keep every row
+      tab[NR + p] = r;
+      tab[2 * NR + p] = gp;
+      ord[p] = p;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return tab[NR + a] < tab[NR + b]; });
+    std::copy(ord.begin(), ord.end(), tab.begin());
+    upvec(ringSym, tab, st);
+    ring_sym_on = true;
+  }
   // the tables of the batched vort2flux: g / (2N) along each axis, T from its
   // four edge blocks (T[:, r0_e + c] = B_e A_e^T), and T's powers
   void build_ring() {
@@ -405,6 +456,7 @@ struct Engine : GroupBase {
       hipLaunchKernelGGL(k_gemm64, dim3((NR + 15) / 16, (NR + 15) / 16), dim3(256), 0, st, NR, NR, NR,
                          ringT.p + (size_t)(j - 2) * NR * NR, (long)NR, 1L, ringT.p, (long)NR, 1L,
                          ringT.p + (size_t)(j - 1) * NR * NR, (long)NR, 0);
+    ring_sym_tables();
     ringR.alloc((size_t)B * Lmax * NR);
     ringR.zero(st);
     ringV.alloc((size_t)B * Lmax * NR);
@@ -1233,7 +1285,7 @@ struct Engine : GroupBase {
 #define IBFS_RING_CHAIN(NP)                                                                                   \
   case NP:                                                                                                    \
     hipLaunchKernelGGL(k_ring_chain<NP>, dim3(NR, B), dim3(256), 0, st, geo, fbs(), NR, ringV.p, ringT.p,     \
-                       ringR.p, sring);                                                                       \
+                       ringR.p, sring, ring_sym_on ? (const int*)ringSym.p : nullptr);                        \
     break;
         IBFS_RING_CHAIN(0)
         IBFS_RING_CHAIN(1)

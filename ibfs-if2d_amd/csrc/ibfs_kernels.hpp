@@ -1117,6 +1117,31 @@ __host__ __device__ __forceinline__ int ring_edge0(int m, int n, int e) {
   const int nb = m / 2 + 1, nl = n / 2 + 1;
   return e == 0 ? 0 : e == 1 ? nb : e == 2 ? 2 * nb : 2 * nb + nl;
 }
+// The symmetries of the ring places: g = 4a + 2b + c applies the flip
+// x -> m-2-x if c (positions along x reverse, left <-> right), then the flip
+// y -> n-2-y if b (bottom <-> top, positions along y reverse), then, on
+// square grids, the transpose if a (bottom <-> left, top <-> right, the
+// same position).  The ring map T is built from the sine transforms, the
+// harmonic extension and get_bc's averages, all symmetric under these, so
+// T(g r, g c) = T(r, c) up to rounding.
+__host__ __device__ __forceinline__ int ring_sym(int m, int n, int g, int p) {
+  const int nb = m / 2 + 1, nl = n / 2 + 1;
+  int side, pos;
+  if (p < nb) side = 0, pos = p;
+  else if (p < 2 * nb) side = 1, pos = p - nb;
+  else if (p < 2 * nb + nl) side = 2, pos = p - 2 * nb;
+  else side = 3, pos = p - 2 * nb - nl;
+  if (g & 1) {
+    if (side < 2) pos = nb - 1 - pos;
+    else side ^= 1;
+  }
+  if (g & 2) {
+    if (side < 2) side ^= 1;
+    else pos = nl - 1 - pos;
+  }
+  if (g & 4) side ^= 2;
+  return side < 2 ? side * nb + pos : 2 * nb + (side - 2) * nl + pos;
+}
 // Interp(R) at interior place t (0-based, t = 0..N-2) of edge e (0 bottom,
 // 1 top, 2 left, 3 right): get_bc's value at 1-based place i = t+1 of that
 // edge, taken from the ring vector R (the same two terms in the same order)
@@ -1383,16 +1408,28 @@ __global__ __launch_bounds__(256) void k_ring_fromY_fft(Geo g, const double* __r
 // kernel's whole HBM traffic (NP NR^2 doubles); one row per workgroup keeps
 // sixteen waves of loads in flight per CU.  With sring, R_1 is also written
 // at the ring places of level 1 of sring (a field in the streamfunction's
-// layout) for get_bc(s_1).  grid (NR, B), 256 threads.
+// layout) for get_bc(s_1).  With sym set (sym[l] the row of workgroup l in
+// XCD-contiguous order, sym[NR + p] the representative r of row p's orbit
+// under ring_sym, sym[2 NR + p] a symmetry g with g r = p), row p is
+// sum_q T^j(r, q) V(g q): the powers are read on the representatives' rows
+// only (an eighth of them on square grids), and the rows of one orbit are
+// adjacent in the order, so they run on one XCD and share its L2.
+// grid (NR, B), 256 threads.
 constexpr int kRingMaxLev = 8;
 constexpr int chain_u(int np) { return np <= 3 ? 5 : 3; }
 template <int NP>
 __global__ __launch_bounds__(256) void k_ring_chain(Geo g, long fbs, int NR, const double* __restrict__ V,
                                                     const double* __restrict__ Tp, double* __restrict__ Rb,
-                                                    double* __restrict__ sring) {
+                                                    double* __restrict__ sring, const int* __restrict__ sym) {
   constexpr int L = NP + 2, U = chain_u(NP), NJ = NP > 0 ? NP : 1;
   __shared__ double red[L][4];
-  const int b = blockIdx.y, p = blockIdx.x;
+  const int b = blockIdx.y;
+  int p = blockIdx.x, r = p, gs = 0;
+  if (sym != nullptr) {
+    p = sym[xcd_contiguous(blockIdx.x, NR)];
+    r = sym[NR + p];
+    gs = sym[2 * NR + p];
+  }
   const double* vb = V + ((long)b * L + 1) * NR;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double acc[L];
@@ -1406,15 +1443,18 @@ __global__ __launch_bounds__(256) void k_ring_chain(Geo g, long fbs, int NR, con
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const int q = q0 + threadIdx.x + 256 * u;
-          t[j][u] = q < NR ? Tp[(long)j * NR * NR + (long)p * NR + q] : 0.0;
+          t[j][u] = q < NR ? Tp[(long)j * NR * NR + (long)r * NR + q] : 0.0;
         }
+      int qv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int q = q0 + threadIdx.x + 256 * u;
+        qv[u] = q < NR ? (gs != 0 ? ring_sym(g.m, g.n, gs, q) : q) : -1;
+      }
 #pragma unroll
       for (int k = 1; k < L; k++)
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int q = q0 + threadIdx.x + 256 * u;
-          vv[k - 1][u] = q < NR ? vb[(k - 1) * NR + q] : 0.0;
-        }
+        for (int u = 0; u < U; u++) vv[k - 1][u] = qv[u] >= 0 ? vb[(k - 1) * NR + qv[u]] : 0.0;
 #pragma unroll
       for (int j = 0; j < NP; j++)  // T^(j+1) V_{k+j+1}
 #pragma unroll

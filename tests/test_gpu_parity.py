@@ -416,6 +416,33 @@ def test_split_streamfunction_is_bit_identical(name, monkeypatch):
         assert np.array_equal(s1[k], s0[k]), k
 
 
+@pytest.mark.parametrize("name", ["rect64x48L5", "cyl64L4"])
+def test_ring_chain_on_symmetry_representatives(name, monkeypatch):
+    """The ring map T commutes with the flips of the grid (and, on square
+    grids, its transpose), so the ring chain reads T's powers on one row per
+    symmetry orbit and permutes the vectors instead (the default) or reads
+    every row (IBFS_RING_SYM=0).  The built T is symmetric only to the
+    rounding of its GEMMs, so the two agree to rounding over 10 steps and
+    are not identical (the symmetric path was taken: four symmetries on the
+    oblong grid, eight on the square one)."""
+    from ibfs_amd import hip
+
+    a = v2f_args(name)
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_RING_SYM", "0")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(10), g0.advance(10)
+    s1, s0 = g1.get_state(), g0.get_state()
+    nb2 = 2 * g1.nbody
+    assert np.max(np.abs(h1[:, :nb2] - h0[:, :nb2])) < 1e-12 * np.max(np.abs(h0[:, :nb2]))
+    for k in ("omega", "q"):
+        assert relerr(s1[k], s0[k]) < 1e-13, k
+    assert relerr(s1["fb"], s0["fb"]) < 1e-12
+    assert not np.array_equal(s1["omega"], s0["omega"])
+
+
 @pytest.mark.parametrize("name", ["rect64x48L5", "cyl64L4", "cyl40r5"])
 def test_coarsify_tiles_are_bit_identical(name, monkeypatch):
     """The coarsify chain two levels per launch in LDS tiles (the default)

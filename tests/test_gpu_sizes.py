@@ -112,13 +112,14 @@ def test_step_switches_at_512_five_levels(monkeypatch):
     and step 4's left/right terms kept apart for the tail or added into the
     streamfunction, give bit-identical states after 3 steps; the band correction in real
     space or as spectral column pairs, step 3 in real space or as the
-    spectral ctci, and the ring vectors' bottom and top lines by one FFT or
-    by direct sums, agree to rounding."""
+    spectral ctci, the ring vectors' bottom and top lines by one FFT or
+    by direct sums, and the ring chain on the representative rows of T's
+    symmetry orbits or on every row, agree to rounding."""
     from ibfs_amd import hip
 
     def run(**env):
         for k in ("IBFS_COARSIFY_TILES", "IBFS_COARSIFY_INC", "IBFS_S_WINDOW", "IBFS_BAND_FFT", "IBFS_STEP3_LOCAL",
-                  "IBFS_RING_FFT", "IBFS_SPLIT_S"):
+                  "IBFS_RING_FFT", "IBFS_SPLIT_S", "IBFS_RING_SYM"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -134,7 +135,7 @@ def test_step_switches_at_512_five_levels(monkeypatch):
         assert np.array_equal(h1, h), env
         for k in st:
             assert np.array_equal(st1[k], st[k]), (env, k)
-    for env in ({"IBFS_BAND_FFT": "1"}, {"IBFS_STEP3_LOCAL": "0"}, {"IBFS_RING_FFT": "0"}):
+    for env in ({"IBFS_BAND_FFT": "1"}, {"IBFS_STEP3_LOCAL": "0"}, {"IBFS_RING_FFT": "0"}, {"IBFS_RING_SYM": "0"}):
         h1, st1 = run(**env)
         for k in ("omega", "q"):
             assert relerr(st1[k], st[k]) < 1e-13, (env, k)
