@@ -1580,7 +1580,7 @@ struct Engine : GroupBase {
       ctci_like(SrcField<T>{RR.p, nw}, true, DstSub<T>{omega.p, FB, nloc > 0 ? ainv_loc.p : nullptr}, T2.p, T3.p, B,
                 nullptr, 0, lam1i.p, tbs());
     if (nloc > 0)
-      hipLaunchKernelGGL(k_ainv_local<T>, dim3(ainv_tiles, B), dim3(64 * kAinvWaves), 0, st, geo, omega.p, FB,
+      hipLaunchKernelGGL(k_ainv_local<T>, dim3(ainv_tiles, B), dim3(kAinvThreads), 0, st, geo, omega.p, FB,
                          (const T*)RR.p,
                          (long)nw, ainv_box.p, ainv_K.p, ainv_loc.p, ainv_R);
     // STEP 4 (src/operators.f90:177): streamfunctions here, fluxes in k_tail
@@ -1700,7 +1700,8 @@ struct Engine : GroupBase {
         const int i0 = std::max(boxv[4 * b] - R, 0), j0 = std::max(boxv[4 * b + 1] - R, 0);
         const int i1 = std::min(boxv[4 * b + 2] + R, m - 2), j1 = std::min(boxv[4 * b + 3] + R, n - 2);
         boxv[4 * b] = i0, boxv[4 * b + 1] = j0, boxv[4 * b + 2] = i1 - i0 + 1, boxv[4 * b + 3] = j1 - j0 + 1;
-        tiles = std::max(tiles, ((boxv[4 * b + 2] + 7) / 8) * ((boxv[4 * b + 3] + 7) / 8));
+        const int ntx = (boxv[4 * b + 2] + kAinvTX - 1) / kAinvTX, nty = (boxv[4 * b + 3] + kAinvTY - 1) / kAinvTY;
+        tiles = std::max(tiles, ntx * nty);
       }
     std::vector<double> kh((size_t)B * (R + 1) * (R + 1), 0.0);
     for (int b = 0; b < B; b++)

@@ -1774,92 +1774,137 @@ __global__ void k_coarsify(Geo g, T* __restrict__ coarse, const T* __restrict__ 
 // host tabulates K for 0 <= a, b <= R (the dropped tail below 1e-17 of a unit
 // source) and takes a case this way only when its active vertices are at
 // least R cells from every edge (then every image term is past R).
-// Workgroup: an 8 x 8 block of the case's target box (the active box
-// widened by R, inside the field); it stages rr over the block and R around
-// it (zero outside the field), and K; each of the sixteen waves sums every
-// sixteenth row of the (2R+1)^2 window for the 64 places (short chains: the
-// sums are latency-bound), and the partial sums are added in a fixed tree.  R is the largest reach of the cases
-// taken; a case's table is zero past its own reach.
+// Workgroup: a kAinvTX x kAinvTY block of the case's target box; it stages
+// rr over the block and R around it (zero outside the field), and K.  Each
+// thread owns four neighbouring places of one block row; the sixteen groups
+// of 32 threads take the window rows dy = grp mod 16 (two rows each at
+// R = 14: short serial chains for the latency-bound single case), and for
+// each row a thread
+// streams the 2R + 4 staged values its four places read once from LDS,
+// feeding four chains of FMAs (each place's taps in ascending dx), so a
+// staged value serves up to four places.  The groups' partial sums are
+// added in a fixed tree.  R is the largest reach of the cases taken; a
+// case's table is zero past its own reach, and the row groups and tap order
+// do not depend on R, so a case in a batch gives its single-case result bit
+// for bit.
 // om: level 1 of case b at b bs; rr (one level per case) at b rbs.
 // box[b] = {i0, j0, nx, ny} (target box), K per case ((R+1)^2, row b = |dy|),
-// loc[b] != 0 for the cases taken.  grid (max tiles, B), 64 kAinvWaves threads.
+// loc[b] != 0 for the cases taken.  grid (max tiles, B), 512 threads.
 constexpr int kAinvRmax = 20;
-constexpr int kAinvW = 8 + 2 * kAinvRmax;
-constexpr int kAinvWaves = 16;  // waves per workgroup; wave w sums the rows dy = w mod 16
+constexpr int kAinvTX = 16, kAinvTY = 8, kAinvPX = 4, kAinvGroups = 16, kAinvThreads = 32 * kAinvGroups;
+constexpr int kAinvWX = kAinvTX + 2 * kAinvRmax, kAinvWY = kAinvTY + 2 * kAinvRmax;
 template <typename T>
-__global__ __launch_bounds__(64 * kAinvWaves) void k_ainv_local(Geo g, T* __restrict__ om, long bs,
-                                                                const T* __restrict__ rr, long rbs,
-                                                                const int* __restrict__ box,
-                                                                const double* __restrict__ K,
-                                                                const int* __restrict__ loc, int R) {
-  constexpr int NT = 64 * kAinvWaves;
-  __shared__ double rs[kAinvW * kAinvW];
+__global__ __launch_bounds__(kAinvThreads) void k_ainv_local(Geo g, T* __restrict__ om, long bs,
+                                                             const T* __restrict__ rr, long rbs,
+                                                             const int* __restrict__ box, const double* __restrict__ K,
+                                                             const int* __restrict__ loc, int R) {
+  constexpr int NP = kAinvTX * kAinvTY;  // places per block
+  __shared__ double rs[kAinvWX * kAinvWY];
   __shared__ double ks[(kAinvRmax + 1) * (kAinvRmax + 1)];
-  __shared__ double red[kAinvWaves][64];
+  __shared__ double red[kAinvGroups][NP];
   const int b = blockIdx.y;
   if (!loc[b]) return;
   const int bi0 = box[4 * b], bj0 = box[4 * b + 1], nx = box[4 * b + 2], ny = box[4 * b + 3];
-  const int ntx = (nx + 7) / 8, tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
-  if (ty * 8 >= ny) return;
-  const int x0 = bi0 + tx * 8, y0 = bj0 + ty * 8, W = 8 + 2 * R, R1 = R + 1;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, lx = lane & 7, ly = lane >> 3;
-  const bool mine = w == 0 && tx * 8 + lx < nx && ty * 8 + ly < ny;
-  T* o = om + b * bs + (x0 + lx) + (long)(y0 + ly) * g.mm1;
+  const int ntx = (nx + kAinvTX - 1) / kAinvTX, tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  if (ty * kAinvTY >= ny) return;
+  const int x0 = bi0 + tx * kAinvTX, y0 = bj0 + ty * kAinvTY, W = kAinvTX + 2 * R, H = kAinvTY + 2 * R, R1 = R + 1;
+  const int tid = threadIdx.x, grp = tid >> 5, lt = tid & 31, ly = lt >> 2, lq = lt & 3;
+  // the place this thread writes at the end (threads < NP): block place tid
+  const int wx = tid % kAinvTX, wy = tid / kAinvTX;
+  const bool mine = tid < NP && tx * kAinvTX + wx < nx && ty * kAinvTY + wy < ny;
+  T* o = om + b * bs + (x0 + wx) + (long)(y0 + wy) * g.mm1;
   const T old = mine ? *o : T(0);
   const T* rb = rr + b * rbs;
   // every staging load of the thread is issued before the first store
-  constexpr int U = (kAinvW * kAinvW + NT - 1) / NT, UK = ((kAinvRmax + 1) * (kAinvRmax + 1) + NT - 1) / NT;
+  constexpr int NT = kAinvThreads;
+  constexpr int U = (kAinvWX * kAinvWY + NT - 1) / NT, UK = ((kAinvRmax + 1) * (kAinvRmax + 1) + NT - 1) / NT;
   double v[U], kv[UK];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const int e = threadIdx.x + NT * u, y = e / W, x = e - y * W, ii = x0 - R + x, jj = y0 - R + y;
-    v[u] = (e < W * W && ii >= 0 && ii < g.mm1 && jj >= 0 && jj < g.nm1) ? (double)rb[ii + (long)jj * g.mm1] : 0.0;
+    const int e = tid + NT * u, y = e / W, x = e - y * W, ii = x0 - R + x, jj = y0 - R + y;
+    v[u] = (e < W * H && ii >= 0 && ii < g.mm1 && jj >= 0 && jj < g.nm1) ? (double)rb[ii + (long)jj * g.mm1] : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < UK; u++) {
-    const int e = threadIdx.x + NT * u;
+    const int e = tid + NT * u;
     kv[u] = e < R1 * R1 ? K[(long)b * R1 * R1 + e] : 0.0;
   }
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const int e = threadIdx.x + NT * u;
-    if (e < W * W) rs[e] = v[u];
+    const int e = tid + NT * u;
+    if (e < W * H) rs[e] = v[u];
   }
 #pragma unroll
   for (int u = 0; u < UK; u++) {
-    const int e = threadIdx.x + NT * u;
+    const int e = tid + NT * u;
     if (e < R1 * R1) ks[e] = kv[u];
   }
   __syncthreads();
-  // wave w takes the rows dy = w mod 16 (two at most while R <= 15), so a
-  // case's sums do not depend on the launch's R (rows and places past the
-  // case's own reach add exact zeros): a case in a batch gives its
-  // single-case result bit for bit.  Even and odd dx go to two chains.
-  double acc = 0.0;
-  for (int dy = -R + ((w + R) & (kAinvWaves - 1)); dy <= R; dy += kAinvWaves) {
+  // group grp takes the rows dy = grp mod 16 in ascending order.  For place
+  // p of the thread (staged centre column c = 4 lq + p + R) and row dy the
+  // sum is two chains: a0 = K(0) rr(c) + the even dx, a1 = the odd dx, each
+  // term K(dx) (rr(c - dx) + rr(c + dx)), dx ascending; then acc += a0 + a1.
+  // The left and right values of the four places are two windows that slide
+  // apart by one column per dx (one new LDS value each).
+  double acc[kAinvPX];
+#pragma unroll
+  for (int p = 0; p < kAinvPX; p++) acc[p] = 0.0;
+  for (int dy = -R + ((grp + R) & (kAinvGroups - 1)); dy <= R; dy += kAinvGroups) {
     const double* kr = ks + (dy < 0 ? -dy : dy) * R1;
-    const double* rrow = rs + (ly + R - dy) * W + lx + R;
-    double a0 = kr[0] * rrow[0], a1 = 0.0;
-    int dx = 1;
-#pragma unroll 4
-    for (; dx + 1 <= R; dx += 2) {
-      a1 = fma(kr[dx], rrow[-dx] + rrow[dx], a1);
-      a0 = fma(kr[dx + 1], rrow[-dx - 1] + rrow[dx + 1], a0);
+    const double* rrow = rs + (ly + R - dy) * W + kAinvPX * lq + R;  // centre of place p = 0
+    double a0[kAinvPX], a1[kAinvPX], lw[kAinvPX], rw[kAinvPX];
+#pragma unroll
+    for (int p = 0; p < kAinvPX; p++) {
+      a0[p] = kr[0] * rrow[p];
+      a1[p] = 0.0;
     }
-    if (dx <= R) a1 = fma(kr[dx], rrow[-dx] + rrow[dx], a1);
-    acc += a0 + a1;
+    if (R >= 1) {
+#pragma unroll
+      for (int p = 0; p < kAinvPX; p++) {  // dx = 1
+        lw[p] = rrow[p - 1];
+        rw[p] = rrow[p + 1];
+      }
+    }
+    // lw[p] = rr(c_p - dx), rw[p] = rr(c_p + dx); the step to dx + 1 reads
+    // rr(c_0 - dx - 1) and rr(c_3 + dx + 1) only
+    auto slide = [&](int dx) {
+#pragma unroll
+      for (int p = kAinvPX - 1; p > 0; p--) lw[p] = lw[p - 1];
+#pragma unroll
+      for (int p = 0; p + 1 < kAinvPX; p++) rw[p] = rw[p + 1];
+      lw[0] = rrow[-dx - 1];
+      rw[kAinvPX - 1] = rrow[kAinvPX - 1 + dx + 1];
+    };
+    int dx = 1;
+    for (; dx + 1 <= R; dx += 2) {
+      const double k1 = kr[dx], k2 = kr[dx + 1];
+#pragma unroll
+      for (int p = 0; p < kAinvPX; p++) a1[p] = fma(k1, lw[p] + rw[p], a1[p]);
+      slide(dx);
+#pragma unroll
+      for (int p = 0; p < kAinvPX; p++) a0[p] = fma(k2, lw[p] + rw[p], a0[p]);
+      if (dx + 2 <= R) slide(dx + 1);
+    }
+    if (dx <= R) {
+      const double k1 = kr[dx];
+#pragma unroll
+      for (int p = 0; p < kAinvPX; p++) a1[p] = fma(k1, lw[p] + rw[p], a1[p]);
+    }
+#pragma unroll
+    for (int p = 0; p < kAinvPX; p++) acc[p] += a0[p] + a1[p];
   }
-  red[w][lane] = acc;
+#pragma unroll
+  for (int p = 0; p < kAinvPX; p++) red[grp][ly * kAinvTX + kAinvPX * lq + p] = acc[p];
   __syncthreads();
   if (mine) {
-    double p[kAinvWaves];
+    double q[kAinvGroups];
 #pragma unroll
-    for (int x = 0; x < kAinvWaves; x++) p[x] = red[x][lane];
+    for (int x = 0; x < kAinvGroups; x++) q[x] = red[x][tid];
 #pragma unroll
-    for (int h = kAinvWaves / 2; h > 0; h /= 2)
+    for (int h = kAinvGroups / 2; h > 0; h /= 2)
 #pragma unroll
-      for (int x = 0; x < h; x++) p[x] = p[x] + p[x + h];
-    *o = old - T(p[0]);
+      for (int x = 0; x < h; x++) q[x] = q[x] + q[x + h];
+    *o = old - T(q[0]);
   }
 }
 
