@@ -81,6 +81,14 @@ struct Engine : GroupBase {
   static constexpr bool kSolve64 = sizeof(T) == 4;
   FftSet<double> fs64;
   DevBuf<double> laminv64, S64, U1, U2;
+  // step 2 of the FP64 streamfunctions needs the levels below the finest
+  // only on their rings (get_bc of the next finer level, the force system's
+  // get_bc(s_2)): there the last row pass gives way to the ring lines of the
+  // column pair's output (ctci64 with ring; r2V: their ring vectors, r2Sin:
+  // sin(pi t / m)).  Step 4 computes every level whole.  IBFS_STEP2_RING=0
+  // keeps the row pass.
+  bool ring2 = true;
+  DevBuf<double> r2V, r2Sin;
 
   DevBuf<int> Lc;
   const int* gate = nullptr;  // Lc.p, or nullptr when every case has Lmax levels
@@ -295,7 +303,14 @@ struct Engine : GroupBase {
       S64.alloc(W);
       S64.zero(st);
       U1.alloc((size_t)B * nw);
-      U2.alloc((size_t)B * nw);
+      U2.alloc((size_t)2 * B * nw);  // two fields per case for ctci64's ring form
+      if (const char* r2 = std::getenv("IBFS_STEP2_RING")) ring2 = r2[0] != '0';
+      ring2 = ring2 && m % 4 == 0 && n % 4 == 0 && Lmax >= 2;
+      if (ring2) {
+        r2V.alloc((size_t)2 * B * ring_count(m, n));
+        r2Sin.alloc((size_t)2 * m);
+        hipLaunchKernelGGL(k_sinpi_table, dim3((2 * m + 255) / 256), dim3(256), 0, st, m, r2Sin.p);
+      }
     }
     T2.alloc((size_t)B * nw);
     T3.alloc((size_t)B * nw);
@@ -1186,17 +1201,24 @@ struct Engine : GroupBase {
   // ctci of level lev in FP64 from the FP32 vorticity om and the coarser
   // level's streamfunction sc (element type TC), into S64 level lev: the
   // three passes of ctci_like in FP64 with the FP64 laminv.
+  // ring: only the ring places of S64 level lev, from the column pair's
+  // output by the ring lines (which read it as level 1 of two-level items:
+  // the pair writes at stride 2 nw)
   template <typename TC>
-  void ctci64(const T* om, const TC* sc, int batch, const int* gate, int lev) {
+  void ctci64(const T* om, const TC* sc, int batch, const int* gate, int lev, bool ring = false) {
     const long bs = fbs();
     launch<0>(RowOp<double, SrcVortBC<T, double, TC>, DstField<double>>{
                   geo, SrcVortBC<T, double, TC>{om + (long)lev * nw, sc, bs, gate, lev}, DstField<double>{U1.p, nw},
                   gate, lev},
               geo.nm1, 2, batch);
     launch_col2(Col2Op<double, SrcField<double>, MidScale<double>, DstField<double>>{
-                    geo, SrcField<double>{U1.p, nw}, MidScale<double>{laminv64.p}, DstField<double>{U2.p, nw}, gate,
-                    lev},
+                    geo, SrcField<double>{U1.p, nw}, MidScale<double>{laminv64.p},
+                    DstField<double>{ring ? U2.p + nw : U2.p, ring ? 2L * nw : (long)nw}, gate, lev},
                 geo.mm1, 2, batch);
+    if (ring) {
+      ring_lines(U2.p, 2, batch, r2V.p, r2Sin.p, S64.p + (long)lev * nw, bs);
+      return;
+    }
     launch<0>(RowOp<double, SrcField<double>, DstField<double>>{geo, SrcField<double>{U2.p, nw},
                                                                 DstField<double>{S64.p + (long)lev * nw, bs}, gate,
                                                                 lev},
@@ -1214,8 +1236,12 @@ struct Engine : GroupBase {
   // pair's output T2L: the bottom and top lines by one FFT per item where
   // the row length has a size-specialised pass of 256 threads
   // (k_ring_fromY_fft), by direct sums otherwise (k_ring_fromY)
-  void ring_fromY() {
-    const int NR = ringN, n2 = fset<double>().pm.n2;
+  void ring_fromY() { ring_lines(T2L.p, Lmax, B * (Lmax - 1), ringV.p, ringSin.p, nullptr, 0); }
+  // the ring lines of the items of Y (lm levels per case, levels 1..lm-1;
+  // nitems = cases (lm - 1)) into V (ring_count entries per level), and at
+  // their places of fld + case * fb when fld is set
+  void ring_lines(const double* Y, int lm, int nitems, double* V, const double* sintab, double* fld, long fb) {
+    const int NR = ring_count(m, n), n2 = fset<double>().pm.n2;
     const dim3 blk(256);
     const size_t lds = sizeof(double) * 2 * m;
     const int nlrg = (n / 2 + 1 + 3) / 4;
@@ -1223,22 +1249,22 @@ struct Engine : GroupBase {
     if (fset<double>().spec_m && ring_fft_lines) {
       switch (n2) {
         case 256:
-          hipLaunchKernelGGL(k_ring_fromY_fft<256>, dim3(nlrg + 1, B * (Lmax - 1)), blk, lds, st, geo, T2L.p, Lmax,
-                             NR, ringSin.p, ringV.p, tw, nlrg);
+          hipLaunchKernelGGL(k_ring_fromY_fft<256>, dim3(nlrg + 1, nitems), blk, lds, st, geo, Y, lm, NR, sintab, V,
+                             tw, nlrg, fld, fb);
           return;
         case 512:
-          hipLaunchKernelGGL(k_ring_fromY_fft<512>, dim3(nlrg + 1, B * (Lmax - 1)), blk, lds, st, geo, T2L.p, Lmax,
-                             NR, ringSin.p, ringV.p, tw, nlrg);
+          hipLaunchKernelGGL(k_ring_fromY_fft<512>, dim3(nlrg + 1, nitems), blk, lds, st, geo, Y, lm, NR, sintab, V,
+                             tw, nlrg, fld, fb);
           return;
         case 1024:
-          hipLaunchKernelGGL(k_ring_fromY_fft<1024>, dim3(nlrg + 1, B * (Lmax - 1)), blk, lds, st, geo, T2L.p, Lmax,
-                             NR, ringSin.p, ringV.p, tw, nlrg);
+          hipLaunchKernelGGL(k_ring_fromY_fft<1024>, dim3(nlrg + 1, nitems), blk, lds, st, geo, Y, lm, NR, sintab, V,
+                             tw, nlrg, fld, fb);
           return;
         default: break;
       }
     }
-    hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, B * (Lmax - 1)), blk, lds, st, geo, T2L.p,
-                       Lmax, NR, ringSin.p, ringV.p);
+    hipLaunchKernelGGL(k_ring_fromY, dim3((ring_fromY_tasks(m, n) + 3) / 4, nitems), blk, lds, st, geo, Y, lm, NR,
+                       sintab, V, fld, fb);
   }
   // the edge spectra of Interp(R_{k+1}) for the items of im: by FFT on
   // square grids whose row length has a size-specialised pass of 256
@@ -1350,10 +1376,11 @@ struct Engine : GroupBase {
   }
   // vort2flux streamfunctions of an FP32 engine with every level solved in
   // FP64 into S64 (steps 2 and 4: the force system and the stored fluxes)
-  void vort2flux64(T* om, int batch, const int* gate, bool inc = false) {
+  void vort2flux64(T* om, int batch, const int* gate, bool inc = false, bool ring = false) {
     coarsify(om, batch, gate, inc);
     for (int lev = Lmax - 1; lev >= 0; lev--)
-      ctci64<double>(om, (lev + 1 < Lmax) ? S64.p + (long)(lev + 1) * nw : nullptr, batch, gate, lev);
+      ctci64<double>(om, (lev + 1 < Lmax) ? S64.p + (long)(lev + 1) * nw : nullptr, batch, gate, lev,
+                     ring && lev > 0);
   }
   // --------------------------------------------------------------- one step
   void enqueue_step(int first) override {
@@ -1556,7 +1583,7 @@ struct Engine : GroupBase {
       }
     };
     if constexpr (kSolve64) {
-      vort2flux64(omega.p, B, gate);
+      vort2flux64(omega.p, B, gate, false, ring2);
       force((const double*)S64.p);
     } else if (v2f_batch) {
       vort2flux_levels(false);

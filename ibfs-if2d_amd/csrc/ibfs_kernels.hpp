@@ -1258,9 +1258,11 @@ __device__ __forceinline__ void stage_lds(double* dst, int n, F&& load) {
   }
 }
 
+// With fld set, every ring value is also written at its place of the
+// field fld + case * fbs (the corners once, by the bottom and top lines).
 __device__ __forceinline__ void ring_fromY_waves(const Geo& g, const double* __restrict__ Y, int lm, int NR,
                                                  const double* __restrict__ sintab, double* __restrict__ V,
-                                                 bool lr_only) {
+                                                 bool lr_only, double* __restrict__ fld = nullptr, long fbs = 0) {
   extern __shared__ double sn[];  // sin(pi t / m), t < 2m
   const int b = blockIdx.y / (lm - 1), k = 1 + blockIdx.y % (lm - 1);
   const double* y = Y + ((long)b * lm + k) * g.nw;
@@ -1314,6 +1316,10 @@ __device__ __forceinline__ void ring_fromY_waves(const Geo& g, const double* __r
     if (lane == 0) {
       out[2 * nb + task] = 2.0 * aL;
       out[2 * nb + nl + task] = 2.0 * aR;
+      if (fld != nullptr && task > 0 && task < nl - 1) {
+        fld[b * fbs + (m / 4 - 1) + (long)jj * K] = 2.0 * aL;
+        fld[b * fbs + (3 * m / 4 - 1) + (long)jj * K] = 2.0 * aR;
+      }
     }
   } else {
     int fr[kRingRowOut], st[kRingRowOut];
@@ -1343,13 +1349,17 @@ __device__ __forceinline__ void ring_fromY_waves(const Geo& g, const double* __r
 #pragma unroll
     for (int u = 0; u < kRingRowOut; u++) {
       const double t = wave_sum(a[u]);
-      if (lane == 0 && c0 + u < nb) out[line * nb + c0 + u] = 2.0 * t;
+      if (lane == 0 && c0 + u < nb) {
+        out[line * nb + c0 + u] = 2.0 * t;
+        if (fld != nullptr) fld[b * fbs + (m / 4 - 1 + c0 + u) + (long)jj * K] = 2.0 * t;
+      }
     }
   }
 }
 __global__ __launch_bounds__(256) void k_ring_fromY(Geo g, const double* __restrict__ Y, int lm, int NR,
-                                                    const double* __restrict__ sintab, double* __restrict__ V) {
-  ring_fromY_waves(g, Y, lm, NR, sintab, V, false);
+                                                    const double* __restrict__ sintab, double* __restrict__ V,
+                                                    double* __restrict__ fld, long fbs) {
+  ring_fromY_waves(g, Y, lm, NR, sintab, V, false, fld, fbs);
 }
 
 // The bottom and top lines of the ring vectors as one DST-I of the two rows
@@ -1369,11 +1379,17 @@ struct SrcRingLines {
 };
 struct DstRingLines {
   double* V; int lm, NR; Geo gf;
+  double* fld = nullptr;  // also at the places of fld + case * fbs when set
+  long fbs = 0;
   __device__ void operator()(const Geo&, int b, int ii, int row, double v) const {
     const int nb = gf.m / 2 + 1, pos = ii - (gf.m / 4 - 1);
     if (pos < 0 || pos >= nb) return;
     const int c = b / (lm - 1), k = 1 + b % (lm - 1);
     V[((long)c * lm + k) * NR + row * nb + pos] = v;
+    if (fld != nullptr) {
+      const int jj = row == 0 ? gf.n / 4 - 1 : 3 * gf.n / 4 - 1;
+      fld[c * fbs + ii + (long)jj * gf.mm1] = v;
+    }
   }
 };
 // k_ring_fromY with the bottom and top lines from one FFT per item: the
@@ -1384,19 +1400,20 @@ struct DstRingLines {
 template <int N2>
 __global__ __launch_bounds__(256) void k_ring_fromY_fft(Geo g, const double* __restrict__ Y, int lm, int NR,
                                                         const double* __restrict__ sintab, double* __restrict__ V,
-                                                        const cplx<double>* __restrict__ tw, int nlrg) {
+                                                        const cplx<double>* __restrict__ tw, int nlrg,
+                                                        double* __restrict__ fld, long fbs) {
   constexpr int P = 1024 / N2;
   static_assert(P * N2 / 4 == 256, "k_ring_fromY_fft: one pass per workgroup of 256 threads");
   if ((int)blockIdx.x >= nlrg) {
     __shared__ cplx<double> z[(spec::kTwoBuf<double, N2, P> ? 2 : 1) * P * N2];
     Geo g2 = g;
     g2.nm1 = 2;  // the pass's two rows
-    const RowOp<double, SrcRingLines, DstRingLines> op{g2, SrcRingLines{Y, lm, g}, DstRingLines{V, lm, NR, g},
-                                                       nullptr, 0};
+    const RowOp<double, SrcRingLines, DstRingLines> op{g2, SrcRingLines{Y, lm, g},
+                                                       DstRingLines{V, lm, NR, g, fld, fbs}, nullptr, 0};
     spec::dst_pass_body<double, N2, P, 0, RowOp<double, SrcRingLines, DstRingLines>>(op, tw, z, 0, 1);
     return;
   }
-  ring_fromY_waves(g, Y, lm, NR, sintab, V, true);
+  ring_fromY_waves(g, Y, lm, NR, sintab, V, true, fld, fbs);
 }
 
 // R_k = V_k + sum_{j=1}^{L-1-k} T^j V_{k+j} for k = 1..L-1 of case b

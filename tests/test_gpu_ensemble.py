@@ -83,6 +83,33 @@ def test_equivalent_forms_in_an_uneven_batch(monkeypatch, switch):
             assert np.array_equal(s1[i][key], s0[i][key]), f"case {i} {key}"
 
 
+@pytest.mark.parametrize("name", ["naca48x40", "cyl40r5"])
+def test_mixed_step2_on_the_rings(name, monkeypatch):
+    """The mixed form's step 2 needs the FP64 streamfunctions of the levels
+    below the finest only on their rings (get_bc of the next finer level):
+    the ring lines of the column pair's output (the default) or the whole
+    last row pass (IBFS_STEP2_RING=0) give those values to FP64 rounding, so
+    after 20 steps every field is within the mixed form's 1e-5 of the other,
+    and the force solution differs (the ring form was taken)."""
+    from ibfs_amd import hip
+
+    a = case_args(name)
+    g64 = hip.Solver(**a)
+    cm, cv = g64.build_cholesky()
+    runs = []
+    for on in ("1", "0"):
+        monkeypatch.setenv("IBFS_STEP2_RING", on)
+        g = hip.Solver(**a, precision=hip.MIXED)
+        g.set_cholesky(cm, cv)
+        runs.append((g.advance(20), g.get_state()))
+    (h1, s1), (h0, s0) = runs
+    nb2 = 2 * g64.nbody
+    assert np.max(np.abs(h1[:, :nb2] - h0[:, :nb2])) < 1e-5 * np.max(np.abs(h0[:, :nb2]))
+    for k in ("omega", "q", "fb"):
+        assert relerr(s1[k], s0[k]) < 1e-5, k
+    assert not np.array_equal(s1["fb"], s0["fb"])
+
+
 def test_ensemble_factor_equals_single_factor():
     from ibfs_amd import hip
 
