@@ -88,7 +88,7 @@ struct Engine : GroupBase {
   // sin(pi t / m)).  Step 4 computes every level whole.  IBFS_STEP2_RING=0
   // keeps the row pass.
   bool ring2 = true;
-  DevBuf<double> r2V, r2Sin;
+  DevBuf<double> r2V, r2Sin, r2Y;  // r2Y: the FP64 engine's column pairs, two fields per case
 
   DevBuf<int> Lc;
   const int* gate = nullptr;  // Lc.p, or nullptr when every case has Lmax levels
@@ -304,13 +304,14 @@ struct Engine : GroupBase {
       S64.zero(st);
       U1.alloc((size_t)B * nw);
       U2.alloc((size_t)2 * B * nw);  // two fields per case for ctci64's ring form
-      if (const char* r2 = std::getenv("IBFS_STEP2_RING")) ring2 = r2[0] != '0';
-      ring2 = ring2 && m % 4 == 0 && n % 4 == 0 && Lmax >= 2;
-      if (ring2) {
-        r2V.alloc((size_t)2 * B * ring_count(m, n));
-        r2Sin.alloc((size_t)2 * m);
-        hipLaunchKernelGGL(k_sinpi_table, dim3((2 * m + 255) / 256), dim3(256), 0, st, m, r2Sin.p);
-      }
+    }
+    if (const char* r2 = std::getenv("IBFS_STEP2_RING")) ring2 = r2[0] != '0';
+    ring2 = ring2 && m % 4 == 0 && n % 4 == 0 && Lmax >= 2;
+    if (ring2) {
+      r2V.alloc((size_t)2 * B * ring_count(m, n));
+      r2Sin.alloc((size_t)2 * m);
+      hipLaunchKernelGGL(k_sinpi_table, dim3((2 * m + 255) / 256), dim3(256), 0, st, m, r2Sin.p);
+      if constexpr (!kSolve64) r2Y.alloc((size_t)2 * B * nw);
     }
     T2.alloc((size_t)B * nw);
     T3.alloc((size_t)B * nw);
@@ -1185,11 +1186,25 @@ struct Engine : GroupBase {
   // vort2flux (src/operators.f90:191-240) on a batch of multi-level fields om
   // (stride Lmax*nw); streamfunctions go to s (same layout); when vel is set,
   // curl(s) is written on every level (stride Lmax*nq).
-  void vort2flux(T* om, T* s, T* vel, int batch, const int* gate, T* t1, T* t2, bool inc = false) {
+  // ring (FP64 engines): the levels below the finest only on their rings,
+  // as ctci64's ring form (step 2)
+  void vort2flux(T* om, T* s, T* vel, int batch, const int* gate, T* t1, T* t2, bool inc = false,
+                 bool ring = false) {
     const long bs = fbs();
     coarsify(om, batch, gate, inc);
     for (int lev = Lmax - 1; lev >= 0; lev--) {
       T* sc = (lev + 1 < Lmax) ? s + (long)(lev + 1) * nw : nullptr;
+      if constexpr (!kSolve64) {
+        if (ring && lev > 0) {
+          row_pass(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, DstField<T>{t1, nw}, batch, gate, lev);
+          launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{geo, SrcField<T>{t1, nw}, MidScale<T>{laminv.p},
+                                                                        DstField<T>{r2Y.p + nw, 2L * nw}, gate,
+                                                                        lev},
+                      geo.mm1, 2, batch);
+          ring_lines(r2Y.p, 2, batch, r2V.p, r2Sin.p, s + (long)lev * nw, bs);
+          continue;
+        }
+      }
       ctci_like(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, false, DstField<T>{s + (long)lev * nw, bs}, t1,
                 t2, batch, gate, lev);
     }
@@ -1589,7 +1604,7 @@ struct Engine : GroupBase {
       vort2flux_levels(false);
       force((const T*)S.p);
     } else {
-      vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p);
+      vort2flux(omega.p, S.p, nullptr, B, gate, T2.p, T3.p, false, ring2);
       force((const T*)S.p);
     }
   }

@@ -443,6 +443,32 @@ def test_ring_chain_on_symmetry_representatives(name, monkeypatch):
     assert not np.array_equal(s1["omega"], s0["omega"])
 
 
+@pytest.mark.parametrize("name", ["rect64x48L5", "cyl64L4"])
+def test_fp64_step2_on_the_rings(name, monkeypatch):
+    """Step 2 of the level-by-level FP64 vort2flux (IBFS_V2F_LEVELS=0, the
+    form of batches) computes the levels below the finest only on their
+    rings, by the ring lines of the column pair's output (the default), or
+    whole (IBFS_STEP2_RING=0): the same values to rounding, so 10 steps agree
+    to rounding and the force solution differs (the ring form was taken)."""
+    from ibfs_amd import hip
+
+    monkeypatch.setenv("IBFS_V2F_LEVELS", "0")
+    a = v2f_args(name)
+    g1 = hip.Solver(**a)
+    cm, cv = g1.build_cholesky()
+    monkeypatch.setenv("IBFS_STEP2_RING", "0")
+    g0 = hip.Solver(**a)
+    g0.set_cholesky(cm, cv)
+    h1, h0 = g1.advance(10), g0.advance(10)
+    s1, s0 = g1.get_state(), g0.get_state()
+    nb2 = 2 * g1.nbody
+    assert np.max(np.abs(h1[:, :nb2] - h0[:, :nb2])) < 1e-12 * np.max(np.abs(h0[:, :nb2]))
+    for k in ("omega", "q"):
+        assert relerr(s1[k], s0[k]) < 1e-13, k
+    assert relerr(s1["fb"], s0["fb"]) < 1e-12
+    assert not np.array_equal(s1["fb"], s0["fb"])
+
+
 @pytest.mark.parametrize("name", ["rect64x48L5", "cyl64L4", "cyl40r5"])
 def test_coarsify_tiles_are_bit_identical(name, monkeypatch):
     """The coarsify chain two levels per launch in LDS tiles (the default)
