@@ -1195,13 +1195,14 @@ struct Engine : GroupBase {
     for (int lev = Lmax - 1; lev >= 0; lev--) {
       T* sc = (lev + 1 < Lmax) ? s + (long)(lev + 1) * nw : nullptr;
       if constexpr (!kSolve64) {
-        if (ring && lev > 0) {
+        if (ring && (lev > 0 || swin_ok)) {
           row_pass(SrcVortBC<T>{om + (long)lev * nw, sc, bs, gate, lev}, DstField<T>{t1, nw}, batch, gate, lev);
           launch_col2(Col2Op<T, SrcField<T>, MidScale<T>, DstField<T>>{geo, SrcField<T>{t1, nw}, MidScale<T>{laminv.p},
                                                                         DstField<T>{r2Y.p + nw, 2L * nw}, gate,
                                                                         lev},
                       geo.mm1, 2, batch);
-          ring_lines(r2Y.p, 2, batch, r2V.p, r2Sin.p, s + (long)lev * nw, bs);
+          if (lev > 0) ring_lines(r2Y.p, 2, batch, r2V.p, r2Sin.p, s + (long)lev * nw, bs);
+          else last_row_window(r2Y.p + nw, 2L * nw, s, bs, batch, gate);
           continue;
         }
       }
@@ -1216,9 +1217,22 @@ struct Engine : GroupBase {
   // ctci of level lev in FP64 from the FP32 vorticity om and the coarser
   // level's streamfunction sc (element type TC), into S64 level lev: the
   // three passes of ctci_like in FP64 with the FP64 laminv.
+  // the last row pass of level 0 in step 2 on the force stencils' rows
+  // swin[0]..swin[1] only (the force right-hand side reads s_1 nowhere
+  // else; the window starts on an even row and ends on an odd one, so it
+  // pairs rows into FFTs as the whole pass does: the same values)
+  // (Y: the column pair's output, ys apart per case)
+  void last_row_window(const double* Y, long ys, double* s0, long bs, int batch, const int* gate) {
+    Geo gr = geo;
+    gr.nm1 = swin[1] - swin[0] + 1;
+    launch<0>(RowOp<double, SrcRowWin<SrcField<double>>, DstRowWin<DstField<double>>>{
+                  gr, {SrcField<double>{Y, ys}, geo, swin[0]}, {DstField<double>{s0, bs}, geo, swin[0]}, gate, 0},
+              gr.nm1, 2, batch);
+  }
   // ring: only the ring places of S64 level lev, from the column pair's
   // output by the ring lines (which read it as level 1 of two-level items:
-  // the pair writes at stride 2 nw)
+  // the pair writes at stride 2 nw); on level 0 the last row pass on the
+  // force stencils' rows
   template <typename TC>
   void ctci64(const T* om, const TC* sc, int batch, const int* gate, int lev, bool ring = false) {
     const long bs = fbs();
@@ -1226,12 +1240,14 @@ struct Engine : GroupBase {
                   geo, SrcVortBC<T, double, TC>{om + (long)lev * nw, sc, bs, gate, lev}, DstField<double>{U1.p, nw},
                   gate, lev},
               geo.nm1, 2, batch);
+    const bool two = ring && (lev > 0 || swin_ok);  // the pair at stride 2 nw
     launch_col2(Col2Op<double, SrcField<double>, MidScale<double>, DstField<double>>{
                     geo, SrcField<double>{U1.p, nw}, MidScale<double>{laminv64.p},
-                    DstField<double>{ring ? U2.p + nw : U2.p, ring ? 2L * nw : (long)nw}, gate, lev},
+                    DstField<double>{two ? U2.p + nw : U2.p, two ? 2L * nw : (long)nw}, gate, lev},
                 geo.mm1, 2, batch);
-    if (ring) {
-      ring_lines(U2.p, 2, batch, r2V.p, r2Sin.p, S64.p + (long)lev * nw, bs);
+    if (two) {
+      if (lev > 0) ring_lines(U2.p, 2, batch, r2V.p, r2Sin.p, S64.p + (long)lev * nw, bs);
+      else last_row_window(U2.p + nw, 2L * nw, S64.p, bs, batch, gate);
       return;
     }
     launch<0>(RowOp<double, SrcField<double>, DstField<double>>{geo, SrcField<double>{U2.p, nw},
@@ -1394,8 +1410,7 @@ struct Engine : GroupBase {
   void vort2flux64(T* om, int batch, const int* gate, bool inc = false, bool ring = false) {
     coarsify(om, batch, gate, inc);
     for (int lev = Lmax - 1; lev >= 0; lev--)
-      ctci64<double>(om, (lev + 1 < Lmax) ? S64.p + (long)(lev + 1) * nw : nullptr, batch, gate, lev,
-                     ring && lev > 0);
+      ctci64<double>(om, (lev + 1 < Lmax) ? S64.p + (long)(lev + 1) * nw : nullptr, batch, gate, lev, ring);
   }
   // --------------------------------------------------------------- one step
   void enqueue_step(int first) override {
