@@ -2717,7 +2717,11 @@ __device__ double dot_in_order(int nf, const double* __restrict__ a, const doubl
 }
 
 // r = b - Ax; d = r; delta_new = r . d; iter = 0 (src/operators.f90:407-413)
-constexpr int kGemvCols = 8;  // columns per slice of k_gemv_dense (nslice = ceil(nf / 8))
+// columns per slice of k_gemv_dense (nslice = ceil(nf / 32)): the single
+// workgroup of k_cg_iter sums nslice partials per force, so few slices keep
+// its chain of loads short (522 forces, CG at 512^2 x 5: 8 columns 320
+// steps/s, 16 406, 32 409, 64 344, too few workgroups for the product)
+constexpr int kGemvCols = 32;
 // q = A x with a dense A (column-major, nf x nf) for the CG form when no
 // body moves: workgroup (r, s) takes 256 rows and the s-th of nslice column
 // slices, and writes its partial sums to part[s * nf + row]; the CG kernels
