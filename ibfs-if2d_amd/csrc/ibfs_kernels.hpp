@@ -2757,7 +2757,7 @@ static __global__ __launch_bounds__(256) void k_gemv_dense(int nf, const double*
 __device__ __forceinline__ void form_q(int nf, const double* __restrict__ part, int nslice, double* __restrict__ q) {
   for (int f = threadIdx.x; f < nf; f += blockDim.x) {
     double acc = 0.0;
-#pragma unroll 8
+#pragma unroll 16
     for (int sl = 0; sl < nslice; sl++) acc += part[(long)sl * nf + f];
     q[f] = acc;
   }
